@@ -1,0 +1,171 @@
+/*
+ * sevp.h -- C ABI of the MI355X SEVP (Stabilized Error Vector Propagation)
+ * barotropic-pressure solver, a drop-in for the TIMCOM solver calls.
+ *
+ * Conventions shared by every entry point
+ *   - Arrays are Fortran (column-major) arrays, passed as plain pointers.
+ *     Coefficients AL/AB/AC/AR/AT (the reference's AX/AY/BB/CX/CY) and the
+ *     right-hand side are float (REAL*4) with shape (I2, J2); the solution X
+ *     and all influence matrices are double (REAL*8).  X carries one halo
+ *     column and row on every side: shape (I2+2, J2+2).
+ *   - IE holds the block row ends IE(1..NB0) with the reference's meaning
+ *     (PREP/template/GLOBAL/src/prep_GLOBAL.f90:147-168): strictly increasing,
+ *     IE(NB0) = J2+1.
+ *   - Every function returns 0 on success or a negative SEVP_E* code;
+ *     sevp_last_error() describes the most recent failure of the calling
+ *     thread.  Numerical problems are not errors: zero pivots met by the
+ *     Gauss-Jordan inversion are counted and reported, as the reference
+ *     divides by them silently.
+ *   - The "_dev" entry points take device pointers and enqueue on the plan's
+ *     stream (or the stream given); the others take host pointers and return
+ *     when the result is back in host memory.
+ *   - A plan is used by one host thread at a time.  Calls on one plan are
+ *     ordered on its stream.
+ */
+#ifndef SEVP_H
+#define SEVP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SEVP_OK 0
+#define SEVP_E_ARG (-1)      /* invalid argument or shape */
+#define SEVP_E_HIP (-2)      /* HIP runtime error */
+#define SEVP_E_NOMEM (-3)    /* device allocation failed */
+#define SEVP_E_STATE (-4)    /* plan not ready (influence matrices missing) */
+#define SEVP_E_NOTCONV (-5)  /* refinement did not reach the tolerance */
+
+/* Solver form (the reference's FL_EVP_STP). */
+#define SEVP_FORM_FULL_PERIODIC 0 /* REP1 + PRE1: full width, periodic east-west (FL_EVP_STP=0) */
+#define SEVP_FORM_BIR 1           /* REPBIR + PRE on NBIR overlapping strips, multiplicative sweeps (FL_EVP_STP=1) */
+#define SEVP_FORM_FULL 2          /* REP + PRE with M2=I2: full width, Dirichlet east-west halo columns */
+
+/* Arithmetic of the influence-matrix corrections. */
+#define SEVP_PREC_FP64 0 /* plain: reference operation order, bitwise equal to the reference */
+#define SEVP_PREC_FP32 1 /* corrections as FP32 MFMA GEMMs; marching and residual FP64 */
+#define SEVP_PREC_BF16 2 /* corrections as BF16-input, FP32-accumulate MFMA GEMMs */
+
+typedef struct sevp_plan_s *sevp_plan;
+
+typedef struct {
+  int i2, j2;      /* interior grid size (I2 x J2) */
+  int nb0;         /* number of block rows NB0 */
+  const int *ie;   /* IE(1..NB0), host pointer, copied */
+  int form;        /* SEVP_FORM_* */
+  int nbir;        /* number of BIR strips (SEVP_FORM_BIR); IBIR = 2*I2/NBIR */
+  int nsweep;      /* BIR sweeps per solve (the reference runs 5) */
+  int periodic;    /* BIR: refresh the periodic halo after the seam strip (FLAG_PERIODIC_WE) */
+  int nrhs_max;    /* largest batch of right-hand sides solved in one call */
+  int precision;   /* SEVP_PREC_* used by the correction products */
+} sevp_desc;
+
+typedef struct {
+  int iterations;        /* SEVP passes run (1 for a plain solve) */
+  double max_rel_resid;  /* max over RHS of ||s - A x||_2 / ||s||_2 at exit, or -1 if not evaluated */
+  double seconds;        /* wall time of the call */
+} sevp_stats;
+
+/* ---------------------------------------------------------------- plans */
+
+/* Create a plan and upload the operator (host pointers, each I2*J2 floats).
+ * Replaces the module allocation MAIN/template/GLOBAL/src/grid_var_GLOBAL.f90:
+ * 537-592 (VAR_ALLOCATE_SEVP) and the operator read of
+ * MAIN/template/GLOBAL/src/input_GLOBAL.f90:438-443. */
+int sevp_plan_create(const sevp_desc *desc, const float *al, const float *ab, const float *ac,
+                     const float *ar, const float *at, sevp_plan *out);
+int sevp_plan_destroy(sevp_plan plan);
+
+/* Build and invert the influence matrices on the device.  Replaces PREP's
+ * INITFS strip loop (PREP/template/GLOBAL/src/prep_GLOBAL.f90:264-317) with
+ * PRE (:457-513), PRE1 (:327-386) and MATINV (:516-568).  zero_pivots (may be
+ * NULL) receives the number of zero pivots met. */
+int sevp_build_influence(sevp_plan plan, int *zero_pivots);
+
+/* Load / export influence matrices in the reference layout (the RINV, RINV1
+ * records of the EVP file, PREP/template/GLOBAL/src/prep_GLOBAL.f90:320):
+ *   BIR : RINV(IBIR*IBIR, NB0, NBIR), RINV1(IBIR*IBIR, NB0-1, NBIR)
+ *   full: RINV(I2, I2, NB0),          RINV1(I2, I2, NB0-1)           */
+int sevp_load_influence(sevp_plan plan, const double *rinv, const double *rinv1);
+int sevp_get_influence(sevp_plan plan, double *rinv, double *rinv1);
+
+/* Zero the seam-strip scratch XX (the reference's module array starts at
+ * zero and keeps its interior between calls). */
+int sevp_reset_seam(sevp_plan plan);
+
+/* ---------------------------------------------------------------- solves */
+
+/* One solve of nrhs independent right-hand sides with the plan's form, in the
+ * reference's operation order (FP64 plans) -- replaces FS_SOLVER's solver
+ * dispatch, MAIN/template/GLOBAL/src/fs_GLOBAL.f90:1850-1966: the 5-sweep BIR
+ * strip loop (FL_EVP_STP=1) or one REP1 pass (FL_EVP_STP=0).
+ *   s : (I2, J2, nrhs) float;  x : (I2+2, J2+2, nrhs) double, initial guess in,
+ *   solution out (halo columns and rows are the Dirichlet / periodic data). */
+int sevp_solve(sevp_plan plan, const float *s, double *x, int nrhs);
+int sevp_solve_dev(sevp_plan plan, const float *s_dev, double *x_dev, int nrhs, void *hip_stream);
+
+/* Iterative refinement x <- x + SEVP(s - A x) with the residual in FP64, until
+ * every RHS has ||s - A x||_2 / ||s||_2 <= tol or max_iter passes ran.  Full
+ * forms only.  Returns SEVP_E_NOTCONV (with x holding the last iterate) if
+ * the tolerance was not met.  rel_resid (host, nrhs doubles, may be NULL)
+ * receives each RHS's final relative residual. */
+int sevp_solve_refined_dev(sevp_plan plan, const float *s_dev, double *x_dev, int nrhs,
+                           double tol, int max_iter, double *rel_resid, sevp_stats *stats);
+
+/* Relative residual ||s - A x||_2 / ||s||_2 per RHS in FP64, with the
+ * 5-point product of p_sqprod (MAIN/template/GLOBAL/src/sevp.f90:696-713). */
+int sevp_residual_dev(sevp_plan plan, const float *s_dev, const double *x_dev, int nrhs,
+                      double *rel_resid);
+
+/* ------------------------------------- one-to-one reference entry points
+ * Host pointers, the reference's argument lists and meaning; each call copies
+ * its operands to the device and back.  Scratch arguments (DUM0, DUM1, DUM2,
+ * H) are accepted for signature compatibility; the device keeps its own
+ * scratch and does not write them. */
+
+/* REPBIR, MAIN/template/GLOBAL/src/sevp.f90:218-328. */
+int sevp_repbir(const float *ax, const float *ay, const float *bb, const float *cx,
+                const float *cy, const double *rinv, const double *rinv1, double *dum0,
+                double *dum1, double *dum2, const float *f, double *h, double *x, const int *ie,
+                int i0, int i2, int m0, int m2, int nblk);
+/* REP1, MAIN/template/GLOBAL/src/sevp.f90:11-107. */
+int sevp_rep1(const float *ax, const float *ay, const float *bb, const float *cx,
+              const float *cy, const double *rinv, const double *rinv1, double *dum0,
+              double *dum1, double *dum2, const float *f, double *h, double *x, const int *ie,
+              int i0, int i2, int nblk);
+/* REP, MAIN/template/GLOBAL/src/sevp.f90:335-438. */
+int sevp_rep(const float *ax, const float *ay, const float *bb, const float *cx,
+             const float *cy, const double *rinv, const double *rinv1, double *dum0,
+             double *dum1, double *dum2, const float *f, double *h, double *x, const int *ie,
+             int i0, int i2, int nblk);
+/* REPBIR_PERIOD, MAIN/template/GLOBAL/src/sevp.f90:109-217.  Its periodic
+ * copies index column I0 of both X and H, which is only meaningful when
+ * M0 = I0 and M2 = I2 (its one call site, commented out at
+ * MAIN/template/GLOBAL/src/fs_GLOBAL.f90:1851); other shapes are rejected. */
+int sevp_repbir_period(const float *ax, const float *ay, const float *bb, const float *cx,
+                       const float *cy, const double *rinv, const double *rinv1, double *dum0,
+                       double *dum1, double *dum2, const float *f, double *h, double *x,
+                       const int *ie, int i0, int i2, int m0, int m2, int nblk);
+/* PRE, PREP/template/GLOBAL/src/prep_GLOBAL.f90:457-513 (one strip). */
+int sevp_pre(const float *ax, const float *ay, const float *bb, const float *cx,
+             const float *cy, double *rinv, double *rinv1, double *h, const int *ie, int i2,
+             int m0, int m2, int nblk);
+/* PRE1, PREP/template/GLOBAL/src/prep_GLOBAL.f90:327-386. */
+int sevp_pre1(const float *ax, const float *ay, const float *bb, const float *cx,
+              const float *cy, double *rinv, double *rinv1, double *h, const int *ie, int i0,
+              int i2, int nblk);
+/* MATINV, PREP/template/GLOBAL/src/prep_GLOBAL.f90:516-568. */
+int sevp_matinv(double *b, int n);
+
+/* ------------------------------------------------------------- misc */
+const char *sevp_last_error(void);
+const char *sevp_version(void);
+/* Device memory the plan holds, in bytes. */
+size_t sevp_plan_device_bytes(sevp_plan plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SEVP_H */

@@ -1,0 +1,826 @@
+// C ABI of the SEVP library: plans, influence-matrix builds, solves and the
+// one-to-one reference entry points.  See include/sevp.h for the contract.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sevp.h"
+#include "sevp_dev.h"
+#include "sevp_internal.h"
+
+using namespace sevp;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPC(expr)                                                                  \
+  do {                                                                              \
+    hipError_t _e = (expr);                                                         \
+    if (_e != hipSuccess)                                                           \
+      return fail(_e == hipErrorOutOfMemory ? SEVP_E_NOMEM : SEVP_E_HIP,            \
+                  std::string(#expr) + ": " + hipGetErrorString(_e));               \
+  } while (0)
+
+#define CHK(expr)              \
+  do {                         \
+    int _r = (expr);           \
+    if (_r != SEVP_OK) return _r; \
+  } while (0)
+
+
+// ---------------------------------------------------------------- device buffers
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  int alloc(size_t count) {
+    release();
+    if (count == 0) return SEVP_OK;
+    hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(SEVP_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    n = count;
+    return SEVP_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+  ~DBuf() { release(); }
+};
+
+struct sevp_plan_s {
+  sevp_desc d{};
+  std::vector<int> ie;
+  int ibir = 0;  // strip width (BIR) or i2 (full)
+  int w = 0;     // influence-matrix order
+  int nmat = 0;  // number of strips (BIR) or 1
+  hipStream_t stream = nullptr;
+  bool have_inv = false;
+  DBuf<float> coef;     // 5 * i2*j2 (AL, AB, AC, AR, AT)
+  DBuf<int> ie_d;
+  DBuf<double> rinv, rinv1;      // reference layout
+  DBuf<float> rinv32, rinv1_32;  // mixed precision copies
+  // solve scratch
+  DBuf<double> e, dum0, hfree, hcpl, g;  // full forms
+  DBuf<float> partial;                   // mixed: split-K partial sums
+  DBuf<int> counters;                    // mixed: split-K arrival counters
+  int splits = 1;
+  DBuf<double> xx, bdum0;                // BIR
+  DBuf<double> r64, dx, part;            // refinement
+  DBuf<double> b1s;                      // MATINV scratch
+  DBuf<int> zp;
+  const float *al() const { return coef.p; }
+  const float *ab() const { return coef.p + (size_t)d.i2 * d.j2; }
+  const float *ac() const { return coef.p + 2 * (size_t)d.i2 * d.j2; }
+  const float *ar() const { return coef.p + 3 * (size_t)d.i2 * d.j2; }
+  const float *at() const { return coef.p + 4 * (size_t)d.i2 * d.j2; }
+  size_t rinv_count() const { return (size_t)w * w * d.nb0 * nmat; }
+  size_t rinv1_count() const { return (size_t)w * w * (d.nb0 > 1 ? d.nb0 - 1 : 1) * nmat; }
+};
+
+static int check_ie(const int *ie, int nb0, int j2) {
+  if (!ie || nb0 < 1) return fail(SEVP_E_ARG, "IE missing or NB0 < 1");
+  int prev = 1;
+  for (int b = 0; b < nb0; ++b) {
+    if (ie[b] <= prev) return fail(SEVP_E_ARG, "IE must be strictly increasing from 2");
+    prev = ie[b];
+  }
+  if (ie[nb0 - 1] != j2 + 1) return fail(SEVP_E_ARG, "IE(NB0) must equal J2+1");
+  return SEVP_OK;
+}
+
+// ------------------------------------------------------------------ PRE driver
+// Influence matrices of `nmat` strips of width w (or one full-width periodic
+// or Dirichlet problem), the block loop of PRE / PRE1.
+struct PreCtx {
+  const float *ax, *ay, *bb, *cx, *cy;
+  int cld, i2, j2, w, nmat, periodic;
+  ColMap map;
+  double *rinv, *rinv1;  // reference layout, nmat strips
+  const int *ie;         // host
+  int nb0;
+  int *zp;               // device counter
+  double *b1s;           // MATINV scratch (nmat * w) for the global-memory path
+};
+
+static int run_pre(const PreCtx &c, hipStream_t st) {
+  const long ww = (long)c.w * c.w;
+  const long sstride = ww * c.nb0;
+  const long s1stride = ww * (c.nb0 > 1 ? c.nb0 - 1 : 1);
+  int jl = 1;
+  for (int nb = 1; nb <= c.nb0; ++nb) {
+    const int jh = c.ie[nb - 1];
+    MarchArgs m{};
+    m.ax = c.ax; m.ay = c.ay; m.bb = c.bb; m.cx = c.cx; m.cy = c.cy;
+    m.cld = c.cld;
+    m.i2 = c.i2; m.j2 = c.j2;
+    m.periodic = c.periodic;
+    m.js = jl; m.jf = jh - 2; m.ieb = jh;
+    m.w = c.w;
+    m.map = c.map;
+    m.nrhs = c.nmat;  // strip count
+    m.rinv_b = c.rinv + (long)(nb - 1) * ww;
+    m.tmat = (nb != c.nb0) ? c.rinv + (long)(c.nb0 - 1) * ww : nullptr;
+    m.rinv1_prev = (nb != 1) ? c.rinv1 + (long)(nb - 2) * ww : nullptr;
+    m.rinv_sstride = sstride;
+    m.rinv1_sstride = s1stride;
+    HIPC(launch_march(MODE_PRE, m, c.w, st));
+    HIPC(launch_matinv(c.rinv + (long)(nb - 1) * ww, sstride, c.w, c.nmat, c.zp, c.b1s, st));
+    if (nb == c.nb0) break;
+    for (int s = 0; s < c.nmat; ++s) {
+      GemmArgs gm{};
+      gm.a = c.rinv + s * sstride + (long)(nb - 1) * ww;
+      gm.a_sm = 1; gm.a_sk = c.w;
+      gm.b = c.rinv + s * sstride + (long)(c.nb0 - 1) * ww;
+      gm.b_sk = 1; gm.b_sn = c.w;
+      gm.M = c.w; gm.N = c.w; gm.K = c.w;
+      gm.ep = EP_NEGSUB;
+      gm.out = c.rinv1 + s * s1stride + (long)(nb - 1) * ww;
+      gm.out_ld = c.w;
+      HIPC(launch_gemm(gm, st));
+    }
+    jl = jh;
+  }
+  return SEVP_OK;
+}
+
+// ----------------------------------------------------------- full-width pass
+// One REP1 (periodic) / REP pass over nrhs right-hand sides (device pointers).
+struct FullCtx {
+  const float *ax, *ay, *bb, *cx, *cy;  // ld i2
+  int i2, j2, periodic, nb0;
+  const int *ie;  // host
+  const double *rinv, *rinv1;
+  const float *rinv32, *rinv1_32;  // non-null selects the FP32 MFMA corrections
+  double *e, *dum0, *hfree, *hcpl, *g;
+  float *partial;
+  int *counters;
+  int splits;
+};
+
+static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, long f_stride,
+                         double *x, long x_stride, int nrhs, hipStream_t st) {
+  const long ww = (long)c.i2 * c.i2;
+  const int i0 = c.i2 + 2, j0 = c.j2 + 2;
+  (void)j0;
+  const long dstride = (long)c.i2 * (c.nb0 > 1 ? c.nb0 - 1 : 1);
+  const bool mixed = c.rinv32 != nullptr;
+  auto corr = [&](const double *a, const double *b64, const float *b32, int ep, int xrow,
+                  double *dum0, double *out, double *g, int cyrow) -> int {
+    if (!mixed) {
+      GemmArgs gm{};
+      gm.a = a; gm.a_sm = c.i2; gm.a_sk = 1;
+      gm.b = b64; gm.b_sk = 1; gm.b_sn = c.i2;
+      gm.M = nrhs; gm.N = c.i2; gm.K = c.i2;
+      gm.ep = ep;
+      gm.x = x; gm.x_stride = x_stride; gm.xrow = xrow;
+      gm.dum0 = dum0; gm.dum0_stride = dstride;
+      gm.out = out; gm.out_ld = c.i2;
+      gm.g = g; gm.cy = c.cy; gm.cyrow = cyrow;
+      gm.i2 = c.i2;
+      HIPC(launch_gemm(gm, st));
+    } else {
+      GemmF32Args gm{};
+      gm.b = b32;
+      gm.M = nrhs; gm.N = c.i2; gm.K = c.i2;
+      gm.ep = ep;
+      gm.x = x; gm.x_stride = x_stride; gm.xrow = xrow;
+      gm.dum0 = dum0; gm.dum0_stride = dstride;
+      gm.out = out; gm.out_ld = c.i2;
+      gm.g = g; gm.cy = c.cy; gm.cyrow = cyrow;
+      gm.i2 = c.i2;
+      HIPC(launch_gemm_mixed(gm, a, c.partial, c.counters, c.splits, st));
+    }
+    return SEVP_OK;
+  };
+  (void)i0;
+  int js = 1;
+  for (int nb = 1; nb <= c.nb0; ++nb) {
+    const int ieb = c.ie[nb - 1];
+    MarchArgs m{};
+    m.ax = c.ax; m.ay = c.ay; m.bb = c.bb; m.cx = c.cx; m.cy = c.cy;
+    m.cld = c.i2;
+    m.f = f32; m.f64 = f64; m.f_stride = f_stride;
+    m.x = x; m.x_stride = x_stride;
+    m.e = c.e;
+    m.i2 = c.i2; m.j2 = c.j2; m.periodic = c.periodic;
+    m.js = js; m.jf = ieb - 2; m.ieb = ieb;
+    m.resid = (nb != c.nb0);
+    m.w = c.i2;
+    m.map = ColMap{1, 0, 0};
+    m.nrhs = nrhs;
+    HIPC(launch_march(MODE_PART, m, nrhs, st));
+    if (nb != c.nb0)
+      CHK(corr(c.e, c.rinv1 + (long)(nb - 1) * ww, mixed ? c.rinv1_32 + (long)(nb - 1) * ww : nullptr,
+               EP_FWD, ieb, c.dum0 + (long)(nb - 1) * c.i2, nullptr, nullptr, 0));
+    js = ieb;
+  }
+  for (int nbs = 1; nbs <= c.nb0; ++nbs) {
+    const int nb = c.nb0 - nbs + 1;
+    const int ieb = c.ie[nb - 1];
+    js = (nb != 1) ? c.ie[nb - 2] : 1;
+    ResidArgs r{};
+    r.ax = c.ax; r.ay = c.ay; r.bb = c.bb; r.cx = c.cx; r.cy = c.cy;
+    r.f = f32; r.f64 = f64; r.f_stride = f_stride;
+    r.x = x; r.x_stride = x_stride;
+    r.dum0 = (nb != c.nb0) ? c.dum0 : nullptr;
+    r.dum0_stride = dstride;
+    r.e = c.e;
+    r.i2 = c.i2; r.j2 = c.j2; r.periodic = c.periodic; r.nb = nb; r.ieb = ieb; r.nrhs = nrhs;
+    HIPC(launch_resid(r, st));
+    CHK(corr(c.e, c.rinv + (long)(nb - 1) * ww, mixed ? c.rinv32 + (long)(nb - 1) * ww : nullptr,
+             EP_BWD, js + 1, nullptr, c.hfree, nb != 1 ? c.g : nullptr, js - 1));
+    if (nb != 1)
+      CHK(corr(c.g, c.rinv1 + (long)(nb - 2) * ww, mixed ? c.rinv1_32 + (long)(nb - 2) * ww : nullptr,
+               EP_CPL, 0, nullptr, c.hcpl, nullptr, 0));
+    if (ieb - 2 >= js) {
+      MarchArgs m{};
+      m.ax = c.ax; m.ay = c.ay; m.bb = c.bb; m.cx = c.cx; m.cy = c.cy;
+      m.cld = c.i2;
+      m.x = x; m.x_stride = x_stride;
+      m.hcpl = (nb != 1) ? c.hcpl : nullptr;
+      m.hfree = c.hfree;
+      m.i2 = c.i2; m.j2 = c.j2; m.periodic = c.periodic;
+      m.js = js; m.jf = ieb - 2; m.ieb = ieb;
+      m.w = c.i2;
+      m.map = ColMap{1, 0, 0};
+      m.nrhs = nrhs;
+      HIPC(launch_march(MODE_HOM, m, nrhs, st));
+    }
+  }
+  if (c.periodic) HIPC(launch_halo_fix(x, x_stride, c.i2, c.j2, nrhs, st));
+  return SEVP_OK;
+}
+
+// ------------------------------------------------------------- residual
+// part[(r*nchunk + chunk)*2 + {0,1}] = partial sums of (s - A x)^2 and s^2.
+namespace sevp {
+__global__ __launch_bounds__(256) void resid_norm_kernel(const float *al, const float *ab,
+                                                         const float *ac, const float *ar,
+                                                         const float *at, const float *s,
+                                                         const double *s64, long s_stride,
+                                                         const double *x, long x_stride,
+                                                         double *rout, int i2, int j2, int nchunk,
+                                                         double *part) {
+  __shared__ double red[2][256];
+  const int r = blockIdx.y, ch = blockIdx.x, t = threadIdx.x;
+  const long n = (long)i2 * j2;
+  const long per = (n + nchunk - 1) / nchunk;
+  const long lo = ch * per, hi = min(n, lo + per);
+  const double *xr = x + (long)r * x_stride;
+  const int i0 = i2 + 2;
+  double a0 = 0.0, a1 = 0.0;
+  for (long q = lo + t; q < hi; q += 256) {
+    const int i = (int)(q % i2) + 1, j = (int)(q / i2) + 1;
+    const double ax = (double)ab[q] * xr[f2(i0, i + 1, j)] + (double)al[q] * xr[f2(i0, i, j + 1)] +
+                      (double)ac[q] * xr[f2(i0, i + 1, j + 1)] + (double)ar[q] * xr[f2(i0, i + 2, j + 1)] +
+                      (double)at[q] * xr[f2(i0, i + 1, j + 2)];
+    const double sv = s64 ? s64[(long)r * s_stride + q] : (double)s[(long)r * s_stride + q];
+    const double rv = sv - ax;
+    if (rout) rout[(long)r * n + q] = rv;
+    a0 += rv * rv;
+    a1 += sv * sv;
+  }
+  red[0][t] = a0;
+  red[1][t] = a1;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) {
+      red[0][t] += red[0][t + o];
+      red[1][t] += red[1][t + o];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    part[((long)r * nchunk + ch) * 2] = red[0][0];
+    part[((long)r * nchunk + ch) * 2 + 1] = red[1][0];
+  }
+}
+
+__global__ void axpy_interior_kernel(double *x, long x_stride, const double *d, long d_stride,
+                                     int i2, int j2, int nrhs) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n = (long)i2 * j2;
+  if (t >= n * nrhs) return;
+  const int r = (int)(t / n);
+  const long q = t % n;
+  const int i = (int)(q % i2) + 2, j = (int)(q / i2) + 2;
+  const long o = f2(i2 + 2, i, j);
+  x[(long)r * x_stride + o] += d[(long)r * d_stride + o];
+}
+}  // namespace sevp
+
+static constexpr int NCHUNK = 64;
+
+static int residual_norms(sevp_plan p, const float *s, const double *s64, long s_stride,
+                          const double *x, long x_stride, double *rout, int nrhs,
+                          std::vector<double> &rel) {
+  const int i2 = p->d.i2, j2 = p->d.j2;
+  hipLaunchKernelGGL(resid_norm_kernel, dim3(NCHUNK, nrhs), dim3(256), 0, p->stream, p->al(),
+                     p->ab(), p->ac(), p->ar(), p->at(), s, s64, s_stride, x, x_stride, rout, i2,
+                     j2, NCHUNK, p->part.p);
+  HIPC(hipGetLastError());
+  std::vector<double> h((size_t)nrhs * NCHUNK * 2);
+  HIPC(hipMemcpyAsync(h.data(), p->part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                      p->stream));
+  HIPC(hipStreamSynchronize(p->stream));
+  rel.assign(nrhs, 0.0);
+  for (int r = 0; r < nrhs; ++r) {
+    double a = 0, b = 0;
+    for (int c = 0; c < NCHUNK; ++c) {
+      a += h[((size_t)r * NCHUNK + c) * 2];
+      b += h[((size_t)r * NCHUNK + c) * 2 + 1];
+    }
+    rel[r] = b > 0 ? std::sqrt(a / b) : std::sqrt(a);
+  }
+  return SEVP_OK;
+}
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+const char *sevp_last_error(void) { return g_err.c_str(); }
+const char *sevp_version(void) { return "sevp-mi355x 0.1 (gfx950)"; }
+
+int sevp_plan_create(const sevp_desc *desc, const float *al, const float *ab, const float *ac,
+                     const float *ar, const float *at, sevp_plan *out) {
+  if (!desc || !out || !al || !ab || !ac || !ar || !at) return fail(SEVP_E_ARG, "null argument");
+  *out = nullptr;
+  const sevp_desc &d = *desc;
+  if (d.i2 < 2 || d.j2 < 2) return fail(SEVP_E_ARG, "grid must be at least 2x2");
+  if (d.nrhs_max < 1) return fail(SEVP_E_ARG, "nrhs_max must be >= 1");
+  if (d.form != SEVP_FORM_BIR && d.form != SEVP_FORM_FULL_PERIODIC && d.form != SEVP_FORM_FULL)
+    return fail(SEVP_E_ARG, "unknown form");
+  if (d.precision < SEVP_PREC_FP64 || d.precision > SEVP_PREC_BF16)
+    return fail(SEVP_E_ARG, "unknown precision");
+  CHK(check_ie(d.ie, d.nb0, d.j2));
+  auto *p = new sevp_plan_s();
+  p->d = d;
+  p->ie.assign(d.ie, d.ie + d.nb0);
+  p->d.ie = p->ie.data();
+  if (d.form == SEVP_FORM_BIR) {
+    if (d.nbir < 2 || (2 * d.i2) % d.nbir != 0) {
+      delete p;
+      return fail(SEVP_E_ARG, "BIR form needs NBIR >= 2 dividing 2*I2");
+    }
+    p->ibir = 2 * d.i2 / d.nbir;
+    if (p->ibir % 2 != 0 || p->ibir > 64) {
+      delete p;
+      return fail(SEVP_E_ARG, "BIR strip width IBIR must be even and <= 64");
+    }
+    if (d.precision != SEVP_PREC_FP64) {
+      delete p;
+      return fail(SEVP_E_ARG, "BIR form runs the plain FP64 arithmetic only");
+    }
+    if (d.nsweep < 1) {
+      delete p;
+      return fail(SEVP_E_ARG, "nsweep must be >= 1");
+    }
+    p->w = p->ibir;
+    p->nmat = d.nbir;
+  } else {
+    p->ibir = d.i2;
+    p->w = d.i2;
+    p->nmat = 1;
+  }
+  int rc = SEVP_OK;
+  auto bail = [&](int code) {
+    sevp_plan_destroy(p);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(SEVP_E_HIP, "hipStreamCreate failed"));
+  const size_t nc = (size_t)d.i2 * d.j2;
+  if ((rc = p->coef.alloc(5 * nc))) return bail(rc);
+  const float *src[5] = {al, ab, ac, ar, at};
+  for (int k = 0; k < 5; ++k)
+    if (hipMemcpy(p->coef.p + k * nc, src[k], nc * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(SEVP_E_HIP, "coefficient upload failed"));
+  if ((rc = p->ie_d.alloc(d.nb0))) return bail(rc);
+  if (hipMemcpy(p->ie_d.p, p->ie.data(), d.nb0 * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(SEVP_E_HIP, "IE upload failed"));
+  if ((rc = p->rinv.alloc(p->rinv_count()))) return bail(rc);
+  if ((rc = p->rinv1.alloc(p->rinv1_count()))) return bail(rc);
+  if ((rc = p->zp.alloc(1))) return bail(rc);
+  if ((rc = p->b1s.alloc((size_t)p->w * p->nmat))) return bail(rc);
+  const int nr = d.nrhs_max;
+  const int nb1 = d.nb0 > 1 ? d.nb0 - 1 : 1;
+  const size_t nx = (size_t)(d.i2 + 2) * (d.j2 + 2);
+  if ((rc = p->part.alloc((size_t)nr * NCHUNK * 2))) return bail(rc);
+  if (d.form == SEVP_FORM_BIR) {
+    if ((rc = p->xx.alloc((size_t)nr * (p->ibir + 2) * (d.j2 + 2)))) return bail(rc);
+    if ((rc = p->bdum0.alloc((size_t)nr * p->ibir * nb1))) return bail(rc);
+    if (hipMemset(p->xx.p, 0, p->xx.bytes()) != hipSuccess) return bail(fail(SEVP_E_HIP, "memset"));
+    if (hipMemset(p->bdum0.p, 0, p->bdum0.bytes()) != hipSuccess) return bail(fail(SEVP_E_HIP, "memset"));
+  } else {
+    if ((rc = p->e.alloc((size_t)nr * d.i2))) return bail(rc);
+    if ((rc = p->dum0.alloc((size_t)nr * d.i2 * nb1))) return bail(rc);
+    if ((rc = p->hfree.alloc((size_t)nr * d.i2))) return bail(rc);
+    if ((rc = p->hcpl.alloc((size_t)nr * d.i2))) return bail(rc);
+    if ((rc = p->g.alloc((size_t)nr * d.i2))) return bail(rc);
+    if ((rc = p->r64.alloc((size_t)nr * nc))) return bail(rc);
+    if ((rc = p->dx.alloc((size_t)nr * nx))) return bail(rc);
+    if (d.precision != SEVP_PREC_FP64) {
+      if (nr > 64) return bail(fail(SEVP_E_ARG, "mixed precision solves at most 64 RHS per call"));
+      const int ntiles = (d.i2 + 15) / 16;
+      p->splits = std::max(1, std::min(16, 1024 / ntiles));
+      if ((rc = p->partial.alloc((size_t)p->splits * nr * d.i2))) return bail(rc);
+      if ((rc = p->counters.alloc((size_t)ntiles))) return bail(rc);
+      if (hipMemset(p->counters.p, 0, p->counters.bytes()) != hipSuccess)
+        return bail(fail(SEVP_E_HIP, "memset"));
+      if ((rc = p->rinv32.alloc(p->rinv_count()))) return bail(rc);
+      if ((rc = p->rinv1_32.alloc(p->rinv1_count()))) return bail(rc);
+    }
+  }
+  *out = p;
+  return SEVP_OK;
+}
+
+int sevp_plan_destroy(sevp_plan p) {
+  if (!p) return SEVP_OK;
+  if (p->stream) {
+    (void)hipStreamSynchronize(p->stream);
+    (void)hipStreamDestroy(p->stream);
+  }
+  delete p;
+  return SEVP_OK;
+}
+
+size_t sevp_plan_device_bytes(sevp_plan p) {
+  if (!p) return 0;
+  return p->coef.bytes() + p->ie_d.bytes() + p->rinv.bytes() + p->rinv1.bytes() +
+         p->rinv32.bytes() + p->rinv1_32.bytes() + p->e.bytes() + p->dum0.bytes() +
+         p->hfree.bytes() + p->hcpl.bytes() + p->g.bytes() + p->partial.bytes() + p->xx.bytes() +
+         p->bdum0.bytes() + p->r64.bytes() + p->dx.bytes() + p->part.bytes() + p->b1s.bytes() +
+         p->zp.bytes();
+}
+
+static int make_low_precision_copies(sevp_plan p) {
+  if (p->d.precision == SEVP_PREC_FP64 || p->d.form == SEVP_FORM_BIR) return SEVP_OK;
+  HIPC(launch_cvt_f32(p->rinv.p, p->rinv32.p, (long)p->rinv.n, p->stream));
+  HIPC(launch_cvt_f32(p->rinv1.p, p->rinv1_32.p, (long)p->rinv1.n, p->stream));
+  return SEVP_OK;
+}
+
+int sevp_build_influence(sevp_plan p, int *zero_pivots) {
+  if (!p) return fail(SEVP_E_ARG, "null plan");
+  HIPC(hipMemsetAsync(p->zp.p, 0, sizeof(int), p->stream));
+  HIPC(hipMemsetAsync(p->rinv1.p, 0, p->rinv1.bytes(), p->stream));
+  PreCtx c{};
+  c.ax = p->al(); c.ay = p->ab(); c.bb = p->ac(); c.cx = p->ar(); c.cy = p->at();
+  c.cld = p->d.i2;
+  c.i2 = p->d.i2; c.j2 = p->d.j2;
+  c.w = p->w;
+  c.nmat = p->nmat;
+  c.rinv = p->rinv.p; c.rinv1 = p->rinv1.p;
+  c.ie = p->ie.data();
+  c.nb0 = p->d.nb0;
+  c.zp = p->zp.p;
+  c.b1s = p->b1s.p;
+  if (p->d.form == SEVP_FORM_BIR) {
+    c.periodic = 0;
+    c.map = ColMap{1, p->ibir / 2, 1};
+  } else {
+    c.periodic = (p->d.form == SEVP_FORM_FULL_PERIODIC);
+    c.map = ColMap{1, 0, 0};
+  }
+  CHK(run_pre(c, p->stream));
+  CHK(make_low_precision_copies(p));
+  int z = 0;
+  HIPC(hipMemcpyAsync(&z, p->zp.p, sizeof(int), hipMemcpyDeviceToHost, p->stream));
+  HIPC(hipStreamSynchronize(p->stream));
+  if (zero_pivots) *zero_pivots = z;
+  p->have_inv = true;
+  return SEVP_OK;
+}
+
+int sevp_load_influence(sevp_plan p, const double *rinv, const double *rinv1) {
+  if (!p || !rinv || !rinv1) return fail(SEVP_E_ARG, "null argument");
+  HIPC(hipMemcpy(p->rinv.p, rinv, p->rinv.bytes(), hipMemcpyHostToDevice));
+  if (p->d.nb0 > 1) HIPC(hipMemcpy(p->rinv1.p, rinv1, p->rinv1.bytes(), hipMemcpyHostToDevice));
+  CHK(make_low_precision_copies(p));
+  HIPC(hipStreamSynchronize(p->stream));
+  p->have_inv = true;
+  return SEVP_OK;
+}
+
+int sevp_get_influence(sevp_plan p, double *rinv, double *rinv1) {
+  if (!p || !rinv || !rinv1) return fail(SEVP_E_ARG, "null argument");
+  if (!p->have_inv) return fail(SEVP_E_STATE, "influence matrices not built or loaded");
+  HIPC(hipStreamSynchronize(p->stream));
+  HIPC(hipMemcpy(rinv, p->rinv.p, p->rinv.bytes(), hipMemcpyDeviceToHost));
+  if (p->d.nb0 > 1) HIPC(hipMemcpy(rinv1, p->rinv1.p, p->rinv1.bytes(), hipMemcpyDeviceToHost));
+  return SEVP_OK;
+}
+
+int sevp_reset_seam(sevp_plan p) {
+  if (!p) return fail(SEVP_E_ARG, "null plan");
+  if (p->xx.p) HIPC(hipMemsetAsync(p->xx.p, 0, p->xx.bytes(), p->stream));
+  return SEVP_OK;
+}
+
+static FullCtx full_ctx(sevp_plan p, bool mixed) {
+  FullCtx c{};
+  c.ax = p->al(); c.ay = p->ab(); c.bb = p->ac(); c.cx = p->ar(); c.cy = p->at();
+  c.i2 = p->d.i2; c.j2 = p->d.j2;
+  c.periodic = (p->d.form == SEVP_FORM_FULL_PERIODIC);
+  c.nb0 = p->d.nb0;
+  c.ie = p->ie.data();
+  c.rinv = p->rinv.p; c.rinv1 = p->rinv1.p;
+  if (mixed) {
+    c.rinv32 = p->rinv32.p;
+    c.rinv1_32 = p->rinv1_32.p;
+  }
+  c.e = p->e.p; c.dum0 = p->dum0.p; c.hfree = p->hfree.p; c.hcpl = p->hcpl.p; c.g = p->g.p;
+  c.partial = p->partial.p;
+  c.counters = p->counters.p;
+  c.splits = p->splits;
+  return c;
+}
+
+int sevp_solve_dev(sevp_plan p, const float *s, double *x, int nrhs, void *stream) {
+  if (!p || !s || !x) return fail(SEVP_E_ARG, "null argument");
+  if (nrhs < 1 || nrhs > p->d.nrhs_max) return fail(SEVP_E_ARG, "nrhs out of range");
+  if (!p->have_inv) return fail(SEVP_E_STATE, "influence matrices not built or loaded");
+  hipStream_t st = stream ? (hipStream_t)stream : p->stream;
+  const long s_stride = (long)p->d.i2 * p->d.j2;
+  const long x_stride = (long)(p->d.i2 + 2) * (p->d.j2 + 2);
+  if (p->d.form == SEVP_FORM_BIR) {
+    BirArgs a{};
+    a.al = p->al(); a.ab = p->ab(); a.ac = p->ac(); a.ar = p->ar(); a.at = p->at();
+    a.s = s; a.rinv = p->rinv.p; a.rinv1 = p->rinv1.p;
+    a.x = x; a.xx = p->xx.p; a.dum0 = p->bdum0.p; a.ie = p->ie_d.p;
+    a.i2 = p->d.i2; a.j2 = p->d.j2; a.nbir = p->d.nbir; a.ibir = p->ibir; a.nb0 = p->d.nb0;
+    a.nsweep = p->d.nsweep; a.periodic = p->d.periodic;
+    a.s_stride = s_stride; a.x_stride = x_stride;
+    a.xx_stride = (long)(p->ibir + 2) * (p->d.j2 + 2);
+    a.dum0_stride = (long)p->ibir * (p->d.nb0 > 1 ? p->d.nb0 - 1 : 1);
+    HIPC(launch_bir_sweeps(a, nrhs, st));
+    return SEVP_OK;
+  }
+  FullCtx c = full_ctx(p, p->d.precision != SEVP_PREC_FP64);
+  return run_full_pass(c, s, nullptr, s_stride, x, x_stride, nrhs, st);
+}
+
+int sevp_solve(sevp_plan p, const float *s, double *x, int nrhs) {
+  if (!p || !s || !x) return fail(SEVP_E_ARG, "null argument");
+  if (nrhs < 1 || nrhs > p->d.nrhs_max) return fail(SEVP_E_ARG, "nrhs out of range");
+  const size_t ns = (size_t)p->d.i2 * p->d.j2 * nrhs;
+  const size_t nx = (size_t)(p->d.i2 + 2) * (p->d.j2 + 2) * nrhs;
+  DBuf<float> sd;
+  DBuf<double> xd;
+  CHK(sd.alloc(ns));
+  CHK(xd.alloc(nx));
+  HIPC(hipMemcpyAsync(sd.p, s, ns * sizeof(float), hipMemcpyHostToDevice, p->stream));
+  HIPC(hipMemcpyAsync(xd.p, x, nx * sizeof(double), hipMemcpyHostToDevice, p->stream));
+  CHK(sevp_solve_dev(p, sd.p, xd.p, nrhs, nullptr));
+  HIPC(hipMemcpyAsync(x, xd.p, nx * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  HIPC(hipStreamSynchronize(p->stream));
+  return SEVP_OK;
+}
+
+int sevp_residual_dev(sevp_plan p, const float *s, const double *x, int nrhs, double *rel) {
+  if (!p || !s || !x || !rel) return fail(SEVP_E_ARG, "null argument");
+  if (nrhs < 1 || nrhs > p->d.nrhs_max) return fail(SEVP_E_ARG, "nrhs out of range");
+  std::vector<double> v;
+  CHK(residual_norms(p, s, nullptr, (long)p->d.i2 * p->d.j2, x, (long)(p->d.i2 + 2) * (p->d.j2 + 2),
+                     nullptr, nrhs, v));
+  std::memcpy(rel, v.data(), sizeof(double) * nrhs);
+  return SEVP_OK;
+}
+
+int sevp_solve_refined_dev(sevp_plan p, const float *s, double *x, int nrhs, double tol,
+                           int max_iter, double *rel_out, sevp_stats *stats) {
+  if (!p || !s || !x) return fail(SEVP_E_ARG, "null argument");
+  if (p->d.form == SEVP_FORM_BIR) return fail(SEVP_E_ARG, "refinement needs a full-width form");
+  if (nrhs < 1 || nrhs > p->d.nrhs_max) return fail(SEVP_E_ARG, "nrhs out of range");
+  if (!p->have_inv) return fail(SEVP_E_STATE, "influence matrices not built or loaded");
+  auto t0 = std::chrono::steady_clock::now();
+  const int i2 = p->d.i2, j2 = p->d.j2;
+  const long ns = (long)i2 * j2, nx = (long)(i2 + 2) * (j2 + 2);
+  FullCtx c = full_ctx(p, p->d.precision != SEVP_PREC_FP64);
+  std::vector<double> rel;
+  int it = 0;
+  double worst = -1.0;
+  for (;;) {
+    CHK(residual_norms(p, s, nullptr, ns, x, nx, p->r64.p, nrhs, rel));
+    worst = 0.0;
+    for (double v : rel) worst = std::max(worst, v);
+    if (!(worst > tol) || it >= max_iter) break;
+    HIPC(hipMemsetAsync(p->dx.p, 0, sizeof(double) * nx * nrhs, p->stream));
+    CHK(run_full_pass(c, nullptr, p->r64.p, ns, p->dx.p, nx, nrhs, p->stream));
+    const long n = ns * nrhs;
+    hipLaunchKernelGGL(axpy_interior_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       p->stream, x, nx, p->dx.p, nx, i2, j2, nrhs);
+    HIPC(hipGetLastError());
+    if (c.periodic) HIPC(launch_halo_fix(x, nx, i2, j2, nrhs, p->stream));
+    ++it;
+  }
+  if (rel_out) std::memcpy(rel_out, rel.data(), sizeof(double) * nrhs);
+  if (stats) {
+    stats->iterations = it;
+    stats->max_rel_resid = worst;
+    stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  if (worst > tol) return fail(SEVP_E_NOTCONV, "refinement did not reach the tolerance");
+  return SEVP_OK;
+}
+
+}  // extern "C"
+
+// --------------------------------------------- one-to-one reference entry points
+
+// Copy rows 1..nrows of `ncols` columns from a column-major array of leading
+// dimension ld into a compact device array of leading dimension ncols.
+template <class T>
+static int up2d(DBuf<T> &dst, const T *src, int ld, int ncols, int nrows) {
+  CHK(dst.alloc((size_t)ncols * nrows));
+  HIPC(hipMemcpy2D(dst.p, ncols * sizeof(T), src, (size_t)ld * sizeof(T), ncols * sizeof(T), nrows,
+                   hipMemcpyHostToDevice));
+  return SEVP_OK;
+}
+
+static hipStream_t aux_stream() { return nullptr; }
+
+extern "C" int sevp_repbir(const float *ax, const float *ay, const float *bb, const float *cx,
+                const float *cy, const double *rinv, const double *rinv1, double *dum0,
+                double *dum1, double *dum2, const float *f, double *h, double *x, const int *ie,
+                int i0, int i2, int m0, int m2, int nblk) {
+  (void)dum0; (void)dum1; (void)dum2; (void)h; (void)m0;
+  if (!ax || !ay || !bb || !cx || !cy || !rinv || !f || !x || !ie) return fail(SEVP_E_ARG, "null argument");
+  if (m2 < 1 || m2 > 64 || nblk < 1 || i2 < m2 || i0 < m2 + 2) return fail(SEVP_E_ARG, "bad REPBIR shape");
+  if (nblk > 1 && !rinv1) return fail(SEVP_E_ARG, "null RINV1");
+  for (int b = 0; b < nblk; ++b)
+    if (ie[b] < 2 || (b && ie[b] <= ie[b - 1])) return fail(SEVP_E_ARG, "IE must be increasing from 2");
+  const int jr = ie[nblk - 1] - 1;  // last coefficient row used
+  const int jx = ie[nblk - 1] + 1;  // last X row used
+  DBuf<float> c[5], fd;
+  const float *cs[5] = {ax, ay, bb, cx, cy};
+  for (int k = 0; k < 5; ++k) CHK(up2d(c[k], cs[k], i2, m2, jr));
+  CHK(up2d(fd, f, m2, m2, jr));
+  DBuf<double> xd, rv, rv1, d0;
+  CHK(up2d(xd, x, i0, m2 + 2, jx));
+  CHK(rv.alloc((size_t)m2 * m2 * nblk));
+  HIPC(hipMemcpy(rv.p, rinv, rv.bytes(), hipMemcpyHostToDevice));
+  CHK(rv1.alloc((size_t)m2 * m2 * (nblk > 1 ? nblk - 1 : 1)));
+  if (nblk > 1) HIPC(hipMemcpy(rv1.p, rinv1, rv1.bytes(), hipMemcpyHostToDevice));
+  CHK(d0.alloc((size_t)m2 * (nblk > 1 ? nblk - 1 : 1)));
+  DBuf<int> ied;
+  CHK(ied.alloc(nblk));
+  HIPC(hipMemcpy(ied.p, ie, nblk * sizeof(int), hipMemcpyHostToDevice));
+  RepbirArgs a{};
+  a.ax = c[0].p; a.ay = c[1].p; a.bb = c[2].p; a.cx = c[3].p; a.cy = c[4].p;
+  a.f = fd.p; a.x = xd.p; a.rinv = rv.p; a.rinv1 = rv1.p; a.dum0 = d0.p; a.ie = ied.p;
+  a.i0 = m2 + 2; a.i2 = m2; a.m2 = m2; a.nblk = nblk;
+  HIPC(launch_repbir(a, 1, aux_stream()));
+  HIPC(hipMemcpy2D(x, (size_t)i0 * sizeof(double), xd.p, (m2 + 2) * sizeof(double),
+                   (m2 + 2) * sizeof(double), jx, hipMemcpyDeviceToHost));
+  return SEVP_OK;
+}
+
+static int full_once(const float *ax, const float *ay, const float *bb, const float *cx,
+                     const float *cy, const double *rinv, const double *rinv1, const float *f,
+                     double *x, const int *ie, int i0, int i2, int nblk, int periodic) {
+  if (!ax || !ay || !bb || !cx || !cy || !rinv || !f || !x || !ie) return fail(SEVP_E_ARG, "null argument");
+  if (i0 != i2 + 2 || i2 < 2 || nblk < 1) return fail(SEVP_E_ARG, "bad shape (I0 must be I2+2)");
+  if (nblk > 1 && !rinv1) return fail(SEVP_E_ARG, "null RINV1");
+  for (int b = 0; b < nblk; ++b)
+    if (ie[b] < 2 || (b && ie[b] <= ie[b - 1])) return fail(SEVP_E_ARG, "IE must be increasing from 2");
+  const int j2 = ie[nblk - 1] - 1;  // rows used: coefficients 1..J2, X 1..J2+2
+  const size_t nc = (size_t)i2 * j2;
+  DBuf<float> c[5], fd;
+  const float *cs[5] = {ax, ay, bb, cx, cy};
+  for (int k = 0; k < 5; ++k) {
+    CHK(c[k].alloc(nc));
+    HIPC(hipMemcpy(c[k].p, cs[k], nc * sizeof(float), hipMemcpyHostToDevice));
+  }
+  CHK(fd.alloc(nc));
+  HIPC(hipMemcpy(fd.p, f, nc * sizeof(float), hipMemcpyHostToDevice));
+  const size_t nx = (size_t)i0 * (j2 + 2);
+  const int nb1 = nblk > 1 ? nblk - 1 : 1;
+  DBuf<double> xd, rv, rv1, e, d0, hf, hc, g;
+  CHK(xd.alloc(nx));
+  HIPC(hipMemcpy(xd.p, x, nx * sizeof(double), hipMemcpyHostToDevice));
+  CHK(rv.alloc((size_t)i2 * i2 * nblk));
+  HIPC(hipMemcpy(rv.p, rinv, rv.bytes(), hipMemcpyHostToDevice));
+  CHK(rv1.alloc((size_t)i2 * i2 * nb1));
+  if (nblk > 1) HIPC(hipMemcpy(rv1.p, rinv1, rv1.bytes(), hipMemcpyHostToDevice));
+  CHK(e.alloc(i2));
+  CHK(d0.alloc((size_t)i2 * nb1));
+  CHK(hf.alloc(i2));
+  CHK(hc.alloc(i2));
+  CHK(g.alloc(i2));
+  FullCtx fc{};
+  fc.ax = c[0].p; fc.ay = c[1].p; fc.bb = c[2].p; fc.cx = c[3].p; fc.cy = c[4].p;
+  fc.i2 = i2; fc.j2 = j2; fc.periodic = periodic; fc.nb0 = nblk; fc.ie = ie;
+  fc.rinv = rv.p; fc.rinv1 = rv1.p;
+  fc.e = e.p; fc.dum0 = d0.p; fc.hfree = hf.p; fc.hcpl = hc.p; fc.g = g.p;
+  CHK(run_full_pass(fc, fd.p, nullptr, (long)nc, xd.p, (long)nx, 1, aux_stream()));
+  HIPC(hipMemcpy(x, xd.p, nx * sizeof(double), hipMemcpyDeviceToHost));
+  return SEVP_OK;
+}
+
+extern "C" int sevp_rep1(const float *ax, const float *ay, const float *bb, const float *cx,
+              const float *cy, const double *rinv, const double *rinv1, double *dum0,
+              double *dum1, double *dum2, const float *f, double *h, double *x, const int *ie,
+              int i0, int i2, int nblk) {
+  (void)dum0; (void)dum1; (void)dum2; (void)h;
+  return full_once(ax, ay, bb, cx, cy, rinv, rinv1, f, x, ie, i0, i2, nblk, 1);
+}
+
+extern "C" int sevp_rep(const float *ax, const float *ay, const float *bb, const float *cx,
+             const float *cy, const double *rinv, const double *rinv1, double *dum0,
+             double *dum1, double *dum2, const float *f, double *h, double *x, const int *ie,
+             int i0, int i2, int nblk) {
+  (void)dum0; (void)dum1; (void)dum2; (void)h;
+  return full_once(ax, ay, bb, cx, cy, rinv, rinv1, f, x, ie, i0, i2, nblk, 0);
+}
+
+extern "C" int sevp_repbir_period(const float *ax, const float *ay, const float *bb, const float *cx,
+                       const float *cy, const double *rinv, const double *rinv1, double *dum0,
+                       double *dum1, double *dum2, const float *f, double *h, double *x,
+                       const int *ie, int i0, int i2, int m0, int m2, int nblk) {
+  if (m0 != i0 || m2 != i2)
+    return fail(SEVP_E_ARG, "REPBIR_PERIOD is defined for M0=I0, M2=I2 only");
+  return sevp_rep1(ax, ay, bb, cx, cy, rinv, rinv1, dum0, dum1, dum2, f, h, x, ie, i0, i2, nblk);
+}
+
+static int pre_once(const float *ax, const float *ay, const float *bb, const float *cx,
+                    const float *cy, double *rinv, double *rinv1, const int *ie, int cld, int w,
+                    int nblk, int periodic) {
+  if (!ax || !ay || !bb || !cx || !cy || !rinv || !ie) return fail(SEVP_E_ARG, "null argument");
+  if (w < 1 || nblk < 1 || cld < w) return fail(SEVP_E_ARG, "bad PRE shape");
+  if (nblk > 1 && !rinv1) return fail(SEVP_E_ARG, "null RINV1");
+  for (int b = 0; b < nblk; ++b)
+    if (ie[b] < 2 || (b && ie[b] <= ie[b - 1])) return fail(SEVP_E_ARG, "IE must be increasing from 2");
+  const int jr = ie[nblk - 1] - 1;
+  DBuf<float> c[5];
+  const float *cs[5] = {ax, ay, bb, cx, cy};
+  for (int k = 0; k < 5; ++k) CHK(up2d(c[k], cs[k], cld, w, jr));
+  const int nb1 = nblk > 1 ? nblk - 1 : 1;
+  DBuf<double> rv, rv1, b1;
+  DBuf<int> zp;
+  CHK(rv.alloc((size_t)w * w * nblk));
+  CHK(rv1.alloc((size_t)w * w * nb1));
+  CHK(b1.alloc(w));
+  CHK(zp.alloc(1));
+  HIPC(hipMemset(zp.p, 0, sizeof(int)));
+  HIPC(hipMemset(rv1.p, 0, rv1.bytes()));
+  PreCtx pc{};
+  pc.ax = c[0].p; pc.ay = c[1].p; pc.bb = c[2].p; pc.cx = c[3].p; pc.cy = c[4].p;
+  pc.cld = w; pc.i2 = w; pc.j2 = jr; pc.w = w; pc.nmat = 1; pc.periodic = periodic;
+  pc.map = ColMap{1, 0, 0};
+  pc.rinv = rv.p; pc.rinv1 = rv1.p; pc.ie = ie; pc.nb0 = nblk; pc.zp = zp.p; pc.b1s = b1.p;
+  CHK(run_pre(pc, aux_stream()));
+  HIPC(hipDeviceSynchronize());
+  HIPC(hipMemcpy(rinv, rv.p, rv.bytes(), hipMemcpyDeviceToHost));
+  if (nblk > 1) HIPC(hipMemcpy(rinv1, rv1.p, rv1.bytes(), hipMemcpyDeviceToHost));
+  return SEVP_OK;
+}
+
+extern "C" int sevp_pre(const float *ax, const float *ay, const float *bb, const float *cx,
+             const float *cy, double *rinv, double *rinv1, double *h, const int *ie, int i2,
+             int m0, int m2, int nblk) {
+  (void)h; (void)m0;
+  return pre_once(ax, ay, bb, cx, cy, rinv, rinv1, ie, i2, m2, nblk, 0);
+}
+
+extern "C" int sevp_pre1(const float *ax, const float *ay, const float *bb, const float *cx,
+              const float *cy, double *rinv, double *rinv1, double *h, const int *ie, int i0,
+              int i2, int nblk) {
+  (void)h;
+  if (i0 != i2 + 2) return fail(SEVP_E_ARG, "I0 must be I2+2");
+  return pre_once(ax, ay, bb, cx, cy, rinv, rinv1, ie, i2, i2, nblk, 1);
+}
+
+extern "C" int sevp_matinv(double *b, int n) {
+  if (!b || n < 1) return fail(SEVP_E_ARG, "bad MATINV argument");
+  DBuf<double> bd, b1;
+  DBuf<int> zp;
+  CHK(bd.alloc((size_t)n * n));
+  CHK(b1.alloc(n));
+  CHK(zp.alloc(1));
+  HIPC(hipMemset(zp.p, 0, sizeof(int)));
+  HIPC(hipMemcpy(bd.p, b, bd.bytes(), hipMemcpyHostToDevice));
+  HIPC(launch_matinv(bd.p, (long)n * n, n, 1, zp.p, b1.p, aux_stream()));
+  HIPC(hipDeviceSynchronize());
+  HIPC(hipMemcpy(b, bd.p, bd.bytes(), hipMemcpyDeviceToHost));
+  return SEVP_OK;
+}
+
+

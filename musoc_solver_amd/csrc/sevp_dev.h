@@ -1,0 +1,41 @@
+// Device-side helpers shared by the SEVP kernels (gfx950, wave64).
+//
+// Arithmetic rule for every kernel that claims parity with the reference:
+// the library is compiled with -ffp-contract=off, every expression is written
+// in the reference's left-to-right order, float coefficients are promoted to
+// double exactly, and every dot product is a sequential sum from zero.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SEVP_WAVE 64
+
+// Column-major (Fortran) 2-D index with 1-based i, j.
+__device__ __forceinline__ long f2(int ld, int i, int j) {
+  return (long)(i - 1) + (long)(j - 1) * (long)ld;
+}
+
+// Lane i receives lane i-1's value (lane 0 receives `fill`): DPP wave_shr:1.
+__device__ __forceinline__ double wave_shr1(double v, double fill) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  int flo = __double2loint(fill), fhi = __double2hiint(fill);
+  lo = __builtin_amdgcn_update_dpp(flo, lo, 0x138, 0xf, 0xf, false);
+  hi = __builtin_amdgcn_update_dpp(fhi, hi, 0x138, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Lane i receives lane i+1's value (lane 63 receives `fill`): DPP wave_shl:1.
+__device__ __forceinline__ double wave_shl1(double v, double fill) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  int flo = __double2loint(fill), fhi = __double2hiint(fill);
+  lo = __builtin_amdgcn_update_dpp(flo, lo, 0x130, 0xf, 0xf, false);
+  hi = __builtin_amdgcn_update_dpp(fhi, hi, 0x130, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Broadcast lane `src` (wave-uniform) of a double.
+__device__ __forceinline__ double read_lane(double v, int src) {
+  int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+  int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+  return __hiloint2double(hi, lo);
+}

@@ -1,0 +1,521 @@
+// Full-width SEVP building blocks (REP1, REP, PRE1, PRE and the strip PRE),
+// batched over independent right-hand sides / unit vectors.
+//
+// March kernels use trapezoid tiles: a wavefront marches 64*K consecutive
+// columns (K per lane, east/west neighbours through DPP wave shifts), and
+// after s rows only the columns at least s away from the tile edge are still
+// exact, so each tile carries `halo` extra columns on each side and writes
+// only its central `tile_out` columns.  Recomputing the halo columns repeats
+// exactly the same operations on exactly the same operands, so the written
+// values are bitwise those of a march over the whole width.  Periodic
+// problems index columns modulo the width; this reproduces the reference's
+// halo copies X(1)=X(I0-1), X(I0)=X(2), which only ever duplicate interior
+// values, and a final pass (halo_fix_kernel) leaves the halo columns as the
+// reference leaves them.
+//
+// The correction products use a tiled FP64 GEMM whose every output is one
+// thread's sequential sum over k from zero with separately rounded multiply
+// and add: the reference's DUM2(N)=DUM2(N)+DUM1(M)*RINV(M,N) loop order.
+//
+// MATINV is the reference's Gauss-Jordan without pivoting
+// (PREP/template/GLOBAL/src/prep_GLOBAL.f90:516-568) with the matrix held in
+// LDS; each element receives the same sequence of operations as in the
+// reference, only the independent element updates of one step run in
+// parallel.
+#include "sevp_dev.h"
+#include "sevp_internal.h"
+
+namespace sevp {
+
+template <int K>
+__device__ __forceinline__ void nbr(const double (&v)[K], int k, double &w_, double &e_) {
+  if (k > 0) w_ = v[k - 1];
+  else w_ = wave_shr1(v[K - 1], 0.0);
+  if (k < K - 1) e_ = v[k + 1];
+  else e_ = wave_shl1(v[0], 0.0);
+}
+
+template <int K, int MODE>
+__global__ __launch_bounds__(64) void march_kernel(MarchArgs p) {
+  const int lane = threadIdx.x;
+  const int tile = blockIdx.x;
+  const int bidx = blockIdx.y;  // RHS (PART/HOM) or unit-vector column g-1 (PRE)
+  const int strip = blockIdx.z;
+  const int w = p.w;
+  const int T = p.tile_out;
+  const int out_lo = 1 + tile * T;
+  const int out_hi = min(out_lo + T, w + 1);  // exclusive
+  ColMap map = p.map;
+  if (MODE == MODE_PRE) {
+    map.il = strip * map.inc + 1;
+    map.seam = (map.seam && strip == (int)gridDim.z - 1) ? 1 : 0;
+  }
+  const int js = p.js, jf = p.jf, ieb = p.ieb;
+
+  int c[K], pc[K], cofs[K];
+  bool out[K], halo_w[K], halo_e[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    c[k] = out_lo - p.halo + lane * K + k;
+    if (p.periodic) {
+      int m = (c[k] - 1) % w;
+      if (m < 0) m += w;
+      pc[k] = m + 1;
+      halo_w[k] = halo_e[k] = false;
+    } else {
+      pc[k] = min(max(c[k], 1), w);
+      halo_w[k] = c[k] <= 0;
+      halo_e[k] = c[k] >= w + 1;
+    }
+    out[k] = c[k] >= out_lo && c[k] < out_hi;
+    cofs[k] = map.phys(pc[k]);
+  }
+#define COEF(arr, k, j) ((double)(arr)[(long)cofs[k] + (long)((j) - 1) * p.cld])
+
+  double *x = nullptr;
+  const float *f = nullptr;
+  const double *f64 = nullptr;
+  const int i0 = p.i2 + 2;
+  if (MODE != MODE_PRE) {
+    x = p.x + (long)bidx * p.x_stride;
+    if (MODE == MODE_PART) {
+      if (p.f64) f64 = p.f64 + (long)bidx * p.f_stride;
+      else f = p.f + (long)bidx * p.f_stride;
+    }
+  }
+  // X index (1-based) of a lane's column: halo lanes read the Dirichlet column.
+  auto xcol = [&](int k) -> int { return halo_w[k] ? 1 : (halo_e[k] ? i0 : pc[k] + 1); };
+
+  double vs[K], vc[K];
+  const int g = bidx + 1;  // PRE: unit column
+  double *rinv_b = nullptr, *tmat = nullptr;
+  const double *r1p = nullptr;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (MODE == MODE_PART) {
+      vs[k] = x[f2(i0, xcol(k), js)];
+      vc[k] = x[f2(i0, xcol(k), js + 1)];
+    } else if (MODE == MODE_HOM) {
+      const bool h = halo_w[k] || halo_e[k];
+      vs[k] = (h || !p.hcpl) ? 0.0 : p.hcpl[(long)bidx * p.i2 + pc[k] - 1];
+      vc[k] = h ? 0.0 : p.hfree[(long)bidx * p.i2 + pc[k] - 1];
+    }
+  }
+  if (MODE == MODE_PRE) {
+    rinv_b = p.rinv_b + (long)strip * p.rinv_sstride;
+    tmat = p.tmat ? p.tmat + (long)strip * p.rinv_sstride : nullptr;
+    r1p = p.rinv1_prev ? p.rinv1_prev + (long)strip * p.rinv1_sstride : nullptr;
+    double cyg = 0.0;
+    if (r1p) cyg = (double)p.cy[(long)map.phys(g) + (long)(js - 2) * p.cld];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const bool h = halo_w[k] || halo_e[k];
+      vs[k] = (h || !r1p) ? 0.0 : r1p[(g - 1) + (long)(pc[k] - 1) * w] * cyg;
+      vc[k] = (!h && pc[k] == g) ? 1.0 : 0.0;
+    }
+  }
+
+  double ca[K], cb[K], cc[K], cd[K], ce[K], cf[K], cx[K];
+  auto load_row = [&](int j, double (&a_)[K], double (&b_)[K], double (&c_)[K],
+                      double (&d_)[K], double (&e_)[K], double (&f_)[K], double (&x_)[K]) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      a_[k] = COEF(p.ax, k, j);
+      b_[k] = COEF(p.ay, k, j);
+      c_[k] = COEF(p.bb, k, j);
+      d_[k] = COEF(p.cx, k, j);
+      e_[k] = COEF(p.cy, k, j);
+      if (MODE == MODE_PART) {
+        const long fo = (long)(pc[k] - 1) + (long)(j - 1) * p.i2;
+        f_[k] = f64 ? f64[fo] : (double)f[fo];
+      } else {
+        f_[k] = 0.0;
+      }
+      // next-row X operand: PART halo lanes need the Dirichlet value of row j+2,
+      // HOM lanes need X(j+2) to accumulate into.
+      if (MODE == MODE_PART) x_[k] = (halo_w[k] || halo_e[k]) ? x[f2(i0, xcol(k), j + 2)] : 0.0;
+      else if (MODE == MODE_HOM) x_[k] = x[f2(i0, pc[k] + 1, j + 2)];
+      else x_[k] = 0.0;
+    }
+  };
+
+  const int jlast = jf + 1;  // closure row, operands needed after the march
+  load_row(js, ca, cb, cc, cd, ce, cf, cx);
+  for (int j = js; j <= jf; ++j) {
+    double na[K], nb_[K], nc[K], nd[K], ne[K], nf[K], nx[K];
+    if (j + 1 <= jlast && (MODE != MODE_HOM || j + 1 <= jf)) load_row(j + 1, na, nb_, nc, nd, ne, nf, nx);
+    double vn[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double xw, xe;
+      nbr<K>(vc, k, xw, xe);
+      double r;
+      if (MODE == MODE_PART) {
+        r = (cf[k] - ca[k] * xw - cb[k] * vs[k] - cc[k] * vc[k] - cd[k] * xe) / ce[k];
+        if (halo_w[k] || halo_e[k]) r = cx[k];
+      } else if (MODE == MODE_HOM) {
+        r = (-(ca[k] * xw) - cb[k] * vs[k] - cc[k] * vc[k] - cd[k] * xe) / ce[k];
+        if (halo_w[k] || halo_e[k]) r = 0.0;
+      } else {
+        r = -((ca[k] * xw + cb[k] * vs[k] + cc[k] * vc[k] + cd[k] * xe) / ce[k]);
+        if (halo_w[k] || halo_e[k]) r = 0.0;
+      }
+      vn[k] = r;
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (out[k]) {
+        if (MODE == MODE_PART) x[f2(i0, pc[k] + 1, j + 2)] = vn[k];
+        else if (MODE == MODE_HOM) x[f2(i0, pc[k] + 1, j + 2)] = cx[k] + vn[k];
+      }
+      vs[k] = vc[k];
+      vc[k] = vn[k];
+    }
+    if (j + 1 <= jlast && (MODE != MODE_HOM || j + 1 <= jf)) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        ca[k] = na[k]; cb[k] = nb_[k]; cc[k] = nc[k]; cd[k] = nd[k]; ce[k] = ne[k];
+        cf[k] = nf[k]; cx[k] = nx[k];
+      }
+    }
+  }
+  // row operands now hold the closure row jlast = ieb - 1
+  if (MODE == MODE_PART && p.resid) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double xw, xe;
+      nbr<K>(vc, k, xw, xe);
+      const double xn = x[f2(i0, pc[k] + 1, ieb + 1)];
+      const double r = cf[k] - ca[k] * xw - cb[k] * vs[k] - cc[k] * vc[k] - cd[k] * xe - ce[k] * xn;
+      if (out[k]) p.e[(long)bidx * p.i2 + pc[k] - 1] = r;
+    }
+  }
+  if (MODE == MODE_PRE) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double xw, xe;
+      nbr<K>(vc, k, xw, xe);
+      const double r = ca[k] * xw + cb[k] * vs[k] + cc[k] * vc[k] + cd[k] * xe;
+      if (out[k]) {
+        rinv_b[(g - 1) + (long)(pc[k] - 1) * w] = r;
+        if (tmat) tmat[(g - 1) + (long)(pc[k] - 1) * w] = vc[k];
+      }
+    }
+  }
+#undef COEF
+}
+
+// Choose lanes-per-column K and the tile size for a march of `steps` rows.
+static void march_shape(int w, int steps, int &K, int &tile, int &halo, int &ntiles) {
+  halo = steps + 1;
+  int best = -1;
+  long bestcost = 0;
+  for (int k : {1, 2, 4}) {
+    const int t = 64 * k - 2 * halo;
+    if (t <= 0) continue;
+    const long n = (w + t - 1) / t;
+    const long cost = n * k;
+    if (best < 0 || cost < bestcost) {
+      best = k;
+      bestcost = cost;
+      tile = t;
+      ntiles = (int)n;
+    }
+  }
+  K = best;
+}
+
+template <int MODE>
+static hipError_t launch_march_mode(const MarchArgs &a0, int nbatch, int nstrips, hipStream_t st) {
+  MarchArgs a = a0;
+  int K, tile, halo, ntiles;
+  march_shape(a.w, a.jf - a.js + 1, K, tile, halo, ntiles);
+  if (K < 0) return hipErrorInvalidValue;
+  a.tile_out = tile;
+  a.halo = halo;
+  dim3 grid(ntiles, nbatch, nstrips);
+  if (K == 1) hipLaunchKernelGGL((march_kernel<1, MODE>), grid, dim3(64), 0, st, a);
+  else if (K == 2) hipLaunchKernelGGL((march_kernel<2, MODE>), grid, dim3(64), 0, st, a);
+  else hipLaunchKernelGGL((march_kernel<4, MODE>), grid, dim3(64), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_march(int mode, const MarchArgs &a, int nbatch, hipStream_t st) {
+  const int nstrips = (mode == MODE_PRE) ? a.nrhs : 1;  // PRE: nrhs carries the strip count
+  if (mode == MODE_PART) return launch_march_mode<MODE_PART>(a, nbatch, 1, st);
+  if (mode == MODE_HOM) return launch_march_mode<MODE_HOM>(a, nbatch, 1, st);
+  return launch_march_mode<MODE_PRE>(a, nbatch, nstrips, st);
+}
+
+// Backward restore of X(:, ieb) from DUM0 and closure residual at row ieb-1.
+__global__ void resid_kernel(ResidArgs p) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)p.nrhs * p.i2) return;
+  const int r = (int)(t / p.i2);
+  const int c = (int)(t % p.i2) + 1;
+  const int i2 = p.i2, i0 = i2 + 2, ieb = p.ieb, jr = ieb - 1;
+  double *x = p.x + (long)r * p.x_stride;
+  const double *d0 = p.dum0 ? p.dum0 + (long)r * p.dum0_stride + (long)(p.nb - 1) * i2 : nullptr;
+  auto rowv = [&](int cc) -> double {  // value of row ieb at column cc (1..i2) after the restore
+    return d0 ? d0[cc - 1] : x[f2(i0, cc + 1, ieb)];
+  };
+  const double center = rowv(c);
+  double west, east;
+  if (p.periodic) {
+    west = rowv(c == 1 ? i2 : c - 1);
+    east = rowv(c == i2 ? 1 : c + 1);
+  } else {
+    west = (c == 1) ? x[f2(i0, 1, ieb)] : rowv(c - 1);
+    east = (c == i2) ? x[f2(i0, i0, ieb)] : rowv(c + 1);
+  }
+  const long co = (long)(c - 1) + (long)(jr - 1) * i2;
+  const double fv = p.f64 ? p.f64[(long)r * p.f_stride + co] : (double)p.f[(long)r * p.f_stride + co];
+  const double e = fv - (double)p.ax[co] * west - (double)p.ay[co] * x[f2(i0, c + 1, jr)] -
+                   (double)p.bb[co] * center - (double)p.cx[co] * east -
+                   (double)p.cy[co] * x[f2(i0, c + 1, ieb + 1)];
+  p.e[(long)r * i2 + c - 1] = e;
+  if (d0) x[f2(i0, c + 1, ieb)] = center;
+}
+
+hipError_t launch_resid(const ResidArgs &a, hipStream_t st) {
+  const long n = (long)a.nrhs * a.i2;
+  hipLaunchKernelGGL(resid_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Sequential-order FP64 GEMM with fused epilogues.
+//   D(m, n) = sum_{k=0..K-1} A(m,k) * B(k,n)   (EP_NEGSUB: 0 - sum ... as
+//   successive subtractions)
+// Tile 16 (m) x 64 (n), 256 threads, 4 outputs per thread, K in chunks of 32
+// staged through LDS.
+constexpr int GT_M = 16, GT_N = 64, GT_K = 32;
+
+__global__ __launch_bounds__(256) void gemm_seq_kernel(GemmArgs p) {
+  __shared__ double As[GT_M][GT_K + 1];
+  __shared__ double Bs[GT_K][GT_N + 1];
+  const int t = threadIdx.x;
+  const int n0 = blockIdx.x * GT_N, m0 = blockIdx.y * GT_M;
+  const int tn = t % GT_N, tm = (t / GT_N) * 4;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const bool neg = (p.ep == EP_NEGSUB);
+  for (int k0 = 0; k0 < p.K; k0 += GT_K) {
+    const int kc = min(GT_K, p.K - k0);
+    for (int idx = t; idx < GT_M * GT_K; idx += 256) {
+      const int kk = idx % GT_K, mm = idx / GT_K;
+      const int m = m0 + mm, k = k0 + kk;
+      As[mm][kk] = (m < p.M && kk < kc) ? p.a[(long)m * p.a_sm + (long)k * p.a_sk] : 0.0;
+    }
+    for (int idx = t; idx < GT_K * GT_N; idx += 256) {
+      const int kk = idx % GT_K, nn = idx / GT_K;
+      const int n = n0 + nn, k = k0 + kk;
+      Bs[kk][nn] = (n < p.N && kk < kc) ? p.b[(long)k * p.b_sk + (long)n * p.b_sn] : 0.0;
+    }
+    __syncthreads();
+    if (neg) {
+      for (int kk = 0; kk < kc; ++kk) {
+        const double bv = Bs[kk][tn];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] = acc[i] - As[tm + i][kk] * bv;
+      }
+    } else {
+      for (int kk = 0; kk < kc; ++kk) {
+        const double bv = Bs[kk][tn];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] = acc[i] + As[tm + i][kk] * bv;
+      }
+    }
+    __syncthreads();
+  }
+  const int n = n0 + tn;
+  if (n >= p.N) return;
+  const int i0 = p.i2 + 2;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int m = m0 + tm + i;
+    if (m >= p.M) continue;
+    const double d = acc[i];
+    switch (p.ep) {
+      case EP_FWD: {
+        double *x = p.x + (long)m * p.x_stride;
+        const long xi = f2(i0, n + 2, p.xrow);
+        const double xv = x[xi];
+        p.dum0[(long)m * p.dum0_stride + n] = xv;
+        x[xi] = xv - d;
+        break;
+      }
+      case EP_BWD: {
+        double *x = p.x + (long)m * p.x_stride;
+        const long xi = f2(i0, n + 2, p.xrow);
+        x[xi] = x[xi] + d;
+        p.out[(long)m * p.out_ld + n] = d;
+        if (p.g) p.g[(long)m * p.out_ld + n] = d * (double)p.cy[(long)n + (long)(p.cyrow - 1) * p.i2];
+        break;
+      }
+      case EP_CPL:
+        p.out[(long)m * p.out_ld + n] = d;
+        break;
+      default:  // EP_NEGSUB: column-major output (m fastest)
+        p.out[(long)m + (long)n * p.out_ld] = d;
+        break;
+    }
+  }
+}
+
+hipError_t launch_gemm(const GemmArgs &a, hipStream_t st) {
+  dim3 grid((a.N + GT_N - 1) / GT_N, (a.M + GT_M - 1) / GT_M);
+  hipLaunchKernelGGL(gemm_seq_kernel, grid, dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// MATINV in LDS for n <= MATINV_LDS_MAX, one workgroup per matrix.
+constexpr int MATINV_LDS_MAX = 128;
+
+__global__ __launch_bounds__(256) void matinv_lds_kernel(double *mats, long stride, int n, int *zp) {
+  extern __shared__ double sm[];
+  double *B = sm;                 // n*n, column-major
+  double *b1 = sm + (long)n * n;  // n
+  __shared__ double piv;
+  __shared__ int zcount;
+  double *g = mats + (long)blockIdx.x * stride;
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int nn = n * n;
+  for (int i = t; i < nn; i += nt) B[i] = g[i];
+  if (t == 0) zcount = 0;
+  __syncthreads();
+#define BM(i, j) B[((i) - 1) + ((j) - 1) * n]
+  for (int i = 1; i <= n - 1; ++i) {
+    if (t == 0) {
+      if (BM(i, i) == 0.0) ++zcount;
+      piv = 1.0 / BM(i, i);
+      BM(i, i) = 1.0;
+    }
+    __syncthreads();
+    const double bp = piv;
+    for (int j = 1 + t; j <= n; j += nt) BM(i, j) = BM(i, j) * bp;
+    for (int ia = i + 1 + t; ia <= n; ia += nt) b1[ia - 1] = BM(ia, i);
+    __syncthreads();
+    // rows ia > i: column i is zeroed then updated like every other column
+    const int rows = n - i;
+    for (int idx = t; idx < rows * n; idx += nt) {
+      const int ia = i + 1 + idx % rows, j = 1 + idx / rows;
+      const double cur = (j == i) ? 0.0 : BM(ia, j);
+      BM(ia, j) = cur - b1[ia - 1] * BM(i, j);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    if (BM(n, n) == 0.0) ++zcount;
+    piv = 1.0 / BM(n, n);
+    BM(n, n) = 1.0;
+  }
+  __syncthreads();
+  {
+    const double bp = piv;
+    for (int j = 1 + t; j <= n; j += nt) BM(n, j) = BM(n, j) * bp;
+  }
+  __syncthreads();
+  for (int i = 2; i <= n; ++i) {
+    for (int ib = 1 + t; ib <= i - 1; ib += nt) b1[ib - 1] = BM(ib, i);
+    __syncthreads();
+    const int rows = i - 1;
+    for (int idx = t; idx < rows * n; idx += nt) {
+      const int ib = 1 + idx % rows, j = 1 + idx / rows;
+      const double cur = (j == i) ? 0.0 : BM(ib, j);
+      BM(ib, j) = cur - b1[ib - 1] * BM(i, j);
+    }
+    __syncthreads();
+  }
+#undef BM
+  for (int i = t; i < nn; i += nt) g[i] = B[i];
+  if (t == 0 && zp && zcount) atomicAdd(zp, zcount);
+}
+
+// Same elimination on a matrix in global memory (any n; one workgroup per
+// matrix, used when the matrix does not fit in LDS).
+__global__ __launch_bounds__(256) void matinv_glb_kernel(double *mats, long stride, int n,
+                                                         double *b1s, int *zp) {
+  __shared__ double piv;
+  __shared__ int zcount;
+  double *B = mats + (long)blockIdx.x * stride;
+  double *b1 = b1s + (long)blockIdx.x * n;
+  const int t = threadIdx.x, nt = blockDim.x;
+  if (t == 0) zcount = 0;
+  __syncthreads();
+#define BM(i, j) B[(long)((i) - 1) + (long)((j) - 1) * n]
+  for (int i = 1; i <= n - 1; ++i) {
+    if (t == 0) {
+      if (BM(i, i) == 0.0) ++zcount;
+      piv = 1.0 / BM(i, i);
+      BM(i, i) = 1.0;
+    }
+    __syncthreads();
+    const double bp = piv;
+    for (int j = 1 + t; j <= n; j += nt) BM(i, j) = BM(i, j) * bp;
+    for (int ia = i + 1 + t; ia <= n; ia += nt) b1[ia - 1] = BM(ia, i);
+    __syncthreads();
+    const int rows = n - i;
+    for (long idx = t; idx < (long)rows * n; idx += nt) {
+      const int ia = i + 1 + (int)(idx % rows), j = 1 + (int)(idx / rows);
+      const double cur = (j == i) ? 0.0 : BM(ia, j);
+      BM(ia, j) = cur - b1[ia - 1] * BM(i, j);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    if (BM(n, n) == 0.0) ++zcount;
+    piv = 1.0 / BM(n, n);
+    BM(n, n) = 1.0;
+  }
+  __syncthreads();
+  {
+    const double bp = piv;
+    for (int j = 1 + t; j <= n; j += nt) BM(n, j) = BM(n, j) * bp;
+  }
+  __syncthreads();
+  for (int i = 2; i <= n; ++i) {
+    for (int ib = 1 + t; ib <= i - 1; ib += nt) b1[ib - 1] = BM(ib, i);
+    __syncthreads();
+    const int rows = i - 1;
+    for (long idx = t; idx < (long)rows * n; idx += nt) {
+      const int ib = 1 + (int)(idx % rows), j = 1 + (int)(idx / rows);
+      const double cur = (j == i) ? 0.0 : BM(ib, j);
+      BM(ib, j) = cur - b1[ib - 1] * BM(i, j);
+    }
+    __syncthreads();
+  }
+#undef BM
+  if (t == 0 && zp && zcount) atomicAdd(zp, zcount);
+}
+
+hipError_t launch_matinv(double *mats, long stride, int n, int count, int *zp, double *b1s,
+                         hipStream_t st) {
+  const size_t lds = sizeof(double) * ((size_t)n * n + n);
+  if (n <= MATINV_LDS_MAX && lds <= 64 * 1024) {
+    hipLaunchKernelGGL(matinv_lds_kernel, dim3(count), dim3(256), lds, st, mats, stride, n, zp);
+  } else {
+    if (!b1s) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(matinv_glb_kernel, dim3(count), dim3(256), 0, st, mats, stride, n, b1s, zp);
+  }
+  return hipGetLastError();
+}
+
+// Periodic halo columns at the end of REP1 (MAIN/template/GLOBAL/src/sevp.f90:103-106).
+__global__ void halo_fix_kernel(double *x, long xs, int i2, int j2, int nrhs) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)nrhs * j2) return;
+  const int r = (int)(t / j2), j = (int)(t % j2) + 2;
+  const int i0 = i2 + 2;
+  double *xr = x + (long)r * xs;
+  xr[f2(i0, 1, j)] = xr[f2(i0, i0 - 1, j)];
+  xr[f2(i0, i0, j)] = xr[f2(i0, 2, j)];
+}
+
+hipError_t launch_halo_fix(double *x, long xs, int i2, int j2, int nrhs, hipStream_t st) {
+  const long n = (long)nrhs * j2;
+  hipLaunchKernelGGL(halo_fix_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, xs, i2, j2, nrhs);
+  return hipGetLastError();
+}
+
+}  // namespace sevp

@@ -1,0 +1,140 @@
+// Host/device argument blocks and launchers shared by the SEVP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace sevp {
+
+// BIR multiplicative sweeps, one wavefront per right-hand side.
+struct BirArgs {
+  const float *al, *ab, *ac, *ar, *at;  // (i2, j2)
+  const float *s;                        // (i2, j2) per RHS
+  const double *rinv, *rinv1;            // (ibir^2, nb0, nbir), (ibir^2, nb0-1, nbir)
+  double *x;                             // (i2+2, j2+2) per RHS
+  double *xx;                            // (ibir+2, j2+2) per RHS, persistent seam scratch
+  double *dum0;                          // (ibir, nb0-1) per RHS
+  const int *ie;                         // (nb0)
+  int i2, j2, nbir, ibir, nb0, nsweep, periodic;
+  long s_stride, x_stride, xx_stride, dum0_stride;
+};
+
+// One REPBIR call with the reference's argument conventions.
+struct RepbirArgs {
+  const float *ax, *ay, *bb, *cx, *cy;  // ld i2, already offset to the strip
+  const float *f;                        // ld m2
+  double *x;                             // ld i0, already offset to the strip
+  const double *rinv, *rinv1;
+  double *dum0;
+  const int *ie;
+  int i0, i2, m2, nblk;
+  long f_stride, x_stride, dum0_stride;
+};
+
+// Column selection of a march over a strip of the coefficient arrays.
+// Physical coefficient column (0-based) of strip column c (1-based, 1..w):
+//   seam == 0 : il - 1 + (c - 1)
+//   seam == 1 : c <= inc ? il - 1 + (c - 1) : c - inc - 1
+struct ColMap {
+  int il, inc, seam;
+  __host__ __device__ int phys(int c) const {
+    return (seam && c > inc) ? (c - inc - 1) : (il - 2 + c);
+  }
+};
+
+// Full-width (or strip) march over one block, batched over independent
+// right-hand sides.  Mode selects the recurrence:
+//   MODE_PART : X(j+2) = (F - AX Xw - AY Xs - BB Xc - CX Xe) / CY        (REP1/REP forward)
+//   MODE_HOM  : H(j+2) = (-AX Hw - AY Hs - BB Hc - CX He) / CY, X += H    (REP1/REP backward)
+//   MODE_PRE  : H(j+2) = -(AX Hw + AY Hs + BB Hc + CX He) / CY            (PRE/PRE1)
+enum { MODE_PART = 0, MODE_HOM = 1, MODE_PRE = 2 };
+
+struct MarchArgs {
+  const float *ax, *ay, *bb, *cx, *cy;  // (cld, j2)
+  int cld;
+  const float *f;   // MODE_PART: (i2, j2) per RHS, ld i2
+  const double *f64;  // MODE_PART: FP64 right-hand side used instead of f when non-null
+  long f_stride;
+  double *x;        // (i2+2, j2+2) per RHS (MODE_PART / MODE_HOM)
+  long x_stride;
+  // MODE_HOM: starting rows H(js) (hcpl, may be null => 0) and H(js+1) (hfree): (nrhs, i2)
+  const double *hcpl, *hfree;
+  // MODE_PRE: coupling rows RINV1(g, :, b-1) (w x w col-major) scaled by CY(g, js-1)
+  const double *rinv1_prev;  // null for the first block
+  // MODE_PRE outputs: RINV(g, i, b) and T(g, n) (both w x w col-major)
+  double *rinv_b, *tmat;
+  long rinv_sstride, rinv1_sstride;  // MODE_PRE: per-strip strides of RINV and RINV1
+  // residual out (MODE_PART, when resid != 0): (nrhs, i2)
+  double *e;
+  int i2, j2, periodic, js, jf, ieb, resid;
+  int w;            // number of columns marched (i2 for full width)
+  int tile_out;     // output columns per wave
+  int halo;         // extra columns each side
+  ColMap map;       // coefficient column map (identity for full width)
+  int nrhs;
+};
+
+struct ResidArgs {  // backward restore + closure residual, one thread per (rhs, column)
+  const float *ax, *ay, *bb, *cx, *cy;
+  const float *f;
+  const double *f64;   // used instead of f when non-null
+  long f_stride;
+  double *x;
+  long x_stride;
+  const double *dum0;  // (i2, nb0-1) per RHS, null for the top block
+  long dum0_stride;
+  double *e;           // (nrhs, i2)
+  int i2, j2, periodic, nb, ieb, nrhs;
+};
+
+enum { EP_FWD = 0, EP_BWD = 1, EP_CPL = 2, EP_NEGSUB = 3 };
+
+struct GemmArgs {  // D(m, n) = sum_k A(m, k) B(k, n), sequential k
+  const double *a;
+  long a_sm, a_sk;
+  const double *b;
+  long b_sk, b_sn;
+  int M, N, K;
+  int ep;
+  // EP_FWD / EP_BWD : X row update, per RHS m
+  double *x;
+  long x_stride;
+  int xrow;           // 1-based X row updated (ieb for FWD, js+1 for BWD)
+  double *dum0;       // EP_FWD: DUM0(:, nb) per RHS
+  long dum0_stride;
+  double *out;        // EP_BWD: hfree (M, N); EP_CPL: hcpl (M, N); EP_NEGSUB: (ld out_ld)
+  long out_ld;
+  double *g;          // EP_BWD: coupling input g = c * CY(n, cyrow) (may be null)
+  const float *cy;
+  int cyrow;          // 1-based coefficient row of CY for the coupling
+  int i2;
+};
+
+struct GemmF32Args {  // mixed precision: A (FP64, rounded on load) x B (FP32), FP32 accumulate
+  const float *b;     // (K, N) col-major, ld K
+  int M, N, K;
+  int ep;
+  double *x;
+  long x_stride;
+  int xrow;
+  double *dum0;
+  long dum0_stride;
+  double *out;
+  long out_ld;
+  double *g;
+  const float *cy;
+  int cyrow;
+  int i2;
+};
+
+hipError_t launch_bir_sweeps(const BirArgs &a, int nrhs, hipStream_t st);
+hipError_t launch_repbir(const RepbirArgs &a, int nrhs, hipStream_t st);
+hipError_t launch_march(int mode, const MarchArgs &a, int nbatch, hipStream_t st);
+hipError_t launch_resid(const ResidArgs &a, hipStream_t st);
+hipError_t launch_gemm(const GemmArgs &a, hipStream_t st);
+hipError_t launch_gemm_mixed(const GemmF32Args &a, const double *a64, float *partial,
+                             int *counters, int splits, hipStream_t st);
+hipError_t launch_cvt_f32(const double *src, float *dst, long n, hipStream_t st);
+hipError_t launch_matinv(double *mats, long mat_stride, int n, int count, int *zero_pivots,
+                         double *b1_scratch, hipStream_t st);
+hipError_t launch_halo_fix(double *x, long x_stride, int i2, int j2, int nrhs, hipStream_t st);
+
+}  // namespace sevp

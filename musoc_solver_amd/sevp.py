@@ -1,0 +1,282 @@
+"""Python front end of the SEVP C ABI (include/sevp.h).
+
+Mirrors the reference's solver interface:
+
+  repbir / rep1 / rep / repbir_period   MAIN/template/GLOBAL/src/sevp.f90
+  pre / pre1 / matinv                   PREP/template/GLOBAL/src/prep_GLOBAL.f90
+  SevpPlan.solve                        FS_SOLVER's dispatch
+                                        (MAIN/template/GLOBAL/src/fs_GLOBAL.f90:1850-1966)
+
+with the same argument meaning: numpy arrays in Fortran order, coefficients
+and right-hand sides float32, solutions and influence matrices float64,
+updated in place where the reference updates its arguments.  Unlike the
+reference (which has no error path), a failure raises SevpError with the
+library's message.
+
+The library is the compiled HIP code in musoc_solver_amd/lib/libsevp_hip.so;
+there is no CPU fallback: if the library cannot be loaded, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libsevp_hip.so")
+
+FORM_FULL_PERIODIC = 0
+FORM_BIR = 1
+FORM_FULL = 2
+PREC_FP64 = 0
+PREC_FP32 = 1
+PREC_BF16 = 2
+
+SEVP_OK = 0
+SEVP_E_NOTCONV = -5
+
+
+class SevpError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"sevp error {code}: {msg}")
+        self.code = code
+
+
+class _Desc(ctypes.Structure):
+    _fields_ = [("i2", ctypes.c_int), ("j2", ctypes.c_int), ("nb0", ctypes.c_int),
+                ("ie", ctypes.POINTER(ctypes.c_int)), ("form", ctypes.c_int),
+                ("nbir", ctypes.c_int), ("nsweep", ctypes.c_int), ("periodic", ctypes.c_int),
+                ("nrhs_max", ctypes.c_int), ("precision", ctypes.c_int)]
+
+
+class SevpStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("max_rel_resid", ctypes.c_double),
+                ("seconds", ctypes.c_double)]
+
+
+_lib = None
+VP = ctypes.c_void_p
+CI = ctypes.c_int
+
+
+def lib():
+    """Load libsevp_hip.so (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SevpError(-2, f"{LIB_PATH} is missing; run python -m musoc_solver_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        L.sevp_last_error.restype = ctypes.c_char_p
+        L.sevp_version.restype = ctypes.c_char_p
+        L.sevp_plan_device_bytes.restype = ctypes.c_size_t
+        L.sevp_plan_device_bytes.argtypes = [VP]
+        L.sevp_plan_create.argtypes = [ctypes.POINTER(_Desc), VP, VP, VP, VP, VP, ctypes.POINTER(VP)]
+        L.sevp_plan_destroy.argtypes = [VP]
+        L.sevp_build_influence.argtypes = [VP, ctypes.POINTER(CI)]
+        L.sevp_load_influence.argtypes = [VP, VP, VP]
+        L.sevp_get_influence.argtypes = [VP, VP, VP]
+        L.sevp_reset_seam.argtypes = [VP]
+        L.sevp_solve.argtypes = [VP, VP, VP, CI]
+        L.sevp_solve_dev.argtypes = [VP, VP, VP, CI, VP]
+        L.sevp_solve_refined_dev.argtypes = [VP, VP, VP, CI, ctypes.c_double, CI, VP,
+                                             ctypes.POINTER(SevpStats)]
+        L.sevp_residual_dev.argtypes = [VP, VP, VP, CI, VP]
+        L.sevp_repbir.argtypes = [VP] * 14 + [CI] * 5
+        L.sevp_repbir_period.argtypes = [VP] * 14 + [CI] * 5
+        L.sevp_rep1.argtypes = [VP] * 14 + [CI] * 3
+        L.sevp_rep.argtypes = [VP] * 14 + [CI] * 3
+        L.sevp_pre.argtypes = [VP] * 9 + [CI] * 4
+        L.sevp_pre1.argtypes = [VP] * 9 + [CI] * 3
+        L.sevp_matinv.argtypes = [VP, CI]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != SEVP_OK:
+        raise SevpError(rc, lib().sevp_last_error().decode())
+
+
+def _p(a, off=0):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        if not (a.flags.f_contiguous or a.flags.c_contiguous):
+            raise ValueError("array must be contiguous")
+        return ctypes.c_void_p(a.ctypes.data + off * a.itemsize)
+    # torch tensor on the device
+    return ctypes.c_void_p(a.data_ptr() + off * a.element_size())
+
+
+def _f32(a):
+    if a.dtype != np.float32 or not a.flags.f_contiguous:
+        raise TypeError("expected a Fortran-ordered float32 array")
+    return a
+
+
+def _f64(a):
+    if a.dtype != np.float64 or not a.flags.f_contiguous:
+        raise TypeError("expected a Fortran-ordered float64 array")
+    return a
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def version() -> str:
+    return lib().sevp_version().decode()
+
+
+# ------------------------------------------------------------------ plans
+
+class SevpPlan:
+    """A device-resident SEVP problem: operator, IE, influence matrices and
+    scratch for up to ``nrhs_max`` simultaneous right-hand sides."""
+
+    def __init__(self, coefs, ie, form=FORM_BIR, nbir=None, nsweep=5, periodic=True,
+                 nrhs_max=1, precision=PREC_FP64):
+        al, ab, ac, ar, at = [_f32(np.asfortranarray(c, dtype=np.float32)) for c in coefs]
+        self.i2, self.j2 = al.shape
+        self.ie = _i32(ie)
+        self.form, self.precision, self.nrhs_max = form, precision, nrhs_max
+        self.nbir = nbir if nbir is not None else 0
+        self.ibir = 2 * self.i2 // self.nbir if form == FORM_BIR else self.i2
+        d = _Desc(self.i2, self.j2, len(self.ie), self.ie.ctypes.data_as(ctypes.POINTER(CI)),
+                  form, self.nbir, nsweep, 1 if periodic else 0, nrhs_max, precision)
+        h = VP()
+        _check(lib().sevp_plan_create(ctypes.byref(d), _p(al), _p(ab), _p(ac), _p(ar), _p(at),
+                                      ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sevp_plan_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def nb0(self):
+        return len(self.ie)
+
+    @property
+    def device_bytes(self):
+        return int(lib().sevp_plan_device_bytes(self._h))
+
+    def influence_shapes(self):
+        nb1 = max(self.nb0 - 1, 1)
+        if self.form == FORM_BIR:
+            w2 = self.ibir * self.ibir
+            return (w2, self.nb0, self.nbir), (w2, nb1, self.nbir)
+        return (self.i2, self.i2, self.nb0), (self.i2, self.i2, nb1)
+
+    def build_influence(self) -> int:
+        z = CI(0)
+        _check(lib().sevp_build_influence(self._h, ctypes.byref(z)))
+        return z.value
+
+    def load_influence(self, rinv, rinv1):
+        _check(lib().sevp_load_influence(self._h, _p(_f64(rinv)), _p(_f64(rinv1))))
+
+    def get_influence(self):
+        s0, s1 = self.influence_shapes()
+        rinv = np.zeros(s0, order="F")
+        rinv1 = np.zeros(s1, order="F")
+        _check(lib().sevp_get_influence(self._h, _p(rinv), _p(rinv1)))
+        return rinv, rinv1
+
+    def reset_seam(self):
+        _check(lib().sevp_reset_seam(self._h))
+
+    def solve(self, s, x):
+        """Host arrays: s (I2, J2[, nrhs]) float32, x (I2+2, J2+2[, nrhs])
+        float64 updated in place."""
+        nrhs = s.shape[2] if s.ndim == 3 else 1
+        _check(lib().sevp_solve(self._h, _p(_f32(s)), _p(_f64(x)), nrhs))
+        return x
+
+    def solve_dev(self, s, x, nrhs=None, stream=None):
+        """Device tensors (torch, cuda): s float32 (nrhs, J2, I2) contiguous
+        (i.e. Fortran (I2, J2) per RHS), x float64 (nrhs, J2+2, I2+2)."""
+        nrhs = nrhs or s.shape[0]
+        _check(lib().sevp_solve_dev(self._h, _p(s), _p(x), nrhs, stream))
+
+    def solve_refined_dev(self, s, x, tol=1e-10, max_iter=20, nrhs=None):
+        nrhs = nrhs or s.shape[0]
+        rel = np.zeros(nrhs)
+        st = SevpStats()
+        rc = lib().sevp_solve_refined_dev(self._h, _p(s), _p(x), nrhs, tol, max_iter, _p(rel),
+                                          ctypes.byref(st))
+        if rc not in (SEVP_OK, SEVP_E_NOTCONV):
+            _check(rc)
+        return rel, st, rc == SEVP_OK
+
+    def residual_dev(self, s, x, nrhs=None):
+        nrhs = nrhs or s.shape[0]
+        rel = np.zeros(nrhs)
+        _check(lib().sevp_residual_dev(self._h, _p(s), _p(x), nrhs, _p(rel)))
+        return rel
+
+
+# ------------------------------------------------- one-to-one reference calls
+
+def repbir(ax, ay, bb, cx, cy, rinv, rinv1, dum0, dum1, dum2, f, h, x, ie, i0, i2, m0, m2, nblk,
+           coef_off=0, x_off=0):
+    """REPBIR (MAIN/template/GLOBAL/src/sevp.f90:218-328).  ``coef_off`` and
+    ``x_off`` are element offsets into the coefficient arrays and X, the
+    equivalent of the Fortran caller passing AL(IL,1) and X(IL,1)."""
+    cs = [_p(_f32(c), coef_off) for c in (ax, ay, bb, cx, cy)]
+    _check(lib().sevp_repbir(*cs, _p(rinv), _p(rinv1), _p(dum0), _p(dum1), _p(dum2), _p(_f32(f)),
+                             _p(h), _p(_f64(x), x_off), _p(_i32(ie)), i0, i2, m0, m2, nblk))
+
+
+def repbir_period(ax, ay, bb, cx, cy, rinv, rinv1, dum0, dum1, dum2, f, h, x, ie, i0, i2, m0, m2,
+                  nblk):
+    """REPBIR_PERIOD (MAIN/template/GLOBAL/src/sevp.f90:109-217), M0=I0, M2=I2."""
+    cs = [_p(_f32(c)) for c in (ax, ay, bb, cx, cy)]
+    _check(lib().sevp_repbir_period(*cs, _p(rinv), _p(rinv1), _p(dum0), _p(dum1), _p(dum2),
+                                    _p(_f32(f)), _p(h), _p(_f64(x)), _p(_i32(ie)), i0, i2, m0, m2,
+                                    nblk))
+
+
+def _full(fn, ax, ay, bb, cx, cy, rinv, rinv1, dum0, dum1, dum2, f, h, x, ie, i0, i2, nblk):
+    cs = [_p(_f32(c)) for c in (ax, ay, bb, cx, cy)]
+    _check(fn(*cs, _p(rinv), _p(rinv1), _p(dum0), _p(dum1), _p(dum2), _p(_f32(f)), _p(h),
+              _p(_f64(x)), _p(_i32(ie)), i0, i2, nblk))
+
+
+def rep1(*args):
+    """REP1 (MAIN/template/GLOBAL/src/sevp.f90:11-107)."""
+    _full(lib().sevp_rep1, *args)
+
+
+def rep(*args):
+    """REP (MAIN/template/GLOBAL/src/sevp.f90:335-438)."""
+    _full(lib().sevp_rep, *args)
+
+
+def pre(ax, ay, bb, cx, cy, rinv, rinv1, h, ie, i2, m0, m2, nblk, coef_off=0):
+    """PRE (PREP/template/GLOBAL/src/prep_GLOBAL.f90:457-513); RINV, RINV1 out."""
+    cs = [_p(_f32(c), coef_off) for c in (ax, ay, bb, cx, cy)]
+    _check(lib().sevp_pre(*cs, _p(_f64(rinv)), _p(_f64(rinv1)), _p(h), _p(_i32(ie)), i2, m0, m2,
+                          nblk))
+
+
+def pre1(ax, ay, bb, cx, cy, rinv, rinv1, h, ie, i0, i2, nblk):
+    """PRE1 (PREP/template/GLOBAL/src/prep_GLOBAL.f90:327-386)."""
+    cs = [_p(_f32(c)) for c in (ax, ay, bb, cx, cy)]
+    _check(lib().sevp_pre1(*cs, _p(_f64(rinv)), _p(_f64(rinv1)), _p(h), _p(_i32(ie)), i0, i2,
+                           nblk))
+
+
+def matinv(b):
+    """MATINV (PREP/template/GLOBAL/src/prep_GLOBAL.f90:516-568), in place."""
+    _check(lib().sevp_matinv(_p(_f64(b)), b.shape[0]))
+    return b

@@ -1,0 +1,552 @@
+/*
+ * sevp_oracle.c -- CPU restatement of the TIMCOM SEVP barotropic solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP library is
+ * compared against.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it; nothing in the product path links or calls it.
+ *
+ * Every routine follows one reference routine statement by statement, with
+ * the reference's arithmetic: REAL(4) coefficients and right-hand sides are
+ * promoted exactly to double, every operation is a separately rounded IEEE
+ * double operation evaluated left to right (build with -ffp-contract=off so no
+ * multiply-add is fused), and every dot product is a sequential sum that
+ * starts from zero.
+ *
+ * Array arguments follow the Fortran conventions of the reference: column
+ * major, 1-based in the index macros below, and the caller passes pointers
+ * already offset to the first element of the strip (as the Fortran caller
+ * passes AL(IL,1) or X(IL,1)).
+ *
+ * Parity pins (see tests/test_oracle_*.py):
+ *   orc_repbir, orc_rep1, orc_rep, orc_repbir_period: bitwise against the
+ *     reference routines compiled from MAIN/template/GLOBAL/src/sevp.f90 into
+ *     oracle/_ref and against the golden vectors in tests/golden/.
+ *   orc_bir_solve: bitwise against a driver that calls the compiled reference
+ *     REPBIR in the reference strip order (tests/golden).
+ *   orc_matinv, orc_pre, orc_pre1, orc_bir_prep: the reference PREP program is
+ *     not buildable here (its host scope needs a header the reference does not
+ *     ship), so these are pinned by mathematics: exact rational Gauss-Jordan on
+ *     small matrices, B*inv(B) = I, brute-force influence matrices, and
+ *     end-to-end solves that must reproduce scipy's sparse direct solution.
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#define IDX2(ld, i, j) (((long)(i) - 1) + ((long)(j) - 1) * (long)(ld))
+
+/* ------------------------------------------------------------------------ */
+/* REPBIR: strip SEVP solve.  Follows MAIN/template/GLOBAL/src/sevp.f90:218-328.
+ * AX..CY have leading dimension i2, F has leading dimension m2, X leading
+ * dimension i0, H leading dimension m0.  RINV(m2,m2,nblk), RINV1(m2,m2,nblk-1),
+ * DUM0(m2,nblk-1), DUM1(m2), DUM2(m2) are caller-owned. */
+void orc_repbir(const float *ax, const float *ay, const float *bb,
+                const float *cx, const float *cy, const double *rinv,
+                const double *rinv1, double *dum0, double *dum1, double *dum2,
+                const float *f, double *h, double *x, const int *ie, int i0,
+                int i2, int m0, int m2, int nblk)
+{
+#define AX(i, j) ((double)ax[IDX2(i2, i, j)])
+#define AY(i, j) ((double)ay[IDX2(i2, i, j)])
+#define BB(i, j) ((double)bb[IDX2(i2, i, j)])
+#define CX(i, j) ((double)cx[IDX2(i2, i, j)])
+#define CY(i, j) ((double)cy[IDX2(i2, i, j)])
+#define F(i, j) ((double)f[IDX2(m2, i, j)])
+#define X(i, j) x[IDX2(i0, i, j)]
+#define H(i, j) h[IDX2(m0, i, j)]
+#define RINV(m, n, b) rinv[((long)(m) - 1) + ((long)(n) - 1) * m2 + ((long)(b) - 1) * (long)m2 * m2]
+#define RINV1(m, n, b) rinv1[((long)(m) - 1) + ((long)(n) - 1) * m2 + ((long)(b) - 1) * (long)m2 * m2]
+#define DUM0(n, b) dum0[((long)(n) - 1) + ((long)(b) - 1) * m2]
+  int js = 1;
+  for (int nb = 1; nb <= nblk; ++nb) {
+    int jf = ie[nb - 1] - 2;
+    for (int j = js; j <= jf; ++j)
+      for (int i = 1; i <= m2; ++i)
+        X(i + 1, j + 2) = (F(i, j) - AX(i, j) * X(i, j + 1) - AY(i, j) * X(i + 1, j) -
+                           BB(i, j) * X(i + 1, j + 1) - CX(i, j) * X(i + 2, j + 1)) / CY(i, j);
+    if (nb != nblk) {
+      int j = ie[nb - 1] - 1;
+      for (int i = 1; i <= m2; ++i)
+        dum1[i - 1] = F(i, j) - AX(i, j) * X(i, j + 1) - AY(i, j) * X(i + 1, j) -
+                      BB(i, j) * X(i + 1, j + 1) - CX(i, j) * X(i + 2, j + 1) -
+                      CY(i, j) * X(i + 1, j + 2);
+      j = ie[nb - 1];
+      for (int n = 1; n <= m2; ++n) {
+        dum2[n - 1] = 0.0;
+        for (int m = 1; m <= m2; ++m)
+          dum2[n - 1] = dum2[n - 1] + dum1[m - 1] * RINV1(m, n, nb);
+        DUM0(n, nb) = X(n + 1, j);
+        X(n + 1, j) = X(n + 1, j) - dum2[n - 1];
+      }
+    }
+    js = ie[nb - 1];
+  }
+  for (int nbs = 1; nbs <= nblk; ++nbs) {
+    int nb = nblk - nbs + 1;
+    js = (nb != 1) ? ie[nb - 2] : 1;
+    int jf = ie[nb - 1] - 2;
+    if (nb != nblk) {
+      int j = ie[nb - 1];
+      for (int n = 1; n <= m2; ++n) X(n + 1, j) = DUM0(n, nb);
+    }
+    for (int j = js; j <= ie[nb - 1]; ++j)
+      for (int i = 1; i <= m0; ++i) H(i, j) = 0.0;
+    int j = ie[nb - 1] - 1;
+    for (int i = 1; i <= m2; ++i)
+      dum1[i - 1] = F(i, j) - AX(i, j) * X(i, j + 1) - AY(i, j) * X(i + 1, j) -
+                    BB(i, j) * X(i + 1, j + 1) - CX(i, j) * X(i + 2, j + 1) -
+                    CY(i, j) * X(i + 1, j + 2);
+    for (int n = 1; n <= m2; ++n) {
+      dum2[n - 1] = 0.0;
+      for (int m = 1; m <= m2; ++m)
+        dum2[n - 1] = dum2[n - 1] + dum1[m - 1] * RINV(m, n, nb);
+      H(n + 1, js + 1) = dum2[n - 1];
+      X(n + 1, js + 1) = X(n + 1, js + 1) + dum2[n - 1];
+    }
+    if (nb != 1) {
+      for (int m = 1; m <= m2; ++m) dum1[m - 1] = H(m + 1, js + 1) * CY(m, js - 1);
+      j = ie[nb - 2];
+      for (int n = 1; n <= m2; ++n) {
+        dum2[n - 1] = 0.0;
+        for (int m = 1; m <= m2; ++m)
+          dum2[n - 1] = dum2[n - 1] + dum1[m - 1] * RINV1(m, n, nb - 1);
+        H(n + 1, j) = dum2[n - 1];
+      }
+    }
+    for (j = js; j <= jf; ++j)
+      for (int i = 1; i <= m2; ++i) {
+        H(i + 1, j + 2) = (-AX(i, j) * H(i, j + 1) - AY(i, j) * H(i + 1, j) -
+                           BB(i, j) * H(i + 1, j + 1) - CX(i, j) * H(i + 2, j + 1)) / CY(i, j);
+        X(i + 1, j + 2) = X(i + 1, j + 2) + H(i + 1, j + 2);
+      }
+  }
+}
+
+/* REP: full-width non-periodic SEVP, MAIN/template/GLOBAL/src/sevp.f90:335-438.
+ * Statement for statement it is REPBIR with M0=I0 and M2=I2 (the H zeroing
+ * runs over I=1..I0, F has leading dimension I2); the tests pin this identity
+ * against both compiled reference routines. */
+void orc_rep(const float *ax, const float *ay, const float *bb, const float *cx,
+             const float *cy, const double *rinv, const double *rinv1,
+             double *dum0, double *dum1, double *dum2, const float *f,
+             double *h, double *x, const int *ie, int i0, int i2, int nblk)
+{
+  orc_repbir(ax, ay, bb, cx, cy, rinv, rinv1, dum0, dum1, dum2, f, h, x, ie,
+             i0, i2, i0, i2, nblk);
+}
+
+/* Periodic full-width SEVP.  REP1 (MAIN/template/GLOBAL/src/sevp.f90:11-107)
+ * with m0 = i0 and m2 = i2; REPBIR_PERIOD (sevp.f90:109-217) is the same
+ * statement sequence with the march over M2 columns, F of leading dimension
+ * M2 and H of leading dimension M0, while the periodic copies index column I0
+ * and I0-1 of both X and H.  Both are expressed by this one routine. */
+static void rep_periodic(const float *ax, const float *ay, const float *bb,
+                         const float *cx, const float *cy, const double *rinv,
+                         const double *rinv1, double *dum0, double *dum1,
+                         double *dum2, const float *f, double *h, double *x,
+                         const int *ie, int i0, int i2, int m0, int m2, int nblk)
+{
+  const int i1 = i0 - 1;
+  int js = 1;
+  for (int nb = 1; nb <= nblk; ++nb) {
+    int jf = ie[nb - 1] - 2;
+    for (int j = js; j <= jf; ++j) {
+      X(1, j + 1) = X(i1, j + 1);
+      X(i0, j + 1) = X(2, j + 1);
+      for (int i = 1; i <= m2; ++i)
+        X(i + 1, j + 2) = (F(i, j) - AX(i, j) * X(i, j + 1) - AY(i, j) * X(i + 1, j) -
+                           BB(i, j) * X(i + 1, j + 1) - CX(i, j) * X(i + 2, j + 1)) / CY(i, j);
+    }
+    X(1, jf + 2) = X(i1, jf + 2);
+    X(i0, jf + 2) = X(2, jf + 2);
+    if (nb != nblk) {
+      int j = ie[nb - 1] - 1;
+      for (int i = 1; i <= m2; ++i)
+        dum1[i - 1] = F(i, j) - AX(i, j) * X(i, j + 1) - AY(i, j) * X(i + 1, j) -
+                      BB(i, j) * X(i + 1, j + 1) - CX(i, j) * X(i + 2, j + 1) -
+                      CY(i, j) * X(i + 1, j + 2);
+      j = ie[nb - 1];
+      for (int n = 1; n <= m2; ++n) {
+        dum2[n - 1] = 0.0;
+        for (int m = 1; m <= m2; ++m)
+          dum2[n - 1] = dum2[n - 1] + dum1[m - 1] * RINV1(m, n, nb);
+        DUM0(n, nb) = X(n + 1, j);
+        X(n + 1, j) = X(n + 1, j) - dum2[n - 1];
+      }
+      X(1, j) = X(i1, j);
+      X(i0, j) = X(2, j);
+    }
+    js = ie[nb - 1];
+  }
+  for (int nbs = 1; nbs <= nblk; ++nbs) {
+    int nb = nblk - nbs + 1;
+    js = (nb != 1) ? ie[nb - 2] : 1;
+    int jf = ie[nb - 1] - 2;
+    if (nb != nblk) {
+      int j = ie[nb - 1];
+      for (int n = 1; n <= m2; ++n) X(n + 1, j) = DUM0(n, nb);
+    }
+    for (int j = js; j <= ie[nb - 1]; ++j)
+      for (int i = 1; i <= m0; ++i) H(i, j) = 0.0;
+    int j = ie[nb - 1] - 1;
+    X(1, j + 1) = X(i1, j + 1);
+    X(i0, j + 1) = X(2, j + 1);
+    for (int i = 1; i <= m2; ++i)
+      dum1[i - 1] = F(i, j) - AX(i, j) * X(i, j + 1) - AY(i, j) * X(i + 1, j) -
+                    BB(i, j) * X(i + 1, j + 1) - CX(i, j) * X(i + 2, j + 1) -
+                    CY(i, j) * X(i + 1, j + 2);
+    for (int n = 1; n <= m2; ++n) {
+      dum2[n - 1] = 0.0;
+      for (int m = 1; m <= m2; ++m)
+        dum2[n - 1] = dum2[n - 1] + dum1[m - 1] * RINV(m, n, nb);
+      H(n + 1, js + 1) = dum2[n - 1];
+      X(n + 1, js + 1) = X(n + 1, js + 1) + dum2[n - 1];
+    }
+    H(1, js + 1) = H(i1, js + 1);
+    H(i0, js + 1) = H(2, js + 1);
+    X(1, js + 1) = X(i1, js + 1);
+    X(i0, js + 1) = X(2, js + 1);
+    if (nb != 1) {
+      for (int m = 1; m <= m2; ++m) dum1[m - 1] = H(m + 1, js + 1) * CY(m, js - 1);
+      j = ie[nb - 2];
+      for (int n = 1; n <= m2; ++n) {
+        dum2[n - 1] = 0.0;
+        for (int m = 1; m <= m2; ++m)
+          dum2[n - 1] = dum2[n - 1] + dum1[m - 1] * RINV1(m, n, nb - 1);
+        H(n + 1, j) = dum2[n - 1];
+      }
+      H(1, j) = H(i1, j);
+      H(i0, j) = H(2, j);
+    }
+    for (j = js; j <= jf; ++j) {
+      H(1, j + 1) = H(i1, j + 1);
+      H(i0, j + 1) = H(2, j + 1);
+      for (int i = 1; i <= m2; ++i) {
+        H(i + 1, j + 2) = (-AX(i, j) * H(i, j + 1) - AY(i, j) * H(i + 1, j) -
+                           BB(i, j) * H(i + 1, j + 1) - CX(i, j) * H(i + 2, j + 1)) / CY(i, j);
+        X(i + 1, j + 2) = X(i + 1, j + 2) + H(i + 1, j + 2);
+      }
+      X(1, j + 2) = X(i1, j + 2);
+      X(i0, j + 2) = X(2, j + 2);
+    }
+  }
+  int j1 = ie[nblk - 1];
+  for (int j = 2; j <= j1; ++j) {
+    X(1, j) = X(i1, j);
+    X(i0, j) = X(2, j);
+  }
+}
+
+void orc_rep1(const float *ax, const float *ay, const float *bb,
+              const float *cx, const float *cy, const double *rinv,
+              const double *rinv1, double *dum0, double *dum1, double *dum2,
+              const float *f, double *h, double *x, const int *ie, int i0,
+              int i2, int nblk)
+{
+  rep_periodic(ax, ay, bb, cx, cy, rinv, rinv1, dum0, dum1, dum2, f, h, x, ie,
+               i0, i2, i0, i2, nblk);
+}
+
+void orc_repbir_period(const float *ax, const float *ay, const float *bb,
+                       const float *cx, const float *cy, const double *rinv,
+                       const double *rinv1, double *dum0, double *dum1,
+                       double *dum2, const float *f, double *h, double *x,
+                       const int *ie, int i0, int i2, int m0, int m2, int nblk)
+{
+  rep_periodic(ax, ay, bb, cx, cy, rinv, rinv1, dum0, dum1, dum2, f, h, x, ie,
+               i0, i2, m0, m2, nblk);
+}
+#undef F
+#undef X
+
+/* ------------------------------------------------------------------------ */
+/* MATINV: in-place Gauss-Jordan inverse without pivoting.
+ * Follows PREP/template/GLOBAL/src/prep_GLOBAL.f90:516-568: the pivot row is
+ * scaled by the reciprocal of its pivot, the pivot column below is eliminated
+ * with separately rounded multiply and subtract, then the same is done above
+ * the diagonal in a second sweep.  Returns the number of zero pivots met (the
+ * reference divides by them silently). */
+int orc_matinv(double *b, int n)
+{
+#define B(i, j) b[IDX2(n, i, j)]
+  int zero_pivots = 0;
+  double *b1 = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+  double *b2 = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+  for (int i = 1; i <= n - 1; ++i) {
+    if (B(i, i) == 0.0) ++zero_pivots;
+    b1[0] = 1.0 / B(i, i);
+    B(i, i) = 1.0;
+    for (int j = 1; j <= n; ++j) B(i, j) = B(i, j) * b1[0];
+    for (int ia = i + 1; ia <= n; ++ia) b1[ia - 1] = B(ia, i);
+    for (int ia = i + 1; ia <= n; ++ia) B(ia, i) = 0.0;
+    for (int j = 1; j <= n; ++j) b2[j - 1] = B(i, j);
+    for (int ia = i + 1; ia <= n; ++ia)
+      for (int j = 1; j <= n; ++j) B(ia, j) = B(ia, j) - b1[ia - 1] * b2[j - 1];
+  }
+  if (B(n, n) == 0.0) ++zero_pivots;
+  b1[0] = 1.0 / B(n, n);
+  B(n, n) = 1.0;
+  for (int j = 1; j <= n; ++j) B(n, j) = B(n, j) * b1[0];
+  for (int i = 2; i <= n; ++i) {
+    for (int ib = 1; ib <= i; ++ib) b1[ib - 1] = B(ib, i);
+    for (int ib = 1; ib <= i - 1; ++ib) B(ib, i) = 0.0;
+    for (int j = 1; j <= n; ++j) b2[j - 1] = B(i, j);
+    for (int ib = 1; ib <= i - 1; ++ib)
+      for (int j = 1; j <= n; ++j) B(ib, j) = B(ib, j) - b1[ib - 1] * b2[j - 1];
+  }
+  free(b1);
+  free(b2);
+  return zero_pivots;
+#undef B
+}
+
+/* PRE: influence matrices of one strip.
+ * Follows PREP/template/GLOBAL/src/prep_GLOBAL.f90:457-513.  AX..CY have
+ * leading dimension i2, H leading dimension m0; RINV(m2,m2,nblk) and
+ * RINV1(m2,m2,nblk-1) are written.  The last-row responses T of block nb are
+ * kept in the RINV(:,:,nblk) slot until that block is processed, as the
+ * reference does.  Returns the number of zero pivots met in MATINV. */
+int orc_pre(const float *ax, const float *ay, const float *bb, const float *cx,
+            const float *cy, double *rinv, double *rinv1, double *h,
+            const int *ie, int i2, int m0, int m2, int nblk)
+{
+  int zp = 0;
+  int jl = 1;
+  for (int nb = 1; nb <= nblk; ++nb) {
+    int jh = ie[nb - 1];
+    int jhp = jh + 1, jhm = jh - 2, jg = jl + 1;
+    for (int ng = 1; ng <= m2; ++ng) {
+      int ig = ng + 1;
+      for (int j = jl; j <= jhp; ++j)
+        for (int i = 1; i <= m0; ++i) H(i, j) = 0.0;
+      H(ig, jg) = 1.0;
+      if (nb != 1)
+        for (int n = 1; n <= m2; ++n) H(n + 1, jl) = RINV1(ng, n, nb - 1) * CY(ig - 1, jg - 2);
+      for (int j = jl; j <= jhm; ++j)
+        for (int i = 1; i <= m2; ++i)
+          H(i + 1, j + 2) = -(AX(i, j) * H(i, j + 1) + AY(i, j) * H(i + 1, j) +
+                              BB(i, j) * H(i + 1, j + 1) + CX(i, j) * H(i + 2, j + 1)) / CY(i, j);
+      int j = jh - 1;
+      for (int i = 1; i <= m2; ++i)
+        RINV(ng, i, nb) = AX(i, j) * H(i, j + 1) + AY(i, j) * H(i + 1, j) +
+                          BB(i, j) * H(i + 1, j + 1) + CX(i, j) * H(i + 2, j + 1);
+      if (nb != nblk) {
+        j = ie[nb - 1];
+        for (int n = 1; n <= m2; ++n) RINV(ng, n, nblk) = H(n + 1, j);
+      }
+      H(ig, jg) = 0.0;
+    }
+    zp += orc_matinv(&RINV(1, 1, nb), m2);
+    if (nb == nblk) return zp;
+    for (int i = 1; i <= m2; ++i)
+      for (int j = 1; j <= m2; ++j) {
+        RINV1(i, j, nb) = 0.0;
+        for (int k = 1; k <= m2; ++k)
+          RINV1(i, j, nb) = RINV1(i, j, nb) - RINV(i, k, nb) * RINV(k, j, nblk);
+      }
+    jl = jh;
+  }
+  return zp;
+}
+
+/* PRE1: periodic full-width influence matrices.
+ * Follows PREP/template/GLOBAL/src/prep_GLOBAL.f90:327-386: as PRE with the
+ * periodic copies H(1)=H(I0-1), H(I0)=H(2) of each marched row. */
+int orc_pre1(const float *ax, const float *ay, const float *bb, const float *cx,
+             const float *cy, double *rinv, double *rinv1, double *h,
+             const int *ie, int i0, int i2, int nblk)
+{
+  const int m0 = i0, m2 = i2, i1 = i0 - 1;
+  int zp = 0;
+  int jl = 1;
+  for (int nb = 1; nb <= nblk; ++nb) {
+    int jh = ie[nb - 1];
+    int jhp = jh + 1, jhm = jh - 2, jg = jl + 1;
+    for (int ng = 1; ng <= i2; ++ng) {
+      int ig = ng + 1;
+      for (int j = jl; j <= jhp; ++j)
+        for (int i = 1; i <= i0; ++i) H(i, j) = 0.0;
+      H(ig, jg) = 1.0;
+      H(1, jg) = H(i1, jg);
+      H(i0, jg) = H(2, jg);
+      if (nb != 1)
+        for (int n = 1; n <= i2; ++n) H(n + 1, jl) = RINV1(ng, n, nb - 1) * CY(ig - 1, jg - 2);
+      for (int j = jl; j <= jhm; ++j) {
+        H(1, j + 1) = H(i1, j + 1);
+        H(i0, j + 1) = H(2, j + 1);
+        for (int i = 1; i <= i2; ++i)
+          H(i + 1, j + 2) = -(AX(i, j) * H(i, j + 1) + AY(i, j) * H(i + 1, j) +
+                              BB(i, j) * H(i + 1, j + 1) + CX(i, j) * H(i + 2, j + 1)) / CY(i, j);
+      }
+      int j = jh - 1;
+      H(1, j + 1) = H(i1, j + 1);
+      H(i0, j + 1) = H(2, j + 1);
+      for (int i = 1; i <= i2; ++i)
+        RINV(ng, i, nb) = AX(i, j) * H(i, j + 1) + AY(i, j) * H(i + 1, j) +
+                          BB(i, j) * H(i + 1, j + 1) + CX(i, j) * H(i + 2, j + 1);
+      if (nb != nblk) {
+        j = ie[nb - 1];
+        for (int n = 1; n <= i2; ++n) RINV(ng, n, nblk) = H(n + 1, j);
+        H(1, jg) = 0.0;
+        H(i0, jg) = 0.0;
+      }
+      H(ig, jg) = 0.0;
+    }
+    zp += orc_matinv(&RINV(1, 1, nb), i2);
+    if (nb == nblk) return zp;
+    for (int i = 1; i <= i2; ++i)
+      for (int j = 1; j <= i2; ++j) {
+        RINV1(i, j, nb) = 0.0;
+        for (int k = 1; k <= i2; ++k)
+          RINV1(i, j, nb) = RINV1(i, j, nb) - RINV(i, k, nb) * RINV(k, j, nblk);
+      }
+    jl = jh;
+  }
+  return zp;
+}
+#undef AX
+#undef AY
+#undef BB
+#undef CX
+#undef CY
+#undef H
+#undef RINV
+#undef RINV1
+#undef DUM0
+
+/* ------------------------------------------------------------------------ */
+/* Seam strip gather: the last BIR strip starts at column il and wraps the
+ * periodic seam, taking inc columns il..il+inc-1 and then columns 1..inc.
+ * Follows PREP/template/GLOBAL/src/prep_GLOBAL.f90:293-313 and
+ * MAIN/template/GLOBAL/src/fs_GLOBAL.f90:1925-1942 (both select the same
+ * columns).  src has leading dimension i2, dst leading dimension ibir. */
+static void seam_gather_f(const float *src, float *dst, int i2, int j2, int il,
+                          int inc, int ibir)
+{
+  for (int j = 1; j <= j2; ++j) {
+    for (int i = 1; i <= inc; ++i) dst[IDX2(ibir, i, j)] = src[IDX2(i2, il + i - 1, j)];
+    for (int i = inc + 1; i <= ibir; ++i) dst[IDX2(ibir, i, j)] = src[IDX2(i2, i - inc, j)];
+  }
+}
+
+/* INITFS strip loop of PREP: influence matrices of every BIR strip.
+ * Follows PREP/template/GLOBAL/src/prep_GLOBAL.f90:286-317.  RINV has the
+ * reference shape (ibir*ibir, nb0, nbir), RINV1 (ibir*ibir, nb0-1, nbir).
+ * h is scratch of (ibir+2) x (j2+2).  Returns the number of zero pivots. */
+int orc_bir_prep(const float *al, const float *ab, const float *ac,
+                 const float *ar, const float *at, double *rinv, double *rinv1,
+                 double *h, const int *ie, int i2, int j2, int nbir, int nb0)
+{
+  const int ibir = 2 * i2 / nbir, inc = ibir / 2;
+  const long ibr2 = (long)ibir * ibir;
+  int zp = 0;
+  float *c[5] = {0};
+  for (int n = 1; n <= nbir; ++n) {
+    int il = (n - 1) * inc + 1;
+    double *rv = rinv + (long)(n - 1) * ibr2 * nb0;
+    double *rv1 = rinv1 + (long)(n - 1) * ibr2 * (nb0 - 1);
+    if (nbir == 1 || n != nbir) {
+      long o = IDX2(i2, il, 1);
+      zp += orc_pre(al + o, ab + o, ac + o, ar + o, at + o, rv, rv1, h, ie, i2,
+                    ibir + 2, ibir, nb0);
+    } else {
+      const float *srcs[5] = {al, ab, ac, ar, at};
+      for (int k = 0; k < 5; ++k) {
+        c[k] = (float *)malloc(sizeof(float) * (size_t)ibir * j2);
+        seam_gather_f(srcs[k], c[k], i2, j2, il, inc, ibir);
+      }
+      zp += orc_pre(c[0], c[1], c[2], c[3], c[4], rv, rv1, h, ie, ibir,
+                    ibir + 2, ibir, nb0);
+      for (int k = 0; k < 5; ++k) free(c[k]);
+    }
+  }
+  return zp;
+}
+
+/* FS_SOLVER, FL_EVP_STP=1 branch: multiplicative sweeps over overlapping BIR
+ * strips.  Follows MAIN/template/GLOBAL/src/fs_GLOBAL.f90:1902-1966.
+ *   x   : (i2+2) x (j2+2) solution with halos, in/out (initial guess in).
+ *   xx  : (ibir+2) x (j2+2) seam scratch, in/out.  The reference keeps it as a
+ *         module array that starts at zero and is never cleared between
+ *         calls, and its halo entries in row j2+1 and its halo rows are never
+ *         refreshed by the gather; pass the same buffer to successive calls.
+ *   periodic: apply the FLAG_PERIODIC_WE halo refresh after the seam strip.
+ * The call order of strips and sweeps is a data dependency and is kept. */
+void orc_bir_solve(const float *al, const float *ab, const float *ac,
+                   const float *ar, const float *at, const float *s,
+                   const double *rinv, const double *rinv1, double *x,
+                   double *xx, const int *ie, int i2, int j2, int nbir,
+                   int nb0, int nsweep, int periodic)
+{
+  const int ibir = 2 * i2 / nbir, inc = ibir / 2;
+  const int i0 = i2 + 2, i1 = i2 + 1, j1 = j2 + 1, j0 = j2 + 2;
+  const long ibr2 = (long)ibir * ibir;
+  const int m0 = ibir + 2;
+  float *src = (float *)malloc(sizeof(float) * (size_t)ibir * j2);
+  float *c[5];
+  for (int k = 0; k < 5; ++k) c[k] = (float *)malloc(sizeof(float) * (size_t)ibir * j2);
+  double *h = (double *)calloc((size_t)m0 * j0, sizeof(double));
+  double *dum0 = (double *)calloc((size_t)ibir * (nb0 > 1 ? nb0 - 1 : 1), sizeof(double));
+  double *dum1 = (double *)calloc((size_t)ibir, sizeof(double));
+  double *dum2 = (double *)calloc((size_t)ibir, sizeof(double));
+  for (int it = 1; it <= nsweep; ++it) {
+    for (int n = 1; n <= nbir; ++n) {
+      int il = (n - 1) * inc + 1;
+      const double *rv = rinv + (long)(n - 1) * ibr2 * nb0;
+      const double *rv1 = rinv1 + (long)(n - 1) * ibr2 * (nb0 - 1);
+      if (n != nbir) {
+        for (int j = 1; j <= j2; ++j)
+          for (int i = 1; i <= ibir; ++i) src[IDX2(ibir, i, j)] = s[IDX2(i2, il + i - 1, j)];
+        long o = IDX2(i2, il, 1);
+        orc_repbir(al + o, ab + o, ac + o, ar + o, at + o, rv, rv1, dum0, dum1,
+                   dum2, src, h, x + IDX2(i0, il, 1), ie, i0, i2, m0, ibir, nb0);
+      } else {
+        for (int j = 1; j <= j2; ++j) {
+          xx[IDX2(m0, 1, j)] = x[IDX2(i0, il, j)];
+          xx[IDX2(m0, ibir + 2, j)] = x[IDX2(i0, inc + 2, j)];
+        }
+        seam_gather_f(s, src, i2, j2, il, inc, ibir);
+        seam_gather_f(al, c[0], i2, j2, il, inc, ibir);
+        seam_gather_f(ab, c[1], i2, j2, il, inc, ibir);
+        seam_gather_f(ac, c[2], i2, j2, il, inc, ibir);
+        seam_gather_f(ar, c[3], i2, j2, il, inc, ibir);
+        seam_gather_f(at, c[4], i2, j2, il, inc, ibir);
+        orc_repbir(c[0], c[1], c[2], c[3], c[4], rv, rv1, dum0, dum1, dum2, src,
+                   h, xx, ie, m0, ibir, m0, ibir, nb0);
+        for (int j = 2; j <= j1; ++j) {
+          for (int i = 1; i <= inc; ++i) x[IDX2(i0, il + i, j)] = xx[IDX2(m0, i + 1, j)];
+          for (int i = inc + 1; i <= ibir; ++i) x[IDX2(i0, i - inc + 1, j)] = xx[IDX2(m0, i + 1, j)];
+        }
+        if (periodic)
+          for (int j = 2; j <= j1; ++j) {
+            x[IDX2(i0, 1, j)] = x[IDX2(i0, i1, j)];
+            x[IDX2(i0, i0, j)] = x[IDX2(i0, 2, j)];
+          }
+      }
+    }
+  }
+  free(src);
+  for (int k = 0; k < 5; ++k) free(c[k]);
+  free(h);
+  free(dum0);
+  free(dum1);
+  free(dum2);
+}
+
+/* 5-point product r = A x on the interior, following p_sqprod
+ * (MAIN/template/GLOBAL/src/sevp.f90:696-713): the terms are summed in the
+ * order AB, AL, AC, AR, AT.  x and r are (i2+2) x (j2+2); the halo of r is not
+ * touched. */
+void orc_sqprod(const float *al, const float *ab, const float *ac,
+                const float *ar, const float *at, const double *x, double *r,
+                int i2, int j2)
+{
+  const int i0 = i2 + 2;
+  for (int j = 1; j <= j2; ++j)
+    for (int i = 1; i <= i2; ++i) {
+      long c = IDX2(i2, i, j);
+      r[IDX2(i0, i + 1, j + 1)] =
+          (double)ab[c] * x[IDX2(i0, i + 1, j)] + (double)al[c] * x[IDX2(i0, i, j + 1)] +
+          (double)ac[c] * x[IDX2(i0, i + 1, j + 1)] + (double)ar[c] * x[IDX2(i0, i + 2, j + 1)] +
+          (double)at[c] * x[IDX2(i0, i + 1, j + 2)];
+    }
+}

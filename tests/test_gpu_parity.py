@@ -1,0 +1,229 @@
+"""GPU parity: the HIP library against the CPU oracle on the same inputs.
+
+Every plain-FP64 path must be bitwise equal to the oracle (which is itself
+pinned bitwise to the compiled reference routines, see test_oracle_ref.py).
+Mixed-precision paths are checked by their contract: relative residual
+<= 1e-10 after refinement, and the distance to the FP64 oracle solution.
+"""
+import numpy as np
+import pytest
+
+from musoc_solver_amd import sevp as S
+from musoc_solver_amd import synth
+from tests.conftest import fields_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def make(i2, j2, n0, seed=11, metric="mercator", lat_max=90.0, h_land=50.0, ideal=False):
+    if ideal:
+        coefs = synth.ideal_operator(i2, j2)
+        mask = np.ones((i2, j2), bool)
+    else:
+        c = synth.make_case(i2, j2, seed, metric=metric, lat_max=lat_max, h_land=h_land)
+        coefs, mask = c.coefs, c.mask
+    ie = synth.reference_ie(j2, synth.nb0_for_height(j2, n0))
+    return coefs, ie, mask
+
+
+def rand_x(i2, j2, nrhs, seed):
+    rng = np.random.default_rng(seed)
+    return np.asfortranarray(rng.standard_normal((i2 + 2, j2 + 2, nrhs)))
+
+
+BIR_CASES = [
+    # (i2, j2, nbir, n0, kwargs)
+    (48, 24, 8, 6, dict(metric="mercator")),
+    (192, 96, 16, 8, dict(metric="mercator")),
+    (192, 96, 16, 24, dict(ideal=True)),  # the 192x96 / 24x24-block layout, reference test operator
+    (120, 60, 8, 5, dict(metric="latlon", lat_max=80.0)),  # IBIR = 30
+]
+
+
+@pytest.mark.parametrize("i2,j2,nbir,n0,kw", BIR_CASES)
+def test_bir_prep_and_solve_bitwise(oracle, device, i2, j2, nbir, n0, kw):
+    import torch
+    coefs, ie, mask = make(i2, j2, n0, **kw)
+    nrhs = 3
+    print(f"[bir] i2={i2} j2={j2} nbir={nbir} n0={n0} {kw}", flush=True)
+    plan = S.SevpPlan(coefs, ie, form=S.FORM_BIR, nbir=nbir, nsweep=5, nrhs_max=nrhs)
+    zp = plan.build_influence()
+    rinv, rinv1 = plan.get_influence()
+    r_o, r1_o, zp_o = oracle.bir_prep(coefs, ie, nbir)
+    assert zp == zp_o
+    assert fields_equal(rinv, r_o)
+    assert fields_equal(rinv1[:, :len(ie) - 1], r1_o[:, :len(ie) - 1])
+    s = synth.make_rhs(mask, 5, nrhs)
+    x0 = rand_x(i2, j2, nrhs, 6)
+    # two successive calls: the seam scratch XX persists between them
+    xg = x0.copy(order="F")
+    plan.solve(s, xg)
+    xo = x0.copy(order="F")
+    xxs = [np.zeros((2 * i2 // nbir + 2, j2 + 2), order="F") for _ in range(nrhs)]
+    for r in range(nrhs):
+        xr = np.asfortranarray(xo[:, :, r])
+        oracle.bir_solve(coefs, s[:, :, r], r_o, r1_o, xr, xxs[r], ie, nbir)
+        xo[:, :, r] = xr
+    assert fields_equal(xg, xo)
+    plan.solve(s, xg)
+    for r in range(nrhs):
+        xr = np.asfortranarray(xo[:, :, r])
+        oracle.bir_solve(coefs, s[:, :, r], r_o, r1_o, xr, xxs[r], ie, nbir)
+        xo[:, :, r] = xr
+    assert fields_equal(xg, xo)
+    plan.close()
+
+
+FULL_CASES = [
+    (96, 48, 3, S.FORM_FULL_PERIODIC),
+    (96, 48, 4, S.FORM_FULL),
+    (40, 24, 6, S.FORM_FULL_PERIODIC),
+    (64, 30, 5, S.FORM_FULL_PERIODIC),  # trapezoid tiles wrap around the periodic width
+]
+
+
+@pytest.mark.parametrize("i2,j2,n0,form", FULL_CASES)
+def test_full_width_bitwise(oracle, device, i2, j2, n0, form):
+    import torch
+    coefs, ie, mask = make(i2, j2, n0, metric="latlon", lat_max=80.0)
+    nrhs = 4
+    print(f"[full] i2={i2} j2={j2} n0={n0} form={form}", flush=True)
+    plan = S.SevpPlan(coefs, ie, form=form, nrhs_max=nrhs)
+    zp = plan.build_influence()
+    rinv, rinv1 = plan.get_influence()
+    if form == S.FORM_FULL_PERIODIC:
+        r_o, r1_o, zp_o = oracle.pre1(coefs, ie)
+    else:
+        r_o, r1_o, zp_o = oracle.pre(coefs, ie)
+    assert zp == zp_o
+    assert fields_equal(rinv, r_o)
+    assert fields_equal(rinv1[:, :, :len(ie) - 1], r1_o[:, :, :len(ie) - 1])
+    s = synth.make_rhs(mask, 7, nrhs)
+    x0 = rand_x(i2, j2, nrhs, 8)
+    xg = x0.copy(order="F")
+    plan.solve(s, xg)
+    for r in range(nrhs):
+        xr = np.asfortranarray(x0[:, :, r])
+        fn = oracle.rep1 if form == S.FORM_FULL_PERIODIC else oracle.rep
+        fn(coefs, r_o, r1_o, s[:, :, r], xr, ie)
+        assert fields_equal(xg[:, :, r], xr), f"rhs {r}"
+    plan.close()
+
+
+def test_entry_points_bitwise(oracle, device):
+    i2, j2, n0 = 48, 24, 4
+    coefs, ie, mask = make(i2, j2, n0, metric="mercator")
+    al, ab, ac, ar, at = coefs
+    nblk = len(ie)
+    s = synth.make_rhs(mask, 9, 1)[:, :, 0].copy(order="F")
+    print("[entry] matinv", flush=True)
+    rng = np.random.default_rng(3)
+    for n in (24, 96):
+        b = np.asfortranarray(rng.standard_normal((n, n)) + n * np.eye(n))
+        bo, _ = oracle.matinv(b)
+        bg = b.copy(order="F")
+        S.matinv(bg)
+        assert fields_equal(bg, bo), f"matinv n={n}"
+    print("[entry] pre1/rep1", flush=True)
+    rinv = np.zeros((i2, i2, nblk), order="F")
+    rinv1 = np.zeros((i2, i2, nblk - 1), order="F")
+    h = np.zeros((i2 + 2, j2 + 2), order="F")
+    S.pre1(al, ab, ac, ar, at, rinv, rinv1, h, ie, i2 + 2, i2, nblk)
+    r_o, r1_o, _ = oracle.pre1(coefs, ie)
+    assert fields_equal(rinv, r_o) and fields_equal(rinv1, r1_o[:, :, :nblk - 1])
+    d0 = np.zeros((i2, nblk), order="F")
+    d1 = np.zeros(i2)
+    x0 = rand_x(i2, j2, 1, 4)[:, :, 0].copy(order="F")
+    for fn_g, fn_o, period in ((S.rep1, oracle.rep1, False), (S.repbir_period, oracle.rep1, True)):
+        xg = x0.copy(order="F")
+        if period:
+            fn_g(al, ab, ac, ar, at, rinv, rinv1, d0, d1, d1, s, h, xg, ie, i2 + 2, i2, i2 + 2, i2, nblk)
+        else:
+            fn_g(al, ab, ac, ar, at, rinv, rinv1, d0, d1, d1, s, h, xg, ie, i2 + 2, i2, nblk)
+        xo = x0.copy(order="F")
+        fn_o(coefs, r_o, r1_o, s, xo, ie)
+        assert fields_equal(xg, xo)
+    print("[entry] pre/rep", flush=True)
+    S.pre(al, ab, ac, ar, at, rinv, rinv1, h, ie, i2, i2 + 2, i2, nblk)
+    r_o, r1_o, _ = oracle.pre(coefs, ie)
+    assert fields_equal(rinv, r_o) and fields_equal(rinv1, r1_o[:, :, :nblk - 1])
+    xg = x0.copy(order="F")
+    S.rep(al, ab, ac, ar, at, rinv, rinv1, d0, d1, d1, s, h, xg, ie, i2 + 2, i2, nblk)
+    xo = x0.copy(order="F")
+    oracle.rep(coefs, r_o, r1_o, s, xo, ie)
+    assert fields_equal(xg, xo)
+    print("[entry] strip pre/repbir", flush=True)
+    # one strip of width 12 starting at column il, as FS_SOLVER calls it
+    nbir, ibir = 8, 12
+    il = 13
+    off = il - 1
+    rv = np.zeros((ibir, ibir, nblk), order="F")
+    rv1 = np.zeros((ibir, ibir, nblk - 1), order="F")
+    S.pre(al, ab, ac, ar, at, rv, rv1, h, ie, i2, ibir + 2, ibir, nblk, coef_off=off)
+    rvo = np.zeros((ibir, ibir, nblk), order="F")
+    rv1o = np.zeros((ibir, ibir, nblk), order="F")
+    ho = np.zeros((ibir + 2, j2 + 2), order="F")
+    from oracle.oracle import P, _ci
+    zpo = oracle.lib().orc_pre(*[P(c, off) for c in coefs], P(rvo), P(rv1o), P(ho),
+                                P(np.ascontiguousarray(ie, np.int32)), _ci(i2), _ci(ibir + 2),
+                                _ci(ibir), _ci(nblk))
+    assert fields_equal(rv, rvo) and fields_equal(rv1, rv1o[:, :, :nblk - 1])
+    src = np.asfortranarray(s[il - 1:il - 1 + ibir, :])
+    xg = x0.copy(order="F")
+    dd0 = np.zeros((ibir, nblk), order="F")
+    dd1 = np.zeros(ibir)
+    S.repbir(al, ab, ac, ar, at, rv, rv1, dd0, dd1, dd1, src, h, xg, ie, i2 + 2, i2, ibir + 2, ibir,
+             nblk, coef_off=off, x_off=off)
+    xo = x0.copy(order="F")
+    oracle.repbir_call([(c, off) for c in coefs], rvo, rv1o, dd0.copy(order="F"), dd1.copy(),
+                       dd1.copy(), src, ho, (xo, off), np.ascontiguousarray(ie, np.int32), i2 + 2,
+                       i2, ibir + 2, ibir, nblk)
+    assert fields_equal(xg, xo)
+
+
+def _device_case(i2, j2, n0, nrhs, precision, seed=21):
+    import torch
+    coefs, ie, mask = make(i2, j2, n0, seed=seed, metric="latlon", lat_max=80.0)
+    plan = S.SevpPlan(coefs, ie, form=S.FORM_FULL_PERIODIC, nrhs_max=nrhs, precision=precision)
+    plan.build_influence()
+    s = synth.make_rhs(mask, seed + 1, nrhs)
+    s_d = torch.from_numpy(np.ascontiguousarray(s.transpose(2, 1, 0))).cuda()
+    x_d = torch.zeros((nrhs, j2 + 2, i2 + 2), dtype=torch.float64, device="cuda")
+    return plan, coefs, ie, s, s_d, x_d
+
+
+@pytest.mark.parametrize("precision", [S.PREC_FP64, S.PREC_FP32])
+def test_refined_solve_reaches_tolerance(oracle, device, precision):
+    import torch
+    i2, j2, n0, nrhs = 96, 48, 3, 8
+    print(f"[refine] precision={precision}", flush=True)
+    plan, coefs, ie, s, s_d, x_d = _device_case(i2, j2, n0, nrhs, precision)
+    rel, st, ok = plan.solve_refined_dev(s_d, x_d, tol=1e-10, max_iter=8)
+    torch.cuda.synchronize()
+    assert ok, (rel, st.iterations)
+    assert rel.max() <= 1e-10
+    # independent check of the residual with the oracle's p_sqprod
+    x = x_d.cpu().numpy().transpose(2, 1, 0)
+    r_o, r1_o, _ = oracle.pre1(coefs, ie)
+    for r in range(nrhs):
+        xr = np.asfortranarray(x[:, :, r])
+        assert oracle.rel_residual(coefs, s[:, :, r], xr) <= 1.5e-10
+        # error against the FP64 oracle solution (one REP1 pass, h=3)
+        xo = np.zeros((i2 + 2, j2 + 2), order="F")
+        oracle.rep1(coefs, r_o, r1_o, s[:, :, r], xo, ie)
+        err = np.linalg.norm(xr[1:-1, 1:-1] - xo[1:-1, 1:-1]) / np.linalg.norm(xo[1:-1, 1:-1])
+        assert err < 1e-8
+    plan.close()
+
+
+def test_residual_dev_matches_oracle(oracle, device):
+    import torch
+    i2, j2 = 64, 30
+    plan, coefs, ie, s, s_d, x_d = _device_case(i2, j2, 3, 2, S.PREC_FP64)
+    x = rand_x(i2, j2, 2, 1)
+    x_d.copy_(torch.from_numpy(np.ascontiguousarray(x.transpose(2, 1, 0))))
+    rel = plan.residual_dev(s_d, x_d)
+    for r in range(2):
+        ro = oracle.rel_residual(coefs, s[:, :, r], np.asfortranarray(x[:, :, r]))
+        assert abs(rel[r] - ro) <= 1e-12 * ro
+    plan.close()
