@@ -82,7 +82,7 @@ struct sevp_plan_s {
   int splits = 1;
   DBuf<double> xx, bdum0;                // BIR
   DBuf<double> r64, dx, part;            // refinement
-  DBuf<double> b1s;                      // MATINV scratch
+  DBuf<double> b1s;                      // MATINV scratch (blocked path)
   DBuf<int> zp;
   const float *al() const { return coef.p; }
   const float *ab() const { return coef.p + (size_t)d.i2 * d.j2; }
@@ -115,7 +115,7 @@ struct PreCtx {
   const int *ie;         // host
   int nb0;
   int *zp;               // device counter
-  double *b1s;           // MATINV scratch (nmat * w) for the global-memory path
+  double *b1s;           // MATINV scratch for the blocked path (null when LDS suffices)
 };
 
 static int run_pre(const PreCtx &c, hipStream_t st) {
@@ -417,7 +417,8 @@ int sevp_plan_create(const sevp_desc *desc, const float *al, const float *ab, co
   if ((rc = p->rinv.alloc(p->rinv_count()))) return bail(rc);
   if ((rc = p->rinv1.alloc(p->rinv1_count()))) return bail(rc);
   if ((rc = p->zp.alloc(1))) return bail(rc);
-  if ((rc = p->b1s.alloc((size_t)p->w * p->nmat))) return bail(rc);
+  if (!matinv_uses_lds(p->w))
+    if ((rc = p->b1s.alloc(matinv_blocked_scratch(p->w, p->nmat)))) return bail(rc);
   const int nr = d.nrhs_max;
   const int nb1 = d.nb0 > 1 ? d.nb0 - 1 : 1;
   const size_t nx = (size_t)(d.i2 + 2) * (d.j2 + 2);
@@ -777,7 +778,7 @@ static int pre_once(const float *ax, const float *ay, const float *bb, const flo
   DBuf<int> zp;
   CHK(rv.alloc((size_t)w * w * nblk));
   CHK(rv1.alloc((size_t)w * w * nb1));
-  CHK(b1.alloc(w));
+  if (!matinv_uses_lds(w)) CHK(b1.alloc(matinv_blocked_scratch(w, 1)));
   CHK(zp.alloc(1));
   HIPC(hipMemset(zp.p, 0, sizeof(int)));
   HIPC(hipMemset(rv1.p, 0, rv1.bytes()));
@@ -813,7 +814,7 @@ extern "C" int sevp_matinv(double *b, int n) {
   DBuf<double> bd, b1;
   DBuf<int> zp;
   CHK(bd.alloc((size_t)n * n));
-  CHK(b1.alloc(n));
+  if (!matinv_uses_lds(n)) CHK(b1.alloc(matinv_blocked_scratch(n, 1)));
   CHK(zp.alloc(1));
   HIPC(hipMemset(zp.p, 0, sizeof(int)));
   HIPC(hipMemcpy(bd.p, b, bd.bytes(), hipMemcpyHostToDevice));

@@ -19,9 +19,9 @@
 //
 // MATINV is the reference's Gauss-Jordan without pivoting
 // (PREP/template/GLOBAL/src/prep_GLOBAL.f90:516-568) with the matrix held in
-// LDS; each element receives the same sequence of operations as in the
-// reference, only the independent element updates of one step run in
-// parallel.
+// LDS when it fits (larger orders use the blocked form in sevp_matinv.hip);
+// each element receives the same sequence of operations as in the reference,
+// only the independent element updates of one step run in parallel.
 #include "sevp_dev.h"
 #include "sevp_internal.h"
 
@@ -432,73 +432,21 @@ __global__ __launch_bounds__(256) void matinv_lds_kernel(double *mats, long stri
   if (t == 0 && zp && zcount) atomicAdd(zp, zcount);
 }
 
-// Same elimination on a matrix in global memory (any n; one workgroup per
-// matrix, used when the matrix does not fit in LDS).
-__global__ __launch_bounds__(256) void matinv_glb_kernel(double *mats, long stride, int n,
-                                                         double *b1s, int *zp) {
-  __shared__ double piv;
-  __shared__ int zcount;
-  double *B = mats + (long)blockIdx.x * stride;
-  double *b1 = b1s + (long)blockIdx.x * n;
-  const int t = threadIdx.x, nt = blockDim.x;
-  if (t == 0) zcount = 0;
-  __syncthreads();
-#define BM(i, j) B[(long)((i) - 1) + (long)((j) - 1) * n]
-  for (int i = 1; i <= n - 1; ++i) {
-    if (t == 0) {
-      if (BM(i, i) == 0.0) ++zcount;
-      piv = 1.0 / BM(i, i);
-      BM(i, i) = 1.0;
-    }
-    __syncthreads();
-    const double bp = piv;
-    for (int j = 1 + t; j <= n; j += nt) BM(i, j) = BM(i, j) * bp;
-    for (int ia = i + 1 + t; ia <= n; ia += nt) b1[ia - 1] = BM(ia, i);
-    __syncthreads();
-    const int rows = n - i;
-    for (long idx = t; idx < (long)rows * n; idx += nt) {
-      const int ia = i + 1 + (int)(idx % rows), j = 1 + (int)(idx / rows);
-      const double cur = (j == i) ? 0.0 : BM(ia, j);
-      BM(ia, j) = cur - b1[ia - 1] * BM(i, j);
-    }
-    __syncthreads();
-  }
-  if (t == 0) {
-    if (BM(n, n) == 0.0) ++zcount;
-    piv = 1.0 / BM(n, n);
-    BM(n, n) = 1.0;
-  }
-  __syncthreads();
-  {
-    const double bp = piv;
-    for (int j = 1 + t; j <= n; j += nt) BM(n, j) = BM(n, j) * bp;
-  }
-  __syncthreads();
-  for (int i = 2; i <= n; ++i) {
-    for (int ib = 1 + t; ib <= i - 1; ib += nt) b1[ib - 1] = BM(ib, i);
-    __syncthreads();
-    const int rows = i - 1;
-    for (long idx = t; idx < (long)rows * n; idx += nt) {
-      const int ib = 1 + (int)(idx % rows), j = 1 + (int)(idx / rows);
-      const double cur = (j == i) ? 0.0 : BM(ib, j);
-      BM(ib, j) = cur - b1[ib - 1] * BM(i, j);
-    }
-    __syncthreads();
-  }
-#undef BM
-  if (t == 0 && zp && zcount) atomicAdd(zp, zcount);
+bool matinv_uses_lds(int n) {
+  return n <= MATINV_LDS_MAX && sizeof(double) * ((size_t)n * n + n) <= 64 * 1024;
 }
 
-hipError_t launch_matinv(double *mats, long stride, int n, int count, int *zp, double *b1s,
+// LDS-resident elimination when the matrix fits, else the blocked form
+// (sevp_matinv.hip) with `scratch` of matinv_blocked_scratch(n, count) doubles.
+hipError_t launch_matinv(double *mats, long stride, int n, int count, int *zp, double *scratch,
                          hipStream_t st) {
-  const size_t lds = sizeof(double) * ((size_t)n * n + n);
-  if (n <= MATINV_LDS_MAX && lds <= 64 * 1024) {
+  if (matinv_uses_lds(n)) {
+    const size_t lds = sizeof(double) * ((size_t)n * n + n);
     hipLaunchKernelGGL(matinv_lds_kernel, dim3(count), dim3(256), lds, st, mats, stride, n, zp);
-  } else {
-    if (!b1s) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(matinv_glb_kernel, dim3(count), dim3(256), 0, st, mats, stride, n, b1s, zp);
+    return hipGetLastError();
   }
-  return hipGetLastError();
+  if (!scratch) return hipErrorInvalidValue;
+  return launch_matinv_blocked(mats, stride, n, count, zp, scratch, st);
 }
 
 // Periodic halo columns at the end of REP1 (MAIN/template/GLOBAL/src/sevp.f90:103-106).

@@ -134,7 +134,11 @@ hipError_t launch_gemm_mixed(const GemmF32Args &a, const double *a64, float *par
                              int *counters, int splits, hipStream_t st);
 hipError_t launch_cvt_f32(const double *src, float *dst, long n, hipStream_t st);
 hipError_t launch_matinv(double *mats, long mat_stride, int n, int count, int *zero_pivots,
-                         double *b1_scratch, hipStream_t st);
+                         double *scratch, hipStream_t st);
+bool matinv_uses_lds(int n);
+size_t matinv_blocked_scratch(int n, int count);
+hipError_t launch_matinv_blocked(double *mats, long ms, int n, int count, int *zp,
+                                 double *scratch, hipStream_t st);
 hipError_t launch_halo_fix(double *x, long x_stride, int i2, int j2, int nrhs, hipStream_t st);
 
 }  // namespace sevp

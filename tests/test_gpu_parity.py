@@ -79,6 +79,7 @@ FULL_CASES = [
     (96, 48, 4, S.FORM_FULL),
     (40, 24, 6, S.FORM_FULL_PERIODIC),
     (64, 30, 5, S.FORM_FULL_PERIODIC),  # trapezoid tiles wrap around the periodic width
+    (192, 40, 3, S.FORM_FULL_PERIODIC),  # order-192 influence matrices: blocked MATINV
 ]
 
 
@@ -179,6 +180,19 @@ def test_entry_points_bitwise(oracle, device):
                        dd1.copy(), src, ho, (xo, off), np.ascontiguousarray(ie, np.int32), i2 + 2,
                        i2, ibir + 2, ibir, nblk)
     assert fields_equal(xg, xo)
+
+
+@pytest.mark.parametrize("n", [89, 96, 150, 300])
+def test_matinv_blocked_bitwise(oracle, device, n):
+    rng = np.random.default_rng(n)
+    print(f"[matinv] n={n}", flush=True)
+    b = np.asfortranarray(rng.standard_normal((n, n)) + 0.5 * n * np.eye(n))
+    bo, zo = oracle.matinv(b)
+    bg = b.copy(order="F")
+    S.matinv(bg)
+    assert fields_equal(bg, bo)
+    # the inverse it computes is an inverse
+    assert np.abs(bg @ b - np.eye(n)).max() < 1e-10
 
 
 def _device_case(i2, j2, n0, nrhs, precision, seed=21):
