@@ -60,6 +60,10 @@ typedef struct {
   int periodic;    /* BIR: refresh the periodic halo after the seam strip (FLAG_PERIODIC_WE) */
   int nrhs_max;    /* largest batch of right-hand sides solved in one call */
   int precision;   /* SEVP_PREC_* used by the correction products */
+  int seam_fix;    /* BIR: 0 = the reference's seam halo gather (rows 1..J2,
+                      MAIN/template/GLOBAL/src/fs_GLOBAL.f90:1921-1923), which
+                      leaves the seam strip a zero halo on row J2+1 so the
+                      sweeps stall; 1 = gather rows 1..J2+1, sweeps converge */
 } sevp_desc;
 
 typedef struct {
@@ -158,6 +162,29 @@ int sevp_pre1(const float *ax, const float *ay, const float *bb, const float *cx
               int i2, int nblk);
 /* MATINV, PREP/template/GLOBAL/src/prep_GLOBAL.f90:516-568. */
 int sevp_matinv(double *b, int n);
+
+/* Enqueue this plan's work on `hip_stream` (NULL: the plan's own stream).
+ * Callers that produce a plan's inputs on another stream set it here so the
+ * solve is ordered after them. */
+int sevp_set_stream(sevp_plan plan, void *hip_stream);
+
+/* Switch the arithmetic of the correction products of an existing
+ * full-width plan (SEVP_PREC_*); FP32 copies of the influence matrices are
+ * made on first use.  Lets one influence-matrix build serve both the plain
+ * and the mixed-precision solve. */
+int sevp_set_precision(sevp_plan plan, int precision);
+
+/* -------------------------------------------------------- measurement
+ * Kernel-class timing with HIP events recorded on the plan's stream around
+ * every launch of that class while enabled (used by the benchmark to measure
+ * the dominant kernel's average duration over its timed region). */
+#define SEVP_KCLASS_CORR 0   /* influence-matrix correction products */
+#define SEVP_KCLASS_MARCH 1  /* row marches (particular and homogeneous) */
+#define SEVP_KCLASS_RESID 2  /* closure residuals and FP64 refinement residuals */
+int sevp_profile_begin(sevp_plan plan, int kclass);
+/* Stops recording; total_ms and launches receive the summed device time and
+ * the number of launches timed since sevp_profile_begin. */
+int sevp_profile_end(sevp_plan plan, double *total_ms, long *launches);
 
 /* ------------------------------------------------------------- misc */
 const char *sevp_last_error(void);

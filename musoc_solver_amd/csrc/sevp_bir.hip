@@ -211,8 +211,10 @@ __global__ __launch_bounds__(64) void bir_sweeps_kernel(BirArgs p) {
         repbir_wave<WMAX>(c);
         __threadfence_block();
       } else {
-        // seam strip: halo columns of XX for rows 1..J2 (row J1 is not refreshed)
-        for (int j = 1 + lane; j <= p.j2; j += SEVP_WAVE) {
+        // seam strip: halo columns of XX for rows 1..J2 as the reference does
+        // (row J1 keeps its old value), or 1..J1 with seam_fix
+        const int jg = p.seam_fix ? j1 : p.j2;
+        for (int j = 1 + lane; j <= jg; j += SEVP_WAVE) {
           xx[f2(m0, 1, j)] = x[f2(i0, il, j)];
           xx[f2(m0, w + 2, j)] = x[f2(i0, inc + 2, j)];
         }

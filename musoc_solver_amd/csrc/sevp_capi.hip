@@ -63,13 +63,57 @@ struct DBuf {
   ~DBuf() { release(); }
 };
 
+// Event pairs around the launches of one kernel class (measurement only).
+struct KProf {
+  int kclass = -1;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+  size_t used = 0;
+  double total_ms = 0.0;
+  long launches = 0;
+  hipStream_t st = nullptr;
+  bool on(int k) const { return kclass == k; }
+  void start(int k) {
+    if (!on(k)) return;
+    if (used == pool.size()) {
+      hipEvent_t a, b;
+      if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+      pool.push_back({a, b});
+    }
+    (void)hipEventRecord(pool[used].first, st);
+  }
+  void stop(int k) {
+    if (!on(k) || used >= pool.size()) return;
+    (void)hipEventRecord(pool[used].second, st);
+    ++used;
+  }
+  // Accumulate the recorded pairs; the stream must be idle.
+  void harvest() {
+    for (size_t i = 0; i < used; ++i) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, pool[i].first, pool[i].second) == hipSuccess) {
+        total_ms += ms;
+        ++launches;
+      }
+    }
+    used = 0;
+  }
+  ~KProf() {
+    for (auto &e : pool) {
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
+  }
+};
+
 struct sevp_plan_s {
+  KProf prof;
   sevp_desc d{};
   std::vector<int> ie;
   int ibir = 0;  // strip width (BIR) or i2 (full)
   int w = 0;     // influence-matrix order
   int nmat = 0;  // number of strips (BIR) or 1
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // stream the plan's work is enqueued on
+  hipStream_t own_stream = nullptr;  // the stream the plan created
   bool have_inv = false;
   DBuf<float> coef;     // 5 * i2*j2 (AL, AB, AC, AR, AT)
   DBuf<int> ie_d;
@@ -171,6 +215,7 @@ struct FullCtx {
   float *partial;
   int *counters;
   int splits;
+  KProf *prof;
 };
 
 static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, long f_stride,
@@ -193,7 +238,9 @@ static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, 
       gm.out = out; gm.out_ld = c.i2;
       gm.g = g; gm.cy = c.cy; gm.cyrow = cyrow;
       gm.i2 = c.i2;
+      if (c.prof) c.prof->start(SEVP_KCLASS_CORR);
       HIPC(launch_gemm(gm, st));
+      if (c.prof) c.prof->stop(SEVP_KCLASS_CORR);
     } else {
       GemmF32Args gm{};
       gm.b = b32;
@@ -204,7 +251,9 @@ static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, 
       gm.out = out; gm.out_ld = c.i2;
       gm.g = g; gm.cy = c.cy; gm.cyrow = cyrow;
       gm.i2 = c.i2;
+      if (c.prof) c.prof->start(SEVP_KCLASS_CORR);
       HIPC(launch_gemm_mixed(gm, a, c.partial, c.counters, c.splits, st));
+      if (c.prof) c.prof->stop(SEVP_KCLASS_CORR);
     }
     return SEVP_OK;
   };
@@ -224,7 +273,9 @@ static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, 
     m.w = c.i2;
     m.map = ColMap{1, 0, 0};
     m.nrhs = nrhs;
+    if (c.prof) c.prof->start(SEVP_KCLASS_MARCH);
     HIPC(launch_march(MODE_PART, m, nrhs, st));
+    if (c.prof) c.prof->stop(SEVP_KCLASS_MARCH);
     if (nb != c.nb0)
       CHK(corr(c.e, c.rinv1 + (long)(nb - 1) * ww, mixed ? c.rinv1_32 + (long)(nb - 1) * ww : nullptr,
                EP_FWD, ieb, c.dum0 + (long)(nb - 1) * c.i2, nullptr, nullptr, 0));
@@ -242,7 +293,9 @@ static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, 
     r.dum0_stride = dstride;
     r.e = c.e;
     r.i2 = c.i2; r.j2 = c.j2; r.periodic = c.periodic; r.nb = nb; r.ieb = ieb; r.nrhs = nrhs;
+    if (c.prof) c.prof->start(SEVP_KCLASS_RESID);
     HIPC(launch_resid(r, st));
+    if (c.prof) c.prof->stop(SEVP_KCLASS_RESID);
     CHK(corr(c.e, c.rinv + (long)(nb - 1) * ww, mixed ? c.rinv32 + (long)(nb - 1) * ww : nullptr,
              EP_BWD, js + 1, nullptr, c.hfree, nb != 1 ? c.g : nullptr, js - 1));
     if (nb != 1)
@@ -260,7 +313,9 @@ static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, 
       m.w = c.i2;
       m.map = ColMap{1, 0, 0};
       m.nrhs = nrhs;
+      if (c.prof) c.prof->start(SEVP_KCLASS_MARCH);
       HIPC(launch_march(MODE_HOM, m, nrhs, st));
+      if (c.prof) c.prof->stop(SEVP_KCLASS_MARCH);
     }
   }
   if (c.periodic) HIPC(launch_halo_fix(x, x_stride, c.i2, c.j2, nrhs, st));
@@ -312,6 +367,11 @@ __global__ __launch_bounds__(256) void resid_norm_kernel(const float *al, const 
   }
 }
 
+__global__ void zero_kernel(unsigned int *p, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0u;
+}
+
 __global__ void axpy_interior_kernel(double *x, long x_stride, const double *d, long d_stride,
                                      int i2, int j2, int nrhs) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -327,18 +387,31 @@ __global__ void axpy_interior_kernel(double *x, long x_stride, const double *d, 
 
 static constexpr int NCHUNK = 64;
 
+// Clear `bytes` (a multiple of 4) of device memory on a stream with a kernel.
+static int zero_async(void *ptr, size_t bytes, hipStream_t st) {
+  const long n = (long)(bytes / 4);
+  if (n == 0) return SEVP_OK;
+  hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                     (unsigned int *)ptr, n);
+  HIPC(hipGetLastError());
+  return SEVP_OK;
+}
+
 static int residual_norms(sevp_plan p, const float *s, const double *s64, long s_stride,
                           const double *x, long x_stride, double *rout, int nrhs,
                           std::vector<double> &rel) {
   const int i2 = p->d.i2, j2 = p->d.j2;
+  p->prof.start(SEVP_KCLASS_RESID);
   hipLaunchKernelGGL(resid_norm_kernel, dim3(NCHUNK, nrhs), dim3(256), 0, p->stream, p->al(),
                      p->ab(), p->ac(), p->ar(), p->at(), s, s64, s_stride, x, x_stride, rout, i2,
                      j2, NCHUNK, p->part.p);
   HIPC(hipGetLastError());
+  p->prof.stop(SEVP_KCLASS_RESID);
   std::vector<double> h((size_t)nrhs * NCHUNK * 2);
   HIPC(hipMemcpyAsync(h.data(), p->part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost,
                       p->stream));
   HIPC(hipStreamSynchronize(p->stream));
+  p->prof.harvest();
   rel.assign(nrhs, 0.0);
   for (int r = 0; r < nrhs; ++r) {
     double a = 0, b = 0;
@@ -350,6 +423,9 @@ static int residual_norms(sevp_plan p, const float *s, const double *s64, long s
   }
   return SEVP_OK;
 }
+
+static int alloc_mixed(sevp_plan p);
+static int make_low_precision_copies(sevp_plan p);
 
 // ------------------------------------------------------------------ C ABI
 extern "C" {
@@ -403,8 +479,9 @@ int sevp_plan_create(const sevp_desc *desc, const float *al, const float *ab, co
     sevp_plan_destroy(p);
     return code;
   };
-  if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
+  if (hipStreamCreateWithFlags(&p->own_stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(SEVP_E_HIP, "hipStreamCreate failed"));
+  p->stream = p->own_stream;
   const size_t nc = (size_t)d.i2 * d.j2;
   if ((rc = p->coef.alloc(5 * nc))) return bail(rc);
   const float *src[5] = {al, ab, ac, ar, at};
@@ -436,17 +513,9 @@ int sevp_plan_create(const sevp_desc *desc, const float *al, const float *ab, co
     if ((rc = p->g.alloc((size_t)nr * d.i2))) return bail(rc);
     if ((rc = p->r64.alloc((size_t)nr * nc))) return bail(rc);
     if ((rc = p->dx.alloc((size_t)nr * nx))) return bail(rc);
-    if (d.precision != SEVP_PREC_FP64) {
-      if (nr > 64) return bail(fail(SEVP_E_ARG, "mixed precision solves at most 64 RHS per call"));
-      const int ntiles = (d.i2 + 15) / 16;
-      p->splits = std::max(1, std::min(16, 1024 / ntiles));
-      if ((rc = p->partial.alloc((size_t)p->splits * nr * d.i2))) return bail(rc);
-      if ((rc = p->counters.alloc((size_t)ntiles))) return bail(rc);
-      if (hipMemset(p->counters.p, 0, p->counters.bytes()) != hipSuccess)
-        return bail(fail(SEVP_E_HIP, "memset"));
-      if ((rc = p->rinv32.alloc(p->rinv_count()))) return bail(rc);
-      if ((rc = p->rinv1_32.alloc(p->rinv1_count()))) return bail(rc);
-    }
+    if (d.precision == SEVP_PREC_BF16) return bail(fail(SEVP_E_ARG, "BF16 corrections are not built"));
+    if (d.precision != SEVP_PREC_FP64)
+      if ((rc = alloc_mixed(p))) return bail(rc);
   }
   *out = p;
   return SEVP_OK;
@@ -454,11 +523,17 @@ int sevp_plan_create(const sevp_desc *desc, const float *al, const float *ab, co
 
 int sevp_plan_destroy(sevp_plan p) {
   if (!p) return SEVP_OK;
-  if (p->stream) {
-    (void)hipStreamSynchronize(p->stream);
-    (void)hipStreamDestroy(p->stream);
-  }
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  if (p->own_stream) (void)hipStreamDestroy(p->own_stream);
   delete p;
+  return SEVP_OK;
+}
+
+int sevp_set_stream(sevp_plan p, void *stream) {
+  if (!p) return fail(SEVP_E_ARG, "null plan");
+  HIPC(hipStreamSynchronize(p->stream));
+  p->stream = stream ? (hipStream_t)stream : p->own_stream;
+  p->prof.st = p->stream;
   return SEVP_OK;
 }
 
@@ -480,8 +555,8 @@ static int make_low_precision_copies(sevp_plan p) {
 
 int sevp_build_influence(sevp_plan p, int *zero_pivots) {
   if (!p) return fail(SEVP_E_ARG, "null plan");
-  HIPC(hipMemsetAsync(p->zp.p, 0, sizeof(int), p->stream));
-  HIPC(hipMemsetAsync(p->rinv1.p, 0, p->rinv1.bytes(), p->stream));
+  CHK(zero_async(p->zp.p, sizeof(int), p->stream));
+  CHK(zero_async(p->rinv1.p, p->rinv1.bytes(), p->stream));
   PreCtx c{};
   c.ax = p->al(); c.ay = p->ab(); c.bb = p->ac(); c.cx = p->ar(); c.cy = p->at();
   c.cld = p->d.i2;
@@ -531,7 +606,7 @@ int sevp_get_influence(sevp_plan p, double *rinv, double *rinv1) {
 
 int sevp_reset_seam(sevp_plan p) {
   if (!p) return fail(SEVP_E_ARG, "null plan");
-  if (p->xx.p) HIPC(hipMemsetAsync(p->xx.p, 0, p->xx.bytes(), p->stream));
+  if (p->xx.p) CHK(zero_async(p->xx.p, p->xx.bytes(), p->stream));
   return SEVP_OK;
 }
 
@@ -551,7 +626,61 @@ static FullCtx full_ctx(sevp_plan p, bool mixed) {
   c.partial = p->partial.p;
   c.counters = p->counters.p;
   c.splits = p->splits;
+  c.prof = &p->prof;
   return c;
+}
+
+static int alloc_mixed(sevp_plan p) {
+  if (p->partial.p) return SEVP_OK;
+  const int nr = p->d.nrhs_max;
+  if (nr > 64) return fail(SEVP_E_ARG, "mixed precision solves at most 64 RHS per call");
+  const int ntiles = (p->d.i2 + 15) / 16;
+  p->splits = std::max(1, std::min(16, 1024 / ntiles));
+  CHK(p->partial.alloc((size_t)p->splits * nr * p->d.i2));
+  CHK(p->counters.alloc((size_t)ntiles));
+  HIPC(hipMemset(p->counters.p, 0, p->counters.bytes()));
+  CHK(p->rinv32.alloc(p->rinv_count()));
+  CHK(p->rinv1_32.alloc(p->rinv1_count()));
+  return SEVP_OK;
+}
+
+int sevp_set_precision(sevp_plan p, int precision) {
+  if (!p) return fail(SEVP_E_ARG, "null plan");
+  if (precision < SEVP_PREC_FP64 || precision > SEVP_PREC_BF16)
+    return fail(SEVP_E_ARG, "unknown precision");
+  if (p->d.form == SEVP_FORM_BIR && precision != SEVP_PREC_FP64)
+    return fail(SEVP_E_ARG, "BIR form runs the plain FP64 arithmetic only");
+  if (precision == SEVP_PREC_BF16) return fail(SEVP_E_ARG, "BF16 corrections are not built");
+  const int old = p->d.precision;
+  p->d.precision = precision;
+  if (precision != SEVP_PREC_FP64 && old == SEVP_PREC_FP64) {
+    CHK(alloc_mixed(p));
+    if (p->have_inv) CHK(make_low_precision_copies(p));
+    HIPC(hipStreamSynchronize(p->stream));
+  }
+  return SEVP_OK;
+}
+
+int sevp_profile_begin(sevp_plan p, int kclass) {
+  if (!p) return fail(SEVP_E_ARG, "null plan");
+  if (kclass < SEVP_KCLASS_CORR || kclass > SEVP_KCLASS_RESID) return fail(SEVP_E_ARG, "bad kernel class");
+  HIPC(hipStreamSynchronize(p->stream));
+  p->prof.harvest();
+  p->prof.kclass = kclass;
+  p->prof.total_ms = 0.0;
+  p->prof.launches = 0;
+  p->prof.st = p->stream;
+  return SEVP_OK;
+}
+
+int sevp_profile_end(sevp_plan p, double *total_ms, long *launches) {
+  if (!p) return fail(SEVP_E_ARG, "null plan");
+  HIPC(hipStreamSynchronize(p->stream));
+  p->prof.harvest();
+  if (total_ms) *total_ms = p->prof.total_ms;
+  if (launches) *launches = p->prof.launches;
+  p->prof.kclass = -1;
+  return SEVP_OK;
 }
 
 int sevp_solve_dev(sevp_plan p, const float *s, double *x, int nrhs, void *stream) {
@@ -567,7 +696,7 @@ int sevp_solve_dev(sevp_plan p, const float *s, double *x, int nrhs, void *strea
     a.s = s; a.rinv = p->rinv.p; a.rinv1 = p->rinv1.p;
     a.x = x; a.xx = p->xx.p; a.dum0 = p->bdum0.p; a.ie = p->ie_d.p;
     a.i2 = p->d.i2; a.j2 = p->d.j2; a.nbir = p->d.nbir; a.ibir = p->ibir; a.nb0 = p->d.nb0;
-    a.nsweep = p->d.nsweep; a.periodic = p->d.periodic;
+    a.nsweep = p->d.nsweep; a.periodic = p->d.periodic; a.seam_fix = p->d.seam_fix;
     a.s_stride = s_stride; a.x_stride = x_stride;
     a.xx_stride = (long)(p->ibir + 2) * (p->d.j2 + 2);
     a.dum0_stride = (long)p->ibir * (p->d.nb0 > 1 ? p->d.nb0 - 1 : 1);
@@ -623,7 +752,7 @@ int sevp_solve_refined_dev(sevp_plan p, const float *s, double *x, int nrhs, dou
     worst = 0.0;
     for (double v : rel) worst = std::max(worst, v);
     if (!(worst > tol) || it >= max_iter) break;
-    HIPC(hipMemsetAsync(p->dx.p, 0, sizeof(double) * nx * nrhs, p->stream));
+    CHK(zero_async(p->dx.p, sizeof(double) * nx * nrhs, p->stream));
     CHK(run_full_pass(c, nullptr, p->r64.p, ns, p->dx.p, nx, nrhs, p->stream));
     const long n = ns * nrhs;
     hipLaunchKernelGGL(axpy_interior_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,

@@ -13,7 +13,7 @@ struct BirArgs {
   double *xx;                            // (ibir+2, j2+2) per RHS, persistent seam scratch
   double *dum0;                          // (ibir, nb0-1) per RHS
   const int *ie;                         // (nb0)
-  int i2, j2, nbir, ibir, nb0, nsweep, periodic;
+  int i2, j2, nbir, ibir, nb0, nsweep, periodic, seam_fix;
   long s_stride, x_stride, xx_stride, dum0_stride;
 };
 

@@ -34,6 +34,9 @@ PREC_BF16 = 2
 
 SEVP_OK = 0
 SEVP_E_NOTCONV = -5
+KCLASS_CORR = 0
+KCLASS_MARCH = 1
+KCLASS_RESID = 2
 
 
 class SevpError(RuntimeError):
@@ -46,7 +49,8 @@ class _Desc(ctypes.Structure):
     _fields_ = [("i2", ctypes.c_int), ("j2", ctypes.c_int), ("nb0", ctypes.c_int),
                 ("ie", ctypes.POINTER(ctypes.c_int)), ("form", ctypes.c_int),
                 ("nbir", ctypes.c_int), ("nsweep", ctypes.c_int), ("periodic", ctypes.c_int),
-                ("nrhs_max", ctypes.c_int), ("precision", ctypes.c_int)]
+                ("nrhs_max", ctypes.c_int), ("precision", ctypes.c_int),
+                ("seam_fix", ctypes.c_int)]
 
 
 class SevpStats(ctypes.Structure):
@@ -88,6 +92,11 @@ def lib():
         L.sevp_pre.argtypes = [VP] * 9 + [CI] * 4
         L.sevp_pre1.argtypes = [VP] * 9 + [CI] * 3
         L.sevp_matinv.argtypes = [VP, CI]
+        L.sevp_set_precision.argtypes = [VP, CI]
+        L.sevp_set_stream.argtypes = [VP, VP]
+        L.sevp_profile_begin.argtypes = [VP, CI]
+        L.sevp_profile_end.argtypes = [VP, ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_long)]
         _lib = L
     return _lib
 
@@ -135,7 +144,7 @@ class SevpPlan:
     scratch for up to ``nrhs_max`` simultaneous right-hand sides."""
 
     def __init__(self, coefs, ie, form=FORM_BIR, nbir=None, nsweep=5, periodic=True,
-                 nrhs_max=1, precision=PREC_FP64):
+                 nrhs_max=1, precision=PREC_FP64, seam_fix=False):
         al, ab, ac, ar, at = [_f32(np.asfortranarray(c, dtype=np.float32)) for c in coefs]
         self.i2, self.j2 = al.shape
         self.ie = _i32(ie)
@@ -143,11 +152,13 @@ class SevpPlan:
         self.nbir = nbir if nbir is not None else 0
         self.ibir = 2 * self.i2 // self.nbir if form == FORM_BIR else self.i2
         d = _Desc(self.i2, self.j2, len(self.ie), self.ie.ctypes.data_as(ctypes.POINTER(CI)),
-                  form, self.nbir, nsweep, 1 if periodic else 0, nrhs_max, precision)
+                  form, self.nbir, nsweep, 1 if periodic else 0, nrhs_max, precision,
+                  1 if seam_fix else 0)
         h = VP()
         _check(lib().sevp_plan_create(ctypes.byref(d), _p(al), _p(ab), _p(ac), _p(ar), _p(at),
                                       ctypes.byref(h)))
         self._h = h
+        self._stream = None
 
     def close(self):
         if getattr(self, "_h", None):
@@ -192,6 +203,19 @@ class SevpPlan:
         _check(lib().sevp_get_influence(self._h, _p(rinv), _p(rinv1)))
         return rinv, rinv1
 
+    def set_precision(self, precision):
+        _check(lib().sevp_set_precision(self._h, precision))
+        self.precision = precision
+
+    def profile_begin(self, kclass):
+        _check(lib().sevp_profile_begin(self._h, kclass))
+
+    def profile_end(self):
+        ms = ctypes.c_double(0.0)
+        n = ctypes.c_long(0)
+        _check(lib().sevp_profile_end(self._h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
     def reset_seam(self):
         _check(lib().sevp_reset_seam(self._h))
 
@@ -202,14 +226,25 @@ class SevpPlan:
         _check(lib().sevp_solve(self._h, _p(_f32(s)), _p(_f64(x)), nrhs))
         return x
 
-    def solve_dev(self, s, x, nrhs=None, stream=None):
+    def _follow_torch_stream(self):
+        """Enqueue on torch's current stream so the solve is ordered after
+        the torch work that produced its inputs."""
+        import torch
+        st = torch.cuda.current_stream().cuda_stream
+        if st != self._stream:
+            _check(lib().sevp_set_stream(self._h, ctypes.c_void_p(st)))
+            self._stream = st
+
+    def solve_dev(self, s, x, nrhs=None):
         """Device tensors (torch, cuda): s float32 (nrhs, J2, I2) contiguous
         (i.e. Fortran (I2, J2) per RHS), x float64 (nrhs, J2+2, I2+2)."""
         nrhs = nrhs or s.shape[0]
-        _check(lib().sevp_solve_dev(self._h, _p(s), _p(x), nrhs, stream))
+        self._follow_torch_stream()
+        _check(lib().sevp_solve_dev(self._h, _p(s), _p(x), nrhs, None))
 
     def solve_refined_dev(self, s, x, tol=1e-10, max_iter=20, nrhs=None):
         nrhs = nrhs or s.shape[0]
+        self._follow_torch_stream()
         rel = np.zeros(nrhs)
         st = SevpStats()
         rc = lib().sevp_solve_refined_dev(self._h, _p(s), _p(x), nrhs, tol, max_iter, _p(rel),
@@ -220,6 +255,7 @@ class SevpPlan:
 
     def residual_dev(self, s, x, nrhs=None):
         nrhs = nrhs or s.shape[0]
+        self._follow_torch_stream()
         rel = np.zeros(nrhs)
         _check(lib().sevp_residual_dev(self._h, _p(s), _p(x), nrhs, _p(rel)))
         return rel

@@ -106,7 +106,7 @@ def bir_prep(coefs, ie, nbir):
     return rinv, rinv1, zp
 
 
-def bir_solve(coefs, s, rinv, rinv1, x, xx, ie, nbir, nsweep=5, periodic=True):
+def bir_solve(coefs, s, rinv, rinv1, x, xx, ie, nbir, nsweep=5, periodic=True, seam_fix=False):
     """FS_SOLVER BIR sweeps on one RHS; x and xx are updated in place."""
     al, ab, ac, ar, at = [f32(c) for c in coefs]
     i2, j2 = al.shape
@@ -116,7 +116,8 @@ def bir_solve(coefs, s, rinv, rinv1, x, xx, ie, nbir, nsweep=5, periodic=True):
     ie = np.ascontiguousarray(ie, np.int32)
     lib().orc_bir_solve(P(al), P(ab), P(ac), P(ar), P(at), P(s), P(f64(rinv)),
                         P(f64(rinv1)), P(x), P(xx), P(ie), _ci(i2), _ci(j2),
-                        _ci(nbir), _ci(len(ie)), _ci(nsweep), _ci(1 if periodic else 0))
+                        _ci(nbir), _ci(len(ie)), _ci(nsweep), _ci(1 if periodic else 0),
+                        _ci(1 if seam_fix else 0))
 
 
 def pre1(coefs, ie):

@@ -53,7 +53,11 @@ void orc_repbir(const float *ax, const float *ay, const float *bb,
 #define F(i, j) ((double)f[IDX2(m2, i, j)])
 #define X(i, j) x[IDX2(i0, i, j)]
 #define H(i, j) h[IDX2(m0, i, j)]
+#ifndef ORC_PLANT_TRANSPOSED_RINV
 #define RINV(m, n, b) rinv[((long)(m) - 1) + ((long)(n) - 1) * m2 + ((long)(b) - 1) * (long)m2 * m2]
+#else /* deliberate mistake, used only by the test that checks the pins catch it */
+#define RINV(m, n, b) rinv[((long)(n) - 1) + ((long)(m) - 1) * m2 + ((long)(b) - 1) * (long)m2 * m2]
+#endif
 #define RINV1(m, n, b) rinv1[((long)(m) - 1) + ((long)(n) - 1) * m2 + ((long)(b) - 1) * (long)m2 * m2]
 #define DUM0(n, b) dum0[((long)(n) - 1) + ((long)(b) - 1) * m2]
   int js = 1;
@@ -470,12 +474,17 @@ int orc_bir_prep(const float *al, const float *ab, const float *ac,
  *         calls, and its halo entries in row j2+1 and its halo rows are never
  *         refreshed by the gather; pass the same buffer to successive calls.
  *   periodic: apply the FLAG_PERIODIC_WE halo refresh after the seam strip.
+ *   seam_fix: 0 reproduces the reference, whose halo gather for the seam
+ *         strip covers rows 1..J2 although REPBIR reads the halo columns on
+ *         rows up to J2+1, so the seam strip sees zero there and the sweeps
+ *         stall short of the solution; 1 also gathers row J2+1, after which
+ *         the sweeps converge to the global periodic solution.
  * The call order of strips and sweeps is a data dependency and is kept. */
 void orc_bir_solve(const float *al, const float *ab, const float *ac,
                    const float *ar, const float *at, const float *s,
                    const double *rinv, const double *rinv1, double *x,
                    double *xx, const int *ie, int i2, int j2, int nbir,
-                   int nb0, int nsweep, int periodic)
+                   int nb0, int nsweep, int periodic, int seam_fix)
 {
   const int ibir = 2 * i2 / nbir, inc = ibir / 2;
   const int i0 = i2 + 2, i1 = i2 + 1, j1 = j2 + 1, j0 = j2 + 2;
@@ -500,7 +509,7 @@ void orc_bir_solve(const float *al, const float *ab, const float *ac,
         orc_repbir(al + o, ab + o, ac + o, ar + o, at + o, rv, rv1, dum0, dum1,
                    dum2, src, h, x + IDX2(i0, il, 1), ie, i0, i2, m0, ibir, nb0);
       } else {
-        for (int j = 1; j <= j2; ++j) {
+        for (int j = 1; j <= (seam_fix ? j1 : j2); ++j) {
           xx[IDX2(m0, 1, j)] = x[IDX2(i0, il, j)];
           xx[IDX2(m0, ibir + 2, j)] = x[IDX2(i0, inc + 2, j)];
         }

@@ -241,3 +241,49 @@ def test_residual_dev_matches_oracle(oracle, device):
         ro = oracle.rel_residual(coefs, s[:, :, r], np.asfortranarray(x[:, :, r]))
         assert abs(rel[r] - ro) <= 1e-12 * ro
     plan.close()
+
+
+GOLDEN_CASES = ["bir_harness_192x96_h24", "bir_harness_192x96_h6", "bir_synth_48x24",
+                "rep1_synth_48x24", "rep_synth_48x24"]
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_golden_vectors_on_gpu(oracle, device, name):
+    """The HIP library reproduces the outputs of the compiled reference
+    routines stored in tests/golden, bitwise."""
+    from tests.test_oracle_golden import coefs_of, load
+    meta, d = load(name)
+    coefs = coefs_of(meta, d)
+    ie = d["ie"]
+    print(f"[golden] {name}", flush=True)
+    if name.startswith("bir"):
+        plan = S.SevpPlan(coefs, ie, form=S.FORM_BIR, nbir=meta["nbir"], nsweep=meta["nsweep"],
+                          periodic=meta["periodic"])
+        plan.build_influence()
+        x = np.asfortranarray(d["x0"]).copy(order="F")
+        plan.solve(np.asfortranarray(d["s"]), x)
+    else:
+        form = S.FORM_FULL_PERIODIC if meta["periodic"] else S.FORM_FULL
+        plan = S.SevpPlan(coefs, ie, form=form)
+        plan.load_influence(np.asfortranarray(d["rinv"]), np.asfortranarray(d["rinv1"]))
+        x = np.asfortranarray(d["x0"]).copy(order="F")
+        plan.solve(np.asfortranarray(d["s"]), x)
+    assert fields_equal(x, d["x"])
+    plan.close()
+
+
+def test_bir_seam_fix_bitwise(oracle, device):
+    i2, j2, nbir, n0 = 96, 48, 8, 4
+    coefs, ie, mask = make(i2, j2, n0, metric="latlon", lat_max=80.0)
+    plan = S.SevpPlan(coefs, ie, form=S.FORM_BIR, nbir=nbir, nsweep=3, nrhs_max=2, seam_fix=True)
+    plan.build_influence()
+    r_o, r1_o, _ = oracle.bir_prep(coefs, ie, nbir)
+    s = synth.make_rhs(mask, 31, 2)
+    xg = np.zeros((i2 + 2, j2 + 2, 2), order="F")
+    plan.solve(s, xg)
+    for r in range(2):
+        xo = np.zeros((i2 + 2, j2 + 2), order="F")
+        xx = np.zeros((2 * i2 // nbir + 2, j2 + 2), order="F")
+        oracle.bir_solve(coefs, s[:, :, r], r_o, r1_o, xo, xx, ie, nbir, nsweep=3, seam_fix=True)
+        assert fields_equal(xg[:, :, r], xo)
+    plan.close()
