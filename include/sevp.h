@@ -18,7 +18,8 @@
  *     divides by them silently.
  *   - The "_dev" entry points take device pointers and enqueue on the plan's
  *     stream (or the stream given); the others take host pointers and return
- *     when the result is back in host memory.
+ *     when the result is back in host memory.  The plan's own stream is a
+ *     blocking stream, ordered with the null stream.
  *   - A plan is used by one host thread at a time.  Calls on one plan are
  *     ordered on its stream.
  */
@@ -163,7 +164,8 @@ int sevp_pre1(const float *ax, const float *ay, const float *bb, const float *cx
 /* MATINV, PREP/template/GLOBAL/src/prep_GLOBAL.f90:516-568. */
 int sevp_matinv(double *b, int n);
 
-/* Enqueue this plan's work on `hip_stream` (NULL: the plan's own stream).
+/* Enqueue this plan's work on `hip_stream` (NULL: the plan's own stream,
+ * which is ordered with the null stream).
  * Callers that produce a plan's inputs on another stream set it here so the
  * solve is ordered after them. */
 int sevp_set_stream(sevp_plan plan, void *hip_stream);

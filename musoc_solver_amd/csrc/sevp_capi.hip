@@ -121,9 +121,7 @@ struct sevp_plan_s {
   DBuf<float> rinv32, rinv1_32;  // mixed precision copies
   // solve scratch
   DBuf<double> e, dum0, hfree, hcpl, g;  // full forms
-  DBuf<float> partial;                   // mixed: split-K partial sums
-  DBuf<int> counters;                    // mixed: split-K arrival counters
-  int splits = 1;
+  DBuf<float> e32, g32;                  // mixed: FP32 copies of the error rows
   DBuf<double> xx, bdum0;                // BIR
   DBuf<double> r64, dx, part;            // refinement
   DBuf<double> b1s;                      // MATINV scratch (blocked path)
@@ -212,9 +210,7 @@ struct FullCtx {
   const double *rinv, *rinv1;
   const float *rinv32, *rinv1_32;  // non-null selects the FP32 MFMA corrections
   double *e, *dum0, *hfree, *hcpl, *g;
-  float *partial;
-  int *counters;
-  int splits;
+  float *e32, *g32;  // FP32 copies of e and g (mixed precision), else null
   KProf *prof;
 };
 
@@ -225,8 +221,8 @@ static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, 
   (void)j0;
   const long dstride = (long)c.i2 * (c.nb0 > 1 ? c.nb0 - 1 : 1);
   const bool mixed = c.rinv32 != nullptr;
-  auto corr = [&](const double *a, const double *b64, const float *b32, int ep, int xrow,
-                  double *dum0, double *out, double *g, int cyrow) -> int {
+  auto corr = [&](const double *a, const float *a32, const double *b64, const float *b32, int ep,
+                  int xrow, double *dum0, double *out, double *g, int cyrow) -> int {
     if (!mixed) {
       GemmArgs gm{};
       gm.a = a; gm.a_sm = c.i2; gm.a_sk = 1;
@@ -243,16 +239,17 @@ static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, 
       if (c.prof) c.prof->stop(SEVP_KCLASS_CORR);
     } else {
       GemmF32Args gm{};
+      gm.a = a32;
       gm.b = b32;
       gm.M = nrhs; gm.N = c.i2; gm.K = c.i2;
       gm.ep = ep;
       gm.x = x; gm.x_stride = x_stride; gm.xrow = xrow;
       gm.dum0 = dum0; gm.dum0_stride = dstride;
       gm.out = out; gm.out_ld = c.i2;
-      gm.g = g; gm.cy = c.cy; gm.cyrow = cyrow;
+      gm.g = g; gm.g32 = c.g32; gm.cy = c.cy; gm.cyrow = cyrow;
       gm.i2 = c.i2;
       if (c.prof) c.prof->start(SEVP_KCLASS_CORR);
-      HIPC(launch_gemm_mixed(gm, a, c.partial, c.counters, c.splits, st));
+      HIPC(launch_gemm_mixed(gm, st));
       if (c.prof) c.prof->stop(SEVP_KCLASS_CORR);
     }
     return SEVP_OK;
@@ -267,6 +264,7 @@ static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, 
     m.f = f32; m.f64 = f64; m.f_stride = f_stride;
     m.x = x; m.x_stride = x_stride;
     m.e = c.e;
+    m.e32 = mixed ? c.e32 : nullptr;
     m.i2 = c.i2; m.j2 = c.j2; m.periodic = c.periodic;
     m.js = js; m.jf = ieb - 2; m.ieb = ieb;
     m.resid = (nb != c.nb0);
@@ -277,8 +275,9 @@ static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, 
     HIPC(launch_march(MODE_PART, m, nrhs, st));
     if (c.prof) c.prof->stop(SEVP_KCLASS_MARCH);
     if (nb != c.nb0)
-      CHK(corr(c.e, c.rinv1 + (long)(nb - 1) * ww, mixed ? c.rinv1_32 + (long)(nb - 1) * ww : nullptr,
-               EP_FWD, ieb, c.dum0 + (long)(nb - 1) * c.i2, nullptr, nullptr, 0));
+      CHK(corr(c.e, c.e32, c.rinv1 + (long)(nb - 1) * ww,
+               mixed ? c.rinv1_32 + (long)(nb - 1) * ww : nullptr, EP_FWD, ieb,
+               c.dum0 + (long)(nb - 1) * c.i2, nullptr, nullptr, 0));
     js = ieb;
   }
   for (int nbs = 1; nbs <= c.nb0; ++nbs) {
@@ -292,15 +291,18 @@ static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, 
     r.dum0 = (nb != c.nb0) ? c.dum0 : nullptr;
     r.dum0_stride = dstride;
     r.e = c.e;
+    r.e32 = mixed ? c.e32 : nullptr;
     r.i2 = c.i2; r.j2 = c.j2; r.periodic = c.periodic; r.nb = nb; r.ieb = ieb; r.nrhs = nrhs;
     if (c.prof) c.prof->start(SEVP_KCLASS_RESID);
     HIPC(launch_resid(r, st));
     if (c.prof) c.prof->stop(SEVP_KCLASS_RESID);
-    CHK(corr(c.e, c.rinv + (long)(nb - 1) * ww, mixed ? c.rinv32 + (long)(nb - 1) * ww : nullptr,
-             EP_BWD, js + 1, nullptr, c.hfree, nb != 1 ? c.g : nullptr, js - 1));
+    CHK(corr(c.e, c.e32, c.rinv + (long)(nb - 1) * ww,
+             mixed ? c.rinv32 + (long)(nb - 1) * ww : nullptr, EP_BWD, js + 1, nullptr, c.hfree,
+             nb != 1 ? c.g : nullptr, js - 1));
     if (nb != 1)
-      CHK(corr(c.g, c.rinv1 + (long)(nb - 2) * ww, mixed ? c.rinv1_32 + (long)(nb - 2) * ww : nullptr,
-               EP_CPL, 0, nullptr, c.hcpl, nullptr, 0));
+      CHK(corr(c.g, c.g32, c.rinv1 + (long)(nb - 2) * ww,
+               mixed ? c.rinv1_32 + (long)(nb - 2) * ww : nullptr, EP_CPL, 0, nullptr, c.hcpl,
+               nullptr, 0));
     if (ieb - 2 >= js) {
       MarchArgs m{};
       m.ax = c.ax; m.ay = c.ay; m.bb = c.bb; m.cx = c.cx; m.cy = c.cy;
@@ -479,7 +481,9 @@ int sevp_plan_create(const sevp_desc *desc, const float *al, const float *ab, co
     sevp_plan_destroy(p);
     return code;
   };
-  if (hipStreamCreateWithFlags(&p->own_stream, hipStreamNonBlocking) != hipSuccess)
+  // a blocking stream: work on it is ordered with the null stream, which is
+  // where callers such as PyTorch's default stream enqueue by default
+  if (hipStreamCreateWithFlags(&p->own_stream, hipStreamDefault) != hipSuccess)
     return bail(fail(SEVP_E_HIP, "hipStreamCreate failed"));
   p->stream = p->own_stream;
   const size_t nc = (size_t)d.i2 * d.j2;
@@ -541,7 +545,8 @@ size_t sevp_plan_device_bytes(sevp_plan p) {
   if (!p) return 0;
   return p->coef.bytes() + p->ie_d.bytes() + p->rinv.bytes() + p->rinv1.bytes() +
          p->rinv32.bytes() + p->rinv1_32.bytes() + p->e.bytes() + p->dum0.bytes() +
-         p->hfree.bytes() + p->hcpl.bytes() + p->g.bytes() + p->partial.bytes() + p->xx.bytes() +
+         p->hfree.bytes() + p->hcpl.bytes() + p->g.bytes() + p->e32.bytes() +
+         p->g32.bytes() + p->xx.bytes() +
          p->bdum0.bytes() + p->r64.bytes() + p->dx.bytes() + p->part.bytes() + p->b1s.bytes() +
          p->zp.bytes();
 }
@@ -623,22 +628,18 @@ static FullCtx full_ctx(sevp_plan p, bool mixed) {
     c.rinv1_32 = p->rinv1_32.p;
   }
   c.e = p->e.p; c.dum0 = p->dum0.p; c.hfree = p->hfree.p; c.hcpl = p->hcpl.p; c.g = p->g.p;
-  c.partial = p->partial.p;
-  c.counters = p->counters.p;
-  c.splits = p->splits;
+  c.e32 = mixed ? p->e32.p : nullptr;
+  c.g32 = mixed ? p->g32.p : nullptr;
   c.prof = &p->prof;
   return c;
 }
 
 static int alloc_mixed(sevp_plan p) {
-  if (p->partial.p) return SEVP_OK;
+  if (p->e32.p) return SEVP_OK;
   const int nr = p->d.nrhs_max;
   if (nr > 64) return fail(SEVP_E_ARG, "mixed precision solves at most 64 RHS per call");
-  const int ntiles = (p->d.i2 + 15) / 16;
-  p->splits = std::max(1, std::min(16, 1024 / ntiles));
-  CHK(p->partial.alloc((size_t)p->splits * nr * p->d.i2));
-  CHK(p->counters.alloc((size_t)ntiles));
-  HIPC(hipMemset(p->counters.p, 0, p->counters.bytes()));
+  CHK(p->e32.alloc((size_t)nr * p->d.i2));
+  CHK(p->g32.alloc((size_t)nr * p->d.i2));
   CHK(p->rinv32.alloc(p->rinv_count()));
   CHK(p->rinv1_32.alloc(p->rinv1_count()));
   return SEVP_OK;

@@ -187,7 +187,10 @@ __global__ __launch_bounds__(64) void march_kernel(MarchArgs p) {
       nbr<K>(vc, k, xw, xe);
       const double xn = x[f2(i0, pc[k] + 1, ieb + 1)];
       const double r = cf[k] - ca[k] * xw - cb[k] * vs[k] - cc[k] * vc[k] - cd[k] * xe - ce[k] * xn;
-      if (out[k]) p.e[(long)bidx * p.i2 + pc[k] - 1] = r;
+      if (out[k]) {
+        p.e[(long)bidx * p.i2 + pc[k] - 1] = r;
+        if (p.e32) p.e32[(long)bidx * p.i2 + pc[k] - 1] = (float)r;
+      }
     }
   }
   if (MODE == MODE_PRE) {
@@ -274,6 +277,7 @@ __global__ void resid_kernel(ResidArgs p) {
                    (double)p.bb[co] * center - (double)p.cx[co] * east -
                    (double)p.cy[co] * x[f2(i0, c + 1, ieb + 1)];
   p.e[(long)r * i2 + c - 1] = e;
+  if (p.e32) p.e32[(long)r * i2 + c - 1] = (float)e;
   if (d0) x[f2(i0, c + 1, ieb)] = center;
 }
 
@@ -287,78 +291,107 @@ hipError_t launch_resid(const ResidArgs &a, hipStream_t st) {
 // Sequential-order FP64 GEMM with fused epilogues.
 //   D(m, n) = sum_{k=0..K-1} A(m,k) * B(k,n)   (EP_NEGSUB: 0 - sum ... as
 //   successive subtractions)
-// Tile 16 (m) x 64 (n), 256 threads, 4 outputs per thread, K in chunks of 32
-// staged through LDS.
-constexpr int GT_M = 16, GT_N = 64, GT_K = 32;
+// One output per thread, so that the independent sequential chains fill the
+// GPU (a 64 x 1440 product has 92160 of them).  Tile 16 (m) x 16 (n), K in
+// chunks of 64 staged through LDS, the next chunk loaded into registers while
+// the current one is consumed.
+constexpr int GT_M = 16, GT_N = 16, GT_K = 64;
 
 __global__ __launch_bounds__(256) void gemm_seq_kernel(GemmArgs p) {
-  __shared__ double As[GT_M][GT_K + 1];
-  __shared__ double Bs[GT_K][GT_N + 1];
+  // both tiles k-contiguous (rows of GT_K + 2 doubles, 16-byte aligned) so a
+  // thread reads two consecutive k of each operand with one 128-bit load
+  __shared__ __align__(16) double As[GT_M][GT_K + 2];
+  __shared__ __align__(16) double Bs[GT_N][GT_K + 2];
   const int t = threadIdx.x;
   const int n0 = blockIdx.x * GT_N, m0 = blockIdx.y * GT_M;
-  const int tn = t % GT_N, tm = (t / GT_N) * 4;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const int tn = t % GT_N, tm = t / GT_N;
   const bool neg = (p.ep == EP_NEGSUB);
+  // staging: each thread moves 4 elements of A and 4 of B per chunk
+  double ra[4], rb[4];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int idx = t + 256 * q;
+      {  // A tile: kk fastest (contiguous when a_sk == 1)
+        const int kk = (p.a_sk == 1) ? idx % GT_K : idx / GT_M;
+        const int mm = (p.a_sk == 1) ? idx / GT_K : idx % GT_M;
+        const int m = m0 + mm, k = k0 + kk;
+        ra[q] = (m < p.M && k < p.K) ? p.a[(long)m * p.a_sm + (long)k * p.a_sk] : 0.0;
+      }
+      {  // B tile: kk fastest when b_sk == 1
+        const int kk = (p.b_sk == 1) ? idx % GT_K : idx / GT_N;
+        const int nn = (p.b_sk == 1) ? idx / GT_K : idx % GT_N;
+        const int n = n0 + nn, k = k0 + kk;
+        rb[q] = (n < p.N && k < p.K) ? p.b[(long)k * p.b_sk + (long)n * p.b_sn] : 0.0;
+      }
+    }
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int idx = t + 256 * q;
+      const int kka = (p.a_sk == 1) ? idx % GT_K : idx / GT_M;
+      const int mma = (p.a_sk == 1) ? idx / GT_K : idx % GT_M;
+      As[mma][kka] = ra[q];
+      const int kkb = (p.b_sk == 1) ? idx % GT_K : idx / GT_N;
+      const int nnb = (p.b_sk == 1) ? idx / GT_K : idx % GT_N;
+      Bs[nnb][kkb] = rb[q];
+    }
+  };
+  double acc = 0.0;
+  fetch(0);
   for (int k0 = 0; k0 < p.K; k0 += GT_K) {
     const int kc = min(GT_K, p.K - k0);
-    for (int idx = t; idx < GT_M * GT_K; idx += 256) {
-      const int kk = idx % GT_K, mm = idx / GT_K;
-      const int m = m0 + mm, k = k0 + kk;
-      As[mm][kk] = (m < p.M && kk < kc) ? p.a[(long)m * p.a_sm + (long)k * p.a_sk] : 0.0;
-    }
-    for (int idx = t; idx < GT_K * GT_N; idx += 256) {
-      const int kk = idx % GT_K, nn = idx / GT_K;
-      const int n = n0 + nn, k = k0 + kk;
-      Bs[kk][nn] = (n < p.N && kk < kc) ? p.b[(long)k * p.b_sk + (long)n * p.b_sn] : 0.0;
-    }
     __syncthreads();
+    stash();
+    __syncthreads();
+    if (k0 + GT_K < p.K) fetch(k0 + GT_K);
+    const double2 *ar = reinterpret_cast<const double2 *>(&As[tm][0]);
+    const double2 *br = reinterpret_cast<const double2 *>(&Bs[tn][0]);
+    const int kc2 = kc / 2;
     if (neg) {
-      for (int kk = 0; kk < kc; ++kk) {
-        const double bv = Bs[kk][tn];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) acc[i] = acc[i] - As[tm + i][kk] * bv;
+      for (int k2 = 0; k2 < kc2; ++k2) {
+        const double2 av = ar[k2], bv = br[k2];
+        acc = acc - av.x * bv.x;
+        acc = acc - av.y * bv.y;
       }
+      if (kc & 1) acc = acc - As[tm][kc - 1] * Bs[tn][kc - 1];
     } else {
-      for (int kk = 0; kk < kc; ++kk) {
-        const double bv = Bs[kk][tn];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) acc[i] = acc[i] + As[tm + i][kk] * bv;
+      for (int k2 = 0; k2 < kc2; ++k2) {
+        const double2 av = ar[k2], bv = br[k2];
+        acc = acc + av.x * bv.x;
+        acc = acc + av.y * bv.y;
       }
+      if (kc & 1) acc = acc + As[tm][kc - 1] * Bs[tn][kc - 1];
     }
-    __syncthreads();
   }
-  const int n = n0 + tn;
-  if (n >= p.N) return;
+  const int n = n0 + tn, m = m0 + tm;
+  if (n >= p.N || m >= p.M) return;
   const int i0 = p.i2 + 2;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int m = m0 + tm + i;
-    if (m >= p.M) continue;
-    const double d = acc[i];
-    switch (p.ep) {
-      case EP_FWD: {
-        double *x = p.x + (long)m * p.x_stride;
-        const long xi = f2(i0, n + 2, p.xrow);
-        const double xv = x[xi];
-        p.dum0[(long)m * p.dum0_stride + n] = xv;
-        x[xi] = xv - d;
-        break;
-      }
-      case EP_BWD: {
-        double *x = p.x + (long)m * p.x_stride;
-        const long xi = f2(i0, n + 2, p.xrow);
-        x[xi] = x[xi] + d;
-        p.out[(long)m * p.out_ld + n] = d;
-        if (p.g) p.g[(long)m * p.out_ld + n] = d * (double)p.cy[(long)n + (long)(p.cyrow - 1) * p.i2];
-        break;
-      }
-      case EP_CPL:
-        p.out[(long)m * p.out_ld + n] = d;
-        break;
-      default:  // EP_NEGSUB: column-major output (m fastest)
-        p.out[(long)m + (long)n * p.out_ld] = d;
-        break;
+  const double d = acc;
+  switch (p.ep) {
+    case EP_FWD: {
+      double *x = p.x + (long)m * p.x_stride;
+      const long xi = f2(i0, n + 2, p.xrow);
+      const double xv = x[xi];
+      p.dum0[(long)m * p.dum0_stride + n] = xv;
+      x[xi] = xv - d;
+      break;
     }
+    case EP_BWD: {
+      double *x = p.x + (long)m * p.x_stride;
+      const long xi = f2(i0, n + 2, p.xrow);
+      x[xi] = x[xi] + d;
+      p.out[(long)m * p.out_ld + n] = d;
+      if (p.g) p.g[(long)m * p.out_ld + n] = d * (double)p.cy[(long)n + (long)(p.cyrow - 1) * p.i2];
+      break;
+    }
+    case EP_CPL:
+      p.out[(long)m * p.out_ld + n] = d;
+      break;
+    default:  // EP_NEGSUB: column-major output (m fastest)
+      p.out[(long)m + (long)n * p.out_ld] = d;
+      break;
   }
 }
 

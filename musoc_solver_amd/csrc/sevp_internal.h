@@ -62,8 +62,10 @@ struct MarchArgs {
   // MODE_PRE outputs: RINV(g, i, b) and T(g, n) (both w x w col-major)
   double *rinv_b, *tmat;
   long rinv_sstride, rinv1_sstride;  // MODE_PRE: per-strip strides of RINV and RINV1
-  // residual out (MODE_PART, when resid != 0): (nrhs, i2)
+  // residual out (MODE_PART, when resid != 0): (nrhs, i2); e32 (may be null)
+  // receives the same values rounded to FP32 for the mixed-precision product
   double *e;
+  float *e32;
   int i2, j2, periodic, js, jf, ieb, resid;
   int w;            // number of columns marched (i2 for full width)
   int tile_out;     // output columns per wave
@@ -82,6 +84,7 @@ struct ResidArgs {  // backward restore + closure residual, one thread per (rhs,
   const double *dum0;  // (i2, nb0-1) per RHS, null for the top block
   long dum0_stride;
   double *e;           // (nrhs, i2)
+  float *e32;          // FP32 copy of e (may be null)
   int i2, j2, periodic, nb, ieb, nrhs;
 };
 
@@ -108,7 +111,8 @@ struct GemmArgs {  // D(m, n) = sum_k A(m, k) B(k, n), sequential k
   int i2;
 };
 
-struct GemmF32Args {  // mixed precision: A (FP64, rounded on load) x B (FP32), FP32 accumulate
+struct GemmF32Args {  // mixed precision: A x B in FP32, FP32 accumulate
+  const float *a;     // (M, K) row-major, ld K
   const float *b;     // (K, N) col-major, ld K
   int M, N, K;
   int ep;
@@ -120,6 +124,7 @@ struct GemmF32Args {  // mixed precision: A (FP64, rounded on load) x B (FP32), 
   double *out;
   long out_ld;
   double *g;
+  float *g32;         // EP_BWD: FP32 copy of g for the coupling product
   const float *cy;
   int cyrow;
   int i2;
@@ -130,8 +135,7 @@ hipError_t launch_repbir(const RepbirArgs &a, int nrhs, hipStream_t st);
 hipError_t launch_march(int mode, const MarchArgs &a, int nbatch, hipStream_t st);
 hipError_t launch_resid(const ResidArgs &a, hipStream_t st);
 hipError_t launch_gemm(const GemmArgs &a, hipStream_t st);
-hipError_t launch_gemm_mixed(const GemmF32Args &a, const double *a64, float *partial,
-                             int *counters, int splits, hipStream_t st);
+hipError_t launch_gemm_mixed(const GemmF32Args &a, hipStream_t st);
 hipError_t launch_cvt_f32(const double *src, float *dst, long n, hipStream_t st);
 hipError_t launch_matinv(double *mats, long mat_stride, int n, int count, int *zero_pivots,
                          double *scratch, hipStream_t st);

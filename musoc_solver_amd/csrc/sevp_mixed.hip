@@ -1,14 +1,15 @@
 // Mixed-precision influence-matrix corrections on the matrix cores.
 //
-// D(m, n) = sum_k A(m, k) B(k, n) with A the FP64 error vectors (one row per
-// right-hand side) rounded to FP32 as they are loaded, B an FP32 copy of
-// RINV / RINV1 (column-major), and FP32 accumulation in
-// v_mfma_f32_16x16x4f32.  A wavefront owns 16 output columns and every
+// D(m, n) = sum_k A(m, k) B(k, n) with A the error vectors (one row per
+// right-hand side) rounded to FP32 by the kernels that produce them, B an
+// FP32 copy of RINV / RINV1 (column-major), and FP32 accumulation in
+// v_mfma_f32_16x16x4f32.  Operands are read as float4 runs of k and the next
+// step's operands are loaded while the current step's MFMAs run.  A wavefront owns 16 output columns and every
 // right-hand side (up to 64 rows = four 16x16 accumulator blocks), so each
-// matrix element is read once per product.  K is split over `splits`
-// workgroups; the last workgroup to finish a column tile (an atomic counter
-// per tile) adds the partial sums in split order and applies the same fused
-// epilogue as the FP64 product, so the result does not depend on timing.
+// matrix element is read once per product.  K is split over the wavefronts of
+// the tile's workgroup, whose partial tiles are added through LDS in a fixed
+// order before the same fused epilogue as the FP64 product, so the result
+// does not depend on timing.
 //
 // Inside one wavefront the k order is permuted: lane group q = lane/16 loads
 // k = k0 + 4q .. k0 + 4q + 3 as one run for both operands and feeds them to
@@ -22,74 +23,87 @@ namespace sevp {
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int MX_MAXM = 64;  // right-hand sides per product
-constexpr int MX_WAVES = 4;  // column tiles per workgroup
 
-__global__ __launch_bounds__(256) void gemm_f32_mfma_kernel(GemmF32Args p, const double *a64,
-                                                            int splits, int kchunk, float *partial,
-                                                            int *counters) {
+// Four consecutive k of one row (or column) starting at k, zero past k_hi.
+__device__ __forceinline__ f32x4 load4(const float *base, int k, int k_hi, bool vec) {
+  if (vec && k + 3 < k_hi) return *reinterpret_cast<const f32x4 *>(base + k);
+  f32x4 v;
+  v.x = (k < k_hi) ? base[k] : 0.f;
+  v.y = (k + 1 < k_hi) ? base[k + 1] : 0.f;
+  v.z = (k + 2 < k_hi) ? base[k + 2] : 0.f;
+  v.w = (k + 3 < k_hi) ? base[k + 3] : 0.f;
+  return v;
+}
+
+// One workgroup per (16-column tile, MBW x 16 rows); its NW wavefronts take
+// consecutive K slices (16-wide MFMA steps), then sum their partial tiles
+// through LDS in wavefront order and apply the epilogue.  The workgroups that
+// share a column tile (and so the same RINV columns) get block indices equal
+// modulo 8, which places them on the same XCD and its L2.
+template <int NW, int MBW>
+__global__ __launch_bounds__(64 * NW) void gemm_f32_mfma_kernel(GemmF32Args p, int kq, int mgroups) {
+  __shared__ float red[NW][16 * MBW][17];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int ntile = blockIdx.x * MX_WAVES + wave;
-  const int s = blockIdx.y;
+  const int wg = blockIdx.x;
+  const int ntile = (wg / (8 * mgroups)) * 8 + (wg % 8);
+  const int mgroup = (wg / 8) % mgroups;
   const int n0 = ntile * 16;
-  if (n0 >= p.N) return;
+  const int mbase = mgroup * 16 * MBW;
   const int K = p.K, M = p.M, N = p.N;
+  if (n0 >= N || mbase >= M) return;
   const int col = lane & 15, q = lane >> 4;
   const int n = n0 + col;
-  const int k_lo = s * kchunk, k_hi = min(K, k_lo + kchunk);
-  f32x4 acc[4];
+  const int k_lo = wave * kq, k_hi = min(K, k_lo + kq);
+  const bool vec = (K % 4) == 0;
+  f32x4 acc[MBW];
 #pragma unroll
-  for (int mb = 0; mb < 4; ++mb) acc[mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int mb = 0; mb < MBW; ++mb) acc[mb] = f32x4{0.f, 0.f, 0.f, 0.f};
   const float *bcol = p.b + (long)min(n, N - 1) * K;
-  for (int k0 = k_lo; k0 < k_hi; k0 += 16) {
+  const float *arow[MBW];
+  bool mok[MBW];
+#pragma unroll
+  for (int mb = 0; mb < MBW; ++mb) {
+    const int m = mbase + mb * 16 + col;
+    mok[mb] = m < M;
+    arow[mb] = p.a + (long)min(m, M - 1) * K;
+  }
+  f32x4 bq, aq[MBW];
+  auto fetch = [&](int k0, f32x4 &b_, f32x4 (&a_)[MBW]) {
     const int kb = k0 + 4 * q;
-    float bv[4], av[4][4];
+    b_ = (n < N) ? load4(bcol, kb, k_hi, vec) : f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int k = kb + t;
-      bv[t] = (k < k_hi && n < N) ? bcol[k] : 0.f;
-    }
+    for (int mb = 0; mb < MBW; ++mb)
+      a_[mb] = mok[mb] ? load4(arow[mb], kb, k_hi, vec) : f32x4{0.f, 0.f, 0.f, 0.f};
+  };
+  if (k_lo < k_hi) fetch(k_lo, bq, aq);
+  for (int k0 = k_lo; k0 < k_hi; k0 += 16) {
+    f32x4 bn = bq, an[MBW];
 #pragma unroll
-    for (int mb = 0; mb < 4; ++mb) {
-      const int m = mb * 16 + col;
-      const double *arow = a64 + (long)m * K;
+    for (int mb = 0; mb < MBW; ++mb) an[mb] = aq[mb];
+    if (k0 + 16 < k_hi) fetch(k0 + 16, bn, an);
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int k = kb + t;
-        av[mb][t] = (m < M && k < k_hi) ? (float)arow[k] : 0.f;
-      }
-    }
+    for (int t = 0; t < 4; ++t)
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
+      for (int mb = 0; mb < MBW; ++mb)
+        acc[mb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aq[mb][t], bq[t], acc[mb], 0, 0, 0);
+    bq = bn;
 #pragma unroll
-      for (int mb = 0; mb < 4; ++mb)
-        acc[mb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[mb][t], bv[t], acc[mb], 0, 0, 0);
-    }
+    for (int mb = 0; mb < MBW; ++mb) aq[mb] = an[mb];
   }
-  // D block (mb): lane holds rows mb*16 + 4q + i, column col
-  float *part = partial + (long)s * M * N;
+  // D block mb: lane holds rows mb*16 + 4q + i, column col
 #pragma unroll
-  for (int mb = 0; mb < 4; ++mb)
+  for (int mb = 0; mb < MBW; ++mb)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int m = mb * 16 + 4 * q + i;
-      if (m < M && n < N) part[(long)m * N + n] = acc[mb][i];
-    }
-  if (splits > 1) {
-    __threadfence();
-    int prev = 0;
-    if (lane == 0) prev = atomicAdd(&counters[ntile], 1);
-    prev = __shfl(prev, 0);
-    if (prev != splits - 1) return;
-    __threadfence();
-    if (lane == 0) counters[ntile] = 0;
-  }
+    for (int i = 0; i < 4; ++i) red[wave][mb * 16 + 4 * q + i][col] = acc[mb][i];
+  __syncthreads();
   const int i0 = p.i2 + 2;
-  for (int idx = lane; idx < 16 * M; idx += 64) {
-    const int m = idx / 16, nn = n0 + idx % 16;
-    if (nn >= N) continue;
+  for (int idx = threadIdx.x; idx < 16 * 16 * MBW; idx += 64 * NW) {
+    const int ml = idx / 16, c = idx % 16, nn = n0 + c, m = mbase + ml;
+    if (nn >= N || m >= M) continue;
     float sum = 0.f;
-    for (int ss = 0; ss < splits; ++ss) sum += partial[(long)ss * M * N + (long)m * N + nn];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) sum += red[w][ml][c];
     const double d = (double)sum;
     switch (p.ep) {
       case EP_FWD: {
@@ -105,7 +119,11 @@ __global__ __launch_bounds__(256) void gemm_f32_mfma_kernel(GemmF32Args p, const
         const long xi = f2(i0, nn + 2, p.xrow);
         x[xi] = x[xi] + d;
         p.out[(long)m * p.out_ld + nn] = d;
-        if (p.g) p.g[(long)m * p.out_ld + nn] = d * (double)p.cy[(long)nn + (long)(p.cyrow - 1) * p.i2];
+        if (p.g) {
+          const double gv = d * (double)p.cy[(long)nn + (long)(p.cyrow - 1) * p.i2];
+          p.g[(long)m * p.out_ld + nn] = gv;
+          p.g32[(long)m * p.out_ld + nn] = (float)gv;
+        }
         break;
       }
       default:
@@ -125,17 +143,23 @@ hipError_t launch_cvt_f32(const double *src, float *dst, long n, hipStream_t st)
   return hipGetLastError();
 }
 
-// partial: splits * M * N floats; counters: ceil(N/16) ints, zero on entry
-// and left zero on exit.
-hipError_t launch_gemm_mixed(const GemmF32Args &a, const double *a64, float *partial,
-                             int *counters, int splits, hipStream_t st) {
+// K is split over the wavefronts of one workgroup; the right-hand sides are
+// split over workgroups in 32-row groups so that a 64 x 1440 product spreads
+// over ~180 workgroups.
+hipError_t launch_gemm_mixed(const GemmF32Args &a, hipStream_t st) {
   if (a.M > MX_MAXM || a.M < 1) return hipErrorInvalidValue;
-  int kchunk = (a.K + splits - 1) / splits;
-  kchunk = (kchunk + 15) / 16 * 16;
+  constexpr int MBW = 2;
+  const int mgroups = (a.M + 16 * MBW - 1) / (16 * MBW);
   const int ntiles = (a.N + 15) / 16;
-  dim3 grid((ntiles + MX_WAVES - 1) / MX_WAVES, splits);
-  hipLaunchKernelGGL(gemm_f32_mfma_kernel, grid, dim3(64 * MX_WAVES), 0, st, a, a64, splits,
-                     kchunk, partial, counters);
+  const int ntiles8 = (ntiles + 7) / 8 * 8;  // whole groups of 8 for the XCD mapping
+  dim3 grid(ntiles8 * mgroups);
+  if (a.K >= 1024) {
+    const int kq = ((a.K + 15) / 16 + 15) / 16 * 16;
+    hipLaunchKernelGGL((gemm_f32_mfma_kernel<16, MBW>), grid, dim3(64 * 16), 0, st, a, kq, mgroups);
+  } else {
+    const int kq = ((a.K + 3) / 4 + 15) / 16 * 16;
+    hipLaunchKernelGGL((gemm_f32_mfma_kernel<4, MBW>), grid, dim3(64 * 4), 0, st, a, kq, mgroups);
+  }
   return hipGetLastError();
 }
 

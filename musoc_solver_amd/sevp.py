@@ -230,9 +230,11 @@ class SevpPlan:
         """Enqueue on torch's current stream so the solve is ordered after
         the torch work that produced its inputs."""
         import torch
+        # handle 0 is the null stream: the plan's own (blocking) stream is
+        # ordered with it; any other stream is used directly
         st = torch.cuda.current_stream().cuda_stream
         if st != self._stream:
-            _check(lib().sevp_set_stream(self._h, ctypes.c_void_p(st)))
+            _check(lib().sevp_set_stream(self._h, ctypes.c_void_p(st) if st else None))
             self._stream = st
 
     def solve_dev(self, s, x, nrhs=None):
