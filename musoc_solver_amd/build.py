@@ -91,9 +91,31 @@ def build_fortran(verbose: bool = False) -> str | None:
     return FLIB
 
 
+DRIVER_SRC = os.path.join(ROOT, "tests", "fortran", "dropin_driver.f90")
+DRIVER_EXE = os.path.join(ROOT, "tests", "fortran", "dropin_driver.exe")
+
+
+def build_fortran_driver(verbose: bool = False) -> str | None:
+    """The test program that calls the drop-in procedures as the reference's
+    callers do (tests/test_gpu_fortran.py runs it)."""
+    fc = _flang()
+    if fc is None or not os.path.exists(DRIVER_SRC) or not os.path.exists(FLIB):
+        return None
+    tmp = DRIVER_EXE + ".tmp"
+    rpath = os.path.relpath(LIBDIR, os.path.dirname(DRIVER_EXE))
+    cmd = [fc, "-O2", DRIVER_SRC, "-o", tmp, f"-L{LIBDIR}", "-lsevp_fortran", "-lsevp_hip",
+           f"-Wl,-rpath,$ORIGIN/{rpath}"]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd, cwd=os.path.dirname(DRIVER_EXE))
+    os.replace(tmp, DRIVER_EXE)
+    return DRIVER_EXE
+
+
 def build(verbose: bool = False) -> None:
     build_hip(verbose)
     build_fortran(verbose)
+    build_fortran_driver(verbose)
 
 
 if __name__ == "__main__":

@@ -293,8 +293,8 @@ hipError_t launch_resid(const ResidArgs &a, hipStream_t st) {
 //   successive subtractions)
 // One output per thread, so that the independent sequential chains fill the
 // GPU (a 64 x 1440 product has 92160 of them).  Tile 16 (m) x 16 (n), K in
-// chunks of 64 staged through LDS, the next chunk loaded into registers while
-// the current one is consumed.
+// chunks of 64 staged through LDS from a ring of register slots filled three
+// chunks ahead.
 constexpr int GT_M = 16, GT_N = 16, GT_K = 64;
 
 __global__ __launch_bounds__(256) void gemm_seq_kernel(GemmArgs p) {
@@ -339,30 +339,56 @@ __global__ __launch_bounds__(256) void gemm_seq_kernel(GemmArgs p) {
     }
   };
   double acc = 0.0;
-  fetch(0);
-  for (int k0 = 0; k0 < p.K; k0 += GT_K) {
-    const int kc = min(GT_K, p.K - k0);
-    __syncthreads();
-    stash();
-    __syncthreads();
-    if (k0 + GT_K < p.K) fetch(k0 + GT_K);
-    const double2 *ar = reinterpret_cast<const double2 *>(&As[tm][0]);
-    const double2 *br = reinterpret_cast<const double2 *>(&Bs[tn][0]);
-    const int kc2 = kc / 2;
-    if (neg) {
-      for (int k2 = 0; k2 < kc2; ++k2) {
-        const double2 av = ar[k2], bv = br[k2];
-        acc = acc - av.x * bv.x;
-        acc = acc - av.y * bv.y;
+  // chunks are staged in a ring of PF register slots, fetched PF chunks
+  // ahead of their use, so the global latency overlaps PF chunks of compute
+  constexpr int PF = 3;
+  double ra_r[PF][4], rb_r[PF][4];
+  const int nchunk = (p.K + GT_K - 1) / GT_K;
+  auto fetch_to = [&](int c, double (&ra_)[4], double (&rb_)[4]) {
+    fetch(c * GT_K);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      ra_[q] = ra[q];
+      rb_[q] = rb[q];
+    }
+  };
+#pragma unroll
+  for (int d = 0; d < PF; ++d)
+    if (d < nchunk) fetch_to(d, ra_r[d], rb_r[d]);
+  for (int c0 = 0; c0 < nchunk; c0 += PF) {
+#pragma unroll
+    for (int d = 0; d < PF; ++d) {
+      const int c = c0 + d;
+      if (c >= nchunk) break;
+      const int k0 = c * GT_K;
+      const int kc = min(GT_K, p.K - k0);
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        ra[q] = ra_r[d][q];
+        rb[q] = rb_r[d][q];
       }
-      if (kc & 1) acc = acc - As[tm][kc - 1] * Bs[tn][kc - 1];
-    } else {
-      for (int k2 = 0; k2 < kc2; ++k2) {
-        const double2 av = ar[k2], bv = br[k2];
-        acc = acc + av.x * bv.x;
-        acc = acc + av.y * bv.y;
+      stash();
+      __syncthreads();
+      if (c + PF < nchunk) fetch_to(c + PF, ra_r[d], rb_r[d]);
+      const double2 *ar = reinterpret_cast<const double2 *>(&As[tm][0]);
+      const double2 *br = reinterpret_cast<const double2 *>(&Bs[tn][0]);
+      const int kc2 = kc / 2;
+      if (neg) {
+        for (int k2 = 0; k2 < kc2; ++k2) {
+          const double2 av = ar[k2], bv = br[k2];
+          acc = acc - av.x * bv.x;
+          acc = acc - av.y * bv.y;
+        }
+        if (kc & 1) acc = acc - As[tm][kc - 1] * Bs[tn][kc - 1];
+      } else {
+        for (int k2 = 0; k2 < kc2; ++k2) {
+          const double2 av = ar[k2], bv = br[k2];
+          acc = acc + av.x * bv.x;
+          acc = acc + av.y * bv.y;
+        }
+        if (kc & 1) acc = acc + As[tm][kc - 1] * Bs[tn][kc - 1];
       }
-      if (kc & 1) acc = acc + As[tm][kc - 1] * Bs[tn][kc - 1];
     }
   }
   const int n = n0 + tn, m = m0 + tm;

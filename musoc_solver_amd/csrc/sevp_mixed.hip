@@ -40,7 +40,7 @@ __device__ __forceinline__ f32x4 load4(const float *base, int k, int k_hi, bool 
 // through LDS in wavefront order and apply the epilogue.  The workgroups that
 // share a column tile (and so the same RINV columns) get block indices equal
 // modulo 8, which places them on the same XCD and its L2.
-template <int NW, int MBW>
+template <int NW, int MBW, int STEPS>
 __global__ __launch_bounds__(64 * NW) void gemm_f32_mfma_kernel(GemmF32Args p, int kq, int mgroups) {
   __shared__ float red[NW][16 * MBW][17];
   const int lane = threadIdx.x & 63;
@@ -68,7 +68,6 @@ __global__ __launch_bounds__(64 * NW) void gemm_f32_mfma_kernel(GemmF32Args p, i
     mok[mb] = m < M;
     arow[mb] = p.a + (long)min(m, M - 1) * K;
   }
-  f32x4 bq, aq[MBW];
   auto fetch = [&](int k0, f32x4 &b_, f32x4 (&a_)[MBW]) {
     const int kb = k0 + 4 * q;
     b_ = (n < N) ? load4(bcol, kb, k_hi, vec) : f32x4{0.f, 0.f, 0.f, 0.f};
@@ -76,20 +75,36 @@ __global__ __launch_bounds__(64 * NW) void gemm_f32_mfma_kernel(GemmF32Args p, i
     for (int mb = 0; mb < MBW; ++mb)
       a_[mb] = mok[mb] ? load4(arow[mb], kb, k_hi, vec) : f32x4{0.f, 0.f, 0.f, 0.f};
   };
-  if (k_lo < k_hi) fetch(k_lo, bq, aq);
-  for (int k0 = k_lo; k0 < k_hi; k0 += 16) {
-    f32x4 bn = bq, an[MBW];
+  if (STEPS > 0) {
+    // the whole K slice (STEPS x 16) is loaded before the first MFMA, so the
+    // wavefront waits for memory once
+    f32x4 bq[STEPS > 0 ? STEPS : 1], aq[STEPS > 0 ? STEPS : 1][MBW];
 #pragma unroll
-    for (int mb = 0; mb < MBW; ++mb) an[mb] = aq[mb];
-    if (k0 + 16 < k_hi) fetch(k0 + 16, bn, an);
+    for (int st = 0; st < STEPS; ++st) fetch(k_lo + 16 * st, bq[st], aq[st]);
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+    for (int st = 0; st < STEPS; ++st)
 #pragma unroll
-      for (int mb = 0; mb < MBW; ++mb)
-        acc[mb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aq[mb][t], bq[t], acc[mb], 0, 0, 0);
-    bq = bn;
+      for (int t = 0; t < 4; ++t)
 #pragma unroll
-    for (int mb = 0; mb < MBW; ++mb) aq[mb] = an[mb];
+        for (int mb = 0; mb < MBW; ++mb)
+          acc[mb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aq[st][mb][t], bq[st][t], acc[mb], 0, 0, 0);
+  } else {
+    f32x4 bq, aq[MBW];
+    if (k_lo < k_hi) fetch(k_lo, bq, aq);
+    for (int k0 = k_lo; k0 < k_hi; k0 += 16) {
+      f32x4 bn = bq, an[MBW];
+#pragma unroll
+      for (int mb = 0; mb < MBW; ++mb) an[mb] = aq[mb];
+      if (k0 + 16 < k_hi) fetch(k0 + 16, bn, an);
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int mb = 0; mb < MBW; ++mb)
+          acc[mb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aq[mb][t], bq[t], acc[mb], 0, 0, 0);
+      bq = bn;
+#pragma unroll
+      for (int mb = 0; mb < MBW; ++mb) aq[mb] = an[mb];
+    }
   }
   // D block mb: lane holds rows mb*16 + 4q + i, column col
 #pragma unroll
@@ -155,10 +170,18 @@ hipError_t launch_gemm_mixed(const GemmF32Args &a, hipStream_t st) {
   dim3 grid(ntiles8 * mgroups);
   if (a.K >= 1024) {
     const int kq = ((a.K + 15) / 16 + 15) / 16 * 16;
-    hipLaunchKernelGGL((gemm_f32_mfma_kernel<16, MBW>), grid, dim3(64 * 16), 0, st, a, kq, mgroups);
+    const int steps = kq / 16;
+    if (steps <= 4)
+      hipLaunchKernelGGL((gemm_f32_mfma_kernel<16, MBW, 4>), grid, dim3(64 * 16), 0, st, a, kq, mgroups);
+    else if (steps <= 6)
+      hipLaunchKernelGGL((gemm_f32_mfma_kernel<16, MBW, 6>), grid, dim3(64 * 16), 0, st, a, kq, mgroups);
+    else if (steps <= 8)
+      hipLaunchKernelGGL((gemm_f32_mfma_kernel<16, MBW, 8>), grid, dim3(64 * 16), 0, st, a, kq, mgroups);
+    else
+      hipLaunchKernelGGL((gemm_f32_mfma_kernel<16, MBW, 0>), grid, dim3(64 * 16), 0, st, a, kq, mgroups);
   } else {
     const int kq = ((a.K + 3) / 4 + 15) / 16 * 16;
-    hipLaunchKernelGGL((gemm_f32_mfma_kernel<4, MBW>), grid, dim3(64 * 4), 0, st, a, kq, mgroups);
+    hipLaunchKernelGGL((gemm_f32_mfma_kernel<4, MBW, 0>), grid, dim3(64 * 4), 0, st, a, kq, mgroups);
   }
   return hipGetLastError();
 }
