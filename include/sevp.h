@@ -170,6 +170,11 @@ int sevp_matinv(double *b, int n);
  * solve is ordered after them. */
 int sevp_set_stream(sevp_plan plan, void *hip_stream);
 
+/* Replay each refinement pass as a captured HIP graph (default on); the
+ * graph is captured on first use for a batch size and precision.  Off: the
+ * pass's kernels are launched one by one. */
+int sevp_set_graphs(sevp_plan plan, int on);
+
 /* Switch the arithmetic of the correction products of an existing
  * full-width plan (SEVP_PREC_*); FP32 copies of the influence matrices are
  * made on first use.  Lets one influence-matrix build serve both the plain

@@ -71,9 +71,20 @@ struct KProf {
   double total_ms = 0.0;
   long launches = 0;
   hipStream_t st = nullptr;
+  // while a graph is captured: event pairs recorded into the graph
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> *capture_into = nullptr;
+  // events of the last graph replay, read at the next harvest
+  const std::vector<std::pair<hipEvent_t, hipEvent_t>> *pending = nullptr;
   bool on(int k) const { return kclass == k; }
   void start(int k) {
     if (!on(k)) return;
+    if (capture_into) {
+      hipEvent_t a, b;
+      if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+      capture_into->push_back({a, b});
+      (void)hipEventRecord(a, st);
+      return;
+    }
     if (used == pool.size()) {
       hipEvent_t a, b;
       if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
@@ -82,7 +93,12 @@ struct KProf {
     (void)hipEventRecord(pool[used].first, st);
   }
   void stop(int k) {
-    if (!on(k) || used >= pool.size()) return;
+    if (!on(k)) return;
+    if (capture_into) {
+      if (!capture_into->empty()) (void)hipEventRecord(capture_into->back().second, st);
+      return;
+    }
+    if (used >= pool.size()) return;
     (void)hipEventRecord(pool[used].second, st);
     ++used;
   }
@@ -96,6 +112,16 @@ struct KProf {
       }
     }
     used = 0;
+    if (pending) {
+      for (auto &e : *pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) {
+          total_ms += ms;
+          ++launches;
+        }
+      }
+      pending = nullptr;
+    }
   }
   ~KProf() {
     for (auto &e : pool) {
@@ -105,8 +131,20 @@ struct KProf {
   }
 };
 
+// One captured refinement pass (zero the correction, one full-width SEVP pass
+// of the FP64 residual) for a batch size and precision.
+struct PassGraph {
+  int nrhs = 0, prec = 0;
+  bool prof = false;
+  hipGraphExec_t exec = nullptr;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+};
+
 struct sevp_plan_s {
   KProf prof;
+  std::vector<PassGraph> graphs;
+  hipStream_t cap_stream = nullptr;
+  bool use_graphs = true;
   sevp_desc d{};
   std::vector<int> ie;
   int ibir = 0;  // strip width (BIR) or i2 (full)
@@ -486,6 +524,8 @@ int sevp_plan_create(const sevp_desc *desc, const float *al, const float *ab, co
   if (hipStreamCreateWithFlags(&p->own_stream, hipStreamDefault) != hipSuccess)
     return bail(fail(SEVP_E_HIP, "hipStreamCreate failed"));
   p->stream = p->own_stream;
+  if (hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(SEVP_E_HIP, "hipStreamCreate failed"));
   const size_t nc = (size_t)d.i2 * d.j2;
   if ((rc = p->coef.alloc(5 * nc))) return bail(rc);
   const float *src[5] = {al, ab, ac, ar, at};
@@ -525,9 +565,23 @@ int sevp_plan_create(const sevp_desc *desc, const float *al, const float *ab, co
   return SEVP_OK;
 }
 
+static void drop_graphs(sevp_plan p) {
+  for (auto &g : p->graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    for (auto &e : g.events) {
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
+  }
+  p->graphs.clear();
+  p->prof.pending = nullptr;
+}
+
 int sevp_plan_destroy(sevp_plan p) {
   if (!p) return SEVP_OK;
   if (p->stream) (void)hipStreamSynchronize(p->stream);
+  drop_graphs(p);
+  if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
   if (p->own_stream) (void)hipStreamDestroy(p->own_stream);
   delete p;
   return SEVP_OK;
@@ -735,6 +789,56 @@ int sevp_residual_dev(sevp_plan p, const float *s, const double *x, int nrhs, do
   return SEVP_OK;
 }
 
+// The captured pass for (nrhs, precision, profiling), captured on first use.
+static int get_pass_graph(sevp_plan p, const FullCtx &c, int nrhs, PassGraph **out) {
+  const bool prof = p->prof.kclass >= 0;
+  for (auto &g : p->graphs)
+    if (g.nrhs == nrhs && g.prec == p->d.precision && g.prof == prof) {
+      *out = &g;
+      return SEVP_OK;
+    }
+  const long ns = (long)p->d.i2 * p->d.j2, nx = (long)(p->d.i2 + 2) * (p->d.j2 + 2);
+  PassGraph g;
+  g.nrhs = nrhs;
+  g.prec = p->d.precision;
+  g.prof = prof;
+  hipStream_t cs = p->cap_stream;
+  HIPC(hipStreamSynchronize(p->stream));
+  HIPC(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+  hipStream_t saved = p->prof.st;
+  p->prof.st = cs;
+  if (prof) p->prof.capture_into = &g.events;
+  int rc = zero_async(p->dx.p, sizeof(double) * nx * nrhs, cs);
+  if (rc == SEVP_OK) rc = run_full_pass(c, nullptr, p->r64.p, ns, p->dx.p, nx, nrhs, cs);
+  p->prof.capture_into = nullptr;
+  p->prof.st = saved;
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(cs, &graph);
+  if (rc != SEVP_OK || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto &ev : g.events) {
+      (void)hipEventDestroy(ev.first);
+      (void)hipEventDestroy(ev.second);
+    }
+    if (rc != SEVP_OK) return rc;
+    HIPC(e);
+  }
+  const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HIPC(ei);
+  p->graphs.push_back(std::move(g));
+  *out = &p->graphs.back();
+  return SEVP_OK;
+}
+
+int sevp_set_graphs(sevp_plan p, int on) {
+  if (!p) return fail(SEVP_E_ARG, "null plan");
+  HIPC(hipStreamSynchronize(p->stream));
+  p->use_graphs = on != 0;
+  if (!p->use_graphs) drop_graphs(p);
+  return SEVP_OK;
+}
+
 int sevp_solve_refined_dev(sevp_plan p, const float *s, double *x, int nrhs, double tol,
                            int max_iter, double *rel_out, sevp_stats *stats) {
   if (!p || !s || !x) return fail(SEVP_E_ARG, "null argument");
@@ -745,6 +849,8 @@ int sevp_solve_refined_dev(sevp_plan p, const float *s, double *x, int nrhs, dou
   const int i2 = p->d.i2, j2 = p->d.j2;
   const long ns = (long)i2 * j2, nx = (long)(i2 + 2) * (j2 + 2);
   FullCtx c = full_ctx(p, p->d.precision != SEVP_PREC_FP64);
+  PassGraph *graph = nullptr;
+  if (p->use_graphs) CHK(get_pass_graph(p, c, nrhs, &graph));
   std::vector<double> rel;
   int it = 0;
   double worst = -1.0;
@@ -753,8 +859,13 @@ int sevp_solve_refined_dev(sevp_plan p, const float *s, double *x, int nrhs, dou
     worst = 0.0;
     for (double v : rel) worst = std::max(worst, v);
     if (!(worst > tol) || it >= max_iter) break;
-    CHK(zero_async(p->dx.p, sizeof(double) * nx * nrhs, p->stream));
-    CHK(run_full_pass(c, nullptr, p->r64.p, ns, p->dx.p, nx, nrhs, p->stream));
+    if (graph) {
+      HIPC(hipGraphLaunch(graph->exec, p->stream));
+      if (graph->prof) p->prof.pending = &graph->events;
+    } else {
+      CHK(zero_async(p->dx.p, sizeof(double) * nx * nrhs, p->stream));
+      CHK(run_full_pass(c, nullptr, p->r64.p, ns, p->dx.p, nx, nrhs, p->stream));
+    }
     const long n = ns * nrhs;
     hipLaunchKernelGGL(axpy_interior_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                        p->stream, x, nx, p->dx.p, nx, i2, j2, nrhs);

@@ -94,6 +94,7 @@ def lib():
         L.sevp_matinv.argtypes = [VP, CI]
         L.sevp_set_precision.argtypes = [VP, CI]
         L.sevp_set_stream.argtypes = [VP, VP]
+        L.sevp_set_graphs.argtypes = [VP, CI]
         L.sevp_profile_begin.argtypes = [VP, CI]
         L.sevp_profile_end.argtypes = [VP, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_long)]
@@ -206,6 +207,9 @@ class SevpPlan:
     def set_precision(self, precision):
         _check(lib().sevp_set_precision(self._h, precision))
         self.precision = precision
+
+    def set_graphs(self, on: bool):
+        _check(lib().sevp_set_graphs(self._h, 1 if on else 0))
 
     def profile_begin(self, kclass):
         _check(lib().sevp_profile_begin(self._h, kclass))

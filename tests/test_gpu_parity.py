@@ -287,3 +287,26 @@ def test_bir_seam_fix_bitwise(oracle, device):
         oracle.bir_solve(coefs, s[:, :, r], r_o, r1_o, xo, xx, ie, nbir, nsweep=3, seam_fix=True)
         assert fields_equal(xg[:, :, r], xo)
     plan.close()
+
+
+@pytest.mark.parametrize("precision", [S.PREC_FP64, S.PREC_FP32])
+def test_graph_replays_match_eager_launches(oracle, device, precision):
+    """The refinement pass replayed as a captured HIP graph gives bitwise the
+    result of launching its kernels one by one, on every replay (each solve
+    replays the graph several times with eager residual kernels between)."""
+    import torch
+    i2, j2, n0, nrhs = 96, 48, 3, 4
+    print(f"[graph] precision={precision}", flush=True)
+    plan, coefs, ie, s, s_d, x_d = _device_case(i2, j2, n0, nrhs, precision)
+    plan.set_graphs(False)
+    rel_e, st_e, ok_e = plan.solve_refined_dev(s_d, x_d, tol=1e-12, max_iter=6)
+    x_eager = x_d.clone()
+    plan.set_graphs(True)
+    for rep in range(3):
+        x_d.zero_()
+        rel_g, st_g, ok_g = plan.solve_refined_dev(s_d, x_d, tol=1e-12, max_iter=6)
+        torch.cuda.synchronize()
+        assert st_g.iterations == st_e.iterations
+        assert torch.equal(x_d, x_eager), f"replay {rep}"
+        assert np.array_equal(rel_g, rel_e)
+    plan.close()
