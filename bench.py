@@ -12,7 +12,7 @@ GPU):
              mask (30% land), log-uniform 50-5500 m depth smoothed 3 times,
              land kept at 50 m, free-surface term 1/(0.5 g dt^2), dt = 1800 s
   solver     full-width periodic SEVP (REP1/PRE1 form), block rows of height
-             3 (NB0 = 240; --n0 / --n0-mixed), FP64 iterative refinement
+             2 (NB0 = 360; --n0 / --n0-mixed), FP64 iterative refinement
   batch      64 N(0,1) right-hand sides, zero mean over the ocean
 
 Legs
@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--i2", type=int, default=1440)
     ap.add_argument("--j2", type=int, default=720)
-    ap.add_argument("--n0", type=int, default=3, help="block height (rows per block), plain leg")
+    ap.add_argument("--n0", type=int, default=2, help="block height (rows per block), plain leg")
     ap.add_argument("--n0-mixed", type=int, default=None,
                     help="block height of the mixed-precision leg (default: --n0)")
     ap.add_argument("--nrhs", type=int, default=64)
@@ -64,8 +64,21 @@ def parse():
     ap.add_argument("--cpu-rhs", type=int, default=3, help="RHS in the CPU baseline sample (0: skip)")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--legs", default="mixed,plain", help="comma list of GPU legs to time")
-    ap.add_argument("--log", default=None, help="progress log file (default stderr)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device-to-device (reference kernel for HBM counters)")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """One device-to-device copy of exactly 2**30 bytes: FETCH_SIZE and
+    WRITE_SIZE of this kernel scale the counters of the measured kernels."""
+    import torch
+    a = torch.ones(2 ** 27, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": 2 ** 30, "ok": bool(b[-1].item() == 1.0)}))
 
 
 def log(msg):
@@ -74,6 +87,9 @@ def log(msg):
 
 def main():
     a = parse()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1",
                           "n_gpus": a.gpus}))

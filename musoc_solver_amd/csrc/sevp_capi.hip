@@ -157,6 +157,7 @@ struct sevp_plan_s {
   DBuf<int> ie_d;
   DBuf<double> rinv, rinv1;      // reference layout
   DBuf<float> rinv32, rinv1_32;  // mixed precision copies
+  DBuf<float> w32;               // mixed: W_b = RINV_b diag(CY(:,IE(b-1)-1)) RINV1_(b-1), b >= 2
   // solve scratch
   DBuf<double> e, dum0, hfree, hcpl, g;  // full forms
   DBuf<float> e32, g32;                  // mixed: FP32 copies of the error rows
@@ -247,6 +248,7 @@ struct FullCtx {
   const int *ie;  // host
   const double *rinv, *rinv1;
   const float *rinv32, *rinv1_32;  // non-null selects the FP32 MFMA corrections
+  const float *w32;                // mixed: coupling operators W_b (I2 x I2, b = 2..NB0)
   double *e, *dum0, *hfree, *hcpl, *g;
   float *e32, *g32;  // FP32 copies of e and g (mixed precision), else null
   KProf *prof;
@@ -334,13 +336,32 @@ static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, 
     if (c.prof) c.prof->start(SEVP_KCLASS_RESID);
     HIPC(launch_resid(r, st));
     if (c.prof) c.prof->stop(SEVP_KCLASS_RESID);
-    CHK(corr(c.e, c.e32, c.rinv + (long)(nb - 1) * ww,
-             mixed ? c.rinv32 + (long)(nb - 1) * ww : nullptr, EP_BWD, js + 1, nullptr, c.hfree,
-             nb != 1 ? c.g : nullptr, js - 1));
-    if (nb != 1)
-      CHK(corr(c.g, c.g32, c.rinv1 + (long)(nb - 2) * ww,
-               mixed ? c.rinv1_32 + (long)(nb - 2) * ww : nullptr, EP_CPL, 0, nullptr, c.hcpl,
-               nullptr, 0));
+    if (mixed) {
+      // one product against [RINV_b | W_b]: the free-row correction and, for
+      // b > 1, the coupling row at once
+      GemmF32Args gm{};
+      gm.a = c.e32;
+      gm.b = c.rinv32 + (long)(nb - 1) * ww;
+      gm.M = nrhs; gm.K = c.i2;
+      gm.N = (nb != 1) ? 2 * c.i2 : c.i2;
+      gm.b2 = (nb != 1) ? c.w32 + (long)(nb - 2) * ww : nullptr;
+      gm.nsplit = c.i2;
+      gm.out2 = c.hcpl;
+      gm.ep = EP_BWD;
+      gm.x = x; gm.x_stride = x_stride; gm.xrow = js + 1;
+      gm.out = c.hfree; gm.out_ld = c.i2;
+      gm.g = nullptr;
+      gm.i2 = c.i2;
+      if (c.prof) c.prof->start(SEVP_KCLASS_CORR);
+      HIPC(launch_gemm_mixed(gm, st));
+      if (c.prof) c.prof->stop(SEVP_KCLASS_CORR);
+    } else {
+      CHK(corr(c.e, c.e32, c.rinv + (long)(nb - 1) * ww, nullptr, EP_BWD, js + 1, nullptr,
+               c.hfree, nb != 1 ? c.g : nullptr, js - 1));
+      if (nb != 1)
+        CHK(corr(c.g, c.g32, c.rinv1 + (long)(nb - 2) * ww, nullptr, EP_CPL, 0, nullptr, c.hcpl,
+                 nullptr, 0));
+    }
     if (ieb - 2 >= js) {
       MarchArgs m{};
       m.ax = c.ax; m.ay = c.ay; m.bb = c.bb; m.cx = c.cx; m.cy = c.cy;
@@ -598,7 +619,7 @@ int sevp_set_stream(sevp_plan p, void *stream) {
 size_t sevp_plan_device_bytes(sevp_plan p) {
   if (!p) return 0;
   return p->coef.bytes() + p->ie_d.bytes() + p->rinv.bytes() + p->rinv1.bytes() +
-         p->rinv32.bytes() + p->rinv1_32.bytes() + p->e.bytes() + p->dum0.bytes() +
+         p->rinv32.bytes() + p->rinv1_32.bytes() + p->w32.bytes() + p->e.bytes() + p->dum0.bytes() +
          p->hfree.bytes() + p->hcpl.bytes() + p->g.bytes() + p->e32.bytes() +
          p->g32.bytes() + p->xx.bytes() +
          p->bdum0.bytes() + p->r64.bytes() + p->dx.bytes() + p->part.bytes() + p->b1s.bytes() +
@@ -609,6 +630,28 @@ static int make_low_precision_copies(sevp_plan p) {
   if (p->d.precision == SEVP_PREC_FP64 || p->d.form == SEVP_FORM_BIR) return SEVP_OK;
   HIPC(launch_cvt_f32(p->rinv.p, p->rinv32.p, (long)p->rinv.n, p->stream));
   HIPC(launch_cvt_f32(p->rinv1.p, p->rinv1_32.p, (long)p->rinv1.n, p->stream));
+  // W_b = RINV_b diag(CY(:, IE(b-1)-1)) RINV1_(b-1): the coupling row of the
+  // backward pass as one product with the error row (FP64, then FP32)
+  const int i2 = p->d.i2;
+  const long ww = (long)i2 * i2;
+  if (p->d.nb0 > 1) {
+    DBuf<double> t, w;
+    CHK(t.alloc(ww));
+    CHK(w.alloc(ww));
+    for (int nb = 2; nb <= p->d.nb0; ++nb) {
+      const int js = p->ie[nb - 2];
+      HIPC(launch_scale_cols(p->rinv.p + (long)(nb - 1) * ww, p->at(), js - 1, i2, t.p, p->stream));
+      GemmArgs gm{};
+      gm.a = t.p; gm.a_sm = 1; gm.a_sk = i2;
+      gm.b = p->rinv1.p + (long)(nb - 2) * ww; gm.b_sk = 1; gm.b_sn = i2;
+      gm.M = i2; gm.N = i2; gm.K = i2;
+      gm.ep = EP_COLMAJ;
+      gm.out = w.p; gm.out_ld = i2;
+      HIPC(launch_gemm(gm, p->stream));
+      HIPC(launch_cvt_f32(w.p, p->w32.p + (long)(nb - 2) * ww, ww, p->stream));
+    }
+    HIPC(hipStreamSynchronize(p->stream));
+  }
   return SEVP_OK;
 }
 
@@ -680,6 +723,7 @@ static FullCtx full_ctx(sevp_plan p, bool mixed) {
   if (mixed) {
     c.rinv32 = p->rinv32.p;
     c.rinv1_32 = p->rinv1_32.p;
+    c.w32 = p->w32.p;
   }
   c.e = p->e.p; c.dum0 = p->dum0.p; c.hfree = p->hfree.p; c.hcpl = p->hcpl.p; c.g = p->g.p;
   c.e32 = mixed ? p->e32.p : nullptr;
@@ -696,6 +740,7 @@ static int alloc_mixed(sevp_plan p) {
   CHK(p->g32.alloc((size_t)nr * p->d.i2));
   CHK(p->rinv32.alloc(p->rinv_count()));
   CHK(p->rinv1_32.alloc(p->rinv1_count()));
+  CHK(p->w32.alloc(p->rinv1_count()));
   return SEVP_OK;
 }
 

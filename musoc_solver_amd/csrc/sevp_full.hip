@@ -305,7 +305,6 @@ __global__ __launch_bounds__(256) void gemm_seq_kernel(GemmArgs p) {
   const int t = threadIdx.x;
   const int n0 = blockIdx.x * GT_N, m0 = blockIdx.y * GT_M;
   const int tn = t % GT_N, tm = t / GT_N;
-  const bool neg = (p.ep == EP_NEGSUB);
   // staging: each thread moves 4 elements of A and 4 of B per chunk
   double ra[4], rb[4];
   auto fetch = [&](int k0) {
@@ -338,6 +337,7 @@ __global__ __launch_bounds__(256) void gemm_seq_kernel(GemmArgs p) {
       Bs[nnb][kkb] = rb[q];
     }
   };
+  const bool neg = (p.ep == EP_NEGSUB);
   double acc = 0.0;
   // chunks are staged in a ring of PF register slots, fetched PF chunks
   // ahead of their use, so the global latency overlaps PF chunks of compute
@@ -415,10 +415,27 @@ __global__ __launch_bounds__(256) void gemm_seq_kernel(GemmArgs p) {
     case EP_CPL:
       p.out[(long)m * p.out_ld + n] = d;
       break;
-    default:  // EP_NEGSUB: column-major output (m fastest)
+    default:  // EP_NEGSUB, EP_COLMAJ: column-major output (m fastest)
       p.out[(long)m + (long)n * p.out_ld] = d;
       break;
   }
+}
+
+// T(i, k) = R(i, k) * CY(k, cyrow): columns of an n x n column-major matrix
+// scaled by one row of the north coefficient.
+__global__ void scale_cols_kernel(const double *r, const float *cy, int cyrow, int i2, double *t) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)i2 * i2) return;
+  const int k = (int)(q / i2);
+  t[q] = r[q] * (double)cy[(long)k + (long)(cyrow - 1) * i2];
+}
+
+hipError_t launch_scale_cols(const double *r, const float *cy, int cyrow, int i2, double *t,
+                             hipStream_t st) {
+  const long n = (long)i2 * i2;
+  hipLaunchKernelGGL(scale_cols_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, r, cy,
+                     cyrow, i2, t);
+  return hipGetLastError();
 }
 
 hipError_t launch_gemm(const GemmArgs &a, hipStream_t st) {

@@ -88,7 +88,7 @@ struct ResidArgs {  // backward restore + closure residual, one thread per (rhs,
   int i2, j2, periodic, nb, ieb, nrhs;
 };
 
-enum { EP_FWD = 0, EP_BWD = 1, EP_CPL = 2, EP_NEGSUB = 3 };
+enum { EP_FWD = 0, EP_BWD = 1, EP_CPL = 2, EP_NEGSUB = 3, EP_COLMAJ = 4 };
 
 struct GemmArgs {  // D(m, n) = sum_k A(m, k) B(k, n), sequential k
   const double *a;
@@ -114,6 +114,11 @@ struct GemmArgs {  // D(m, n) = sum_k A(m, k) B(k, n), sequential k
 struct GemmF32Args {  // mixed precision: A x B in FP32, FP32 accumulate
   const float *a;     // (M, K) row-major, ld K
   const float *b;     // (K, N) col-major, ld K
+  // optional second operand: columns n >= nsplit come from b2 (col-major,
+  // ld K, column n - nsplit) and are written to out2 (M, nsplit) as they are
+  const float *b2;
+  int nsplit;
+  double *out2;
   int M, N, K;
   int ep;
   double *x;
@@ -143,6 +148,8 @@ bool matinv_uses_lds(int n);
 size_t matinv_blocked_scratch(int n, int count);
 hipError_t launch_matinv_blocked(double *mats, long ms, int n, int count, int *zp,
                                  double *scratch, hipStream_t st);
+hipError_t launch_scale_cols(const double *r, const float *cy, int cyrow, int i2, double *t,
+                             hipStream_t st);
 hipError_t launch_halo_fix(double *x, long x_stride, int i2, int j2, int nrhs, hipStream_t st);
 
 }  // namespace sevp

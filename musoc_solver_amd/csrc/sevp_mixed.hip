@@ -59,7 +59,8 @@ __global__ __launch_bounds__(64 * NW) void gemm_f32_mfma_kernel(GemmF32Args p, i
   f32x4 acc[MBW];
 #pragma unroll
   for (int mb = 0; mb < MBW; ++mb) acc[mb] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const float *bcol = p.b + (long)min(n, N - 1) * K;
+  const int nc = min(n, N - 1);
+  const float *bcol = (p.b2 && nc >= p.nsplit) ? p.b2 + (long)(nc - p.nsplit) * K : p.b + (long)nc * K;
   const float *arow[MBW];
   bool mok[MBW];
 #pragma unroll
@@ -120,6 +121,10 @@ __global__ __launch_bounds__(64 * NW) void gemm_f32_mfma_kernel(GemmF32Args p, i
 #pragma unroll
     for (int w = 0; w < NW; ++w) sum += red[w][ml][c];
     const double d = (double)sum;
+    if (p.b2 && nn >= p.nsplit) {
+      p.out2[(long)m * p.nsplit + (nn - p.nsplit)] = d;
+      continue;
+    }
     switch (p.ep) {
       case EP_FWD: {
         double *x = p.x + (long)m * p.x_stride;
