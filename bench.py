@@ -64,6 +64,10 @@ def parse():
     ap.add_argument("--cpu-rhs", type=int, default=3, help="RHS in the CPU baseline sample (0: skip)")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--legs", default="mixed,plain", help="comma list of GPU legs to time")
+    ap.add_argument("--kernel-timing", choices=("after", "timed"), default="after",
+                    help="time the correction kernel with HIP events in one extra solve after "
+                         "the timed steps (after) or inside them (timed: event nodes in the "
+                         "captured pass lengthen the timed region)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device (reference kernel for HBM counters)")
     return ap.parse_args()
@@ -124,7 +128,8 @@ def main():
             x.zero_()
             plan.solve_refined_dev(s, x, tol=a.tol, max_iter=a.max_iter)
         torch.cuda.synchronize()
-        plan.profile_begin(S.KCLASS_CORR)
+        if a.kernel_timing == "timed":
+            plan.profile_begin(S.KCLASS_CORR)
         iters, worst, fails = [], 0.0, 0
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -136,21 +141,22 @@ def main():
             fails += 0 if ok else 1
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        corr_ms, corr_n = plan.profile_end()
-        if corr_n == 0:
-            # event nodes did not record inside the graph: time the same
-            # solves once more with eager launches for the kernel duration
-            plan.set_graphs(False)
+        if a.kernel_timing == "timed":
+            corr_ms, corr_n = plan.profile_end()
+        else:
+            # the same solve once more, its correction products bracketed by
+            # HIP events (event-record nodes in the captured pass)
             plan.profile_begin(S.KCLASS_CORR)
             x.zero_()
             plan.solve_refined_dev(s, x, tol=a.tol, max_iter=a.max_iter)
             corr_ms, corr_n = plan.profile_end()
-            plan.set_graphs(True)
         r = {"solves_per_s": a.steps * nrhs / dt, "ms_per_step": 1e3 * dt / a.steps,
              "block_height": n0_of[name], "nb0": plan.nb0,
              "passes": float(np.mean(iters)), "max_rel_resid": worst, "failed_steps": fails,
              "corr_avg_us": 1e3 * corr_ms / max(corr_n, 1), "corr_launches": corr_n,
-             "corr_share": corr_ms / (1e3 * dt)}
+             "corr_share": (corr_ms * (a.steps if a.kernel_timing == "after" else 1)
+                            / (1e3 * dt)),
+             "kernel_timing": a.kernel_timing}
         log(f"{name}: {r}")
         return r
 

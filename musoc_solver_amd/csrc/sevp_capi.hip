@@ -71,20 +71,15 @@ struct KProf {
   double total_ms = 0.0;
   long launches = 0;
   hipStream_t st = nullptr;
-  // while a graph is captured: event pairs recorded into the graph
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> *capture_into = nullptr;
+  // while a graph is captured: the kernel nodes of the profiled class, which
+  // get event-record nodes around them once the capture ends (recording an
+  // event on a capturing stream only orders work, it adds no timing node)
+  std::vector<hipGraphNode_t> *capture_into = nullptr;
   // events of the last graph replay, read at the next harvest
   const std::vector<std::pair<hipEvent_t, hipEvent_t>> *pending = nullptr;
   bool on(int k) const { return kclass == k; }
   void start(int k) {
-    if (!on(k)) return;
-    if (capture_into) {
-      hipEvent_t a, b;
-      if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
-      capture_into->push_back({a, b});
-      (void)hipEventRecord(a, st);
-      return;
-    }
+    if (!on(k) || capture_into) return;
     if (used == pool.size()) {
       hipEvent_t a, b;
       if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
@@ -95,7 +90,13 @@ struct KProf {
   void stop(int k) {
     if (!on(k)) return;
     if (capture_into) {
-      if (!capture_into->empty()) (void)hipEventRecord(capture_into->back().second, st);
+      // the node just captured is the stream's only current dependency
+      hipStreamCaptureStatus cs;
+      const hipGraphNode_t *deps = nullptr;
+      size_t ndeps = 0;
+      if (hipStreamGetCaptureInfo_v2(st, &cs, nullptr, nullptr, &deps, &ndeps) == hipSuccess &&
+          cs == hipStreamCaptureStatusActive && ndeps == 1)
+        capture_into->push_back(deps[0]);
       return;
     }
     if (used >= pool.size()) return;
@@ -122,6 +123,7 @@ struct KProf {
       }
       pending = nullptr;
     }
+    (void)hipGetLastError();  // a pair that never ran must not fail a later launch check
   }
   ~KProf() {
     for (auto &e : pool) {
@@ -161,6 +163,7 @@ struct sevp_plan_s {
   // solve scratch
   DBuf<double> e, dum0, hfree, hcpl, g;  // full forms
   DBuf<float> e32, g32;                  // mixed: FP32 copies of the error rows
+  DBuf<float> kpart;                     // mixed: split-K partial products (S, nrhs, N)
   DBuf<double> xx, bdum0;                // BIR
   DBuf<double> r64, dx, part;            // refinement
   DBuf<double> b1s;                      // MATINV scratch (blocked path)
@@ -251,11 +254,118 @@ struct FullCtx {
   const float *w32;                // mixed: coupling operators W_b (I2 x I2, b = 2..NB0)
   double *e, *dum0, *hfree, *hcpl, *g;
   float *e32, *g32;  // FP32 copies of e and g (mixed precision), else null
+  float *kpart;      // mixed: split-K partial products
   KProf *prof;
 };
 
+// One REP1 / REP pass with the corrections as split-K FP32 MFMA products.
+// The forward march of block b reads the previous block's correction of row
+// IE(b-1) straight from the partial products (the X array keeps the
+// uncorrected value, which is what the backward pass restores anyway); the
+// backward march of block b reads its free-row and coupling corrections the
+// same way, and the launch that carries it adds the free-row correction to
+// X(JS+1) and forms the closure residual of block b-1, the only reader of
+// that row in the launch.
+static int run_mixed_pass(const FullCtx &c, const float *f32, const double *f64, long f_stride,
+                          double *x, long x_stride, int nrhs, hipStream_t st) {
+  const long ww = (long)c.i2 * c.i2;
+  int ks = 0;
+  const int ns1 = gemm_part_slices(c.i2, c.i2, &ks);
+  const int ns2 = gemm_part_slices(2 * c.i2, c.i2, &ks);
+  auto part_of = [&](int n_prod) {
+    PartRef r{};
+    r.p = c.kpart;
+    r.ld = n_prod;
+    r.sstride = (long)nrhs * n_prod;
+    r.ns = (n_prod == c.i2) ? ns1 : ns2;
+    return r;
+  };
+  auto product = [&](const float *b, const float *b2, int n_prod) -> int {
+    GemmPartArgs g{};
+    g.a = c.e32;
+    g.b = b;
+    g.b2 = b2;
+    g.nsplit = c.i2;
+    g.M = nrhs; g.N = n_prod; g.K = c.i2;
+    g.part = c.kpart;
+    if (c.prof) c.prof->start(SEVP_KCLASS_CORR);
+    HIPC(launch_gemm_part(g, st));
+    if (c.prof) c.prof->stop(SEVP_KCLASS_CORR);
+    return SEVP_OK;
+  };
+  int js = 1;
+  for (int nb = 1; nb <= c.nb0; ++nb) {
+    const int ieb = c.ie[nb - 1];
+    MarchArgs m{};
+    m.ax = c.ax; m.ay = c.ay; m.bb = c.bb; m.cx = c.cx; m.cy = c.cy;
+    m.cld = c.i2;
+    m.f = f32; m.f64 = f64; m.f_stride = f_stride;
+    m.x = x; m.x_stride = x_stride;
+    m.e = nullptr;
+    m.e32 = c.e32;
+    m.i2 = c.i2; m.j2 = c.j2; m.periodic = c.periodic;
+    m.js = js; m.jf = ieb - 2; m.ieb = ieb;
+    m.resid = (nb != c.nb0);
+    if (nb != 1) m.corr = part_of(c.i2);
+    m.cpl_off = -1;
+    m.w = c.i2;
+    m.map = ColMap{1, 0, 0};
+    m.nrhs = nrhs;
+    if (c.prof) c.prof->start(SEVP_KCLASS_MARCH);
+    HIPC(launch_march(MODE_PART, m, nrhs, st));
+    if (c.prof) c.prof->stop(SEVP_KCLASS_MARCH);
+    if (nb != c.nb0) CHK(product(c.rinv1_32 + (long)(nb - 1) * ww, nullptr, c.i2));
+    js = ieb;
+  }
+  auto resid_args = [&](int nb) {
+    ResidArgs r{};
+    r.ax = c.ax; r.ay = c.ay; r.bb = c.bb; r.cx = c.cx; r.cy = c.cy;
+    r.f = f32; r.f64 = f64; r.f_stride = f_stride;
+    r.x = x; r.x_stride = x_stride;
+    r.e32 = c.e32;
+    r.i2 = c.i2; r.j2 = c.j2; r.periodic = c.periodic; r.nb = nb; r.ieb = c.ie[nb - 1];
+    r.nrhs = nrhs;
+    return r;
+  };
+  {
+    const ResidArgs r = resid_args(c.nb0);
+    if (c.prof) c.prof->start(SEVP_KCLASS_RESID);
+    HIPC(launch_resid(r, st));
+    if (c.prof) c.prof->stop(SEVP_KCLASS_RESID);
+  }
+  for (int nb = c.nb0; nb >= 1; --nb) {
+    const int ieb = c.ie[nb - 1];
+    js = (nb != 1) ? c.ie[nb - 2] : 1;
+    // [RINV_b | W_b]: free-row correction and, for b > 1, the coupling row
+    const int n_prod = (nb != 1) ? 2 * c.i2 : c.i2;
+    CHK(product(c.rinv32 + (long)(nb - 1) * ww, nb != 1 ? c.w32 + (long)(nb - 2) * ww : nullptr,
+                n_prod));
+    MarchArgs m{};
+    m.ax = c.ax; m.ay = c.ay; m.bb = c.bb; m.cx = c.cx; m.cy = c.cy;
+    m.cld = c.i2;
+    m.x = x; m.x_stride = x_stride;
+    m.corr = part_of(n_prod);
+    m.cpl_off = (nb != 1) ? c.i2 : -1;
+    m.i2 = c.i2; m.j2 = c.j2; m.periodic = c.periodic;
+    m.js = js; m.jf = ieb - 2; m.ieb = ieb;
+    m.w = c.i2;
+    m.map = ColMap{1, 0, 0};
+    m.nrhs = nrhs;
+    ResidArgs rb = (nb != 1) ? resid_args(nb - 1) : resid_args(1);
+    rb.corr = part_of(n_prod);
+    rb.apply_row = js + 1;
+    rb.do_resid = (nb != 1);
+    if (c.prof) c.prof->start(SEVP_KCLASS_MARCH);
+    HIPC(launch_hom_resid(m, &rb, st));
+    if (c.prof) c.prof->stop(SEVP_KCLASS_MARCH);
+  }
+  if (c.periodic) HIPC(launch_halo_fix(x, x_stride, c.i2, c.j2, nrhs, st));
+  return SEVP_OK;
+}
+
 static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, long f_stride,
                          double *x, long x_stride, int nrhs, hipStream_t st) {
+  if (c.kpart) return run_mixed_pass(c, f32, f64, f_stride, x, x_stride, nrhs, st);
   const long ww = (long)c.i2 * c.i2;
   const int i0 = c.i2 + 2, j0 = c.j2 + 2;
   (void)j0;
@@ -320,10 +430,10 @@ static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, 
                c.dum0 + (long)(nb - 1) * c.i2, nullptr, nullptr, 0));
     js = ieb;
   }
-  for (int nbs = 1; nbs <= c.nb0; ++nbs) {
-    const int nb = c.nb0 - nbs + 1;
-    const int ieb = c.ie[nb - 1];
-    js = (nb != 1) ? c.ie[nb - 2] : 1;
+  // backward: restore + closure residual of the top block, then per block the
+  // corrections and one launch for its H march fused with the next block
+  // down's restore + residual
+  auto resid_args = [&](int nb) {
     ResidArgs r{};
     r.ax = c.ax; r.ay = c.ay; r.bb = c.bb; r.cx = c.cx; r.cy = c.cy;
     r.f = f32; r.f64 = f64; r.f_stride = f_stride;
@@ -332,10 +442,20 @@ static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, 
     r.dum0_stride = dstride;
     r.e = c.e;
     r.e32 = mixed ? c.e32 : nullptr;
-    r.i2 = c.i2; r.j2 = c.j2; r.periodic = c.periodic; r.nb = nb; r.ieb = ieb; r.nrhs = nrhs;
+    r.i2 = c.i2; r.j2 = c.j2; r.periodic = c.periodic; r.nb = nb; r.ieb = c.ie[nb - 1];
+    r.nrhs = nrhs;
+    return r;
+  };
+  {
+    const ResidArgs r = resid_args(c.nb0);
     if (c.prof) c.prof->start(SEVP_KCLASS_RESID);
     HIPC(launch_resid(r, st));
     if (c.prof) c.prof->stop(SEVP_KCLASS_RESID);
+  }
+  for (int nbs = 1; nbs <= c.nb0; ++nbs) {
+    const int nb = c.nb0 - nbs + 1;
+    const int ieb = c.ie[nb - 1];
+    js = (nb != 1) ? c.ie[nb - 2] : 1;
     if (mixed) {
       // one product against [RINV_b | W_b]: the free-row correction and, for
       // b > 1, the coupling row at once
@@ -362,22 +482,22 @@ static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, 
         CHK(corr(c.g, c.g32, c.rinv1 + (long)(nb - 2) * ww, nullptr, EP_CPL, 0, nullptr, c.hcpl,
                  nullptr, 0));
     }
-    if (ieb - 2 >= js) {
-      MarchArgs m{};
-      m.ax = c.ax; m.ay = c.ay; m.bb = c.bb; m.cx = c.cx; m.cy = c.cy;
-      m.cld = c.i2;
-      m.x = x; m.x_stride = x_stride;
-      m.hcpl = (nb != 1) ? c.hcpl : nullptr;
-      m.hfree = c.hfree;
-      m.i2 = c.i2; m.j2 = c.j2; m.periodic = c.periodic;
-      m.js = js; m.jf = ieb - 2; m.ieb = ieb;
-      m.w = c.i2;
-      m.map = ColMap{1, 0, 0};
-      m.nrhs = nrhs;
-      if (c.prof) c.prof->start(SEVP_KCLASS_MARCH);
-      HIPC(launch_march(MODE_HOM, m, nrhs, st));
-      if (c.prof) c.prof->stop(SEVP_KCLASS_MARCH);
-    }
+    MarchArgs m{};
+    m.ax = c.ax; m.ay = c.ay; m.bb = c.bb; m.cx = c.cx; m.cy = c.cy;
+    m.cld = c.i2;
+    m.x = x; m.x_stride = x_stride;
+    m.hcpl = (nb != 1) ? c.hcpl : nullptr;
+    m.hfree = c.hfree;
+    m.i2 = c.i2; m.j2 = c.j2; m.periodic = c.periodic;
+    m.js = js; m.jf = ieb - 2; m.ieb = ieb;
+    m.w = c.i2;
+    m.map = ColMap{1, 0, 0};
+    m.nrhs = nrhs;
+    ResidArgs rb{};
+    if (nb != 1) rb = resid_args(nb - 1);
+    if (c.prof) c.prof->start(SEVP_KCLASS_MARCH);
+    HIPC(launch_hom_resid(m, nb != 1 ? &rb : nullptr, st));
+    if (c.prof) c.prof->stop(SEVP_KCLASS_MARCH);
   }
   if (c.periodic) HIPC(launch_halo_fix(x, x_stride, c.i2, c.j2, nrhs, st));
   return SEVP_OK;
@@ -621,7 +741,7 @@ size_t sevp_plan_device_bytes(sevp_plan p) {
   return p->coef.bytes() + p->ie_d.bytes() + p->rinv.bytes() + p->rinv1.bytes() +
          p->rinv32.bytes() + p->rinv1_32.bytes() + p->w32.bytes() + p->e.bytes() + p->dum0.bytes() +
          p->hfree.bytes() + p->hcpl.bytes() + p->g.bytes() + p->e32.bytes() +
-         p->g32.bytes() + p->xx.bytes() +
+         p->g32.bytes() + p->kpart.bytes() + p->xx.bytes() +
          p->bdum0.bytes() + p->r64.bytes() + p->dx.bytes() + p->part.bytes() + p->b1s.bytes() +
          p->zp.bytes();
 }
@@ -728,6 +848,7 @@ static FullCtx full_ctx(sevp_plan p, bool mixed) {
   c.e = p->e.p; c.dum0 = p->dum0.p; c.hfree = p->hfree.p; c.hcpl = p->hcpl.p; c.g = p->g.p;
   c.e32 = mixed ? p->e32.p : nullptr;
   c.g32 = mixed ? p->g32.p : nullptr;
+  c.kpart = mixed ? p->kpart.p : nullptr;
   c.prof = &p->prof;
   return c;
 }
@@ -736,8 +857,15 @@ static int alloc_mixed(sevp_plan p) {
   if (p->e32.p) return SEVP_OK;
   const int nr = p->d.nrhs_max;
   if (nr > 64) return fail(SEVP_E_ARG, "mixed precision solves at most 64 RHS per call");
+  if (p->d.i2 % 4 != 0) return fail(SEVP_E_ARG, "mixed precision needs I2 divisible by 4");
   CHK(p->e32.alloc((size_t)nr * p->d.i2));
   CHK(p->g32.alloc((size_t)nr * p->d.i2));
+  {
+    const int i2 = p->d.i2;
+    const size_t n1 = (size_t)gemm_part_slices(i2, i2, nullptr) * i2;
+    const size_t n2 = (size_t)gemm_part_slices(2 * i2, i2, nullptr) * 2 * i2;
+    CHK(p->kpart.alloc(std::max(n1, n2) * nr));
+  }
   CHK(p->rinv32.alloc(p->rinv_count()));
   CHK(p->rinv1_32.alloc(p->rinv1_count()));
   CHK(p->w32.alloc(p->rinv1_count()));
@@ -834,6 +962,41 @@ int sevp_residual_dev(sevp_plan p, const float *s, const double *x, int nrhs, do
   return SEVP_OK;
 }
 
+// Put an event-record node before and after each of `knodes` in `graph`: the
+// first waits for the kernel's dependencies and the kernel waits for it, the
+// second follows the kernel and its dependents wait for it.
+static int add_timing_nodes(hipGraph_t graph, const std::vector<hipGraphNode_t> &knodes,
+                            std::vector<std::pair<hipEvent_t, hipEvent_t>> &events) {
+  std::vector<hipGraphNode_t> deps;
+  for (hipGraphNode_t k : knodes) {
+    hipEvent_t a, b;
+    HIPC(hipEventCreate(&a));
+    if (hipEventCreate(&b) != hipSuccess) {
+      (void)hipEventDestroy(a);
+      return fail(SEVP_E_HIP, "hipEventCreate failed");
+    }
+    events.push_back({a, b});
+    size_t n = 0;
+    HIPC(hipGraphNodeGetDependencies(k, nullptr, &n));
+    deps.resize(n);
+    if (n) HIPC(hipGraphNodeGetDependencies(k, deps.data(), &n));
+    hipGraphNode_t ra, rb;
+    HIPC(hipGraphAddEventRecordNode(&ra, graph, deps.data(), n, a));
+    for (size_t i = 0; i < n; ++i) HIPC(hipGraphRemoveDependencies(graph, &deps[i], &k, 1));
+    HIPC(hipGraphAddDependencies(graph, &ra, &k, 1));
+    n = 0;
+    HIPC(hipGraphNodeGetDependentNodes(k, nullptr, &n));
+    deps.resize(n);
+    if (n) HIPC(hipGraphNodeGetDependentNodes(k, deps.data(), &n));
+    HIPC(hipGraphAddEventRecordNode(&rb, graph, &k, 1, b));
+    for (size_t i = 0; i < n; ++i) {
+      HIPC(hipGraphRemoveDependencies(graph, &k, &deps[i], 1));
+      HIPC(hipGraphAddDependencies(graph, &rb, &deps[i], 1));
+    }
+  }
+  return SEVP_OK;
+}
+
 // The captured pass for (nrhs, precision, profiling), captured on first use.
 static int get_pass_graph(sevp_plan p, const FullCtx &c, int nrhs, PassGraph **out) {
   const bool prof = p->prof.kclass >= 0;
@@ -852,7 +1015,8 @@ static int get_pass_graph(sevp_plan p, const FullCtx &c, int nrhs, PassGraph **o
   HIPC(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
   hipStream_t saved = p->prof.st;
   p->prof.st = cs;
-  if (prof) p->prof.capture_into = &g.events;
+  std::vector<hipGraphNode_t> knodes;
+  if (prof) p->prof.capture_into = &knodes;
   int rc = zero_async(p->dx.p, sizeof(double) * nx * nrhs, cs);
   if (rc == SEVP_OK) rc = run_full_pass(c, nullptr, p->r64.p, ns, p->dx.p, nx, nrhs, cs);
   p->prof.capture_into = nullptr;
@@ -861,16 +1025,29 @@ static int get_pass_graph(sevp_plan p, const FullCtx &c, int nrhs, PassGraph **o
   const hipError_t e = hipStreamEndCapture(cs, &graph);
   if (rc != SEVP_OK || e != hipSuccess) {
     if (graph) (void)hipGraphDestroy(graph);
+    if (rc != SEVP_OK) return rc;
+    HIPC(e);
+  }
+  if (prof) {
+    rc = add_timing_nodes(graph, knodes, g.events);
+    if (rc != SEVP_OK) {
+      (void)hipGraphDestroy(graph);
+      for (auto &ev : g.events) {
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+      }
+      return rc;
+    }
+  }
+  const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) {
     for (auto &ev : g.events) {
       (void)hipEventDestroy(ev.first);
       (void)hipEventDestroy(ev.second);
     }
-    if (rc != SEVP_OK) return rc;
-    HIPC(e);
+    HIPC(ei);
   }
-  const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(graph);
-  HIPC(ei);
   p->graphs.push_back(std::move(g));
   *out = &p->graphs.back();
   return SEVP_OK;

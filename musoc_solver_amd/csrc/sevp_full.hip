@@ -36,11 +36,9 @@ __device__ __forceinline__ void nbr(const double (&v)[K], int k, double &w_, dou
 }
 
 template <int K, int MODE>
-__global__ __launch_bounds__(64) void march_kernel(MarchArgs p) {
+__device__ void march_body(const MarchArgs &p, int tile, int bidx, int strip, int nstrips) {
   const int lane = threadIdx.x;
-  const int tile = blockIdx.x;
-  const int bidx = blockIdx.y;  // RHS (PART/HOM) or unit-vector column g-1 (PRE)
-  const int strip = blockIdx.z;
+  // bidx: RHS (PART/HOM) or unit-vector column g-1 (PRE)
   const int w = p.w;
   const int T = p.tile_out;
   const int out_lo = 1 + tile * T;
@@ -48,7 +46,7 @@ __global__ __launch_bounds__(64) void march_kernel(MarchArgs p) {
   ColMap map = p.map;
   if (MODE == MODE_PRE) {
     map.il = strip * map.inc + 1;
-    map.seam = (map.seam && strip == (int)gridDim.z - 1) ? 1 : 0;
+    map.seam = (map.seam && strip == nstrips - 1) ? 1 : 0;
   }
   const int js = p.js, jf = p.jf, ieb = p.ieb;
 
@@ -92,11 +90,11 @@ __global__ __launch_bounds__(64) void march_kernel(MarchArgs p) {
   const double *r1p = nullptr;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
+    const bool h = halo_w[k] || halo_e[k];
     if (MODE == MODE_PART) {
       vs[k] = x[f2(i0, xcol(k), js)];
       vc[k] = x[f2(i0, xcol(k), js + 1)];
-    } else if (MODE == MODE_HOM) {
-      const bool h = halo_w[k] || halo_e[k];
+    } else if (MODE == MODE_HOM && !p.corr.p) {
       vs[k] = (h || !p.hcpl) ? 0.0 : p.hcpl[(long)bidx * p.i2 + pc[k] - 1];
       vc[k] = h ? 0.0 : p.hfree[(long)bidx * p.i2 + pc[k] - 1];
     }
@@ -141,6 +139,19 @@ __global__ __launch_bounds__(64) void march_kernel(MarchArgs p) {
 
   const int jlast = jf + 1;  // closure row, operands needed after the march
   load_row(js, ca, cb, cc, cd, ce, cf, cx);
+  // split-K corrections, read after the row loads above are in flight
+  if (MODE != MODE_PRE && p.corr.p) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const bool h = halo_w[k] || halo_e[k];
+      if (MODE == MODE_PART) {
+        if (!h) vs[k] = vs[k] - p.corr.sum(bidx, pc[k] - 1);
+      } else {
+        vs[k] = (h || p.cpl_off < 0) ? 0.0 : p.corr.sum(bidx, p.cpl_off + pc[k] - 1);
+        vc[k] = h ? 0.0 : p.corr.sum(bidx, pc[k] - 1);
+      }
+    }
+  }
   for (int j = js; j <= jf; ++j) {
     double na[K], nb_[K], nc[K], nd[K], ne[K], nf[K], nx[K];
     if (j + 1 <= jlast && (MODE != MODE_HOM || j + 1 <= jf)) load_row(j + 1, na, nb_, nc, nd, ne, nf, nx);
@@ -188,7 +199,7 @@ __global__ __launch_bounds__(64) void march_kernel(MarchArgs p) {
       const double xn = x[f2(i0, pc[k] + 1, ieb + 1)];
       const double r = cf[k] - ca[k] * xw - cb[k] * vs[k] - cc[k] * vc[k] - cd[k] * xe - ce[k] * xn;
       if (out[k]) {
-        p.e[(long)bidx * p.i2 + pc[k] - 1] = r;
+        if (p.e) p.e[(long)bidx * p.i2 + pc[k] - 1] = r;
         if (p.e32) p.e32[(long)bidx * p.i2 + pc[k] - 1] = (float)r;
       }
     }
@@ -206,6 +217,11 @@ __global__ __launch_bounds__(64) void march_kernel(MarchArgs p) {
     }
   }
 #undef COEF
+}
+
+template <int K, int MODE>
+__global__ __launch_bounds__(64) void march_kernel(MarchArgs p) {
+  march_body<K, MODE>(p, blockIdx.x, blockIdx.y, blockIdx.z, gridDim.z);
 }
 
 // Choose lanes-per-column K and the tile size for a march of `steps` rows.
@@ -250,20 +266,24 @@ hipError_t launch_march(int mode, const MarchArgs &a, int nbatch, hipStream_t st
   return launch_march_mode<MODE_PRE>(a, nbatch, nstrips, st);
 }
 
-// Backward restore of X(:, ieb) from DUM0 and closure residual at row ieb-1.
-__global__ void resid_kernel(ResidArgs p) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)p.nrhs * p.i2) return;
-  const int r = (int)(t / p.i2);
-  const int c = (int)(t % p.i2) + 1;
-  const int i2 = p.i2, i0 = i2 + 2, ieb = p.ieb, jr = ieb - 1;
-  double *x = p.x + (long)r * p.x_stride;
-  const double *d0 = p.dum0 ? p.dum0 + (long)r * p.dum0_stride + (long)(p.nb - 1) * i2 : nullptr;
-  auto rowv = [&](int cc) -> double {  // value of row ieb at column cc (1..i2) after the restore
-    return d0 ? d0[cc - 1] : x[f2(i0, cc + 1, ieb)];
-  };
-  const double center = rowv(c);
-  double west, east;
+// Closure residual of equation row ieb-1 at column c (1-based) for RHS r,
+// given row ieb's west/centre/east values and row ieb+1's value at c.
+__device__ __forceinline__ double closure_resid(const ResidArgs &p, int r, int c, const double *x,
+                                                double west, double center, double east,
+                                                double north) {
+  const int i0 = p.i2 + 2, jr = p.ieb - 1;
+  const long co = (long)(c - 1) + (long)(jr - 1) * p.i2;
+  const double fv = p.f64 ? p.f64[(long)r * p.f_stride + co] : (double)p.f[(long)r * p.f_stride + co];
+  return fv - (double)p.ax[co] * west - (double)p.ay[co] * x[f2(i0, c + 1, jr)] -
+         (double)p.bb[co] * center - (double)p.cx[co] * east - (double)p.cy[co] * north;
+}
+
+// West and east neighbours in row ieb of column c, reading row values
+// through rowv (which applies the DUM0 restore where there is one).
+template <class RowV>
+__device__ __forceinline__ void row_nbrs(const ResidArgs &p, const double *x, int c, RowV rowv,
+                                         double &west, double &east) {
+  const int i2 = p.i2, i0 = i2 + 2, ieb = p.ieb;
   if (p.periodic) {
     west = rowv(c == 1 ? i2 : c - 1);
     east = rowv(c == i2 ? 1 : c + 1);
@@ -271,14 +291,92 @@ __global__ void resid_kernel(ResidArgs p) {
     west = (c == 1) ? x[f2(i0, 1, ieb)] : rowv(c - 1);
     east = (c == i2) ? x[f2(i0, i0, ieb)] : rowv(c + 1);
   }
-  const long co = (long)(c - 1) + (long)(jr - 1) * i2;
-  const double fv = p.f64 ? p.f64[(long)r * p.f_stride + co] : (double)p.f[(long)r * p.f_stride + co];
-  const double e = fv - (double)p.ax[co] * west - (double)p.ay[co] * x[f2(i0, c + 1, jr)] -
-                   (double)p.bb[co] * center - (double)p.cx[co] * east -
-                   (double)p.cy[co] * x[f2(i0, c + 1, ieb + 1)];
-  p.e[(long)r * i2 + c - 1] = e;
+}
+
+// Backward restore of X(:, ieb) from DUM0 and closure residual at row ieb-1.
+__device__ void resid_body(const ResidArgs &p, long t) {
+  if (t >= (long)p.nrhs * p.i2) return;
+  const int r = (int)(t / p.i2);
+  const int c = (int)(t % p.i2) + 1;
+  const int i2 = p.i2, i0 = i2 + 2, ieb = p.ieb;
+  double *x = p.x + (long)r * p.x_stride;
+  const double *d0 = p.dum0 ? p.dum0 + (long)r * p.dum0_stride + (long)(p.nb - 1) * i2 : nullptr;
+  auto rowv = [&](int cc) -> double {  // value of row ieb at column cc (1..i2) after the restore
+    return d0 ? d0[cc - 1] : x[f2(i0, cc + 1, ieb)];
+  };
+  const double center = rowv(c);
+  double west, east;
+  row_nbrs(p, x, c, rowv, west, east);
+  const double e = closure_resid(p, r, c, x, west, center, east, x[f2(i0, c + 1, ieb + 1)]);
+  if (p.e) p.e[(long)r * i2 + c - 1] = e;
   if (p.e32) p.e32[(long)r * i2 + c - 1] = (float)e;
   if (d0) x[f2(i0, c + 1, ieb)] = center;
+}
+
+// Mixed-precision backward, per (rhs, column): X(apply_row) += the free-row
+// correction of the block above (summed from its partial products), then,
+// when do_resid, the closure residual of the block below (row ieb-1 =
+// apply_row-2), whose only read of apply_row is this column's.  X(ieb) was
+// never overwritten by the forward pass here, so no restore is needed.
+__device__ void apply_resid_body(const ResidArgs &p, long t) {
+  if (t >= (long)p.nrhs * p.i2) return;
+  const int r = (int)(t / p.i2);
+  const int c = (int)(t % p.i2) + 1;
+  const int i2 = p.i2, i0 = i2 + 2, ieb = p.ieb;
+  double *x = p.x + (long)r * p.x_stride;
+  const long xa = f2(i0, c + 1, p.apply_row);
+  const double north = x[xa] + p.corr.sum(r, c - 1);
+  x[xa] = north;
+  if (!p.do_resid) return;
+  auto rowv = [&](int cc) -> double { return x[f2(i0, cc + 1, ieb)]; };
+  double west, east;
+  row_nbrs(p, x, c, rowv, west, east);
+  const double e = closure_resid(p, r, c, x, west, rowv(c), east, north);
+  if (p.e) p.e[(long)r * i2 + c - 1] = e;
+  if (p.e32) p.e32[(long)r * i2 + c - 1] = (float)e;
+}
+
+__global__ void resid_kernel(ResidArgs p) {
+  resid_body(p, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// The backward H march of block b and the restore + closure residual of
+// block b-1 touch disjoint rows and read nothing the other writes: one
+// launch runs both (march tiles first, then 64-column residual groups).
+template <int K>
+__global__ __launch_bounds__(64) void hom_resid_kernel(MarchArgs m, ResidArgs r, int ntiles,
+                                                       int has_resid) {
+  const long b = blockIdx.x;
+  const long nm = (long)ntiles * m.nrhs;
+  if (b < nm) {
+    march_body<K, MODE_HOM>(m, (int)(b % ntiles), (int)(b / ntiles), 0, 1);
+    return;
+  }
+  if (!has_resid) return;
+  if (r.corr.p) apply_resid_body(r, (b - nm) * 64 + threadIdx.x);
+  else resid_body(r, (b - nm) * 64 + threadIdx.x);
+}
+
+hipError_t launch_hom_resid(const MarchArgs &a0, const ResidArgs *r, hipStream_t st) {
+  MarchArgs a = a0;
+  int K, tile, halo, ntiles;
+  march_shape(a.w, a.jf - a.js + 1, K, tile, halo, ntiles);
+  if (K < 0) return hipErrorInvalidValue;
+  a.tile_out = tile;
+  a.halo = halo;
+  const long nm = (long)ntiles * a.nrhs;
+  const long nr = r ? ((long)r->nrhs * r->i2 + 63) / 64 : 0;
+  const bool march = a.jf >= a.js;
+  const long nmb = march ? nm : 0;
+  ResidArgs rr = r ? *r : ResidArgs{};
+  dim3 grid((unsigned)(nmb + nr));
+  if (nmb + nr == 0) return hipSuccess;
+  MarchArgs am = a;
+  if (!march) am.nrhs = 0;  // no march blocks: every block is a residual group
+  if (K == 1) hipLaunchKernelGGL((hom_resid_kernel<1>), grid, dim3(64), 0, st, am, rr, ntiles, r != nullptr);
+  else if (K == 2) hipLaunchKernelGGL((hom_resid_kernel<2>), grid, dim3(64), 0, st, am, rr, ntiles, r != nullptr);
+  else hipLaunchKernelGGL((hom_resid_kernel<4>), grid, dim3(64), 0, st, am, rr, ntiles, r != nullptr);
+  return hipGetLastError();
 }
 
 hipError_t launch_resid(const ResidArgs &a, hipStream_t st) {

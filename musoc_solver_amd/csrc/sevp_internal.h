@@ -40,6 +40,30 @@ struct ColMap {
   }
 };
 
+// A correction held as split-K partial products of the mixed-precision
+// GEMM: value(m, n) = sum over slices s of P[s][m][n], added in FP64 in slice
+// order (so every reader of one element computes the same double).
+struct PartRef {
+  const float *p;  // null: no correction
+  long sstride;    // elements per slice (M * ld)
+  int ns;          // number of slices
+  int ld;          // N of the product (elements per right-hand side)
+  // eight loads in flight per group; the additions stay in slice order
+  __device__ double sum(int m, int n) const {
+    const float *q = p + (long)m * ld + n;
+    double acc = 0.0;
+    for (int s0 = 0; s0 < ns; s0 += 8) {
+      float v[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = q[(long)min(s0 + i, ns - 1) * sstride];  // no branch
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        if (s0 + i < ns) acc += (double)v[i];
+    }
+    return acc;
+  }
+};
+
 // Full-width (or strip) march over one block, batched over independent
 // right-hand sides.  Mode selects the recurrence:
 //   MODE_PART : X(j+2) = (F - AX Xw - AY Xs - BB Xc - CX Xe) / CY        (REP1/REP forward)
@@ -67,6 +91,13 @@ struct MarchArgs {
   double *e;
   float *e32;
   int i2, j2, periodic, js, jf, ieb, resid;
+  // mixed precision, split-K corrections read directly from the partials:
+  //   MODE_PART: subtracted from row js (the row the previous block corrected;
+  //              the X array keeps the uncorrected value)
+  //   MODE_HOM : H(js+1) at column n, and, when cpl_off >= 0, H(js) at
+  //              column cpl_off + n (used instead of hfree / hcpl)
+  PartRef corr;
+  int cpl_off;
   int w;            // number of columns marched (i2 for full width)
   int tile_out;     // output columns per wave
   int halo;         // extra columns each side
@@ -83,9 +114,14 @@ struct ResidArgs {  // backward restore + closure residual, one thread per (rhs,
   long x_stride;
   const double *dum0;  // (i2, nb0-1) per RHS, null for the top block
   long dum0_stride;
-  double *e;           // (nrhs, i2)
+  double *e;           // (nrhs, i2), may be null
   float *e32;          // FP32 copy of e (may be null)
   int i2, j2, periodic, nb, ieb, nrhs;
+  // mixed precision backward: X(apply_row) += corr (the free-row correction
+  // of the block above) before the residual, which reads that row as its
+  // north neighbour; do_resid == 0 applies the correction only
+  PartRef corr;
+  int apply_row, do_resid;
 };
 
 enum { EP_FWD = 0, EP_BWD = 1, EP_CPL = 2, EP_NEGSUB = 3, EP_COLMAJ = 4 };
@@ -135,10 +171,27 @@ struct GemmF32Args {  // mixed precision: A x B in FP32, FP32 accumulate
   int i2;
 };
 
+// Split-K partial products of the mixed-precision corrections:
+// P[s][m][n] = sum over k in slice s of A(m, k) B(k, n) in FP32 (MFMA).
+struct GemmPartArgs {
+  const float *a;   // (M, K) row-major, ld K
+  const float *b;   // (K, nsplit) col-major, ld K
+  const float *b2;  // columns n >= nsplit: col-major, ld K, column n - nsplit (may be null)
+  int nsplit;
+  int M, N, K;
+  float *part;      // (S, M, N)
+};
+// Slice length and number of slices used for an (N, K) product.
+int gemm_part_slices(int N, int K, int *kslice);
+hipError_t launch_gemm_part(const GemmPartArgs &a, hipStream_t st);
+
 hipError_t launch_bir_sweeps(const BirArgs &a, int nrhs, hipStream_t st);
 hipError_t launch_repbir(const RepbirArgs &a, int nrhs, hipStream_t st);
 hipError_t launch_march(int mode, const MarchArgs &a, int nbatch, hipStream_t st);
 hipError_t launch_resid(const ResidArgs &a, hipStream_t st);
+// backward H march of one block fused with the restore + residual of the
+// block below (r may be null for the bottom block)
+hipError_t launch_hom_resid(const MarchArgs &m, const ResidArgs *r, hipStream_t st);
 hipError_t launch_gemm(const GemmArgs &a, hipStream_t st);
 hipError_t launch_gemm_mixed(const GemmF32Args &a, hipStream_t st);
 hipError_t launch_cvt_f32(const double *src, float *dst, long n, hipStream_t st);

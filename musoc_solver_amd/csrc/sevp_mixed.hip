@@ -15,6 +15,8 @@
 // k = k0 + 4q .. k0 + 4q + 3 as one run for both operands and feeds them to
 // four successive MFMA steps.  A and B use the same permutation, so the
 // product is unchanged.
+#include <algorithm>
+
 #include "sevp_dev.h"
 #include "sevp_internal.h"
 
@@ -151,6 +153,130 @@ __global__ __launch_bounds__(64 * NW) void gemm_f32_mfma_kernel(GemmF32Args p, i
         break;
     }
   }
+}
+
+// ------------------------------------------------ split-K partial products
+//
+// Workgroup (g, s): K slice s = [s*KS, min(K, (s+1)*KS)) and four
+// 16-column tiles, one per wavefront, each over every right-hand side (up to
+// four 16-row accumulator blocks).  The A slice is staged once in LDS with a
+// row pitch of KS+4 floats (an odd number of 16-byte units, so the 16
+// rows one ds_read_b128 group touches fall in distinct banks) and shared by
+// the four wavefronts; B columns come from HBM as float4 runs of k, the whole
+// slice held in registers.  The FP32 slice sums go to P[s][m][n] and the
+// consumers add them in slice order in FP64.
+constexpr int GP_WAVES = 4;
+
+// Four consecutive k at k (a multiple of 4) of a run of klen (a multiple of
+// 4) floats, zero at or past klen: one unconditional float4 load (from k = 0
+// when out of range) and a select, so the load never branches.
+__device__ __forceinline__ f32x4 load4m(const float *base, int k, int klen) {
+  const bool ok = k < klen;
+  const f32x4 v = *reinterpret_cast<const f32x4 *>(base + (ok ? k : 0));
+  return ok ? v : f32x4{0.f, 0.f, 0.f, 0.f};
+}
+
+// KS (the slice length, a multiple of 16) is a compile-time constant, so the
+// loads, the LDS staging and the 4 x KS/4 MFMA steps are straight-line code.
+// Every global load of the workgroup is issued before the first wait: the B
+// columns of the slice into registers, then the A slice, which goes to LDS.
+// All four 16-row accumulator blocks always run; LDS rows past M are zero.
+// Requires K % 4 == 0 (float4 runs of k are aligned).
+template <int KS>
+__global__ __launch_bounds__(64 * GP_WAVES) void gemm_part_kernel(GemmPartArgs p) {
+  constexpr int LDA = KS + 4;
+  constexpr int KQ = KS / 4;
+  constexpr int NCH = KS / 16;
+  constexpr int ALD = (64 * KQ + 64 * GP_WAVES - 1) / (64 * GP_WAVES);
+  __shared__ float sa[64 * LDA];
+  const int s = blockIdx.y;
+  const int k_lo = s * KS;
+  const int klen = min(p.K - k_lo, KS);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n0 = (blockIdx.x * GP_WAVES + wave) * 16;
+  const int col = lane & 15, q = lane >> 4;
+  const int n = n0 + col;
+  const bool nok = n < p.N;
+  const int nc = nok ? n : p.N - 1;
+  const float *bcol =
+      ((p.b2 && nc >= p.nsplit) ? p.b2 + (long)(nc - p.nsplit) * p.K : p.b + (long)nc * p.K) + k_lo;
+  const f32x4 zero = f32x4{0.f, 0.f, 0.f, 0.f};
+  f32x4 bq[NCH];
+#pragma unroll
+  for (int i = 0; i < NCH; ++i) bq[i] = load4m(bcol, 16 * i + 4 * q, klen);
+  f32x4 at[ALD];
+#pragma unroll
+  for (int j = 0; j < ALD; ++j) {
+    const int idx = threadIdx.x + j * 64 * GP_WAVES;
+    const int m = idx / KQ, k = (idx % KQ) * 4;
+    const bool ok = idx < 64 * KQ && m < p.M;
+    at[j] = load4m(p.a + (long)(ok ? m : 0) * p.K + k_lo, k, ok ? klen : 0);
+  }
+#pragma unroll
+  for (int j = 0; j < ALD; ++j) {
+    const int idx = threadIdx.x + j * 64 * GP_WAVES;
+    if (idx < 64 * KQ) *reinterpret_cast<f32x4 *>(&sa[(idx / KQ) * LDA + (idx % KQ) * 4]) = at[j];
+  }
+  __syncthreads();
+  if (n0 >= p.N) return;
+  f32x4 acc[4] = {zero, zero, zero, zero};
+  const float *arow = sa + col * LDA + 4 * q;
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) {
+    f32x4 av[4];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) av[mb] = *reinterpret_cast<const f32x4 *>(arow + mb * 16 * LDA + 16 * ch);
+    const f32x4 bv = nok ? bq[ch] : zero;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb)
+        acc[mb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[mb][t], bv[t], acc[mb], 0, 0, 0);
+  }
+  if (!nok) return;
+  float *out = p.part + (long)s * p.M * p.N + n;
+#pragma unroll
+  for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int m = mb * 16 + 4 * q + i;
+      if (m < p.M) out[(long)m * p.N] = acc[mb][i];
+    }
+}
+
+// Slice lengths with a compiled kernel, and the one used for a product: the
+// shortest (most workgroups) that leaves at most 16 partial products per
+// output, since every consumer reads all of them.  For 64 x 1440 x N this is
+// 96 (15 slices; 345 workgroups at N = 1440, 675 at N = 2880).
+static const int kGpSlices[] = {32, 48, 64, 96, 128};
+constexpr int GP_MAX_SLICES = 16;
+
+int gemm_part_slices(int N, int K, int *kslice) {
+  (void)N;
+  int ks = kGpSlices[4];
+  for (int c : kGpSlices)
+    if ((K + c - 1) / c <= GP_MAX_SLICES) {
+      ks = c;
+      break;
+    }
+  if (kslice) *kslice = ks;
+  return (K + ks - 1) / ks;
+}
+
+hipError_t launch_gemm_part(const GemmPartArgs &a, hipStream_t st) {
+  if (a.M > MX_MAXM || a.M < 1 || a.N < 1 || a.K < 1 || a.K % 4 != 0) return hipErrorInvalidValue;
+  int ks = 0;
+  const int ns = gemm_part_slices(a.N, a.K, &ks);
+  const int groups = (a.N + 16 * GP_WAVES - 1) / (16 * GP_WAVES);
+  dim3 grid(groups, ns), block(64 * GP_WAVES);
+  switch (ks) {
+    case 32: hipLaunchKernelGGL(gemm_part_kernel<32>, grid, block, 0, st, a); break;
+    case 48: hipLaunchKernelGGL(gemm_part_kernel<48>, grid, block, 0, st, a); break;
+    case 64: hipLaunchKernelGGL(gemm_part_kernel<64>, grid, block, 0, st, a); break;
+    case 96: hipLaunchKernelGGL(gemm_part_kernel<96>, grid, block, 0, st, a); break;
+    default: hipLaunchKernelGGL(gemm_part_kernel<128>, grid, block, 0, st, a); break;
+  }
+  return hipGetLastError();
 }
 
 __global__ void cvt_f32_kernel(const double *src, float *dst, long n) {

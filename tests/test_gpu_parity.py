@@ -309,4 +309,18 @@ def test_graph_replays_match_eager_launches(oracle, device, precision):
         assert st_g.iterations == st_e.iterations
         assert torch.equal(x_d, x_eager), f"replay {rep}"
         assert np.array_equal(rel_g, rel_e)
+    # a profiled replay times its correction products through event nodes in
+    # the graph and still gives the same result
+    x_d.zero_()
+    plan.profile_begin(S.KCLASS_CORR)
+    rel_p, st_p, ok_p = plan.solve_refined_dev(s_d, x_d, tol=1e-12, max_iter=6)
+    ms, n = plan.profile_end()
+    torch.cuda.synchronize()
+    assert torch.equal(x_d, x_eager)
+    nb0 = len(ie)
+    # forward: one product per closed block; backward: the free-row product of
+    # every block plus, in FP64, a separate coupling product below the top
+    per_pass = 3 * nb0 - 2 if precision == S.PREC_FP64 else 2 * nb0 - 1
+    assert n == st_p.iterations * per_pass, (n, st_p.iterations)
+    assert ms > 0.0
     plan.close()
