@@ -64,6 +64,8 @@ def parse():
     ap.add_argument("--cpu-rhs", type=int, default=3, help="RHS in the CPU baseline sample (0: skip)")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--legs", default="mixed,plain", help="comma list of GPU legs to time")
+    ap.add_argument("--rhs-groups", default="2,2",
+                    help="RHS groups whose passes run concurrently, mixed,plain legs")
     ap.add_argument("--kernel-timing", choices=("after", "timed"), default="after",
                     help="time the correction kernel with HIP events in one extra solve after "
                          "the timed steps (after) or inside them (timed: event nodes in the "
@@ -123,6 +125,9 @@ def main():
             f"zero pivots {zp}, {plan.device_bytes / 2**30:.2f} GiB on device)")
         return plan, ie, dt
 
+    def plan_groups(name):
+        return dict(zip(("mixed", "plain"), (int(v) for v in a.rhs_groups.split(",")))).get(name, 1)
+
     def run_leg(plan, name):
         for _ in range(a.warmup):
             x.zero_()
@@ -156,7 +161,7 @@ def main():
              "corr_avg_us": 1e3 * corr_ms / max(corr_n, 1), "corr_launches": corr_n,
              "corr_share": (corr_ms * (a.steps if a.kernel_timing == "after" else 1)
                             / (1e3 * dt)),
-             "kernel_timing": a.kernel_timing}
+             "kernel_timing": a.kernel_timing, "rhs_groups": plan_groups(name)}
         log(f"{name}: {r}")
         return r
 
@@ -168,6 +173,7 @@ def main():
     for name in order:
         prec = {"mixed": S.PREC_FP32, "plain": S.PREC_FP64}[name]
         plan, ie, dt = make_plan(n0_of[name], prec)
+        plan.set_rhs_groups(plan_groups(name))
         setup[name] = dt
         legs[name] = run_leg(plan, name)
         xs[name] = x.clone()

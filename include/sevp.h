@@ -175,6 +175,12 @@ int sevp_set_stream(sevp_plan plan, void *hip_stream);
  * pass's kernels are launched one by one. */
 int sevp_set_graphs(sevp_plan plan, int on);
 
+/* Split the right-hand sides of sevp_solve_refined_dev into `groups` (1..16)
+ * independent groups whose SEVP passes run concurrently on separate streams
+ * (default 1).  The result is bitwise the same for any grouping of FP64
+ * passes; mixed-precision passes are deterministic for a given grouping. */
+int sevp_set_rhs_groups(sevp_plan plan, int groups);
+
 /* Switch the arithmetic of the correction products of an existing
  * full-width plan (SEVP_PREC_*); FP32 copies of the influence matrices are
  * made on first use.  Lets one influence-matrix build serve both the plain

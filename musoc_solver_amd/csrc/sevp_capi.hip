@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <deque>
 #include <vector>
 
 #include "../../include/sevp.h"
@@ -75,8 +76,8 @@ struct KProf {
   // get event-record nodes around them once the capture ends (recording an
   // event on a capturing stream only orders work, it adds no timing node)
   std::vector<hipGraphNode_t> *capture_into = nullptr;
-  // events of the last graph replay, read at the next harvest
-  const std::vector<std::pair<hipEvent_t, hipEvent_t>> *pending = nullptr;
+  // events of the graph replays since the last harvest (one per RHS group)
+  std::vector<const std::vector<std::pair<hipEvent_t, hipEvent_t>> *> pending;
   bool on(int k) const { return kclass == k; }
   void start(int k) {
     if (!on(k) || capture_into) return;
@@ -113,16 +114,15 @@ struct KProf {
       }
     }
     used = 0;
-    if (pending) {
-      for (auto &e : *pending) {
+    for (const auto *ev : pending)
+      for (auto &e : *ev) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) {
           total_ms += ms;
           ++launches;
         }
       }
-      pending = nullptr;
-    }
+    pending.clear();
     (void)hipGetLastError();  // a pair that never ran must not fail a later launch check
   }
   ~KProf() {
@@ -137,6 +137,7 @@ struct KProf {
 // of the FP64 residual) for a batch size and precision.
 struct PassGraph {
   int nrhs = 0, prec = 0;
+  int r0 = 0, count = 0;  // the RHS group this pass covers
   bool prof = false;
   hipGraphExec_t exec = nullptr;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -144,7 +145,7 @@ struct PassGraph {
 
 struct sevp_plan_s {
   KProf prof;
-  std::vector<PassGraph> graphs;
+  std::deque<PassGraph> graphs;  // deque: PassGraph addresses stay valid as graphs are added
   hipStream_t cap_stream = nullptr;
   bool use_graphs = true;
   sevp_desc d{};
@@ -154,6 +155,11 @@ struct sevp_plan_s {
   int nmat = 0;  // number of strips (BIR) or 1
   hipStream_t stream = nullptr;      // stream the plan's work is enqueued on
   hipStream_t own_stream = nullptr;  // the stream the plan created
+  // refinement passes of independent RHS groups run concurrently: group 0 on
+  // `stream`, group g > 0 on gstreams[g-1], joined through events
+  int rhs_groups = 1;
+  std::vector<hipStream_t> gstreams;
+  std::vector<hipEvent_t> gevents;
   bool have_inv = false;
   DBuf<float> coef;     // 5 * i2*j2 (AL, AB, AC, AR, AT)
   DBuf<int> ie_d;
@@ -715,13 +721,15 @@ static void drop_graphs(sevp_plan p) {
     }
   }
   p->graphs.clear();
-  p->prof.pending = nullptr;
+  p->prof.pending.clear();
 }
 
 int sevp_plan_destroy(sevp_plan p) {
   if (!p) return SEVP_OK;
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   drop_graphs(p);
+  for (auto st : p->gstreams) (void)hipStreamDestroy(st);
+  for (auto ev : p->gevents) (void)hipEventDestroy(ev);
   if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
   if (p->own_stream) (void)hipStreamDestroy(p->own_stream);
   delete p;
@@ -853,6 +861,23 @@ static FullCtx full_ctx(sevp_plan p, bool mixed) {
   return c;
 }
 
+// The context of RHS r0.. of a batch: every per-RHS scratch row offset by
+// r0 right-hand sides (the split-K partials by r0 times their per-RHS size).
+static FullCtx group_ctx(const FullCtx &c0, int r0) {
+  FullCtx c = c0;
+  const long i2 = c.i2;
+  const long dstride = i2 * (c.nb0 > 1 ? c.nb0 - 1 : 1);
+  c.e += r0 * i2;
+  c.hfree += r0 * i2;
+  c.hcpl += r0 * i2;
+  c.g += r0 * i2;
+  c.dum0 += r0 * dstride;
+  if (c.e32) c.e32 += r0 * i2;
+  if (c.g32) c.g32 += r0 * i2;
+  if (c.kpart) c.kpart += r0 * (long)gemm_part_slices((int)i2, (int)i2, nullptr) * 2 * i2;
+  return c;
+}
+
 static int alloc_mixed(sevp_plan p) {
   if (p->e32.p) return SEVP_OK;
   const int nr = p->d.nrhs_max;
@@ -962,6 +987,22 @@ int sevp_residual_dev(sevp_plan p, const float *s, const double *x, int nrhs, do
   return SEVP_OK;
 }
 
+// Streams (default priority, non-blocking) and join events for `groups` RHS
+// groups, created on first use.
+static int ensure_group_streams(sevp_plan p, int groups) {
+  while ((int)p->gstreams.size() < groups - 1) {
+    hipStream_t st;
+    HIPC(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    p->gstreams.push_back(st);
+  }
+  while ((int)p->gevents.size() < groups) {
+    hipEvent_t ev;
+    HIPC(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    p->gevents.push_back(ev);
+  }
+  return SEVP_OK;
+}
+
 // Put an event-record node before and after each of `knodes` in `graph`: the
 // first waits for the kernel's dependencies and the kernel waits for it, the
 // second follows the kernel and its dependents wait for it.
@@ -998,16 +1039,21 @@ static int add_timing_nodes(hipGraph_t graph, const std::vector<hipGraphNode_t> 
 }
 
 // The captured pass for (nrhs, precision, profiling), captured on first use.
-static int get_pass_graph(sevp_plan p, const FullCtx &c, int nrhs, PassGraph **out) {
+static int get_pass_graph(sevp_plan p, const FullCtx &c0, int nrhs, int r0, int count,
+                          PassGraph **out) {
   const bool prof = p->prof.kclass >= 0;
   for (auto &g : p->graphs)
-    if (g.nrhs == nrhs && g.prec == p->d.precision && g.prof == prof) {
+    if (g.nrhs == nrhs && g.r0 == r0 && g.count == count && g.prec == p->d.precision &&
+        g.prof == prof) {
       *out = &g;
       return SEVP_OK;
     }
   const long ns = (long)p->d.i2 * p->d.j2, nx = (long)(p->d.i2 + 2) * (p->d.j2 + 2);
+  const FullCtx c = group_ctx(c0, r0);
   PassGraph g;
   g.nrhs = nrhs;
+  g.r0 = r0;
+  g.count = count;
   g.prec = p->d.precision;
   g.prof = prof;
   hipStream_t cs = p->cap_stream;
@@ -1017,8 +1063,9 @@ static int get_pass_graph(sevp_plan p, const FullCtx &c, int nrhs, PassGraph **o
   p->prof.st = cs;
   std::vector<hipGraphNode_t> knodes;
   if (prof) p->prof.capture_into = &knodes;
-  int rc = zero_async(p->dx.p, sizeof(double) * nx * nrhs, cs);
-  if (rc == SEVP_OK) rc = run_full_pass(c, nullptr, p->r64.p, ns, p->dx.p, nx, nrhs, cs);
+  int rc = zero_async(p->dx.p + r0 * nx, sizeof(double) * nx * count, cs);
+  if (rc == SEVP_OK)
+    rc = run_full_pass(c, nullptr, p->r64.p + r0 * ns, ns, p->dx.p + r0 * nx, nx, count, cs);
   p->prof.capture_into = nullptr;
   p->prof.st = saved;
   hipGraph_t graph = nullptr;
@@ -1053,6 +1100,15 @@ static int get_pass_graph(sevp_plan p, const FullCtx &c, int nrhs, PassGraph **o
   return SEVP_OK;
 }
 
+int sevp_set_rhs_groups(sevp_plan p, int groups) {
+  if (!p) return fail(SEVP_E_ARG, "null plan");
+  if (groups < 1 || groups > 16) return fail(SEVP_E_ARG, "RHS groups must be 1..16");
+  HIPC(hipStreamSynchronize(p->stream));
+  for (auto st : p->gstreams) HIPC(hipStreamSynchronize(st));
+  p->rhs_groups = groups;
+  return SEVP_OK;
+}
+
 int sevp_set_graphs(sevp_plan p, int on) {
   if (!p) return fail(SEVP_E_ARG, "null plan");
   HIPC(hipStreamSynchronize(p->stream));
@@ -1071,8 +1127,16 @@ int sevp_solve_refined_dev(sevp_plan p, const float *s, double *x, int nrhs, dou
   const int i2 = p->d.i2, j2 = p->d.j2;
   const long ns = (long)i2 * j2, nx = (long)(i2 + 2) * (j2 + 2);
   FullCtx c = full_ctx(p, p->d.precision != SEVP_PREC_FP64);
-  PassGraph *graph = nullptr;
-  if (p->use_graphs) CHK(get_pass_graph(p, c, nrhs, &graph));
+  // RHS groups: ng right-hand sides each (the last may be smaller)
+  const int ngroups = std::min(p->rhs_groups, nrhs);
+  const int ng = (nrhs + ngroups - 1) / ngroups;
+  CHK(ensure_group_streams(p, ngroups));
+  std::vector<PassGraph *> graph(ngroups, nullptr);
+  if (p->use_graphs)
+    for (int g = 0; g < ngroups; ++g) {
+      const int r0 = g * ng;
+      if (r0 < nrhs) CHK(get_pass_graph(p, c, nrhs, r0, std::min(ng, nrhs - r0), &graph[g]));
+    }
   std::vector<double> rel;
   int it = 0;
   double worst = -1.0;
@@ -1081,12 +1145,30 @@ int sevp_solve_refined_dev(sevp_plan p, const float *s, double *x, int nrhs, dou
     worst = 0.0;
     for (double v : rel) worst = std::max(worst, v);
     if (!(worst > tol) || it >= max_iter) break;
-    if (graph) {
-      HIPC(hipGraphLaunch(graph->exec, p->stream));
-      if (graph->prof) p->prof.pending = &graph->events;
-    } else {
-      CHK(zero_async(p->dx.p, sizeof(double) * nx * nrhs, p->stream));
-      CHK(run_full_pass(c, nullptr, p->r64.p, ns, p->dx.p, nx, nrhs, p->stream));
+    if (ngroups > 1) HIPC(hipEventRecord(p->gevents[0], p->stream));
+    for (int g = 0; g < ngroups; ++g) {
+      const int r0 = g * ng;
+      if (r0 >= nrhs) break;
+      const int count = std::min(ng, nrhs - r0);
+      hipStream_t st = g == 0 ? p->stream : p->gstreams[g - 1];
+      if (g > 0) HIPC(hipStreamWaitEvent(st, p->gevents[0], 0));
+      if (graph[g]) {
+        HIPC(hipGraphLaunch(graph[g]->exec, st));
+        if (graph[g]->prof) p->prof.pending.push_back(&graph[g]->events);
+      } else {
+        const FullCtx cg = group_ctx(c, r0);
+        hipStream_t saved = p->prof.st;
+        p->prof.st = st;
+        CHK(zero_async(p->dx.p + r0 * nx, sizeof(double) * nx * count, st));
+        const int rc = run_full_pass(cg, nullptr, p->r64.p + r0 * ns, ns, p->dx.p + r0 * nx, nx,
+                                     count, st);
+        p->prof.st = saved;
+        CHK(rc);
+      }
+      if (g > 0) {
+        HIPC(hipEventRecord(p->gevents[g], st));
+        HIPC(hipStreamWaitEvent(p->stream, p->gevents[g], 0));
+      }
     }
     const long n = ns * nrhs;
     hipLaunchKernelGGL(axpy_interior_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,

@@ -180,15 +180,16 @@ __device__ __forceinline__ f32x4 load4m(const float *base, int k, int klen) {
 // loads, the LDS staging and the 4 x KS/4 MFMA steps are straight-line code.
 // Every global load of the workgroup is issued before the first wait: the B
 // columns of the slice into registers, then the A slice, which goes to LDS.
-// All four 16-row accumulator blocks always run; LDS rows past M are zero.
-// Requires K % 4 == 0 (float4 runs of k are aligned).
-template <int KS>
+// MB (1, 2 or 4) 16-row accumulator blocks cover the right-hand sides; LDS
+// rows past M are zero.  Requires K % 4 == 0 (float4 runs of k are aligned).
+template <int KS, int MB>
 __global__ __launch_bounds__(64 * GP_WAVES) void gemm_part_kernel(GemmPartArgs p) {
   constexpr int LDA = KS + 4;
   constexpr int KQ = KS / 4;
   constexpr int NCH = KS / 16;
-  constexpr int ALD = (64 * KQ + 64 * GP_WAVES - 1) / (64 * GP_WAVES);
-  __shared__ float sa[64 * LDA];
+  constexpr int ROWS = 16 * MB;
+  constexpr int ALD = (ROWS * KQ + 64 * GP_WAVES - 1) / (64 * GP_WAVES);
+  __shared__ float sa[ROWS * LDA];
   const int s = blockIdx.y;
   const int k_lo = s * KS;
   const int klen = min(p.K - k_lo, KS);
@@ -209,34 +210,36 @@ __global__ __launch_bounds__(64 * GP_WAVES) void gemm_part_kernel(GemmPartArgs p
   for (int j = 0; j < ALD; ++j) {
     const int idx = threadIdx.x + j * 64 * GP_WAVES;
     const int m = idx / KQ, k = (idx % KQ) * 4;
-    const bool ok = idx < 64 * KQ && m < p.M;
+    const bool ok = idx < ROWS * KQ && m < p.M;
     at[j] = load4m(p.a + (long)(ok ? m : 0) * p.K + k_lo, k, ok ? klen : 0);
   }
 #pragma unroll
   for (int j = 0; j < ALD; ++j) {
     const int idx = threadIdx.x + j * 64 * GP_WAVES;
-    if (idx < 64 * KQ) *reinterpret_cast<f32x4 *>(&sa[(idx / KQ) * LDA + (idx % KQ) * 4]) = at[j];
+    if (idx < ROWS * KQ) *reinterpret_cast<f32x4 *>(&sa[(idx / KQ) * LDA + (idx % KQ) * 4]) = at[j];
   }
   __syncthreads();
   if (n0 >= p.N) return;
-  f32x4 acc[4] = {zero, zero, zero, zero};
+  f32x4 acc[MB];
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb) acc[mb] = zero;
   const float *arow = sa + col * LDA + 4 * q;
 #pragma unroll
   for (int ch = 0; ch < NCH; ++ch) {
-    f32x4 av[4];
+    f32x4 av[MB];
 #pragma unroll
-    for (int mb = 0; mb < 4; ++mb) av[mb] = *reinterpret_cast<const f32x4 *>(arow + mb * 16 * LDA + 16 * ch);
+    for (int mb = 0; mb < MB; ++mb) av[mb] = *reinterpret_cast<const f32x4 *>(arow + mb * 16 * LDA + 16 * ch);
     const f32x4 bv = nok ? bq[ch] : zero;
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
-      for (int mb = 0; mb < 4; ++mb)
+      for (int mb = 0; mb < MB; ++mb)
         acc[mb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[mb][t], bv[t], acc[mb], 0, 0, 0);
   }
   if (!nok) return;
   float *out = p.part + (long)s * p.M * p.N + n;
 #pragma unroll
-  for (int mb = 0; mb < 4; ++mb)
+  for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int m = mb * 16 + 4 * q + i;
@@ -269,13 +272,21 @@ hipError_t launch_gemm_part(const GemmPartArgs &a, hipStream_t st) {
   const int ns = gemm_part_slices(a.N, a.K, &ks);
   const int groups = (a.N + 16 * GP_WAVES - 1) / (16 * GP_WAVES);
   dim3 grid(groups, ns), block(64 * GP_WAVES);
+  const int mb = (a.M + 15) / 16;
+#define GP_LAUNCH(KS_)                                                                             \
+  do {                                                                                             \
+    if (mb == 1) hipLaunchKernelGGL((gemm_part_kernel<KS_, 1>), grid, block, 0, st, a);             \
+    else if (mb == 2) hipLaunchKernelGGL((gemm_part_kernel<KS_, 2>), grid, block, 0, st, a);        \
+    else hipLaunchKernelGGL((gemm_part_kernel<KS_, 4>), grid, block, 0, st, a);                     \
+  } while (0)
   switch (ks) {
-    case 32: hipLaunchKernelGGL(gemm_part_kernel<32>, grid, block, 0, st, a); break;
-    case 48: hipLaunchKernelGGL(gemm_part_kernel<48>, grid, block, 0, st, a); break;
-    case 64: hipLaunchKernelGGL(gemm_part_kernel<64>, grid, block, 0, st, a); break;
-    case 96: hipLaunchKernelGGL(gemm_part_kernel<96>, grid, block, 0, st, a); break;
-    default: hipLaunchKernelGGL(gemm_part_kernel<128>, grid, block, 0, st, a); break;
+    case 32: GP_LAUNCH(32); break;
+    case 48: GP_LAUNCH(48); break;
+    case 64: GP_LAUNCH(64); break;
+    case 96: GP_LAUNCH(96); break;
+    default: GP_LAUNCH(128); break;
   }
+#undef GP_LAUNCH
   return hipGetLastError();
 }
 

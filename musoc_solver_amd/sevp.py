@@ -95,6 +95,7 @@ def lib():
         L.sevp_set_precision.argtypes = [VP, CI]
         L.sevp_set_stream.argtypes = [VP, VP]
         L.sevp_set_graphs.argtypes = [VP, CI]
+        L.sevp_set_rhs_groups.argtypes = [VP, CI]
         L.sevp_profile_begin.argtypes = [VP, CI]
         L.sevp_profile_end.argtypes = [VP, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_long)]
@@ -210,6 +211,10 @@ class SevpPlan:
 
     def set_graphs(self, on: bool):
         _check(lib().sevp_set_graphs(self._h, 1 if on else 0))
+
+    def set_rhs_groups(self, groups: int):
+        """Run the refinement passes of `groups` RHS groups concurrently."""
+        _check(lib().sevp_set_rhs_groups(self._h, groups))
 
     def profile_begin(self, kclass):
         _check(lib().sevp_profile_begin(self._h, kclass))

@@ -324,3 +324,28 @@ def test_graph_replays_match_eager_launches(oracle, device, precision):
     assert n == st_p.iterations * per_pass, (n, st_p.iterations)
     assert ms > 0.0
     plan.close()
+
+
+@pytest.mark.parametrize("precision", [S.PREC_FP64, S.PREC_FP32])
+def test_rhs_groups_match_one_group(oracle, device, precision):
+    """Refinement passes of RHS groups on concurrent streams give bitwise the
+    result of one group: every right-hand side's pass is independent of the
+    others, and the split-K slicing depends only on I2."""
+    import torch
+    i2, j2, n0, nrhs = 96, 48, 3, 5
+    print(f"[groups] precision={precision}", flush=True)
+    plan, coefs, ie, s, s_d, x_d = _device_case(i2, j2, n0, nrhs, precision)
+    rel1, st1, ok1 = plan.solve_refined_dev(s_d, x_d, tol=1e-12, max_iter=6)
+    torch.cuda.synchronize()
+    x1 = x_d.clone()
+    for groups in (2, 3):
+        for graphs in (True, False):
+            plan.set_graphs(graphs)
+            plan.set_rhs_groups(groups)
+            x_d.zero_()
+            rel, st, ok = plan.solve_refined_dev(s_d, x_d, tol=1e-12, max_iter=6)
+            torch.cuda.synchronize()
+            assert st.iterations == st1.iterations, (groups, graphs)
+            assert torch.equal(x_d, x1), (groups, graphs)
+            assert np.array_equal(rel, rel1)
+    plan.close()
