@@ -61,15 +61,17 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--max-iter", type=int, default=12)
     ap.add_argument("--seed", type=int, default=3000)
-    ap.add_argument("--cpu-rhs", type=int, default=3, help="RHS in the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-rhs", type=int, default=16, help="RHS in the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=16, help="host threads of the CPU baseline")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--legs", default="mixed,plain", help="comma list of GPU legs to time")
     ap.add_argument("--rhs-groups", default="2,2",
                     help="RHS groups whose passes run concurrently, mixed,plain legs")
     ap.add_argument("--kernel-timing", choices=("after", "timed"), default="after",
                     help="time the correction kernel with HIP events in one extra solve after "
-                         "the timed steps (after) or inside them (timed: event nodes in the "
-                         "captured pass lengthen the timed region)")
+                         "the timed steps, launched eagerly with events from the kernel's own "
+                         "timestamps (after), or inside the timed steps through event nodes in "
+                         "the captured pass, which also count the dispatch gap (timed)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device (reference kernel for HBM counters)")
     return ap.parse_args()
@@ -149,18 +151,29 @@ def main():
         if a.kernel_timing == "timed":
             corr_ms, corr_n = plan.profile_end()
         else:
-            # the same solve once more, its correction products bracketed by
-            # HIP events (event-record nodes in the captured pass)
+            # the same solve once more with its kernels launched one by one,
+            # every correction product carrying HIP events recorded from its
+            # own start and end timestamps (hipExtLaunchKernelGGL)
+            plan.set_graphs(False)
             plan.profile_begin(S.KCLASS_CORR)
             x.zero_()
             plan.solve_refined_dev(s, x, tol=a.tol, max_iter=a.max_iter)
+            span_ms, span_n = plan.profile_spans()
             corr_ms, corr_n = plan.profile_end()
+            plan.set_graphs(True)
+        # HIP-event duration per launch (the kernel's own dispatch timestamps
+        # in the eager solve, event nodes in the timed graph replays); the
+        # span from first wavefront start to last wavefront end, from the GPU
+        # clock, is reported beside it
+        avg_us = 1e3 * corr_ms / max(corr_n, 1)
+        share_ms = corr_ms * (a.steps if a.kernel_timing == "after" else 1)
+        span_us = 1e3 * span_ms / span_n if a.kernel_timing == "after" and span_n else None
         r = {"solves_per_s": a.steps * nrhs / dt, "ms_per_step": 1e3 * dt / a.steps,
              "block_height": n0_of[name], "nb0": plan.nb0,
              "passes": float(np.mean(iters)), "max_rel_resid": worst, "failed_steps": fails,
-             "corr_avg_us": 1e3 * corr_ms / max(corr_n, 1), "corr_launches": corr_n,
-             "corr_share": (corr_ms * (a.steps if a.kernel_timing == "after" else 1)
-                            / (1e3 * dt)),
+             "corr_avg_us": avg_us, "corr_span_us": span_us,
+             "corr_launches": corr_n,
+             "corr_share": share_ms / (1e3 * dt),
              "kernel_timing": a.kernel_timing, "rhs_groups": plan_groups(name)}
         log(f"{name}: {r}")
         return r
@@ -197,16 +210,9 @@ def main():
         log(f"mixed vs plain FP64: max rel error {err:.3e}")
 
     main_leg = legs.get("mixed") or legs.get("plain")
-    # roofline of the dominant kernel: the correction product (FP32 MFMA GEMM)
-    flops = 2.0 * nrhs * i2 * i2
-    achieved = flops / (main_leg["corr_avg_us"] * 1e-6)
-    mixed_main = "mixed" in legs
-    roof = {"bound": "mfma", "achieved": achieved / 1e12,
-            "peak": (PEAK_FP32_MFMA if mixed_main else PEAK_FP64) / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / (PEAK_FP32_MFMA if mixed_main else PEAK_FP64),
-            "traffic": None, "kernel": "gemm_f32_mfma_kernel" if mixed_main else "gemm_seq_kernel",
-            "algorithmic_flops_per_launch": flops, "avg_launch_us": main_leg["corr_avg_us"]}
-
+    main_name = "mixed" if "mixed" in legs else "plain"
+    mixed_main = main_name == "mixed"
+    roof = roofline(main_leg, main_name, nrhs, i2, a)
     cpu = None
     if a.cpu_rhs > 0:
         cpu = cpu_baseline(plan_keep[0], case, plan_keep[1], s_np, a)
@@ -240,38 +246,125 @@ def main():
     plan_keep[0].close()
 
 
+def roofline(leg, name, nrhs, i2, a):
+    """Roofline of the dominant kernel, the influence-matrix correction
+    product, from its average launch duration (HIP events around every launch
+    of one solve).  Per refinement pass and RHS group of m right-hand sides
+    there are NB0-1 forward products (N = I2) and NB0 backward ones (N = 2 I2
+    for the NB0-1 blocks with a coupling row, N = I2 for the bottom block),
+    each with K = I2.  Algorithmic work per product: 2 m K N flops and
+    4 (m K + K N + m N) bytes (FP32 operands and result; the FP64 plain
+    product moves 8-byte elements).  The bound is whichever of MFMA time and
+    HBM time is larger for the average product."""
+    nb0, groups = leg["nb0"], leg["rhs_groups"]
+    m = -(-nrhs // groups)
+    n_fwd, n_bwd2, n_bwd1 = nb0 - 1, nb0 - 1, 1
+    launches = n_fwd + n_bwd2 + n_bwd1
+    ncols = [(i2, n_fwd), (2 * i2, n_bwd2), (i2, n_bwd1)]
+    esz = 4 if name == "mixed" else 8
+    flops = sum(2.0 * m * i2 * n * c for n, c in ncols) / launches
+    nbytes = sum(esz * (m * i2 + i2 * n + m * n) * c for n, c in ncols) / launches
+    t = leg["corr_avg_us"] * 1e-6
+    peak_f = PEAK_FP32_MFMA if name == "mixed" else PEAK_FP64
+    hbm_bound = nbytes / PEAK_HBM > flops / peak_f
+    traffic = measured_traffic(name, m, i2)
+    if hbm_bound:
+        r = {"bound": "hbm", "achieved": nbytes / t / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+             "frac": nbytes / t / PEAK_HBM}
+    else:
+        r = {"bound": "mfma", "achieved": flops / t / 1e12, "peak": peak_f / 1e12,
+             "unit": "TFLOP/s", "frac": flops / t / peak_f}
+    r.update({"traffic": traffic,
+              "kernel": "gemm_part_kernel" if name == "mixed" else "gemm_seq_kernel",
+              "algorithmic_bytes_per_launch": nbytes, "algorithmic_flops_per_launch": flops,
+              "avg_launch_us": leg["corr_avg_us"], "rhs_per_product": m})
+    return r
+
+
+def measured_traffic(name, m, i2):
+    """HBM bytes per launch of the correction product as measured by the
+    FETCH_SIZE / WRITE_SIZE counter runs recorded under profiles/ (scaled by
+    the same counters of a copy of known size), for this configuration; null
+    when no recorded run matches."""
+    path = os.path.join(ROOT, "profiles", "correction_product_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for r in rec.get("runs", []):
+        if r.get("leg") == name and r.get("rhs_per_product") == m and r.get("i2") == i2:
+            return r.get("hbm_bytes_per_launch")
+    return None
+
+
 def cpu_baseline(plan, case, ie, s_np, a):
-    """The oracle on a bounded sample of the same workload: each RHS solved to
-    the same tolerance by FP64 REP1 passes with residual refinement (the
-    residual passed to the next pass as the float32 right-hand side the
-    reference interface takes)."""
+    """The CPU path on a bounded sample of the same workload: right-hand
+    sides solved to the same tolerance by FP64 REP1 passes with residual
+    refinement (the residual passed to the next pass as the float32
+    right-hand side the reference interface takes), one right-hand side per
+    host thread.  REP1 is the reference's own routine compiled from
+    MAIN/template/GLOBAL/src/sevp.f90 (oracle/_ref) when that library is
+    present, else the C restatement; the influence matrices come from the
+    GPU build (bitwise the reference PRE1)."""
+    import threading
     from oracle import oracle as O
     O.build()
+    use_ref = O.ref_available()
     log("cpu baseline: copying influence matrices to the host")
     rinv, rinv1 = plan.get_influence()
     i2, j2 = case.shape
-    t_all, worst, passes = 0.0, 0.0, 0
-    for r in range(a.cpu_rhs):
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 1
+    threads = max(1, min(a.cpu_threads, ncpu, a.nrhs))
+    nrhs = min(a.cpu_rhs, a.nrhs)
+    results = [None] * nrhs
+
+    def solve(r):
         sr = np.asarray(s_np[:, :, r], np.float64)
         x = np.zeros((i2 + 2, j2 + 2), order="F")
-        t0 = time.perf_counter()
-        for _ in range(a.max_iter):
+        passes, rel = 0, 1.0
+        for _ in range(a.max_iter + 1):
             res = sr - O.sqprod(case.coefs, x)[1:-1, 1:-1]
             rel = float(np.linalg.norm(res) / np.linalg.norm(sr))
-            if rel <= a.tol:
+            if rel <= a.tol or passes >= a.max_iter:
                 break
             d = np.zeros((i2 + 2, j2 + 2), order="F")
-            O.rep1(case.coefs, rinv, rinv1, np.asfortranarray(res.astype(np.float32)), d, ie)
+            O.rep1(case.coefs, rinv, rinv1, np.asfortranarray(res.astype(np.float32)), d, ie,
+                   use_ref=use_ref)
             x += d
             passes += 1
-        t_all += time.perf_counter() - t0
-        worst = max(worst, rel)
-        log(f"cpu rhs {r}: {t_all:.2f} s cumulative, rel resid {rel:.2e}")
-    return {"value": a.cpu_rhs / t_all, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": f"{a.cpu_rhs} of the {a.nrhs} RHS, each refined to {a.tol:g} with FP64 "
-                      f"REP1 passes ({passes / a.cpu_rhs:.1f} per RHS, max rel resid {worst:.2e}); "
-                      "influence matrices from the GPU build",
-            "rel_resid": worst}
+        results[r] = (passes, rel)
+
+    def worker(t):
+        for r in range(t, nrhs, threads):
+            solve(r)
+
+    # one right-hand side alone first: the single-core time per solve
+    t0 = time.perf_counter()
+    solve(0)
+    t_one = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pool = [threading.Thread(target=worker, args=(t,)) for t in range(threads)]
+    for t in pool:
+        t.start()
+    for t in pool:
+        t.join()
+    t_all = time.perf_counter() - t0
+    passes = [p for p, _ in results]
+    worst = max(rel for _, rel in results)
+    log(f"cpu: 1 rhs in {t_one:.2f} s on one core; {nrhs} rhs in {t_all:.2f} s on {threads} threads, "
+        f"max rel resid {worst:.2e}")
+    return {"value": nrhs / t_all, "unit": "solves/s", "cores": threads,
+            "kind": "reference" if use_ref else "port",
+            "sample": f"{nrhs} of the {a.nrhs} RHS, one per thread, each refined to {a.tol:g} with "
+                      f"FP64 REP1 passes ({np.mean(passes):.1f} per RHS, max rel resid {worst:.2e}); "
+                      f"REP1 {'compiled from the reference source' if use_ref else 'C restatement'}, "
+                      "residuals by the C restatement of p_sqprod, influence matrices from the "
+                      "GPU build",
+            "single_core_s_per_solve": t_one, "rel_resid": worst}
 
 
 if __name__ == "__main__":

@@ -198,6 +198,12 @@ int sevp_profile_begin(sevp_plan plan, int kclass);
 /* Stops recording; total_ms and launches receive the summed device time and
  * the number of launches timed since sevp_profile_begin. */
 int sevp_profile_end(sevp_plan plan, double *total_ms, long *launches);
+/* Correction products launched outside a captured graph also time
+ * themselves from the GPU's real-time clock (first wavefront's start to last
+ * wavefront's end, the span a kernel trace reports; their HIP events, which
+ * sevp_profile_end sums, also span the dispatch latency).  Sum and count of
+ * those spans since sevp_profile_begin. */
+int sevp_profile_spans(sevp_plan plan, double *total_ms, long *launches);
 
 /* ------------------------------------------------------------- misc */
 const char *sevp_last_error(void);

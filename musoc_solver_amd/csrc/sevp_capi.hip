@@ -1,5 +1,6 @@
 // C ABI of the SEVP library: plans, influence-matrix builds, solves and the
 // one-to-one reference entry points.  See include/sevp.h for the contract.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -104,6 +105,41 @@ struct KProf {
     (void)hipEventRecord(pool[used].second, st);
     ++used;
   }
+  // Kernel spans from the kernel's own clock (self-timed launches outside
+  // graph capture): a (start, end) pair of 100 MHz ticks per workgroup.
+  static constexpr size_t kSpanWords = 8u << 20;
+  unsigned long long *spans = nullptr;  // device, kSpanWords
+  std::vector<std::pair<size_t, long>> span_runs;  // (offset, workgroups) per launch
+  size_t span_next = 0;
+  double span_ms = 0.0;
+  long span_launches = 0;
+  unsigned long long *next_span(int k, long wgs) {
+    if (!on(k) || capture_into || !spans || span_next + 2 * wgs > kSpanWords) return nullptr;
+    span_runs.push_back({span_next, wgs});
+    unsigned long long *t = spans + span_next;
+    span_next += 2 * wgs;
+    return t;
+  }
+  // Launches that time themselves (hipExtLaunchKernelGGL): the event pair for
+  // the next launch of class k outside graph capture, else nulls.  Each such
+  // launch is followed by stop_ext(k).
+  std::pair<hipEvent_t, hipEvent_t> start_ext(int k) {
+    if (!on(k) || capture_into) return {nullptr, nullptr};
+    if (used == pool.size()) {
+      hipEvent_t a, b;
+      if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return {nullptr, nullptr};
+      pool.push_back({a, b});
+    }
+    return pool[used];
+  }
+  void stop_ext(int k, bool launched_with_events) {
+    if (!on(k)) return;
+    if (capture_into) {
+      stop(k);  // records the captured node
+      return;
+    }
+    if (launched_with_events) ++used;
+  }
   // Accumulate the recorded pairs; the stream must be idle.
   void harvest() {
     for (size_t i = 0; i < used; ++i) {
@@ -125,7 +161,31 @@ struct KProf {
     pending.clear();
     (void)hipGetLastError();  // a pair that never ran must not fail a later launch check
   }
+  // Add up the launches timed since the last call; the stream must be idle.
+  void harvest_spans() {
+    if (span_runs.empty()) return;
+    std::vector<unsigned long long> h(span_next);
+    if (hipMemcpy(h.data(), spans, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost) ==
+        hipSuccess)
+      for (const auto &r : span_runs) {
+        unsigned long long t0 = ~0ull, t1 = 0;
+        bool all = true;
+        for (long w = 0; w < r.second; ++w) {
+          const unsigned long long a = h[r.first + 2 * w], b = h[r.first + 2 * w + 1];
+          if (!a || !b) all = false;
+          t0 = std::min(t0, a);
+          t1 = std::max(t1, b);
+        }
+        if (all && t1 > t0) {
+          span_ms += 1e-5 * (double)(t1 - t0);
+          ++span_launches;
+        }
+      }
+    span_runs.clear();
+    span_next = 0;
+  }
   ~KProf() {
+    if (spans) (void)hipFree(spans);
     for (auto &e : pool) {
       (void)hipEventDestroy(e.first);
       (void)hipEventDestroy(e.second);
@@ -168,7 +228,7 @@ struct sevp_plan_s {
   DBuf<float> w32;               // mixed: W_b = RINV_b diag(CY(:,IE(b-1)-1)) RINV1_(b-1), b >= 2
   // solve scratch
   DBuf<double> e, dum0, hfree, hcpl, g;  // full forms
-  DBuf<float> e32, g32;                  // mixed: FP32 copies of the error rows
+  DBuf<float> e32;                       // mixed: FP32 copy of the error rows
   DBuf<float> kpart;                     // mixed: split-K partial products (S, nrhs, N)
   DBuf<double> xx, bdum0;                // BIR
   DBuf<double> r64, dx, part;            // refinement
@@ -259,7 +319,7 @@ struct FullCtx {
   const float *rinv32, *rinv1_32;  // non-null selects the FP32 MFMA corrections
   const float *w32;                // mixed: coupling operators W_b (I2 x I2, b = 2..NB0)
   double *e, *dum0, *hfree, *hcpl, *g;
-  float *e32, *g32;  // FP32 copies of e and g (mixed precision), else null
+  float *e32;        // FP32 copy of e (mixed precision), else null
   float *kpart;      // mixed: split-K partial products
   KProf *prof;
 };
@@ -294,9 +354,11 @@ static int run_mixed_pass(const FullCtx &c, const float *f32, const double *f64,
     g.nsplit = c.i2;
     g.M = nrhs; g.N = n_prod; g.K = c.i2;
     g.part = c.kpart;
-    if (c.prof) c.prof->start(SEVP_KCLASS_CORR);
-    HIPC(launch_gemm_part(g, st));
-    if (c.prof) c.prof->stop(SEVP_KCLASS_CORR);
+    g.tspan = c.prof ? c.prof->next_span(SEVP_KCLASS_CORR, gemm_part_workgroups(g)) : nullptr;
+    const auto ev = c.prof ? c.prof->start_ext(SEVP_KCLASS_CORR)
+                           : std::pair<hipEvent_t, hipEvent_t>{nullptr, nullptr};
+    HIPC(launch_gemm_part(g, st, ev.first, ev.second));
+    if (c.prof) c.prof->stop_ext(SEVP_KCLASS_CORR, ev.first != nullptr);
     return SEVP_OK;
   };
   int js = 1;
@@ -376,38 +438,25 @@ static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, 
   const int i0 = c.i2 + 2, j0 = c.j2 + 2;
   (void)j0;
   const long dstride = (long)c.i2 * (c.nb0 > 1 ? c.nb0 - 1 : 1);
-  const bool mixed = c.rinv32 != nullptr;
-  auto corr = [&](const double *a, const float *a32, const double *b64, const float *b32, int ep,
-                  int xrow, double *dum0, double *out, double *g, int cyrow) -> int {
-    if (!mixed) {
-      GemmArgs gm{};
-      gm.a = a; gm.a_sm = c.i2; gm.a_sk = 1;
-      gm.b = b64; gm.b_sk = 1; gm.b_sn = c.i2;
-      gm.M = nrhs; gm.N = c.i2; gm.K = c.i2;
-      gm.ep = ep;
-      gm.x = x; gm.x_stride = x_stride; gm.xrow = xrow;
-      gm.dum0 = dum0; gm.dum0_stride = dstride;
-      gm.out = out; gm.out_ld = c.i2;
-      gm.g = g; gm.cy = c.cy; gm.cyrow = cyrow;
-      gm.i2 = c.i2;
-      if (c.prof) c.prof->start(SEVP_KCLASS_CORR);
-      HIPC(launch_gemm(gm, st));
-      if (c.prof) c.prof->stop(SEVP_KCLASS_CORR);
-    } else {
-      GemmF32Args gm{};
-      gm.a = a32;
-      gm.b = b32;
-      gm.M = nrhs; gm.N = c.i2; gm.K = c.i2;
-      gm.ep = ep;
-      gm.x = x; gm.x_stride = x_stride; gm.xrow = xrow;
-      gm.dum0 = dum0; gm.dum0_stride = dstride;
-      gm.out = out; gm.out_ld = c.i2;
-      gm.g = g; gm.g32 = c.g32; gm.cy = c.cy; gm.cyrow = cyrow;
-      gm.i2 = c.i2;
-      if (c.prof) c.prof->start(SEVP_KCLASS_CORR);
-      HIPC(launch_gemm_mixed(gm, st));
-      if (c.prof) c.prof->stop(SEVP_KCLASS_CORR);
-    }
+  // the plain FP64 pass (bitwise the reference REP1 / REP); the mixed pass is
+  // run_mixed_pass
+  auto corr = [&](const double *a, const double *b64, int ep, int xrow, double *dum0, double *out,
+                  double *g, int cyrow) -> int {
+    GemmArgs gm{};
+    gm.a = a; gm.a_sm = c.i2; gm.a_sk = 1;
+    gm.b = b64; gm.b_sk = 1; gm.b_sn = c.i2;
+    gm.M = nrhs; gm.N = c.i2; gm.K = c.i2;
+    gm.ep = ep;
+    gm.x = x; gm.x_stride = x_stride; gm.xrow = xrow;
+    gm.dum0 = dum0; gm.dum0_stride = dstride;
+    gm.out = out; gm.out_ld = c.i2;
+    gm.g = g; gm.cy = c.cy; gm.cyrow = cyrow;
+    gm.i2 = c.i2;
+    gm.tspan = c.prof ? c.prof->next_span(SEVP_KCLASS_CORR, gemm_workgroups(gm)) : nullptr;
+    const auto ev = c.prof ? c.prof->start_ext(SEVP_KCLASS_CORR)
+                           : std::pair<hipEvent_t, hipEvent_t>{nullptr, nullptr};
+    HIPC(launch_gemm(gm, st, ev.first, ev.second));
+    if (c.prof) c.prof->stop_ext(SEVP_KCLASS_CORR, ev.first != nullptr);
     return SEVP_OK;
   };
   (void)i0;
@@ -420,7 +469,7 @@ static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, 
     m.f = f32; m.f64 = f64; m.f_stride = f_stride;
     m.x = x; m.x_stride = x_stride;
     m.e = c.e;
-    m.e32 = mixed ? c.e32 : nullptr;
+    m.e32 = nullptr;
     m.i2 = c.i2; m.j2 = c.j2; m.periodic = c.periodic;
     m.js = js; m.jf = ieb - 2; m.ieb = ieb;
     m.resid = (nb != c.nb0);
@@ -431,9 +480,8 @@ static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, 
     HIPC(launch_march(MODE_PART, m, nrhs, st));
     if (c.prof) c.prof->stop(SEVP_KCLASS_MARCH);
     if (nb != c.nb0)
-      CHK(corr(c.e, c.e32, c.rinv1 + (long)(nb - 1) * ww,
-               mixed ? c.rinv1_32 + (long)(nb - 1) * ww : nullptr, EP_FWD, ieb,
-               c.dum0 + (long)(nb - 1) * c.i2, nullptr, nullptr, 0));
+      CHK(corr(c.e, c.rinv1 + (long)(nb - 1) * ww, EP_FWD, ieb, c.dum0 + (long)(nb - 1) * c.i2,
+               nullptr, nullptr, 0));
     js = ieb;
   }
   // backward: restore + closure residual of the top block, then per block the
@@ -447,7 +495,7 @@ static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, 
     r.dum0 = (nb != c.nb0) ? c.dum0 : nullptr;
     r.dum0_stride = dstride;
     r.e = c.e;
-    r.e32 = mixed ? c.e32 : nullptr;
+    r.e32 = nullptr;
     r.i2 = c.i2; r.j2 = c.j2; r.periodic = c.periodic; r.nb = nb; r.ieb = c.ie[nb - 1];
     r.nrhs = nrhs;
     return r;
@@ -462,32 +510,10 @@ static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, 
     const int nb = c.nb0 - nbs + 1;
     const int ieb = c.ie[nb - 1];
     js = (nb != 1) ? c.ie[nb - 2] : 1;
-    if (mixed) {
-      // one product against [RINV_b | W_b]: the free-row correction and, for
-      // b > 1, the coupling row at once
-      GemmF32Args gm{};
-      gm.a = c.e32;
-      gm.b = c.rinv32 + (long)(nb - 1) * ww;
-      gm.M = nrhs; gm.K = c.i2;
-      gm.N = (nb != 1) ? 2 * c.i2 : c.i2;
-      gm.b2 = (nb != 1) ? c.w32 + (long)(nb - 2) * ww : nullptr;
-      gm.nsplit = c.i2;
-      gm.out2 = c.hcpl;
-      gm.ep = EP_BWD;
-      gm.x = x; gm.x_stride = x_stride; gm.xrow = js + 1;
-      gm.out = c.hfree; gm.out_ld = c.i2;
-      gm.g = nullptr;
-      gm.i2 = c.i2;
-      if (c.prof) c.prof->start(SEVP_KCLASS_CORR);
-      HIPC(launch_gemm_mixed(gm, st));
-      if (c.prof) c.prof->stop(SEVP_KCLASS_CORR);
-    } else {
-      CHK(corr(c.e, c.e32, c.rinv + (long)(nb - 1) * ww, nullptr, EP_BWD, js + 1, nullptr,
-               c.hfree, nb != 1 ? c.g : nullptr, js - 1));
-      if (nb != 1)
-        CHK(corr(c.g, c.g32, c.rinv1 + (long)(nb - 2) * ww, nullptr, EP_CPL, 0, nullptr, c.hcpl,
-                 nullptr, 0));
-    }
+    CHK(corr(c.e, c.rinv + (long)(nb - 1) * ww, EP_BWD, js + 1, nullptr, c.hfree,
+             nb != 1 ? c.g : nullptr, js - 1));
+    if (nb != 1)
+      CHK(corr(c.g, c.rinv1 + (long)(nb - 2) * ww, EP_CPL, 0, nullptr, c.hcpl, nullptr, 0));
     MarchArgs m{};
     m.ax = c.ax; m.ay = c.ay; m.bb = c.bb; m.cx = c.cx; m.cy = c.cy;
     m.cld = c.i2;
@@ -749,7 +775,7 @@ size_t sevp_plan_device_bytes(sevp_plan p) {
   return p->coef.bytes() + p->ie_d.bytes() + p->rinv.bytes() + p->rinv1.bytes() +
          p->rinv32.bytes() + p->rinv1_32.bytes() + p->w32.bytes() + p->e.bytes() + p->dum0.bytes() +
          p->hfree.bytes() + p->hcpl.bytes() + p->g.bytes() + p->e32.bytes() +
-         p->g32.bytes() + p->kpart.bytes() + p->xx.bytes() +
+         p->kpart.bytes() + p->xx.bytes() +
          p->bdum0.bytes() + p->r64.bytes() + p->dx.bytes() + p->part.bytes() + p->b1s.bytes() +
          p->zp.bytes();
 }
@@ -855,7 +881,6 @@ static FullCtx full_ctx(sevp_plan p, bool mixed) {
   }
   c.e = p->e.p; c.dum0 = p->dum0.p; c.hfree = p->hfree.p; c.hcpl = p->hcpl.p; c.g = p->g.p;
   c.e32 = mixed ? p->e32.p : nullptr;
-  c.g32 = mixed ? p->g32.p : nullptr;
   c.kpart = mixed ? p->kpart.p : nullptr;
   c.prof = &p->prof;
   return c;
@@ -873,7 +898,6 @@ static FullCtx group_ctx(const FullCtx &c0, int r0) {
   c.g += r0 * i2;
   c.dum0 += r0 * dstride;
   if (c.e32) c.e32 += r0 * i2;
-  if (c.g32) c.g32 += r0 * i2;
   if (c.kpart) c.kpart += r0 * (long)gemm_part_slices((int)i2, (int)i2, nullptr) * 2 * i2;
   return c;
 }
@@ -884,7 +908,6 @@ static int alloc_mixed(sevp_plan p) {
   if (nr > 64) return fail(SEVP_E_ARG, "mixed precision solves at most 64 RHS per call");
   if (p->d.i2 % 4 != 0) return fail(SEVP_E_ARG, "mixed precision needs I2 divisible by 4");
   CHK(p->e32.alloc((size_t)nr * p->d.i2));
-  CHK(p->g32.alloc((size_t)nr * p->d.i2));
   {
     const int i2 = p->d.i2;
     const size_t n1 = (size_t)gemm_part_slices(i2, i2, nullptr) * i2;
@@ -914,17 +937,46 @@ int sevp_set_precision(sevp_plan p, int precision) {
   return SEVP_OK;
 }
 
+__global__ void span_init_kernel(unsigned long long *t, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) t[i] = 0ull;
+}
+
 int sevp_profile_begin(sevp_plan p, int kclass) {
   if (!p) return fail(SEVP_E_ARG, "null plan");
   if (kclass < SEVP_KCLASS_CORR || kclass > SEVP_KCLASS_RESID) return fail(SEVP_E_ARG, "bad kernel class");
   HIPC(hipStreamSynchronize(p->stream));
   p->prof.harvest();
-  p->prof.kclass = kclass;
-  p->prof.total_ms = 0.0;
-  p->prof.launches = 0;
-  p->prof.st = p->stream;
+  p->prof.harvest_spans();
+  KProf &k = p->prof;
+  if (!k.spans) HIPC(hipMalloc(&k.spans, KProf::kSpanWords * sizeof(unsigned long long)));
+  const long n = (long)KProf::kSpanWords;
+  hipLaunchKernelGGL(span_init_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream,
+                     k.spans, n);
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(p->stream));
+  k.kclass = kclass;
+  k.total_ms = 0.0;
+  k.launches = 0;
+  k.span_ms = 0.0;
+  k.span_launches = 0;
+  k.span_runs.clear();
+  k.span_next = 0;
+  k.st = p->stream;
   return SEVP_OK;
 }
+
+int sevp_profile_spans(sevp_plan p, double *total_ms, long *launches) {
+  if (!p) return fail(SEVP_E_ARG, "null plan");
+  HIPC(hipStreamSynchronize(p->stream));
+  for (auto st : p->gstreams) HIPC(hipStreamSynchronize(st));
+  p->prof.harvest_spans();
+  if (total_ms) *total_ms = p->prof.span_ms;
+  if (launches) *launches = p->prof.span_launches;
+  return SEVP_OK;
+}
+
+__global__ void empty_kernel() {}
 
 int sevp_profile_end(sevp_plan p, double *total_ms, long *launches) {
   if (!p) return fail(SEVP_E_ARG, "null plan");

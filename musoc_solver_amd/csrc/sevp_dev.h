@@ -39,3 +39,20 @@ __device__ __forceinline__ double read_lane(double v, int src) {
   int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
   return __hiloint2double(hi, lo);
 }
+
+// Kernel span from the kernel's own clock (profiling only, t null otherwise):
+// workgroup w stores the 100 MHz real-time counter at its entry into t[2w]
+// and, after its memory operations have completed, at its exit into
+// t[2w + 1] (plain stores, one slot pair per workgroup, so timing adds no
+// contention).  The launch's span is max(t[2w + 1]) - min(t[2w]).
+__device__ __forceinline__ long span_wg() {
+  return blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+}
+__device__ __forceinline__ void span_begin(unsigned long long *t) {
+  if (t && threadIdx.x == 0) t[2 * span_wg()] = __builtin_amdgcn_s_memrealtime();
+}
+__device__ __forceinline__ void span_end(unsigned long long *t) {
+  if (!t) return;
+  __syncthreads();
+  if (threadIdx.x == 0) t[2 * span_wg() + 1] = __builtin_amdgcn_s_memrealtime();
+}

@@ -22,6 +22,8 @@
 // LDS when it fits (larger orders use the blocked form in sevp_matinv.hip);
 // each element receives the same sequence of operations as in the reference,
 // only the independent element updates of one step run in parallel.
+#include <hip/hip_ext.h>
+
 #include "sevp_dev.h"
 #include "sevp_internal.h"
 
@@ -396,6 +398,7 @@ hipError_t launch_resid(const ResidArgs &a, hipStream_t st) {
 constexpr int GT_M = 16, GT_N = 16, GT_K = 64;
 
 __global__ __launch_bounds__(256) void gemm_seq_kernel(GemmArgs p) {
+  span_begin(p.tspan);
   // both tiles k-contiguous (rows of GT_K + 2 doubles, 16-byte aligned) so a
   // thread reads two consecutive k of each operand with one 128-bit load
   __shared__ __align__(16) double As[GT_M][GT_K + 2];
@@ -490,10 +493,9 @@ __global__ __launch_bounds__(256) void gemm_seq_kernel(GemmArgs p) {
     }
   }
   const int n = n0 + tn, m = m0 + tm;
-  if (n >= p.N || m >= p.M) return;
   const int i0 = p.i2 + 2;
   const double d = acc;
-  switch (p.ep) {
+  if (n < p.N && m < p.M) switch (p.ep) {
     case EP_FWD: {
       double *x = p.x + (long)m * p.x_stride;
       const long xi = f2(i0, n + 2, p.xrow);
@@ -517,6 +519,7 @@ __global__ __launch_bounds__(256) void gemm_seq_kernel(GemmArgs p) {
       p.out[(long)m + (long)n * p.out_ld] = d;
       break;
   }
+  span_end(p.tspan);
 }
 
 // T(i, k) = R(i, k) * CY(k, cyrow): columns of an n x n column-major matrix
@@ -536,9 +539,14 @@ hipError_t launch_scale_cols(const double *r, const float *cy, int cyrow, int i2
   return hipGetLastError();
 }
 
-hipError_t launch_gemm(const GemmArgs &a, hipStream_t st) {
+long gemm_workgroups(const GemmArgs &a) {
+  return (long)((a.N + GT_N - 1) / GT_N) * ((a.M + GT_M - 1) / GT_M);
+}
+
+hipError_t launch_gemm(const GemmArgs &a, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
   dim3 grid((a.N + GT_N - 1) / GT_N, (a.M + GT_M - 1) / GT_M);
-  hipLaunchKernelGGL(gemm_seq_kernel, grid, dim3(256), 0, st, a);
+  if (ev0) hipExtLaunchKernelGGL(gemm_seq_kernel, grid, dim3(256), 0, st, ev0, ev1, 0, a);
+  else hipLaunchKernelGGL(gemm_seq_kernel, grid, dim3(256), 0, st, a);
   return hipGetLastError();
 }
 

@@ -145,30 +145,7 @@ struct GemmArgs {  // D(m, n) = sum_k A(m, k) B(k, n), sequential k
   const float *cy;
   int cyrow;          // 1-based coefficient row of CY for the coupling
   int i2;
-};
-
-struct GemmF32Args {  // mixed precision: A x B in FP32, FP32 accumulate
-  const float *a;     // (M, K) row-major, ld K
-  const float *b;     // (K, N) col-major, ld K
-  // optional second operand: columns n >= nsplit come from b2 (col-major,
-  // ld K, column n - nsplit) and are written to out2 (M, nsplit) as they are
-  const float *b2;
-  int nsplit;
-  double *out2;
-  int M, N, K;
-  int ep;
-  double *x;
-  long x_stride;
-  int xrow;
-  double *dum0;
-  long dum0_stride;
-  double *out;
-  long out_ld;
-  double *g;
-  float *g32;         // EP_BWD: FP32 copy of g for the coupling product
-  const float *cy;
-  int cyrow;
-  int i2;
+  unsigned long long *tspan;  // profiling: kernel span slot (null: off)
 };
 
 // Split-K partial products of the mixed-precision corrections:
@@ -180,10 +157,15 @@ struct GemmPartArgs {
   int nsplit;
   int M, N, K;
   float *part;      // (S, M, N)
+  unsigned long long *tspan;  // profiling: kernel span slot (null: off)
 };
 // Slice length and number of slices used for an (N, K) product.
 int gemm_part_slices(int N, int K, int *kslice);
-hipError_t launch_gemm_part(const GemmPartArgs &a, hipStream_t st);
+// ev0 / ev1 (optional): events recorded with the kernel's own start and end
+// timestamps (hipExtLaunchKernelGGL), for profiling outside graph capture.
+hipError_t launch_gemm_part(const GemmPartArgs &a, hipStream_t st, hipEvent_t ev0 = nullptr,
+                            hipEvent_t ev1 = nullptr);
+long gemm_part_workgroups(const GemmPartArgs &a);  // grid size of launch_gemm_part
 
 hipError_t launch_bir_sweeps(const BirArgs &a, int nrhs, hipStream_t st);
 hipError_t launch_repbir(const RepbirArgs &a, int nrhs, hipStream_t st);
@@ -192,8 +174,9 @@ hipError_t launch_resid(const ResidArgs &a, hipStream_t st);
 // backward H march of one block fused with the restore + residual of the
 // block below (r may be null for the bottom block)
 hipError_t launch_hom_resid(const MarchArgs &m, const ResidArgs *r, hipStream_t st);
-hipError_t launch_gemm(const GemmArgs &a, hipStream_t st);
-hipError_t launch_gemm_mixed(const GemmF32Args &a, hipStream_t st);
+hipError_t launch_gemm(const GemmArgs &a, hipStream_t st, hipEvent_t ev0 = nullptr,
+                       hipEvent_t ev1 = nullptr);
+long gemm_workgroups(const GemmArgs &a);  // grid size of launch_gemm
 hipError_t launch_cvt_f32(const double *src, float *dst, long n, hipStream_t st);
 hipError_t launch_matinv(double *mats, long mat_stride, int n, int count, int *zero_pivots,
                          double *scratch, hipStream_t st);

@@ -99,6 +99,8 @@ def lib():
         L.sevp_profile_begin.argtypes = [VP, CI]
         L.sevp_profile_end.argtypes = [VP, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_long)]
+        L.sevp_profile_spans.argtypes = [VP, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_long)]
         _lib = L
     return _lib
 
@@ -215,6 +217,13 @@ class SevpPlan:
     def set_rhs_groups(self, groups: int):
         """Run the refinement passes of `groups` RHS groups concurrently."""
         _check(lib().sevp_set_rhs_groups(self._h, groups))
+
+    def profile_spans(self):
+        """(total ms, launches) of the self-timed correction products since
+        profile_begin, from the kernels' own clock."""
+        ms, n = ctypes.c_double(), ctypes.c_long()
+        _check(lib().sevp_profile_spans(self._h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
 
     def profile_begin(self, kclass):
         _check(lib().sevp_profile_begin(self._h, kclass))

@@ -349,3 +349,27 @@ def test_rhs_groups_match_one_group(oracle, device, precision):
             assert torch.equal(x_d, x1), (groups, graphs)
             assert np.array_equal(rel, rel1)
     plan.close()
+
+
+@pytest.mark.parametrize("precision", [S.PREC_FP64, S.PREC_FP32])
+def test_self_timed_products(oracle, device, precision):
+    """Profiled eager solves time every correction product from the GPU clock
+    and with HIP events; timing changes no result."""
+    import torch
+    i2, j2, n0, nrhs = 96, 48, 3, 4
+    print(f"[spans] precision={precision}", flush=True)
+    plan, coefs, ie, s, s_d, x_d = _device_case(i2, j2, n0, nrhs, precision)
+    plan.set_graphs(False)
+    rel0, st0, ok0 = plan.solve_refined_dev(s_d, x_d, tol=1e-12, max_iter=6)
+    torch.cuda.synchronize()
+    x0 = x_d.clone()
+    plan.profile_begin(S.KCLASS_CORR)
+    x_d.zero_()
+    rel, st, ok = plan.solve_refined_dev(s_d, x_d, tol=1e-12, max_iter=6)
+    span_ms, span_n = plan.profile_spans()
+    ev_ms, ev_n = plan.profile_end()
+    torch.cuda.synchronize()
+    assert torch.equal(x_d, x0)
+    assert span_n == ev_n > 0
+    assert 0.0 < span_ms <= ev_ms
+    plan.close()

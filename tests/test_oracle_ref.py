@@ -128,3 +128,33 @@ def test_planted_mistake_is_caught(oracle, tmp_path):
     assert not fields_equal(x_bad, x_ref)
     assert oracle.rel_residual(c.coefs, s, x_ref) < 1e-9
     assert oracle.rel_residual(c.coefs, s, x_bad) > 1e-6
+
+
+def test_reference_rep1_is_reentrant_across_threads(oracle):
+    """The CPU baseline times the compiled reference REP1 on several host
+    threads at once; concurrent calls on independent arrays must give
+    bitwise the sequential results (the routines keep no static state)."""
+    import threading
+    i2, j2, nrhs = 48, 24, 8
+    c, ie = case(i2, j2, 3, 9, "latlon")
+    rinv, rinv1, _ = oracle.pre1(c.coefs, ie)
+    s = synth.make_rhs(c.mask, 10, nrhs)
+    seq = []
+    for r in range(nrhs):
+        x = np.zeros((i2 + 2, j2 + 2), order="F")
+        oracle.rep1(c.coefs, rinv, rinv1, np.asfortranarray(s[:, :, r]), x, ie, use_ref=True)
+        seq.append(x)
+    out = [np.zeros((i2 + 2, j2 + 2), order="F") for _ in range(nrhs)]
+
+    def work(r):
+        for _ in range(10):
+            out[r][:] = 0.0
+            oracle.rep1(c.coefs, rinv, rinv1, np.asfortranarray(s[:, :, r]), out[r], ie, use_ref=True)
+
+    threads = [threading.Thread(target=work, args=(r,)) for r in range(nrhs)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    for r in range(nrhs):
+        assert np.array_equal(out[r], seq[r]), r
