@@ -334,16 +334,17 @@ struct FullCtx {
 // that row in the launch.
 static int run_mixed_pass(const FullCtx &c, const float *f32, const double *f64, long f_stride,
                           double *x, long x_stride, int nrhs, hipStream_t st) {
-  const long ww = (long)c.i2 * c.i2;
+  const long ww = lane_tiles_floats(c.i2, c.i2);  // one FP32 matrix in lane tiles
+  const int np = (c.i2 + 15) / 16 * 16;           // W's columns start at np in [RINV | W]
   int ks = 0;
   const int ns1 = gemm_part_slices(c.i2, c.i2, &ks);
-  const int ns2 = gemm_part_slices(2 * c.i2, c.i2, &ks);
+  const int ns2 = gemm_part_slices(2 * np, c.i2, &ks);
   auto part_of = [&](int n_prod) {
     PartRef r{};
     r.p = c.kpart;
     r.ld = n_prod;
     r.sstride = (long)nrhs * n_prod;
-    r.ns = (n_prod == c.i2) ? ns1 : ns2;
+    r.ns = (n_prod == 2 * np) ? ns2 : ns1;
     return r;
   };
   auto product = [&](const float *b, const float *b2, int n_prod) -> int {
@@ -351,7 +352,7 @@ static int run_mixed_pass(const FullCtx &c, const float *f32, const double *f64,
     g.a = c.e32;
     g.b = b;
     g.b2 = b2;
-    g.nsplit = c.i2;
+    g.nsplit = np;
     g.M = nrhs; g.N = n_prod; g.K = c.i2;
     g.part = c.kpart;
     g.tspan = c.prof ? c.prof->next_span(SEVP_KCLASS_CORR, gemm_part_workgroups(g)) : nullptr;
@@ -405,7 +406,7 @@ static int run_mixed_pass(const FullCtx &c, const float *f32, const double *f64,
     const int ieb = c.ie[nb - 1];
     js = (nb != 1) ? c.ie[nb - 2] : 1;
     // [RINV_b | W_b]: free-row correction and, for b > 1, the coupling row
-    const int n_prod = (nb != 1) ? 2 * c.i2 : c.i2;
+    const int n_prod = (nb != 1) ? 2 * np : c.i2;
     CHK(product(c.rinv32 + (long)(nb - 1) * ww, nb != 1 ? c.w32 + (long)(nb - 2) * ww : nullptr,
                 n_prod));
     MarchArgs m{};
@@ -413,7 +414,7 @@ static int run_mixed_pass(const FullCtx &c, const float *f32, const double *f64,
     m.cld = c.i2;
     m.x = x; m.x_stride = x_stride;
     m.corr = part_of(n_prod);
-    m.cpl_off = (nb != 1) ? c.i2 : -1;
+    m.cpl_off = (nb != 1) ? np : -1;
     m.i2 = c.i2; m.j2 = c.j2; m.periodic = c.periodic;
     m.js = js; m.jf = ieb - 2; m.ieb = ieb;
     m.w = c.i2;
@@ -782,8 +783,10 @@ size_t sevp_plan_device_bytes(sevp_plan p) {
 
 static int make_low_precision_copies(sevp_plan p) {
   if (p->d.precision == SEVP_PREC_FP64 || p->d.form == SEVP_FORM_BIR) return SEVP_OK;
-  HIPC(launch_cvt_f32(p->rinv.p, p->rinv32.p, (long)p->rinv.n, p->stream));
-  HIPC(launch_cvt_f32(p->rinv1.p, p->rinv1_32.p, (long)p->rinv1.n, p->stream));
+  const int nb1 = p->d.nb0 > 1 ? p->d.nb0 - 1 : 1;
+  HIPC(launch_cvt_lane_tiles(p->rinv.p, p->rinv32.p, p->d.i2, p->d.i2, p->d.nb0, p->stream));
+  HIPC(launch_cvt_lane_tiles(p->rinv1.p, p->rinv1_32.p, p->d.i2, p->d.i2, nb1, p->stream));
+  const long tw = lane_tiles_floats(p->d.i2, p->d.i2);
   // W_b = RINV_b diag(CY(:, IE(b-1)-1)) RINV1_(b-1): the coupling row of the
   // backward pass as one product with the error row (FP64, then FP32)
   const int i2 = p->d.i2;
@@ -802,7 +805,7 @@ static int make_low_precision_copies(sevp_plan p) {
       gm.ep = EP_COLMAJ;
       gm.out = w.p; gm.out_ld = i2;
       HIPC(launch_gemm(gm, p->stream));
-      HIPC(launch_cvt_f32(w.p, p->w32.p + (long)(nb - 2) * ww, ww, p->stream));
+      HIPC(launch_cvt_lane_tiles(w.p, p->w32.p + (long)(nb - 2) * tw, i2, i2, 1, p->stream));
     }
     HIPC(hipStreamSynchronize(p->stream));
   }
@@ -898,7 +901,7 @@ static FullCtx group_ctx(const FullCtx &c0, int r0) {
   c.g += r0 * i2;
   c.dum0 += r0 * dstride;
   if (c.e32) c.e32 += r0 * i2;
-  if (c.kpart) c.kpart += r0 * (long)gemm_part_slices((int)i2, (int)i2, nullptr) * 2 * i2;
+  if (c.kpart) c.kpart += r0 * (long)gemm_part_slices((int)i2, (int)i2, nullptr) * 2 * ((i2 + 15) / 16 * 16);
   return c;
 }
 
@@ -909,14 +912,16 @@ static int alloc_mixed(sevp_plan p) {
   if (p->d.i2 % 4 != 0) return fail(SEVP_E_ARG, "mixed precision needs I2 divisible by 4");
   CHK(p->e32.alloc((size_t)nr * p->d.i2));
   {
-    const int i2 = p->d.i2;
+    const int i2 = p->d.i2, np = (i2 + 15) / 16 * 16;
     const size_t n1 = (size_t)gemm_part_slices(i2, i2, nullptr) * i2;
-    const size_t n2 = (size_t)gemm_part_slices(2 * i2, i2, nullptr) * 2 * i2;
+    const size_t n2 = (size_t)gemm_part_slices(2 * np, i2, nullptr) * 2 * np;
     CHK(p->kpart.alloc(std::max(n1, n2) * nr));
   }
-  CHK(p->rinv32.alloc(p->rinv_count()));
-  CHK(p->rinv1_32.alloc(p->rinv1_count()));
-  CHK(p->w32.alloc(p->rinv1_count()));
+  const size_t tw = (size_t)lane_tiles_floats(p->d.i2, p->d.i2);
+  const size_t nb1 = p->d.nb0 > 1 ? p->d.nb0 - 1 : 1;
+  CHK(p->rinv32.alloc(tw * p->d.nb0));
+  CHK(p->rinv1_32.alloc(tw * nb1));
+  CHK(p->w32.alloc(tw * nb1));
   return SEVP_OK;
 }
 

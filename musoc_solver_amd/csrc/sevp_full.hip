@@ -141,6 +141,13 @@ __device__ void march_body(const MarchArgs &p, int tile, int bidx, int strip, in
 
   const int jlast = jf + 1;  // closure row, operands needed after the march
   load_row(js, ca, cb, cc, cd, ce, cf, cx);
+  // the closure residual's north row (the next block's free row) does not
+  // depend on the march: load it with the first rows
+  double xnorth[K];
+  if (MODE == MODE_PART && p.resid) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) xnorth[k] = x[f2(i0, pc[k] + 1, ieb + 1)];
+  }
   // split-K corrections, read after the row loads above are in flight
   if (MODE != MODE_PRE && p.corr.p) {
 #pragma unroll
@@ -198,8 +205,7 @@ __device__ void march_body(const MarchArgs &p, int tile, int bidx, int strip, in
     for (int k = 0; k < K; ++k) {
       double xw, xe;
       nbr<K>(vc, k, xw, xe);
-      const double xn = x[f2(i0, pc[k] + 1, ieb + 1)];
-      const double r = cf[k] - ca[k] * xw - cb[k] * vs[k] - cc[k] * vc[k] - cd[k] * xe - ce[k] * xn;
+      const double r = cf[k] - ca[k] * xw - cb[k] * vs[k] - cc[k] * vc[k] - cd[k] * xe - ce[k] * xnorth[k];
       if (out[k]) {
         if (p.e) p.e[(long)bidx * p.i2 + pc[k] - 1] = r;
         if (p.e32) p.e32[(long)bidx * p.i2 + pc[k] - 1] = (float)r;

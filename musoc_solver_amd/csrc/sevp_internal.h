@@ -152,9 +152,9 @@ struct GemmArgs {  // D(m, n) = sum_k A(m, k) B(k, n), sequential k
 // P[s][m][n] = sum over k in slice s of A(m, k) B(k, n) in FP32 (MFMA).
 struct GemmPartArgs {
   const float *a;   // (M, K) row-major, ld K
-  const float *b;   // (K, nsplit) col-major, ld K
-  const float *b2;  // columns n >= nsplit: col-major, ld K, column n - nsplit (may be null)
-  int nsplit;
+  const float *b;   // (K, nsplit) in lane tiles (launch_cvt_lane_tiles)
+  const float *b2;  // columns n >= nsplit: lane tiles, column n - nsplit (may be null)
+  int nsplit;       // a multiple of 16 when b2 is set
   int M, N, K;
   float *part;      // (S, M, N)
   unsigned long long *tspan;  // profiling: kernel span slot (null: off)
@@ -178,6 +178,11 @@ hipError_t launch_gemm(const GemmArgs &a, hipStream_t st, hipEvent_t ev0 = nullp
                        hipEvent_t ev1 = nullptr);
 long gemm_workgroups(const GemmArgs &a);  // grid size of launch_gemm
 hipError_t launch_cvt_f32(const double *src, float *dst, long n, hipStream_t st);
+// FP32 "lane tiles" of nmat consecutive (K x N) column-major FP64 matrices
+// (the B layout of launch_gemm_part); lane_tiles_floats is one matrix's size.
+long lane_tiles_floats(int K, int N);
+hipError_t launch_cvt_lane_tiles(const double *src, float *dst, int K, int N, long nmat,
+                                 hipStream_t st);
 hipError_t launch_matinv(double *mats, long mat_stride, int n, int count, int *zero_pivots,
                          double *scratch, hipStream_t st);
 bool matinv_uses_lds(int n);

@@ -48,8 +48,8 @@ __device__ __forceinline__ f32x4 load4m(const float *base, int k, int klen) {
 
 // KS (the slice length, a multiple of 16) is a compile-time constant, so the
 // loads, the LDS staging and the 4 x KS/4 MFMA steps are straight-line code.
-// Every global load of the workgroup is issued before the first wait: the B
-// columns of the slice into registers, then the A slice, which goes to LDS.
+// Every global load of the workgroup is issued before the first wait: the A
+// slice (to LDS), then the B chunks of the slice into registers.
 // MB (1, 2 or 4) 16-row accumulator blocks cover the right-hand sides; LDS
 // rows past M are zero.  Requires K % 4 == 0 (float4 runs of k are aligned).
 template <int KS, int MB>
@@ -69,13 +69,9 @@ __global__ __launch_bounds__(64 * GP_WAVES) void gemm_part_kernel(GemmPartArgs p
   const int col = lane & 15, q = lane >> 4;
   const int n = n0 + col;
   const bool nok = n < p.N;
-  const int nc = nok ? n : p.N - 1;
-  const float *bcol =
-      ((p.b2 && nc >= p.nsplit) ? p.b2 + (long)(nc - p.nsplit) * p.K : p.b + (long)nc * p.K) + k_lo;
-  const f32x4 zero = f32x4{0.f, 0.f, 0.f, 0.f};
-  f32x4 bq[NCH];
-#pragma unroll
-  for (int i = 0; i < NCH; ++i) bq[i] = load4m(bcol, 16 * i + 4 * q, klen);
+  const int nc = nok ? n0 : ((p.N - 1) / 16) * 16;
+  // A slice first (the LDS staging waits only for these), then the B
+  // chunks, which the MFMA loop consumes in order as they arrive
   f32x4 at[ALD];
 #pragma unroll
   for (int j = 0; j < ALD; ++j) {
@@ -83,6 +79,23 @@ __global__ __launch_bounds__(64 * GP_WAVES) void gemm_part_kernel(GemmPartArgs p
     const int m = idx / KQ, k = (idx % KQ) * 4;
     const bool ok = idx < ROWS * KQ && m < p.M;
     at[j] = load4m(p.a + (long)(ok ? m : 0) * p.K + k_lo, k, ok ? klen : 0);
+  }
+  // B in lane tiles: this wavefront's 16 columns are tile nt of b (or of b2
+  // past nsplit); chunk k_lo/16 + i is one 1 KiB block, this lane's float4 at
+  // lane position `lane`
+  const int nchk = (p.K + 15) / 16;
+  const int nt = nc / 16;
+  const bool second = p.b2 && nt >= p.nsplit / 16;
+  const f32x4 *btile = reinterpret_cast<const f32x4 *>(second ? p.b2 : p.b) +
+                       (long)(second ? nt - p.nsplit / 16 : nt) * nchk * 64;
+  const int ch0 = k_lo / 16;
+  const f32x4 zero = f32x4{0.f, 0.f, 0.f, 0.f};
+  f32x4 bq[NCH];
+#pragma unroll
+  for (int i = 0; i < NCH; ++i) {
+    const bool ok = ch0 + i < nchk;
+    const f32x4 v = btile[(long)(ok ? ch0 + i : 0) * 64 + lane];
+    bq[i] = ok ? v : zero;
   }
 #pragma unroll
   for (int j = 0; j < ALD; ++j) {
@@ -172,6 +185,37 @@ hipError_t launch_gemm_part(const GemmPartArgs &a, hipStream_t st, hipEvent_t ev
     case 96: gp_launch_mb<96>(a, grid, st, ev0, ev1); break;
     default: gp_launch_mb<128>(a, grid, st, ev0, ev1); break;
   }
+  return hipGetLastError();
+}
+
+// FP32 operand layout of the products ("lane tiles"): an (K x N) column-major
+// matrix becomes tiles of 16 columns; tile t holds ceil(K/16) chunks of 16 k,
+// each a 1 KiB block in the order the 64 lanes of a wavefront load it, lane
+// (q, c) = (l / 16, l % 16) holding k = 16 ch + 4q .. 4q + 3 of column
+// 16 t + c as one float4.  One chunk is one coalesced 1 KiB load, and the
+// chunks of a tile are consecutive; entries past K or N are zero.
+__global__ void cvt_lane_tiles_kernel(const double *src, float *dst, int K, int N, long nmat,
+                                      long src_stride, long dst_stride) {
+  const int nchk = (K + 15) / 16, ntile = (N + 15) / 16;
+  const long per = (long)ntile * nchk * 256;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= per * nmat) return;
+  const long mat = i / per, r = i % per;
+  const int t = (int)(r & 3), l = (int)((r >> 2) & 63);
+  const long blk = r >> 8;
+  const int ch = (int)(blk % nchk), tile = (int)(blk / nchk);
+  const int k = 16 * ch + 4 * (l >> 4) + t, n = 16 * tile + (l & 15);
+  dst[mat * dst_stride + r] =
+      (k < K && n < N) ? (float)src[mat * src_stride + (long)k + (long)n * K] : 0.f;
+}
+
+long lane_tiles_floats(int K, int N) { return (long)((N + 15) / 16) * ((K + 15) / 16) * 256; }
+
+hipError_t launch_cvt_lane_tiles(const double *src, float *dst, int K, int N, long nmat,
+                                 hipStream_t st) {
+  const long per = lane_tiles_floats(K, N), n = per * nmat;
+  hipLaunchKernelGGL(cvt_lane_tiles_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                     src, dst, K, N, nmat, (long)K * N, per);
   return hipGetLastError();
 }
 
