@@ -38,7 +38,7 @@ def lib():
         if not os.path.exists(LIB_PATH):
             build()
         _lib = ctypes.CDLL(LIB_PATH)
-        for name in ("orc_matinv", "orc_pre", "orc_pre1", "orc_bir_prep"):
+        for name in ("orc_matinv", "orc_pre", "orc_pre1", "orc_bir_prep", "orc_bir_prep_strips"):
             getattr(_lib, name).restype = ctypes.c_int
     return _lib
 
@@ -103,6 +103,25 @@ def bir_prep(coefs, ie, nbir):
     ie = np.ascontiguousarray(ie, np.int32)
     zp = lib().orc_bir_prep(P(al), P(ab), P(ac), P(ar), P(at), P(rinv), P(rinv1),
                             P(h), P(ie), _ci(i2), _ci(j2), _ci(nbir), _ci(nb0))
+    return rinv, rinv1, zp
+
+
+def bir_prep_strips(coefs, ie, nbir, strips):
+    """bir_prep for the listed strips only (0-based); the other strips' slices
+    are zero.  Each strip is computed exactly as in the full loop."""
+    al, ab, ac, ar, at = [f32(c) for c in coefs]
+    i2, j2 = al.shape
+    nb0 = len(ie)
+    ibir, _ = strip_geometry(i2, nbir)
+    rinv = np.zeros((ibir * ibir, nb0, nbir), order="F")
+    rinv1 = np.zeros((ibir * ibir, max(nb0 - 1, 1), nbir), order="F")
+    h = np.zeros((ibir + 2, j2 + 2), order="F")
+    ie = np.ascontiguousarray(ie, np.int32)
+    zp = 0
+    for n in strips:
+        zp += lib().orc_bir_prep_strips(P(al), P(ab), P(ac), P(ar), P(at), P(rinv), P(rinv1),
+                                        P(h), P(ie), _ci(i2), _ci(j2), _ci(nbir), _ci(nb0),
+                                        _ci(n + 1), _ci(n + 1))
     return rinv, rinv1, zp
 
 

@@ -436,15 +436,30 @@ static void seam_gather_f(const float *src, float *dst, int i2, int j2, int il,
  * Follows PREP/template/GLOBAL/src/prep_GLOBAL.f90:286-317.  RINV has the
  * reference shape (ibir*ibir, nb0, nbir), RINV1 (ibir*ibir, nb0-1, nbir).
  * h is scratch of (ibir+2) x (j2+2).  Returns the number of zero pivots. */
+int orc_bir_prep_strips(const float *al, const float *ab, const float *ac,
+                        const float *ar, const float *at, double *rinv, double *rinv1,
+                        double *h, const int *ie, int i2, int j2, int nbir, int nb0,
+                        int n_lo, int n_hi);
+
 int orc_bir_prep(const float *al, const float *ab, const float *ac,
                  const float *ar, const float *at, double *rinv, double *rinv1,
                  double *h, const int *ie, int i2, int j2, int nbir, int nb0)
+{
+  return orc_bir_prep_strips(al, ab, ac, ar, at, rinv, rinv1, h, ie, i2, j2, nbir, nb0, 1, nbir);
+}
+
+/* The same loop over strips n_lo..n_hi only (1-based, inclusive); the other
+ * strips' slices of RINV / RINV1 are left as they are. */
+int orc_bir_prep_strips(const float *al, const float *ab, const float *ac,
+                        const float *ar, const float *at, double *rinv, double *rinv1,
+                        double *h, const int *ie, int i2, int j2, int nbir, int nb0,
+                        int n_lo, int n_hi)
 {
   const int ibir = 2 * i2 / nbir, inc = ibir / 2;
   const long ibr2 = (long)ibir * ibir;
   int zp = 0;
   float *c[5] = {0};
-  for (int n = 1; n <= nbir; ++n) {
+  for (int n = n_lo; n <= n_hi; ++n) {
     int il = (n - 1) * inc + 1;
     double *rv = rinv + (long)(n - 1) * ibr2 * nb0;
     double *rv1 = rinv1 + (long)(n - 1) * ibr2 * (nb0 - 1);
