@@ -40,6 +40,7 @@ def lib():
         _lib = ctypes.CDLL(LIB_PATH)
         for name in ("orc_matinv", "orc_pre", "orc_pre1", "orc_bir_prep", "orc_bir_prep_strips"):
             getattr(_lib, name).restype = ctypes.c_int
+        _lib.orc_step14_correct.restype = ctypes.c_float
     return _lib
 
 
@@ -263,3 +264,33 @@ def rel_residual(coefs, s, x):
     r = sqprod(coefs, x)[1:-1, 1:-1]
     s = np.asarray(s, np.float64)
     return float(np.linalg.norm(s - r) / np.linalg.norm(s))
+
+
+def _i16(a):
+    assert a.dtype == np.int16 and a.flags.f_contiguous
+    return a
+
+
+def _f32w(a):
+    assert a.dtype == np.float32 and a.flags.f_contiguous, "REAL(4) Fortran-ordered array expected"
+    return a
+
+
+def step14_prepare(oc, u, v, w, s, mask=True, lopenw=0, harness=False, free_surface=True):
+    """First half of one step-14 ITMASK iteration (land masking, optional
+    harness override, W integration, RHS S); u, v, w, s updated in place."""
+    lib().orc_step14_prepare(
+        _ci(oc.i2), _ci(oc.j2), _ci(oc.k1), _ci(int(mask)), _ci(lopenw), _ci(int(harness)),
+        _ci(int(free_surface)), P(_i16(oc.kb)), P(_i16(oc.iu0)), P(_i16(oc.iv0)), P(oc.odx),
+        P(oc.ody), P(oc.ocs), P(oc.csv), P(oc.odz), P(_f32w(u)), P(_f32w(v)), P(_f32w(w)),
+        P(_f32w(s)))
+
+
+def step14_correct(oc, x, p0, scr, u, v, odt, periodic_we=False):
+    """Second half: P0, U, V and SCR updated in place; returns the largest
+    top-layer speed on land faces."""
+    assert x.dtype == np.float64 and x.flags.f_contiguous
+    return float(lib().orc_step14_correct(
+        _ci(oc.i2), _ci(oc.j2), _ci(oc.k1), _ci(int(periodic_we)), ctypes.c_float(odt),
+        P(_i16(oc.iu)), P(_i16(oc.iv)), P(_i16(oc.iu0)), P(_i16(oc.iv0)), P(oc.odx), P(oc.odyv),
+        P(x), P(_f32w(p0)), P(_f32w(scr)), P(_f32w(u)), P(_f32w(v))))

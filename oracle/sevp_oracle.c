@@ -29,6 +29,7 @@
  *     small matrices, B*inv(B) = I, brute-force influence matrices, and
  *     end-to-end solves that must reproduce scipy's sparse direct solution.
  */
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -574,3 +575,151 @@ void orc_sqprod(const float *al, const float *ab, const float *ac,
           (double)at[c] * x[IDX2(i0, i + 1, j + 2)];
     }
 }
+
+/* ---------------------------------------------------------------- step 14
+ * One ITMASK iteration of FS_GLOBAL step 14
+ * (MAIN/template/GLOBAL/src/fs_GLOBAL.f90:1154-1283), in two halves around
+ * the FS_SOLVER call.  Arrays have the reference shapes (Fortran order):
+ *   U (I0,J0,K1), V (I0,J1,K1), W (I0,J0,K0), P0 (I0,J0), SCR (I0,J0,>=4)
+ *   REAL(4); X (I0,J0) REAL(8); KB, IU0, IV0 (I0,J0), IU (I0,J0,K1),
+ *   IV (I0,J1,K1) INTEGER(2); ODX, ODY, OCS (J0), CSV, ODYV (J1), ODZ (K1)
+ *   REAL(4); S (I2,J2) REAL(4).
+ * Mixed-kind expressions follow Fortran promotion: an INTEGER(2) operand
+ * of a REAL(4) product is converted to REAL(4); REAL(4)*REAL(8) is
+ * evaluated in REAL(8) and rounded when stored into a REAL(4) array.
+ * Options (the reference selects them with cpp flags and namelist values):
+ *   mask          apply the land masking of U, V (:1158-1171) first
+ *   lopenw        LOPENW == 1: also mask the periodic faces I = 1, I1
+ *   harness       the shipped source's override U = 1, V = 0, W(:,:,K0) = 0
+ *                 (:1173-1176)
+ *   free_surface  FLAG_FS: integrate W upward, S = W(:,:,1) (:1178-1192);
+ *                 else downward, S = -W(:,:,KB+1) (:1193-1209)
+ *   periodic_we   FLAG_PERIODIC_WE: seam-face correction (:1230-1233)
+ */
+#define U3(a, i, j, k) (a)[((long)(i) - 1) + ((long)(j) - 1) * I0 + ((long)(k) - 1) * I0 * J0]
+#define V3(a, i, j, k) (a)[((long)(i) - 1) + ((long)(j) - 1) * I0 + ((long)(k) - 1) * I0 * J1]
+#define A2(a, i, j) (a)[((long)(i) - 1) + ((long)(j) - 1) * I0]
+
+void orc_step14_prepare(int i2, int j2, int k1, int mask, int lopenw, int harness,
+                        int free_surface, const short *kb, const short *iu0,
+                        const short *iv0, const float *odx, const float *ody,
+                        const float *ocs, const float *csv, const float *odz, float *u,
+                        float *v, float *w, float *s)
+{
+  const int I0 = i2 + 2, J0 = j2 + 2, I1 = I0 - 1, J1 = J0 - 1, I2 = i2, J2 = j2;
+  const int K1 = k1, K0 = k1 + 1;
+  if (mask) {
+    for (int J = 2; J <= J1; ++J) {
+      for (int I = 2; I <= I2; ++I) U3(u, I, J, 1) = (float)A2(iu0, I, J) * U3(u, I, J, 1);
+      if (lopenw == 1) {
+        U3(u, 1, J, 1) = (float)A2(iu0, 1, J) * U3(u, 1, J, 1);
+        U3(u, I1, J, 1) = (float)A2(iu0, I1, J) * U3(u, I1, J, 1);
+      }
+    }
+    for (int J = 2; J <= J2; ++J)
+      for (int I = 2; I <= I1; ++I) V3(v, I, J, 1) = (float)A2(iv0, I, J) * V3(v, I, J, 1);
+  }
+  if (harness) {
+    for (long n = 0; n < (long)I0 * J0 * K1; ++n) u[n] = 1.0f;
+    for (long n = 0; n < (long)I0 * J1 * K1; ++n) v[n] = 0.0f;
+    for (int J = 1; J <= J0; ++J)
+      for (int I = 1; I <= I0; ++I) U3(w, I, J, K0) = 0.0f;
+  }
+  if (free_surface) {
+    for (int K = K1; K >= 1; --K) {
+      const float temp1 = 1.0f / odz[K - 1];
+      for (int J = 2; J <= J1; ++J) {
+        const float temp = ocs[J - 1] * ody[J - 1];
+        for (int I = 2; I <= I1; ++I) {
+          const float a = (U3(u, I, J, K) - U3(u, I - 1, J, K)) * odx[J - 1];
+          const float b = (csv[J - 1] * V3(v, I, J, K) - csv[J - 2] * V3(v, I, J - 1, K)) * temp;
+          U3(w, I, J, K) = U3(w, I, J, K + 1) + (a + b) * temp1;
+        }
+      }
+    }
+    for (int J = 2; J <= J1; ++J)
+      for (int I = 2; I <= I1; ++I) s[(I - 2) + (long)(J - 2) * I2] = U3(w, I, J, 1);
+  } else {
+    for (int K = 1; K <= K1; ++K) {
+      const float temp1 = 1.0f / odz[K - 1];
+      for (int J = 2; J <= J1; ++J) {
+        const float temp = ocs[J - 1] * ody[J - 1];
+        for (int I = 2; I <= I1; ++I) {
+          const float a = (U3(u, I, J, K) - U3(u, I - 1, J, K)) * odx[J - 1];
+          const float b = (csv[J - 1] * V3(v, I, J, K) - csv[J - 2] * V3(v, I, J - 1, K)) * temp;
+          U3(w, I, J, K + 1) = U3(w, I, J, K) - (a + b) * temp1;
+        }
+      }
+    }
+    for (int J = 2; J <= J1; ++J)
+      for (int I = 2; I <= I1; ++I)
+        s[(I - 2) + (long)(J - 2) * I2] = -U3(w, I, J, A2(kb, I, J) + 1);
+  }
+}
+
+/* The half after the solve (:1214-1283): P0 += ODT*X, the U and V
+ * corrections and their running sums SCR(:,:,3:4), and the largest
+ * top-layer speed on land faces (the value compared with TOLERANCE). */
+float orc_step14_correct(int i2, int j2, int k1, int periodic_we, float odt,
+                         const short *iu, const short *iv, const short *iu0,
+                         const short *iv0, const float *odx, const float *odyv,
+                         const double *x, float *p0, float *scr, float *u, float *v)
+{
+  const int I0 = i2 + 2, J0 = j2 + 2, I1 = I0 - 1, J1 = J0 - 1, I2 = i2, J2 = j2;
+  const int K1 = k1;
+  for (int J = 2; J <= J1; ++J)
+    for (int I = 2; I <= I1; ++I)
+      A2(p0, I, J) = (float)((double)A2(p0, I, J) + (double)odt * A2(x, I, J));
+  for (int J = 2; J <= J1; ++J) {
+    for (int I = 2; I <= I2; ++I)
+#ifndef ORC_PLANT_STEP14_ROW
+      U3(scr, I, J, 1) = (float)(-((A2(x, I + 1, J) - A2(x, I, J)) * (double)odx[J - 1]));
+#else
+      /* planted slip for the tests: the metric of the next row */
+      U3(scr, I, J, 1) = (float)(-((A2(x, I + 1, J) - A2(x, I, J)) * (double)odx[J]));
+#endif
+    U3(scr, 1, J, 1) = 0.0f;
+    U3(scr, I1, J, 1) = 0.0f;
+    if (periodic_we) {
+      U3(scr, 1, J, 1) = (float)(-((A2(x, 2, J) - A2(x, I1, J)) * (double)odx[J - 1]));
+      U3(scr, I1, J, 1) = U3(scr, 1, J, 1);
+    }
+    for (int I = 1; I <= I1; ++I) {
+      U3(scr, I, J, 3) = U3(scr, I, J, 3) + U3(scr, I, J, 1);
+      U3(u, I, J, 1) = U3(u, I, J, 1) + U3(scr, I, J, 1);
+      for (int K = 2; K <= K1; ++K)
+        U3(u, I, J, K) = U3(u, I, J, K) + U3(scr, I, J, 1) * (float)U3(iu, I, J, K);
+    }
+  }
+  for (int J = 2; J <= J2; ++J)
+    for (int I = 2; I <= I1; ++I)
+      U3(scr, I, J, 2) = (float)(-((A2(x, I, J + 1) - A2(x, I, J)) * (double)odyv[J - 1]));
+  for (int I = 2; I <= I1; ++I) {
+    U3(scr, I, 1, 2) = 0.0f;
+    U3(scr, I, J1, 2) = 0.0f;
+  }
+  for (int J = 1; J <= J1; ++J)
+    for (int I = 2; I <= I1; ++I) {
+      U3(scr, I, J, 4) = U3(scr, I, J, 4) + U3(scr, I, J, 2);
+      V3(v, I, J, 1) = V3(v, I, J, 1) + U3(scr, I, J, 2);
+    }
+  for (int K = 2; K <= K1; ++K)
+    for (int J = 1; J <= J1; ++J)
+      for (int I = 2; I <= I1; ++I)
+        V3(v, I, J, K) = V3(v, I, J, K) + U3(scr, I, J, 2) * (float)V3(iv, I, J, K);
+  float tmp = 0.0f;
+  for (int J = 2; J <= J1; ++J)
+    for (int I = 2; I <= I2; ++I) {
+      const float t = (float)(1 - A2(iu0, I, J)) * fabsf(U3(u, I, J, 1));
+      tmp = tmp > t ? tmp : t;
+    }
+  for (int J = 2; J <= J2; ++J)
+    for (int I = 2; I <= I1; ++I) {
+      const float t = (float)(1 - A2(iv0, I, J)) * fabsf(V3(v, I, J, 1));
+      tmp = tmp > t ? tmp : t;
+    }
+  return tmp;
+}
+#undef U3
+#undef V3
+#undef A2
