@@ -187,6 +187,52 @@ int sevp_set_rhs_groups(sevp_plan plan, int groups);
  * and the mixed-precision solve. */
 int sevp_set_precision(sevp_plan plan, int precision);
 
+/* ---------------------------------------------------------- step 14
+ * The barotropic correction of FS_GLOBAL around the solve
+ * (MAIN/template/GLOBAL/src/fs_GLOBAL.f90:1148-1284) on the device: per
+ * ITMASK iteration, land masking of the top-layer velocities, W integrated
+ * over the layers into the RHS S, the plan's solve (the FS_SOLVER call, BIR
+ * or full width, warm-started from X), P0 += ODT*X, the U and V corrections
+ * with their running sums SCR(:,:,3:4), and the largest top-layer land speed,
+ * until it is <= TOLERANCE or MXMASK iterations have run.  The plan must have
+ * been built for one right-hand side of this grid (i2, j2) or more.  All
+ * array pointers are device pointers in the reference's shapes (Fortran
+ * order), updated in place; values are bitwise those of the reference's
+ * loops (REAL(4) arithmetic, REAL(8) where Fortran promotes). */
+typedef struct {
+  int i2, j2, k1;     /* I2, J2, K1 (K0 = K1 + 1) */
+  int free_surface;   /* FLAG_FS: S = W(:,:,1) integrated upward, else S = -W(:,:,KB+1) */
+  int periodic_we;    /* FLAG_PERIODIC_WE: correction of the seam face */
+  int lopenw;         /* LOPENW: 1 also masks the periodic faces I = 1, I1 */
+  int harness;        /* 1 reproduces the shipped source's U = 1, V = 0, W(:,:,K0) = 0 */
+  int mxmask;         /* MXMASK, >= 1 */
+  float tolerance;    /* TOLERANCE */
+  float odt;          /* ODT */
+} sevp_step14_desc;
+
+typedef struct {      /* INTEGER(2) masks and REAL(4) metrics, read only */
+  const short *kb, *iu0, *iv0;   /* (I0, J0) */
+  const short *iu;               /* (I0, J0, K1) */
+  const short *iv;               /* (I0, J1, K1) */
+  const float *odx, *ody, *ocs;  /* (J0) */
+  const float *csv, *odyv;       /* (J1) */
+  const float *odz;              /* (K1) */
+} sevp_step14_grid;
+
+typedef struct {      /* updated in place */
+  float *u;           /* (I0, J0, K1) */
+  float *v;           /* (I0, J1, K1) */
+  float *w;           /* (I0, J0, K0) */
+  float *p0;          /* (I0, J0) */
+  float *scr;         /* (I0, J0, 4); cleared at entry, as the reference does */
+  float *s;           /* (I2, J2) the RHS */
+  double *x;          /* (I0, J0) the solution, carried from call to call */
+} sevp_step14_state;
+
+/* itmask, tmp (optional): the iterations run and the final land speed. */
+int sevp_step14_dev(sevp_plan plan, const sevp_step14_desc *desc, const sevp_step14_grid *grid,
+                    const sevp_step14_state *state, int *itmask, float *tmp);
+
 /* -------------------------------------------------------- measurement
  * Kernel-class timing with HIP events recorded on the plan's stream around
  * every launch of that class while enabled (used by the benchmark to measure

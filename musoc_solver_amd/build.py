@@ -22,7 +22,8 @@ FSRC = os.path.join(PKG, "fortran")
 LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libsevp_hip.so")
 FLIB = os.path.join(LIBDIR, "libsevp_fortran.so")
-HIP_SOURCES = ["sevp_bir.hip", "sevp_full.hip", "sevp_matinv.hip", "sevp_mixed.hip", "sevp_capi.hip"]
+HIP_SOURCES = ["sevp_bir.hip", "sevp_full.hip", "sevp_matinv.hip", "sevp_mixed.hip", "sevp_caller.hip",
+               "sevp_capi.hip"]
 ARCH = "gfx950"
 
 # -ffp-contract=off: every multiply and add of the parity path is separately

@@ -234,6 +234,7 @@ struct sevp_plan_s {
   DBuf<double> r64, dx, part;            // refinement
   DBuf<double> b1s;                      // MATINV scratch (blocked path)
   DBuf<int> zp;
+  DBuf<float> s14;                       // step 14: per-workgroup maxima + the maximum
   const float *al() const { return coef.p; }
   const float *ab() const { return coef.p + (size_t)d.i2 * d.j2; }
   const float *ac() const { return coef.p + 2 * (size_t)d.i2 * d.j2; }
@@ -777,7 +778,7 @@ size_t sevp_plan_device_bytes(sevp_plan p) {
          p->rinv32.bytes() + p->rinv1_32.bytes() + p->w32.bytes() + p->e.bytes() + p->dum0.bytes() +
          p->hfree.bytes() + p->hcpl.bytes() + p->g.bytes() + p->e32.bytes() +
          p->kpart.bytes() + p->xx.bytes() +
-         p->bdum0.bytes() + p->r64.bytes() + p->dx.bytes() + p->part.bytes() + p->b1s.bytes() +
+         p->bdum0.bytes() + p->s14.bytes() + p->r64.bytes() + p->dx.bytes() + p->part.bytes() + p->b1s.bytes() +
          p->zp.bytes();
 }
 
@@ -1154,6 +1155,45 @@ static int get_pass_graph(sevp_plan p, const FullCtx &c0, int nrhs, int r0, int 
   }
   p->graphs.push_back(std::move(g));
   *out = &p->graphs.back();
+  return SEVP_OK;
+}
+
+int sevp_step14_dev(sevp_plan p, const sevp_step14_desc *d, const sevp_step14_grid *g,
+                    const sevp_step14_state *st, int *itmask, float *tmp) {
+  if (!p || !d || !g || !st) return fail(SEVP_E_ARG, "null argument");
+  if (d->i2 != p->d.i2 || d->j2 != p->d.j2) return fail(SEVP_E_ARG, "step 14 grid differs from the plan's");
+  if (d->k1 < 1 || d->mxmask < 1) return fail(SEVP_E_ARG, "K1 and MXMASK must be >= 1");
+  if (!p->have_inv) return fail(SEVP_E_STATE, "influence matrices not built or loaded");
+  if (!g->kb || !g->iu0 || !g->iv0 || !g->iu || !g->iv || !g->odx || !g->ody || !g->ocs ||
+      !g->csv || !g->odyv || !g->odz || !st->u || !st->v || !st->w || !st->p0 || !st->scr ||
+      !st->s || !st->x)
+    return fail(SEVP_E_ARG, "null array");
+  const long np = step14_partials(d->i2, d->j2);
+  if (!p->s14.p) CHK(p->s14.alloc((size_t)np + 1));
+  Step14Args a{};
+  a.i2 = d->i2; a.j2 = d->j2; a.k1 = d->k1;
+  a.mask = 1; a.lopenw = d->lopenw; a.harness = d->harness;
+  a.free_surface = d->free_surface; a.periodic_we = d->periodic_we;
+  a.odt = d->odt;
+  a.kb = g->kb; a.iu0 = g->iu0; a.iv0 = g->iv0; a.iu = g->iu; a.iv = g->iv;
+  a.odx = g->odx; a.ody = g->ody; a.ocs = g->ocs; a.csv = g->csv; a.odyv = g->odyv; a.odz = g->odz;
+  a.u = st->u; a.v = st->v; a.w = st->w; a.p0 = st->p0; a.scr = st->scr; a.s = st->s; a.x = st->x;
+  a.tmp_part = p->s14.p;
+  hipStream_t s = p->stream;
+  HIPC(launch_step14_zero_scr(st->scr, 4L * (d->i2 + 2) * (d->j2 + 2), s));
+  int it = 0;
+  float t = 0.f;
+  for (;;) {
+    ++it;
+    HIPC(launch_step14_prepare(a, s));
+    CHK(sevp_solve_dev(p, st->s, st->x, 1, s));
+    HIPC(launch_step14_correct(a, p->s14.p + np, s));
+    HIPC(hipMemcpyAsync(&t, p->s14.p + np, sizeof(float), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    if (t <= d->tolerance || it >= d->mxmask) break;
+  }
+  if (itmask) *itmask = it;
+  if (tmp) *tmp = t;
   return SEVP_OK;
 }
 

@@ -167,6 +167,25 @@ hipError_t launch_gemm_part(const GemmPartArgs &a, hipStream_t st, hipEvent_t ev
                             hipEvent_t ev1 = nullptr);
 long gemm_part_workgroups(const GemmPartArgs &a);  // grid size of launch_gemm_part
 
+// Step 14 of FS_GLOBAL (the barotropic correction around the solve); arrays
+// have the reference shapes, Fortran order (see include/sevp.h).
+struct Step14Args {
+  int i2, j2, k1;
+  int mask, lopenw, harness, free_surface, periodic_we;
+  float odt;
+  const short *kb, *iu0, *iv0, *iu, *iv;
+  const float *odx, *ody, *ocs, *csv, *odyv, *odz;
+  float *u, *v, *w, *p0, *scr, *s;
+  const double *x;
+  float *tmp_part;  // step14_partials(i2, j2) per-workgroup maxima
+};
+long step14_partials(int i2, int j2);
+// masking (if mask), harness override (if harness), W integration and S
+hipError_t launch_step14_prepare(const Step14Args &p, hipStream_t st);
+// P0, U, V, SCR corrections; the largest top-layer land speed to *tmp_dev
+hipError_t launch_step14_correct(const Step14Args &p, float *tmp_dev, hipStream_t st);
+hipError_t launch_step14_zero_scr(float *scr, long n, hipStream_t st);
+
 hipError_t launch_bir_sweeps(const BirArgs &a, int nrhs, hipStream_t st);
 hipError_t launch_repbir(const RepbirArgs &a, int nrhs, hipStream_t st);
 hipError_t launch_march(int mode, const MarchArgs &a, int nbatch, hipStream_t st);

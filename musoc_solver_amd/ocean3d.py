@@ -101,7 +101,7 @@ def assemble_operator_3d(odx, ody, ocs, csv, odyv, odz, iu, iv, i2, j2, nbir=1,
 
 def make_ocean3d(i2: int, j2: int, k1: int, seed: int, lat_max: float = 60.0,
                  land_frac: float = 0.3, periodic: bool = False, free_surface: bool = True,
-                 dz=None, dt: float = 1800.0) -> Ocean3D:
+                 dz=None, dt: float = 1800.0, nbir: int = 1) -> Ocean3D:
     i0, j0 = i2 + 2, j2 + 2
     i1, j1 = i0 - 1, j0 - 1
     g = make_grid(i2, j2, "latlon", lat_max)
@@ -159,7 +159,7 @@ def make_ocean3d(i2: int, j2: int, k1: int, seed: int, lat_max: float = 60.0,
             iv0[i, j] = iv[i, j, 0]
             iv[i, j, 0] = 1
     coefs, gamma = assemble_operator_3d(odx, ody, ocs, csv, odyv, odz, iu, iv, i2, j2,
-                                        nbir=1, free_surface=free_surface, dt=dt)
+                                        nbir=nbir, free_surface=free_surface, dt=dt)
     f = lambda a, t: _fa(a, t)
     return Ocean3D(i2, j2, k1, periodic, f(odx, F32), f(ody, F32), f(ocs, F32), f(csv, F32),
                    f(odyv, F32), f(odz, F32), f(kb, np.int16), f(iu0, np.int16),

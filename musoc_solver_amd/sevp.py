@@ -96,6 +96,9 @@ def lib():
         L.sevp_set_stream.argtypes = [VP, VP]
         L.sevp_set_graphs.argtypes = [VP, CI]
         L.sevp_set_rhs_groups.argtypes = [VP, CI]
+        L.sevp_step14_dev.argtypes = [VP, ctypes.POINTER(Step14Desc), ctypes.POINTER(Step14Grid),
+                                      ctypes.POINTER(Step14State), ctypes.POINTER(CI),
+                                      ctypes.POINTER(ctypes.c_float)]
         L.sevp_profile_begin.argtypes = [VP, CI]
         L.sevp_profile_end.argtypes = [VP, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_long)]
@@ -142,6 +145,23 @@ def version() -> str:
 
 
 # ------------------------------------------------------------------ plans
+
+class Step14Desc(ctypes.Structure):
+    """sevp_step14_desc (include/sevp.h)."""
+    _fields_ = [("i2", ctypes.c_int), ("j2", ctypes.c_int), ("k1", ctypes.c_int),
+                ("free_surface", ctypes.c_int), ("periodic_we", ctypes.c_int),
+                ("lopenw", ctypes.c_int), ("harness", ctypes.c_int), ("mxmask", ctypes.c_int),
+                ("tolerance", ctypes.c_float), ("odt", ctypes.c_float)]
+
+
+class Step14Grid(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in
+                ("kb", "iu0", "iv0", "iu", "iv", "odx", "ody", "ocs", "csv", "odyv", "odz")]
+
+
+class Step14State(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("u", "v", "w", "p0", "scr", "s", "x")]
+
 
 class SevpPlan:
     """A device-resident SEVP problem: operator, IE, influence matrices and
@@ -261,6 +281,20 @@ class SevpPlan:
         nrhs = nrhs or s.shape[0]
         self._follow_torch_stream()
         _check(lib().sevp_solve_dev(self._h, _p(s), _p(x), nrhs, None))
+
+    def step14_dev(self, desc: Step14Desc, grid: dict, state: dict):
+        """FS_GLOBAL step 14 on the device (sevp_step14_dev).  grid and state
+        map the sevp_step14_grid / sevp_step14_state field names to torch
+        device tensors laid out in the reference's Fortran shapes (a Fortran
+        (A, B, C) array is a contiguous tensor of shape (C, B, A)).  Returns
+        (iterations, final land speed)."""
+        self._follow_torch_stream()
+        g = Step14Grid(**{k: _p(grid[k]).value for k, _ in Step14Grid._fields_})
+        st = Step14State(**{k: _p(state[k]).value for k, _ in Step14State._fields_})
+        it, tmp = ctypes.c_int(), ctypes.c_float()
+        _check(lib().sevp_step14_dev(self._h, ctypes.byref(desc), ctypes.byref(g),
+                                     ctypes.byref(st), ctypes.byref(it), ctypes.byref(tmp)))
+        return it.value, tmp.value
 
     def solve_refined_dev(self, s, x, tol=1e-10, max_iter=20, nrhs=None):
         nrhs = nrhs or s.shape[0]
