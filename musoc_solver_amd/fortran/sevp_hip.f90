@@ -28,10 +28,25 @@ module sevp_hip
     integer(c_int) :: form, nbir, nsweep, periodic, nrhs_max, precision, seam_fix
   end type
 
+  ! Step 14 of FS_GLOBAL on the device (sevp_step14_dev): all array members
+  ! are device addresses of the reference's arrays (see include/sevp.h).
+  type, bind(c), public :: sevp_step14_desc
+    integer(c_int) :: i2, j2, k1, free_surface, periodic_we, lopenw, harness, mxmask
+    real(c_float) :: tolerance, odt
+  end type
+
+  type, bind(c), public :: sevp_step14_grid
+    type(c_ptr) :: kb, iu0, iv0, iu, iv, odx, ody, ocs, csv, odyv, odz
+  end type
+
+  type, bind(c), public :: sevp_step14_state
+    type(c_ptr) :: u, v, w, p0, scr, s, x
+  end type
+
   public :: c_sevp_repbir, c_sevp_rep1, c_sevp_rep, c_sevp_repbir_period, c_sevp_pre, &
             c_sevp_pre1, c_sevp_matinv, sevp_plan_create, sevp_plan_destroy, &
             sevp_build_influence, sevp_load_influence, sevp_get_influence, sevp_solve, &
-            sevp_reset_seam, sevp_check
+            sevp_reset_seam, sevp_step14_dev, sevp_check
 
   interface
     integer(c_int) function c_sevp_repbir(ax, ay, bb, cx, cy, rinv, rinv1, dum0, dum1, dum2, &
@@ -145,6 +160,17 @@ module sevp_hip
     integer(c_int) function sevp_reset_seam(plan) bind(c, name="sevp_reset_seam")
       import :: c_int, c_ptr
       type(c_ptr), value :: plan
+    end function
+
+    integer(c_int) function sevp_step14_dev(plan, desc, grid, state, itmask, tmp) &
+        bind(c, name="sevp_step14_dev")
+      import :: c_int, c_float, c_ptr, sevp_step14_desc, sevp_step14_grid, sevp_step14_state
+      type(c_ptr), value :: plan
+      type(sevp_step14_desc), intent(in) :: desc
+      type(sevp_step14_grid), intent(in) :: grid
+      type(sevp_step14_state), intent(in) :: state
+      integer(c_int), intent(out) :: itmask
+      real(c_float), intent(out) :: tmp
     end function
 
     type(c_ptr) function sevp_last_error_c() bind(c, name="sevp_last_error")
