@@ -38,6 +38,7 @@ extern "C" {
 #define SEVP_E_NOMEM (-3)    /* device allocation failed */
 #define SEVP_E_STATE (-4)    /* plan not ready (influence matrices missing) */
 #define SEVP_E_NOTCONV (-5)  /* refinement did not reach the tolerance */
+#define SEVP_E_IO (-6)       /* file missing, unreadable or not the expected record */
 
 /* Solver form (the reference's FL_EVP_STP). */
 #define SEVP_FORM_FULL_PERIODIC 0 /* REP1 + PRE1: full width, periodic east-west (FL_EVP_STP=0) */
@@ -250,6 +251,34 @@ int sevp_profile_end(sevp_plan plan, double *total_ms, long *launches);
  * sevp_profile_end sums, also span the dispatch latency).  Sum and count of
  * those spans since sevp_profile_begin. */
 int sevp_profile_spans(sevp_plan plan, double *total_ms, long *launches);
+
+/* ------------------------------------------------------------- EVP file
+ * The operator and influence matrices as one big-endian Fortran unformatted
+ * sequential record (OPEN(..., CONVERT='BIG_ENDIAN', FORM='unformatted')).
+ * PREP writes AL,AR,AB,AT,AC,RINV,RINV1,IE
+ * (PREP/template/GLOBAL/src/prep_GLOBAL.f90:320); MAIN reads
+ * AL,AR,AB,AT,AC,OAC,RINV,RINV1,IE (MAIN/template/GLOBAL/src/input_GLOBAL.f90:
+ * 438-443), so the two layouts differ by OAC.  Shapes follow desc (i2, j2,
+ * nb0, ie, form, nbir): coefficients (I2, J2) REAL(4); RINV, RINV1 REAL(8) as
+ * in sevp_load_influence; IE(NB0) INTEGER(4).  Records longer than
+ * 2147483639 bytes are split into subrecords with negated continuation
+ * lengths (gfortran's convention); max_subrecord > 0 sets a smaller split
+ * size (0 = the largest).  These calls run on the host only (no device is
+ * touched); failures return SEVP_E_ARG (shapes) or SEVP_E_IO (the file) with
+ * the message in sevp_last_error. */
+
+/* Write the record; oac = NULL writes PREP's layout, else MAIN's.  desc->ie
+ * supplies IE. */
+int sevp_evp_write(const char *path, const sevp_desc *desc, const float *al, const float *ab,
+                   const float *ac, const float *ar, const float *at, const double *rinv,
+                   const double *rinv1, const float *oac, long max_subrecord);
+/* Read the record.  with_oac: 0 = PREP's layout, 1 = MAIN's, -1 = whichever
+ * the record length matches (*had_oac, if given, says which).  oac may be
+ * NULL (the values are skipped).  A record whose length matches neither
+ * layout is rejected with SEVP_E_IO. */
+int sevp_evp_read(const char *path, const sevp_desc *desc, float *al, float *ab, float *ac,
+                  float *ar, float *at, double *rinv, double *rinv1, int *ie, float *oac,
+                  int with_oac, int *had_oac);
 
 /* ------------------------------------------------------------- misc */
 const char *sevp_last_error(void);

@@ -25,6 +25,10 @@ static int fail(int code, const std::string &msg) {
   return code;
 }
 
+namespace sevp {
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+}  // namespace sevp
+
 #define HIPC(expr)                                                                  \
   do {                                                                              \
     hipError_t _e = (expr);                                                         \

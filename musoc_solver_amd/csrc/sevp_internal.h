@@ -2,7 +2,12 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 namespace sevp {
+
+// Record msg as the calling thread's sevp_last_error text; returns code.
+int set_error(int code, const std::string &msg);
 
 // BIR multiplicative sweeps, one wavefront per right-hand side.
 struct BirArgs {

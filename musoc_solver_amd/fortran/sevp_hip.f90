@@ -46,7 +46,7 @@ module sevp_hip
   public :: c_sevp_repbir, c_sevp_rep1, c_sevp_rep, c_sevp_repbir_period, c_sevp_pre, &
             c_sevp_pre1, c_sevp_matinv, sevp_plan_create, sevp_plan_destroy, &
             sevp_build_influence, sevp_load_influence, sevp_get_influence, sevp_solve, &
-            sevp_reset_seam, sevp_step14_dev, sevp_check
+            sevp_reset_seam, sevp_step14_dev, sevp_evp_write, sevp_evp_read, sevp_check
 
   interface
     integer(c_int) function c_sevp_repbir(ax, ay, bb, cx, cy, rinv, rinv1, dum0, dum1, dum2, &
@@ -171,6 +171,32 @@ module sevp_hip
       type(sevp_step14_state), intent(in) :: state
       integer(c_int), intent(out) :: itmask
       real(c_float), intent(out) :: tmp
+    end function
+
+    ! The EVP record (include/sevp.h): path is a NUL-terminated string
+    ! (trim(name)//c_null_char); oac is c_loc(OAC) or c_null_ptr.
+    integer(c_int) function sevp_evp_write(path, desc, al, ab, ac, ar, at, rinv, rinv1, oac, &
+                                           max_subrecord) bind(c, name="sevp_evp_write")
+      import :: c_int, c_char, c_float, c_double, c_long, c_ptr, sevp_desc
+      character(kind=c_char), intent(in) :: path(*)
+      type(sevp_desc), intent(in) :: desc
+      real(c_float), intent(in) :: al(*), ab(*), ac(*), ar(*), at(*)
+      real(c_double), intent(in) :: rinv(*), rinv1(*)
+      type(c_ptr), value :: oac
+      integer(c_long), value :: max_subrecord
+    end function
+
+    integer(c_int) function sevp_evp_read(path, desc, al, ab, ac, ar, at, rinv, rinv1, ie, oac, &
+                                          with_oac, had_oac) bind(c, name="sevp_evp_read")
+      import :: c_int, c_char, c_float, c_double, c_ptr, sevp_desc
+      character(kind=c_char), intent(in) :: path(*)
+      type(sevp_desc), intent(in) :: desc
+      real(c_float), intent(out) :: al(*), ab(*), ac(*), ar(*), at(*)
+      real(c_double), intent(out) :: rinv(*), rinv1(*)
+      integer(c_int), intent(out) :: ie(*)
+      type(c_ptr), value :: oac
+      integer(c_int), value :: with_oac
+      integer(c_int), intent(out) :: had_oac
     end function
 
     type(c_ptr) function sevp_last_error_c() bind(c, name="sevp_last_error")

@@ -34,6 +34,7 @@ PREC_BF16 = 2
 
 SEVP_OK = 0
 SEVP_E_NOTCONV = -5
+SEVP_E_IO = -6
 KCLASS_CORR = 0
 KCLASS_MARCH = 1
 KCLASS_RESID = 2
@@ -100,6 +101,9 @@ def lib():
                                       ctypes.POINTER(Step14State), ctypes.POINTER(CI),
                                       ctypes.POINTER(ctypes.c_float)]
         L.sevp_profile_begin.argtypes = [VP, CI]
+        L.sevp_evp_write.argtypes = [ctypes.c_char_p, ctypes.POINTER(_Desc)] + [VP] * 8 + [ctypes.c_long]
+        L.sevp_evp_read.argtypes = ([ctypes.c_char_p, ctypes.POINTER(_Desc)] + [VP] * 9
+                                    + [CI, ctypes.POINTER(CI)])
         L.sevp_profile_end.argtypes = [VP, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_long)]
         L.sevp_profile_spans.argtypes = [VP, ctypes.POINTER(ctypes.c_double),
@@ -170,6 +174,7 @@ class SevpPlan:
     def __init__(self, coefs, ie, form=FORM_BIR, nbir=None, nsweep=5, periodic=True,
                  nrhs_max=1, precision=PREC_FP64, seam_fix=False):
         al, ab, ac, ar, at = [_f32(np.asfortranarray(c, dtype=np.float32)) for c in coefs]
+        self.coefs = (al, ab, ac, ar, at)
         self.i2, self.j2 = al.shape
         self.ie = _i32(ie)
         self.form, self.precision, self.nrhs_max = form, precision, nrhs_max
@@ -226,6 +231,25 @@ class SevpPlan:
         rinv1 = np.zeros(s1, order="F")
         _check(lib().sevp_get_influence(self._h, _p(rinv), _p(rinv1)))
         return rinv, rinv1
+
+    def write_evp(self, path, oac=None, max_subrecord=0):
+        """Write the plan's operator, influence matrices and IE as the EVP
+        record PREP writes (PREP/template/GLOBAL/src/prep_GLOBAL.f90:320);
+        with ``oac`` the record carries OAC after AC, as MAIN reads it
+        (MAIN/template/GLOBAL/src/input_GLOBAL.f90:442)."""
+        rinv, rinv1 = self.get_influence()
+        evp_write(path, self.coefs, rinv, rinv1, self.ie, self.form, self.nbir, oac=oac,
+                  max_subrecord=max_subrecord)
+
+    @classmethod
+    def from_evp(cls, path, i2, j2, nb0, form=FORM_BIR, nbir=None, with_oac=None, **kw):
+        """A plan loaded from an EVP file, as MAIN starts from PREP's output
+        (MAIN/template/GLOBAL/src/input_GLOBAL.f90:438-443): the operator and IE
+        come from the record, the influence matrices are uploaded, not rebuilt."""
+        rec = evp_read(path, i2, j2, nb0, form, nbir or 0, with_oac=with_oac)
+        plan = cls(rec["coefs"], rec["ie"], form=form, nbir=nbir, **kw)
+        plan.load_influence(rec["rinv"], rec["rinv1"])
+        return plan
 
     def set_precision(self, precision):
         _check(lib().sevp_set_precision(self._h, precision))
@@ -370,3 +394,62 @@ def matinv(b):
     """MATINV (PREP/template/GLOBAL/src/prep_GLOBAL.f90:516-568), in place."""
     _check(lib().sevp_matinv(_p(_f64(b)), b.shape[0]))
     return b
+
+
+# ---------------------------------------------------------------- EVP file
+
+def _evp_desc(i2, j2, ie, form, nbir):
+    return _Desc(i2, j2, len(ie), ie.ctypes.data_as(ctypes.POINTER(CI)), form, nbir or 0, 0, 0,
+                 1, PREC_FP64, 0)
+
+
+def evp_shapes(i2, nb0, form, nbir=0):
+    """RINV, RINV1 shapes of the EVP record: (IBIR*IBIR, NB0, NBIR) and
+    (IBIR*IBIR, NB0-1, NBIR) for BIR (PREP/template/GLOBAL/src/grid_var_GLOBAL.f90:431),
+    (I2, I2, NB0) and (I2, I2, NB0-1) otherwise (:436)."""
+    if form == FORM_BIR:
+        w = 2 * i2 // nbir
+        return (w * w, nb0, nbir), (w * w, nb0 - 1, nbir)
+    return (i2, i2, nb0), (i2, i2, nb0 - 1)
+
+
+def evp_write(path, coefs, rinv, rinv1, ie, form=FORM_BIR, nbir=0, oac=None, max_subrecord=0):
+    """Write the EVP record: AL,AR,AB,AT,AC,[OAC,]RINV,RINV1,IE, big-endian,
+    Fortran unformatted sequential.  ``coefs`` is (AL, AB, AC, AR, AT) as
+    everywhere in this package; the file order is the reference's."""
+    al, ab, ac, ar, at = [_f32(np.asfortranarray(c, dtype=np.float32)) for c in coefs]
+    i2, j2 = al.shape
+    ie = _i32(ie)
+    s0, s1 = evp_shapes(i2, len(ie), form, nbir)
+    rinv = _f64(np.asfortranarray(rinv))
+    rinv1 = _f64(np.asfortranarray(rinv1))
+    if rinv.size != int(np.prod(s0)) or rinv1.size < int(np.prod(s1)):
+        raise ValueError(f"RINV {rinv.shape} / RINV1 {rinv1.shape} do not match {s0} / {s1}")
+    if oac is not None:
+        oac = _f32(np.asfortranarray(oac, dtype=np.float32))
+        if oac.shape != (i2, j2):
+            raise ValueError("OAC must be (I2, J2)")
+    d = _evp_desc(i2, j2, ie, form, nbir)
+    _check(lib().sevp_evp_write(os.fsencode(path), ctypes.byref(d), _p(al), _p(ab), _p(ac),
+                                    _p(ar), _p(at), _p(rinv), _p(rinv1), _p(oac), max_subrecord))
+
+
+def evp_read(path, i2, j2, nb0, form=FORM_BIR, nbir=0, with_oac=None):
+    """Read an EVP record written by PREP (no OAC) or in MAIN's layout (with
+    OAC); ``with_oac=None`` accepts whichever the record length matches.
+    Returns a dict with coefs (AL, AB, AC, AR, AT), rinv, rinv1, ie and oac
+    (None when the record has none)."""
+    s0, s1 = evp_shapes(i2, nb0, form, nbir)
+    coefs = [np.zeros((i2, j2), np.float32, order="F") for _ in range(5)]
+    oac = np.zeros((i2, j2), np.float32, order="F")
+    rinv = np.zeros(s0, order="F")
+    rinv1 = np.zeros(s1, order="F")
+    ie = np.zeros(nb0, np.int32)
+    had = CI(0)
+    mode = -1 if with_oac is None else (1 if with_oac else 0)
+    d = _evp_desc(i2, j2, ie, form, nbir)
+    _check(lib().sevp_evp_read(os.fsencode(path), ctypes.byref(d), *[_p(c) for c in coefs],
+                                   _p(rinv), _p(rinv1) if rinv1.size else None, _p(ie), _p(oac),
+                                   mode, ctypes.byref(had)))
+    return {"coefs": tuple(coefs), "rinv": rinv, "rinv1": rinv1, "ie": ie,
+            "oac": oac if had.value else None}
