@@ -18,6 +18,8 @@
 // first being negated (gfortran's convention for records over 2 GiB; a
 // record that fits in one subrecord carries two equal positive lengths, as
 // every Fortran compiler writes it).
+#include <sys/types.h>
+
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -33,26 +35,18 @@ int evp_fail(const std::string &m) { return sevp::set_error(SEVP_E_ARG, m); }
 int io_fail(const std::string &m) { return sevp::set_error(SEVP_E_IO, m); }
 
 constexpr long kMaxSub = 2147483639L;
+constexpr size_t kChunk = 1 << 20;  // elements converted per buffer fill
 
-template <class T>
-void put_be(std::vector<unsigned char> &buf, const T *src, size_t n) {
-  const size_t at = buf.size();
-  buf.resize(at + n * sizeof(T));
-  unsigned char *d = buf.data() + at;
-  for (size_t i = 0; i < n; ++i) {
-    unsigned char b[sizeof(T)];
-    std::memcpy(b, src + i, sizeof(T));
-    for (size_t k = 0; k < sizeof(T); ++k) d[i * sizeof(T) + k] = b[sizeof(T) - 1 - k];
-  }
-}
+// One array of the record's item list.
+struct Item {
+  const void *src;  // writer: the values
+  void *dst;        // reader: where they go (null: skip them)
+  size_t n, size;   // element count and size in bytes
+};
 
-template <class T>
-void get_be(const unsigned char *s, T *dst, size_t n) {
-  for (size_t i = 0; i < n; ++i) {
-    unsigned char b[sizeof(T)];
-    for (size_t k = 0; k < sizeof(T); ++k) b[k] = s[i * sizeof(T) + sizeof(T) - 1 - k];
-    std::memcpy(dst + i, b, sizeof(T));
-  }
+void swap_copy(unsigned char *d, const unsigned char *s, size_t n, size_t size) {
+  for (size_t i = 0; i < n; ++i)
+    for (size_t k = 0; k < size; ++k) d[i * size + k] = s[i * size + size - 1 - k];
 }
 
 bool put_marker(FILE *f, int32_t v) {
@@ -69,6 +63,39 @@ bool get_marker(FILE *f, int32_t &v) {
   v = (int32_t)(((uint32_t)b[0] << 24) | ((uint32_t)b[1] << 16) | ((uint32_t)b[2] << 8) | b[3]);
   return true;
 }
+
+// Writes the payload of one record of known length n, opening and closing
+// subrecords at their boundaries as bytes arrive.
+struct RecordWriter {
+  FILE *f;
+  long n, maxsub, done = 0, len = 0, left = 0;
+  bool ok = true;
+  RecordWriter(FILE *f_, long n_, long maxsub_) : f(f_), n(n_), maxsub(maxsub_) { open_sub(); }
+  void open_sub() {
+    len = n - done < maxsub ? n - done : maxsub;
+    left = len;
+    ok = ok && put_marker(f, done + len == n ? (int32_t)len : -(int32_t)len);
+  }
+  void close_sub() { ok = ok && put_marker(f, done == len ? (int32_t)len : -(int32_t)len); }
+  void put(const unsigned char *b, long k) {
+    while (ok && k > 0) {
+      if (left == 0) {
+        close_sub();
+        open_sub();
+      }
+      const long m = k < left ? k : left;
+      ok = ok && std::fwrite(b, 1, (size_t)m, f) == (size_t)m;
+      b += m, k -= m, left -= m, done += m;
+    }
+  }
+  // The trailing length of the last subrecord (positive only if it is also
+  // the first).
+  bool finish() {
+    if (ok && (done != n || left != 0)) ok = false;
+    if (ok) ok = put_marker(f, done == len ? (int32_t)len : -(int32_t)len);
+    return ok;
+  }
+};
 
 struct Layout {
   long ncoef, nrinv, nrinv1, nie;
@@ -92,6 +119,59 @@ int layout_of(const sevp_desc *d, Layout &l) {
   return SEVP_OK;
 }
 
+// Walks the subrecords of the file's first record without reading their
+// payload: checks that every leading length has its matching trailing
+// length and sign, and returns the payload length (or -1).  limit bounds
+// the total so that a corrupt length is refused (-2).
+long scan_record(FILE *f, long limit) {
+  long n = 0;
+  for (bool first = true;; first = false) {
+    int32_t m1 = 0, m2 = 0;
+    if (!get_marker(f, m1)) return -1;
+    const long len = m1 < 0 ? -(long)m1 : (long)m1;
+    if (n + len > limit) return -2;
+    if (fseeko(f, (off_t)len, SEEK_CUR) != 0 || !get_marker(f, m2)) return -1;
+    const long len2 = m2 < 0 ? -(long)m2 : (long)m2;
+    if (len2 != len || (m2 < 0) == first) return -1;
+    n += len;
+    if (m1 >= 0) return n;
+  }
+}
+
+// Reads the payload of the record scan_record accepted into the items, in
+// order, skipping the subrecord markers.
+bool read_items(FILE *f, const std::vector<Item> &items) {
+  std::vector<unsigned char> buf;
+  long left = 0;         // payload bytes left in the current subrecord
+  bool started = false;  // past the first leading length
+  auto take = [&](unsigned char *d, long k) {
+    while (k > 0) {
+      if (left == 0) {
+        int32_t m = 0;
+        if (started && !get_marker(f, m)) return false;  // trailing length
+        if (!get_marker(f, m)) return false;              // next leading length
+        started = true;
+        left = m < 0 ? -(long)m : (long)m;
+        continue;
+      }
+      const long c = k < left ? k : left;
+      if (std::fread(d, 1, (size_t)c, f) != (size_t)c) return false;
+      d += c, k -= c, left -= c;
+    }
+    return true;
+  };
+  for (const Item &it : items) {
+    for (size_t i = 0; i < it.n; i += kChunk) {
+      const size_t m = it.n - i < kChunk ? it.n - i : kChunk;
+      buf.resize(m * it.size);
+      if (!take(buf.data(), (long)(m * it.size))) return false;
+      if (it.dst)
+        swap_copy(static_cast<unsigned char *>(it.dst) + i * it.size, buf.data(), m, it.size);
+    }
+  }
+  return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -105,27 +185,25 @@ int sevp_evp_write(const char *path, const sevp_desc *d, const float *al, const 
     return evp_fail("null argument");
   const long maxsub = max_subrecord > 0 ? max_subrecord : kMaxSub;
   if (maxsub > kMaxSub) return evp_fail("subrecords are at most 2147483639 bytes");
-  std::vector<unsigned char> rec;
-  rec.reserve((size_t)l.bytes(oac != nullptr));
-  for (const float *c : {al, ar, ab, at, ac}) put_be(rec, c, (size_t)l.ncoef);
-  if (oac) put_be(rec, oac, (size_t)l.ncoef);
-  put_be(rec, rinv, (size_t)l.nrinv);
-  if (l.nrinv1) put_be(rec, rinv1, (size_t)l.nrinv1);
-  std::vector<int32_t> ie(d->ie, d->ie + d->nb0);
-  put_be(rec, ie.data(), ie.size());
+  static_assert(sizeof(int) == 4, "IE is INTEGER(4)");
+  std::vector<Item> items;
+  for (const float *c : {al, ar, ab, at, ac}) items.push_back({c, nullptr, (size_t)l.ncoef, 4});
+  if (oac) items.push_back({oac, nullptr, (size_t)l.ncoef, 4});
+  items.push_back({rinv, nullptr, (size_t)l.nrinv, 8});
+  items.push_back({rinv1, nullptr, (size_t)l.nrinv1, 8});
+  items.push_back({d->ie, nullptr, (size_t)l.nie, 4});
   FILE *f = std::fopen(path, "wb");
   if (!f) return io_fail(std::string("cannot open ") + path + " for writing");
-  bool ok = true;
-  const long n = (long)rec.size();
-  long at_ = 0;
-  do {
-    const long len = n - at_ < maxsub ? n - at_ : maxsub;
-    const bool first = at_ == 0, last = at_ + len == n;
-    ok = ok && put_marker(f, last ? (int32_t)len : -(int32_t)len);
-    ok = ok && std::fwrite(rec.data() + at_, 1, (size_t)len, f) == (size_t)len;
-    ok = ok && put_marker(f, first ? (int32_t)len : -(int32_t)len);
-    at_ += len;
-  } while (ok && at_ < n);
+  RecordWriter w(f, l.bytes(oac != nullptr), maxsub);
+  std::vector<unsigned char> buf;
+  for (const Item &it : items)
+    for (size_t i = 0; i < it.n && w.ok; i += kChunk) {
+      const size_t m = it.n - i < kChunk ? it.n - i : kChunk;
+      buf.resize(m * it.size);
+      swap_copy(buf.data(), static_cast<const unsigned char *>(it.src) + i * it.size, m, it.size);
+      w.put(buf.data(), (long)(m * it.size));
+    }
+  bool ok = w.finish();
   ok = (std::fclose(f) == 0) && ok;
   return ok ? SEVP_OK : io_fail(std::string("write to ") + path + " failed");
 }
@@ -139,63 +217,43 @@ int sevp_evp_read(const char *path, const sevp_desc *d, float *al, float *ab, fl
     return evp_fail("null argument");
   FILE *f = std::fopen(path, "rb");
   if (!f) return io_fail(std::string("cannot open ") + path);
-  // No valid record is longer than the larger layout: a length beyond it is
-  // refused before anything is allocated for it.
+  // No valid record is longer than the larger layout.
   const long limit = l.bytes(with_oac != 0);
-  std::vector<unsigned char> rec;
-  bool ok = true, first = true, too_long = false;
-  for (;;) {
-    int32_t m1 = 0, m2 = 0;
-    if (!get_marker(f, m1)) { ok = false; break; }
-    const long len = m1 < 0 ? -(long)m1 : (long)m1;
-    if ((long)rec.size() + len > limit) { ok = false; too_long = true; break; }
-    const size_t at_ = rec.size();
-    rec.resize(at_ + (size_t)len);
-    if (std::fread(rec.data() + at_, 1, (size_t)len, f) != (size_t)len || !get_marker(f, m2)) {
-      ok = false;
-      break;
-    }
-    const long len2 = m2 < 0 ? -(long)m2 : (long)m2;
-    if (len2 != len || (m2 < 0) == first) { ok = false; break; }
-    first = false;
-    if (m1 >= 0) break;
+  const long n = scan_record(f, limit);
+  if (n < 0) {
+    std::fclose(f);
+    if (n == -2)
+      return io_fail(std::string(path) + ": record longer than the " + std::to_string(limit) +
+                     " bytes this grid's layout holds");
+    return io_fail(std::string(path) + ": not one well-formed unformatted record");
   }
-  std::fclose(f);
-  if (too_long)
-    return io_fail(std::string(path) + ": record longer than the " + std::to_string(limit) +
-                   " bytes this grid's layout holds");
-  if (!ok) return io_fail(std::string(path) + ": not one well-formed unformatted record");
-  const long n = (long)rec.size();
   bool use_oac;
   if (with_oac < 0) {
-    if (n == l.bytes(false)) use_oac = false;
-    else if (n == l.bytes(true)) use_oac = true;
-    else
+    use_oac = n == l.bytes(true);
+    if (!use_oac && n != l.bytes(false)) {
+      std::fclose(f);
       return io_fail("record of " + std::to_string(n) + " bytes matches neither layout (" +
-                      std::to_string(l.bytes(false)) + " without OAC, " +
-                      std::to_string(l.bytes(true)) + " with)");
+                     std::to_string(l.bytes(false)) + " without OAC, " +
+                     std::to_string(l.bytes(true)) + " with)");
+    }
   } else {
     use_oac = with_oac != 0;
-    if (n != l.bytes(use_oac))
+    if (n != l.bytes(use_oac)) {
+      std::fclose(f);
       return io_fail("record of " + std::to_string(n) + " bytes, expected " +
-                      std::to_string(l.bytes(use_oac)) + (use_oac ? " with OAC" : " without OAC"));
+                     std::to_string(l.bytes(use_oac)) + (use_oac ? " with OAC" : " without OAC"));
+    }
   }
-  const unsigned char *p = rec.data();
-  for (float *c : {al, ar, ab, at, ac}) {
-    get_be(p, c, (size_t)l.ncoef);
-    p += 4 * l.ncoef;
-  }
-  if (use_oac) {
-    if (oac) get_be(p, oac, (size_t)l.ncoef);
-    p += 4 * l.ncoef;
-  }
-  get_be(p, rinv, (size_t)l.nrinv);
-  p += 8 * l.nrinv;
-  if (l.nrinv1) get_be(p, rinv1, (size_t)l.nrinv1);
-  p += 8 * l.nrinv1;
-  std::vector<int32_t> iev((size_t)l.nie);
-  get_be(p, iev.data(), iev.size());
-  for (long i = 0; i < l.nie; ++i) ie[i] = iev[(size_t)i];
+  std::vector<Item> items;
+  for (float *c : {al, ar, ab, at, ac}) items.push_back({nullptr, c, (size_t)l.ncoef, 4});
+  if (use_oac) items.push_back({nullptr, oac, (size_t)l.ncoef, 4});
+  items.push_back({nullptr, rinv, (size_t)l.nrinv, 8});
+  items.push_back({nullptr, rinv1, (size_t)l.nrinv1, 8});
+  items.push_back({nullptr, ie, (size_t)l.nie, 4});
+  std::rewind(f);
+  const bool ok = read_items(f, items);
+  std::fclose(f);
+  if (!ok) return io_fail(std::string(path) + ": read failed");
   if (had_oac) *had_oac = use_oac ? 1 : 0;
   return SEVP_OK;
 }

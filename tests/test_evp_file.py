@@ -260,3 +260,34 @@ def test_fortran_binding(evp_io, tmp_path):
         assert (tmp_path / "b.raw").read_bytes() == raw_bytes(r0)
         assert run_io(evp_io, "read-main", tmp_path / "m.raw", tmp_path / "main.evp", r) == 0
         assert (tmp_path / "m.raw").read_bytes() == raw_bytes(r0)
+
+
+def test_large_record_chunks_and_odd_splits(evp_io, tmp_path):
+    """RINV of 1.3 M values (more than one conversion chunk), written whole
+    and with subrecords of an odd length that cut values in two: the payload
+    is the same bytes, flang's WRITE gives the same whole record, and both
+    files read back exactly."""
+    i2, j2, nb0 = 256, 40, 20
+    CASES["big"] = (i2, j2, nb0, S.FORM_FULL_PERIODIC, 0)
+    try:
+        r = make_record("big", 8)
+    finally:
+        del CASES["big"]
+    assert r["rinv"].size > (1 << 20)
+    whole, split = tmp_path / "whole.evp", tmp_path / "split.evp"
+    c_write(whole, r, False)
+    c_write(split, r, False, max_subrecord=3_000_001)
+    wb, sb = whole.read_bytes(), split.read_bytes()
+    subs = parse_subrecords(sb)
+    assert len(subs) == -(-(len(wb) - 8) // 3_000_001) > 2
+    payload, at = b"", 0
+    for _, _, n in subs:
+        payload += sb[at + 4:at + 4 + n]
+        at += 8 + n
+    assert payload == wb[4:-4]
+    raw = tmp_path / "in.raw"
+    raw.write_bytes(raw_bytes(r))
+    assert run_io(evp_io, "write-prep", raw, tmp_path / "f.evp", r) == 0
+    assert (tmp_path / "f.evp").read_bytes() == wb
+    assert_record_equal(c_read(whole, r), r, False)
+    assert_record_equal(c_read(split, r), r, False)
