@@ -280,6 +280,22 @@ int sevp_evp_read(const char *path, const sevp_desc *desc, float *al, float *ab,
                   float *ar, float *at, double *rinv, double *rinv1, int *ie, float *oac,
                   int with_oac, int *had_oac);
 
+/* ----------------------------------------------------------- plan cache
+ * Building the influence matrices is the expensive part of a plan (PREP's
+ * whole job).  sevp_operator_hash hashes what they depend on: I2, J2, NB0,
+ * IE, form, NBIR (BIR) and the bits of AL..AT (64-bit, host only).
+ * sevp_plan_create_cached creates the plan and looks for
+ * <cache_dir>/sevp_<hash>.evp, an EVP record in PREP's layout: when the
+ * record's operator and IE equal the caller's bit for bit, its matrices are
+ * loaded (*hit = 1); otherwise they are built and the file is written
+ * (*hit = 0; -1 if it could not be written, which is not an error).  No
+ * counterpart in the reference, where MAIN always reads the file PREP wrote. */
+int sevp_operator_hash(const sevp_desc *desc, const float *al, const float *ab, const float *ac,
+                       const float *ar, const float *at, unsigned long long *hash);
+int sevp_plan_create_cached(const sevp_desc *desc, const float *al, const float *ab,
+                            const float *ac, const float *ar, const float *at,
+                            const char *cache_dir, sevp_plan *out, int *hit);
+
 /* ------------------------------------------------------------- misc */
 const char *sevp_last_error(void);
 const char *sevp_version(void);

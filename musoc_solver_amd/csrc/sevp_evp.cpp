@@ -19,6 +19,7 @@
 // record that fits in one subrecord carries two equal positive lengths, as
 // every Fortran compiler writes it).
 #include <sys/types.h>
+#include <unistd.h>
 
 #include <cstdint>
 #include <cstdio>
@@ -255,6 +256,114 @@ int sevp_evp_read(const char *path, const sevp_desc *d, float *al, float *ab, fl
   std::fclose(f);
   if (!ok) return io_fail(std::string(path) + ": read failed");
   if (had_oac) *had_oac = use_oac ? 1 : 0;
+  return SEVP_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ plan cache
+
+namespace {
+
+uint64_t mix64(uint64_t h) {  // splitmix64 finalizer
+  h ^= h >> 30;
+  h *= 0xbf58476d1ce4e5b9ULL;
+  h ^= h >> 27;
+  h *= 0x94d049bb133111ebULL;
+  return h ^ (h >> 31);
+}
+
+struct Hasher {
+  uint64_t h = 0x5345565045565031ULL;  // format tag: change it to invalidate every cache
+  void word(uint64_t w) { h = mix64(h ^ w) + 0x9e3779b97f4a7c15ULL; }
+  void bytes(const void *p, size_t n) {
+    const unsigned char *b = static_cast<const unsigned char *>(p);
+    size_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+      uint64_t w;
+      std::memcpy(&w, b + i, 8);
+      word(w);
+    }
+    uint64_t w = 0;
+    std::memcpy(&w, b + i, n - i);
+    word(w ^ ((uint64_t)(n - i) << 56));
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int sevp_operator_hash(const sevp_desc *d, const float *al, const float *ab, const float *ac,
+                       const float *ar, const float *at, unsigned long long *out) {
+  Layout l{};
+  if (int rc = layout_of(d, l)) return rc;
+  if (!al || !ab || !ac || !ar || !at || !d->ie || !out) return evp_fail("null argument");
+  Hasher h;
+  h.word((uint64_t)d->i2);
+  h.word((uint64_t)d->j2);
+  h.word((uint64_t)d->nb0);
+  h.word((uint64_t)d->form);
+  h.word((uint64_t)(d->form == SEVP_FORM_BIR ? d->nbir : 0));
+  h.bytes(d->ie, sizeof(int) * (size_t)d->nb0);
+  for (const float *c : {al, ab, ac, ar, at}) h.bytes(c, sizeof(float) * (size_t)l.ncoef);
+  *out = h.h;
+  return SEVP_OK;
+}
+
+int sevp_plan_create_cached(const sevp_desc *d, const float *al, const float *ab, const float *ac,
+                            const float *ar, const float *at, const char *cache_dir,
+                            sevp_plan *out, int *hit) {
+  if (!cache_dir || !out) return evp_fail("null argument");
+  unsigned long long key = 0;
+  if (int rc = sevp_operator_hash(d, al, ab, ac, ar, at, &key)) return rc;
+  Layout l{};
+  layout_of(d, l);
+  sevp_plan plan = nullptr;
+  if (int rc = sevp_plan_create(d, al, ab, ac, ar, at, &plan)) return rc;
+  char name[32];
+  std::snprintf(name, sizeof name, "sevp_%016llx.evp", key);
+  const std::string path = std::string(cache_dir) + "/" + name;
+  std::vector<float> c[5];
+  for (auto &v : c) v.resize((size_t)l.ncoef);
+  std::vector<double> rinv((size_t)l.nrinv), rinv1((size_t)(l.nrinv1 > 0 ? l.nrinv1 : 1));
+  std::vector<int> ie((size_t)l.nie);
+  // A hit needs the stored operator and IE to equal the caller's bit for bit.
+  if (sevp_evp_read(path.c_str(), d, c[0].data(), c[1].data(), c[2].data(), c[3].data(),
+                    c[4].data(), rinv.data(), rinv1.data(), ie.data(), nullptr, -1,
+                    nullptr) == SEVP_OK) {
+    bool same = std::memcmp(ie.data(), d->ie, sizeof(int) * (size_t)l.nie) == 0;
+    const float *mine[5] = {al, ab, ac, ar, at};
+    for (int k = 0; k < 5 && same; ++k)
+      same = std::memcmp(c[k].data(), mine[k], sizeof(float) * (size_t)l.ncoef) == 0;
+    if (same) {
+      if (int rc = sevp_load_influence(plan, rinv.data(), rinv1.data())) {
+        sevp_plan_destroy(plan);
+        return rc;
+      }
+      if (hit) *hit = 1;
+      *out = plan;
+      return SEVP_OK;
+    }
+  }
+  int zp = 0;
+  int rc = sevp_build_influence(plan, &zp);
+  if (rc == SEVP_OK) rc = sevp_get_influence(plan, rinv.data(), rinv1.data());
+  if (rc != SEVP_OK) {
+    sevp_plan_destroy(plan);
+    return rc;
+  }
+  // Written under a private name and renamed, so that a reader never sees
+  // a partial file and concurrent writers of one key leave a whole one.
+  char tmp_suffix[48];
+  std::snprintf(tmp_suffix, sizeof tmp_suffix, ".tmp%ld.%p", (long)getpid(), (void *)plan);
+  const std::string tmp = path + tmp_suffix;
+  int stored = sevp_evp_write(tmp.c_str(), d, al, ab, ac, ar, at, rinv.data(), rinv1.data(),
+                              nullptr, 0) == SEVP_OK &&
+               std::rename(tmp.c_str(), path.c_str()) == 0;
+  if (!stored) std::remove(tmp.c_str());
+  if (hit) *hit = stored ? 0 : -1;
+  *out = plan;
   return SEVP_OK;
 }
 

@@ -101,6 +101,11 @@ def lib():
                                       ctypes.POINTER(Step14State), ctypes.POINTER(CI),
                                       ctypes.POINTER(ctypes.c_float)]
         L.sevp_profile_begin.argtypes = [VP, CI]
+        L.sevp_operator_hash.argtypes = ([ctypes.POINTER(_Desc)] + [VP] * 5
+                                         + [ctypes.POINTER(ctypes.c_ulonglong)])
+        L.sevp_plan_create_cached.argtypes = ([ctypes.POINTER(_Desc)] + [VP] * 5
+                                              + [ctypes.c_char_p, ctypes.POINTER(VP),
+                                                 ctypes.POINTER(CI)])
         L.sevp_evp_write.argtypes = [ctypes.c_char_p, ctypes.POINTER(_Desc)] + [VP] * 8 + [ctypes.c_long]
         L.sevp_evp_read.argtypes = ([ctypes.c_char_p, ctypes.POINTER(_Desc)] + [VP] * 9
                                     + [CI, ctypes.POINTER(CI)])
@@ -172,7 +177,10 @@ class SevpPlan:
     scratch for up to ``nrhs_max`` simultaneous right-hand sides."""
 
     def __init__(self, coefs, ie, form=FORM_BIR, nbir=None, nsweep=5, periodic=True,
-                 nrhs_max=1, precision=PREC_FP64, seam_fix=False):
+                 nrhs_max=1, precision=PREC_FP64, seam_fix=False, cache_dir=None):
+        """With ``cache_dir`` the influence matrices come from the plan cache
+        (sevp_plan_create_cached): ``cache_hit`` is 1 when they were loaded,
+        0 when built and stored, -1 when built but not stored."""
         al, ab, ac, ar, at = [_f32(np.asfortranarray(c, dtype=np.float32)) for c in coefs]
         self.coefs = (al, ab, ac, ar, at)
         self.i2, self.j2 = al.shape
@@ -184,8 +192,16 @@ class SevpPlan:
                   form, self.nbir, nsweep, 1 if periodic else 0, nrhs_max, precision,
                   1 if seam_fix else 0)
         h = VP()
-        _check(lib().sevp_plan_create(ctypes.byref(d), _p(al), _p(ab), _p(ac), _p(ar), _p(at),
-                                      ctypes.byref(h)))
+        self.cache_hit = None
+        if cache_dir is None:
+            _check(lib().sevp_plan_create(ctypes.byref(d), _p(al), _p(ab), _p(ac), _p(ar),
+                                          _p(at), ctypes.byref(h)))
+        else:
+            hit = CI(0)
+            _check(lib().sevp_plan_create_cached(ctypes.byref(d), _p(al), _p(ab), _p(ac), _p(ar),
+                                                 _p(at), os.fsencode(str(cache_dir)),
+                                                 ctypes.byref(h), ctypes.byref(hit)))
+            self.cache_hit = hit.value
         self._h = h
         self._stream = None
 
@@ -453,3 +469,15 @@ def evp_read(path, i2, j2, nb0, form=FORM_BIR, nbir=0, with_oac=None):
                                    mode, ctypes.byref(had)))
     return {"coefs": tuple(coefs), "rinv": rinv, "rinv1": rinv1, "ie": ie,
             "oac": oac if had.value else None}
+
+
+def operator_hash(coefs, ie, form=FORM_BIR, nbir=0) -> int:
+    """The plan cache's key: a 64-bit hash of the shape, IE, form, NBIR and
+    the coefficient bits (sevp_operator_hash; host only)."""
+    al, ab, ac, ar, at = [_f32(np.asfortranarray(c, dtype=np.float32)) for c in coefs]
+    ie = _i32(ie)
+    d = _evp_desc(al.shape[0], al.shape[1], ie, form, nbir)
+    out = ctypes.c_ulonglong(0)
+    _check(lib().sevp_operator_hash(ctypes.byref(d), _p(al), _p(ab), _p(ac), _p(ar), _p(at),
+                                    ctypes.byref(out)))
+    return out.value
