@@ -41,6 +41,8 @@ def lib():
         for name in ("orc_matinv", "orc_pre", "orc_pre1", "orc_bir_prep", "orc_bir_prep_strips"):
             getattr(_lib, name).restype = ctypes.c_int
         _lib.orc_step14_correct.restype = ctypes.c_float
+        for name in ("orc_inprod", "orc_tridiag_min_eig"):
+            getattr(_lib, name).restype = ctypes.c_double
     return _lib
 
 
@@ -294,3 +296,67 @@ def step14_correct(oc, x, p0, scr, u, v, odt, periodic_we=False):
         _ci(oc.i2), _ci(oc.j2), _ci(oc.k1), _ci(int(periodic_we)), ctypes.c_float(odt),
         P(_i16(oc.iu)), P(_i16(oc.iv)), P(_i16(oc.iu0)), P(_i16(oc.iv0)), P(oc.odx), P(oc.odyv),
         P(x), P(_f32w(p0)), P(_f32w(scr)), P(_f32w(u)), P(_f32w(v))))
+
+
+# ------------------------------------------------------------------ Krylov
+# The reference's stop thresholds are REAL(4) literals compared in double.
+BICG_TOL = float(np.float32(1e-8))
+PCSI_TOL = float(np.float32(1e-7))
+
+
+def sqprod_t(coefs, x):
+    """p_sqprod_T (MAIN/template/GLOBAL/src/sevp.f90:714-740)."""
+    al, ab, ac, ar, at = [f32(c) for c in coefs]
+    i2, j2 = al.shape
+    r = np.zeros((i2 + 2, j2 + 2), order="F")
+    lib().orc_sqprod_t(P(al), P(ab), P(ac), P(ar), P(at), P(f64(x)), P(r), _ci(i2), _ci(j2))
+    return r
+
+
+def inprod(r, p):
+    """p_inprod (:741-755): sequential interior sum."""
+    i2, j2 = r.shape[0] - 2, r.shape[1] - 2
+    return lib().orc_inprod(P(f64(r)), P(f64(p)), _ci(i2), _ci(j2))
+
+
+def bicgstab(coefs, b, x, periodic=True, max_iter=1000, restart=100, tol=BICG_TOL,
+             rtol=BICG_TOL):
+    """p_bicgstab_le (:443-552) on X in place; returns (iterations, ||r||)."""
+    al, ab, ac, ar, at = [f32(c) for c in coefs]
+    i2, j2 = al.shape
+    res = ctypes.c_double(0.0)
+    it = lib().orc_bicgstab(P(al), P(ab), P(ac), P(ar), P(at), P(f32(b)), P(f64(x)), _ci(i2),
+                            _ci(j2), _ci(1 if periodic else 0), _ci(max_iter), _ci(restart),
+                            ctypes.c_double(tol), ctypes.c_double(rtol), ctypes.byref(res))
+    return it, res.value
+
+
+def pcsi(coefs, mineig, maxeig, b, x, periodic=True, max_iter=1000, tol=PCSI_TOL):
+    """p_csi (:553-616) on X in place; returns (iterations, ||r||)."""
+    al, ab, ac, ar, at = [f32(c) for c in coefs]
+    i2, j2 = al.shape
+    res = ctypes.c_double(0.0)
+    it = lib().orc_pcsi(P(al), P(ab), P(ac), P(ar), P(at), ctypes.c_double(mineig),
+                        ctypes.c_double(maxeig), P(f32(b)), P(f64(x)), _ci(i2), _ci(j2),
+                        _ci(1 if periodic else 0), _ci(max_iter), ctypes.c_double(tol),
+                        ctypes.byref(res))
+    return it, res.value
+
+
+def lanczos(coefs, m=10):
+    """lanczos (:618-694): returns (mineig, maxeig, alpha[m], beta[m+1], sigma[m+1])."""
+    al, ab, ac, ar, at = [f32(c) for c in coefs]
+    i2, j2 = al.shape
+    alpha = np.zeros(m)
+    beta = np.zeros(m + 1)
+    sigma = np.zeros(m + 1)
+    mn, mx = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    lib().orc_lanczos(P(al), P(ab), P(ac), P(ar), P(at), _ci(i2), _ci(j2), _ci(m), P(alpha),
+                      P(beta), P(sigma), ctypes.byref(mn), ctypes.byref(mx))
+    return mn.value, mx.value, alpha, beta, sigma
+
+
+def tridiag_min_eig(d, e):
+    d = np.ascontiguousarray(d, np.float64)
+    e = np.ascontiguousarray(e, np.float64)
+    return lib().orc_tridiag_min_eig(P(d), P(e), _ci(len(d)))

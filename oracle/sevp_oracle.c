@@ -723,3 +723,279 @@ float orc_step14_correct(int i2, int j2, int k1, int periodic_we, float odt,
 #undef U3
 #undef V3
 #undef A2
+
+/* ---------------------------------------------------------------- Krylov
+ * The cross-check solvers of MAIN/template/GLOBAL/src/sevp.f90:443-795:
+ * p_bicgstab_le, p_csi and lanczos with their helpers p_sqprod_T, p_inprod
+ * and solver_boundary_update.  `preconditioning` (:757-767) is a copy
+ * (PH = P), so PH and SH are the arrays they copy after the periodic halo
+ * update; every work array is (i2+2) x (j2+2), zero at entry, and the
+ * whole-array statements update every element, halos included.  The stop
+ * thresholds are the reference's REAL(4) literals promoted to double
+ * (callers pass (double)1e-8f etc.).
+ *
+ * Where the reference is undefined the restatement picks the reading that
+ * makes it defined, and says so:
+ *   - lanczos writes SIGMA(M+1) and BETA(M+1) past their DIMENSION(M)
+ *     arrays on its last step; here they have M+1 entries.
+ *   - lanczos's local arrays S, Q and WORK are never initialised, and only
+ *     their interiors are written; their halos are taken as zero.
+ *   - FIGI leaves E(1) unset and lanczos reads it in its Gershgorin bound;
+ *     it is taken as zero.
+ *   - FIGI and RATQR are EISPACK routines the reference does not ship (its
+ *     calls match neither EISPACK's nor J. Burkardt's eispack.f90 argument
+ *     lists).  FIGI's symmetrisation is restated (E(I) = SQRT(T(I,1)*
+ *     T(I-1,3)), D = T(:,2)); for RATQR, which returns the smallest
+ *     eigenvalue of the symmetric tridiagonal matrix to a tolerance, the
+ *     eigenvalue itself is computed by bisection on Sturm counts. */
+
+static void krylov_bnd(double *x, int i2, int j2, int periodic)
+{
+  const int i0 = i2 + 2;
+  if (!periodic) return;
+  for (int j = 2; j <= j2 + 1; ++j) {
+    x[IDX2(i0, 1, j)] = x[IDX2(i0, i2 + 1, j)];
+    x[IDX2(i0, i2 + 2, j)] = x[IDX2(i0, 2, j)];
+  }
+}
+
+/* p_inprod (:741-755): sequential, j outer, i inner, over the interior. */
+double orc_inprod(const double *r, const double *p, int i2, int j2)
+{
+  const int i0 = i2 + 2;
+  double tp = 0.0;
+  for (int j = 2; j <= j2 + 1; ++j)
+    for (int i = 2; i <= i2 + 1; ++i) tp = tp + r[IDX2(i0, i, j)] * p[IDX2(i0, i, j)];
+  return tp;
+}
+
+/* p_sqprod_T (:714-740): the transposed 5-point product, with the
+ * coefficients read from zero-padded copies TAB, TAL, TAR, TAT. */
+void orc_sqprod_t(const float *al, const float *ab, const float *ac,
+                  const float *ar, const float *at, const double *x, double *r,
+                  int i2, int j2)
+{
+  const int i0 = i2 + 2;
+  for (int j = 1; j <= j2; ++j)
+    for (int i = 1; i <= i2; ++i) {
+      const double tat = j >= 2 ? (double)at[IDX2(i2, i, j - 1)] : 0.0;      /* TAT(I+1,J) */
+      const double tar = i >= 2 ? (double)ar[IDX2(i2, i - 1, j)] : 0.0;      /* TAR(I,J+1) */
+      const double tal = i <= i2 - 1 ? (double)al[IDX2(i2, i + 1, j)] : 0.0; /* TAL(I+2,J+1) */
+      const double tab = j <= j2 - 1 ? (double)ab[IDX2(i2, i, j + 1)] : 0.0; /* TAB(I+1,J+2) */
+      r[IDX2(i0, i + 1, j + 1)] =
+          tat * x[IDX2(i0, i + 1, j)] + tar * x[IDX2(i0, i, j + 1)] +
+          (double)ac[IDX2(i2, i, j)] * x[IDX2(i0, i + 1, j + 1)] + tal * x[IDX2(i0, i + 2, j + 1)] +
+          tab * x[IDX2(i0, i + 1, j + 2)];
+    }
+}
+
+static void b_minus(const float *b, double *r, int i2, int j2)
+{
+  const int i0 = i2 + 2;
+  for (int j = 1; j <= j2; ++j)
+    for (int i = 1; i <= i2; ++i)
+      r[IDX2(i0, i + 1, j + 1)] = (double)b[IDX2(i2, i, j)] - r[IDX2(i0, i + 1, j + 1)];
+}
+
+/* p_bicgstab_le (:443-552).  Restarts every `restart` iterations from the
+ * true residual with RHO = ALPHA = W = 1 but P and V kept.  Returns the
+ * iterations run; *res_out = SQRT(TP) at exit. */
+int orc_bicgstab(const float *al, const float *ab, const float *ac, const float *ar,
+                 const float *at, const float *b, double *x, int i2, int j2, int periodic,
+                 int max_iter, int restart, double tol, double rtol, double *res_out)
+{
+  const long n = (long)(i2 + 2) * (j2 + 2);
+  double *r = calloc(n, 8), *rh = calloc(n, 8), *p = calloc(n, 8), *v = calloc(n, 8);
+  double *s = calloc(n, 8), *t = calloc(n, 8), *ph = calloc(n, 8), *sh = calloc(n, 8);
+  int iter = 0;
+  double tp = 1.0, rho, rhn, alpha, beta, w, res, tmp;
+  int early = 0;
+  krylov_bnd(x, i2, j2, periodic);
+  while (sqrt(tp) > tol && iter < max_iter) {
+    rho = 1.0;
+    alpha = 1.0;
+    w = 1.0;
+    orc_sqprod(al, ab, ac, ar, at, x, r, i2, j2);
+    b_minus(b, r, i2, j2);
+    krylov_bnd(r, i2, j2, periodic);
+    memcpy(rh, r, n * 8);
+    tp = orc_inprod(r, r, i2, j2);
+    res = sqrt(tp);
+    if (res < tol) {
+      early = 1;
+      break;
+    }
+    while (sqrt(tp) / res > rtol) {
+      iter = iter + 1;
+      rhn = orc_inprod(r, rh, i2, j2);
+      beta = (rhn / rho) * (alpha / w);
+      for (long k = 0; k < n; ++k) p[k] = r[k] + beta * (p[k] - w * v[k]);
+      memcpy(ph, p, n * 8);
+      krylov_bnd(ph, i2, j2, periodic);
+      orc_sqprod(al, ab, ac, ar, at, ph, v, i2, j2);
+      krylov_bnd(v, i2, j2, periodic);
+      tmp = orc_inprod(rh, v, i2, j2);
+      alpha = rhn / tmp;
+      for (long k = 0; k < n; ++k) s[k] = r[k] - alpha * v[k];
+      memcpy(sh, s, n * 8);
+      krylov_bnd(sh, i2, j2, periodic);
+      orc_sqprod(al, ab, ac, ar, at, sh, t, i2, j2);
+      krylov_bnd(t, i2, j2, periodic);
+      tmp = orc_inprod(t, t, i2, j2);
+      w = orc_inprod(t, s, i2, j2);
+      w = w / tmp;
+#ifdef ORC_PLANT_BICG_UPDATE /* a planted slip for the tests: SH in place of PH */
+      for (long k = 0; k < n; ++k) x[k] = x[k] + alpha * sh[k] + w * sh[k];
+#else
+      for (long k = 0; k < n; ++k) x[k] = x[k] + alpha * ph[k] + w * sh[k];
+#endif
+      for (long k = 0; k < n; ++k) r[k] = s[k] - w * t[k];
+      tp = orc_inprod(r, r, i2, j2);
+      rho = rhn;
+      if (iter % restart == 0) break;
+    }
+  }
+  if (!early) krylov_bnd(x, i2, j2, periodic);
+  *res_out = sqrt(tp);
+  free(r); free(rh); free(p); free(v); free(s); free(t); free(ph); free(sh);
+  return iter;
+}
+
+/* p_csi (:553-616): Chebyshev iteration on the interval [mineig, maxeig]. */
+int orc_pcsi(const float *al, const float *ab, const float *ac, const float *ar,
+             const float *at, double mineig, double maxeig, const float *b, double *x,
+             int i2, int j2, int periodic, int max_iter, double tol, double *res_out)
+{
+  const long n = (long)(i2 + 2) * (j2 + 2);
+  double *r = calloc(n, 8), *s = calloc(n, 8), *q = calloc(n, 8), *work1 = calloc(n, 8);
+  int iter = 0;
+  double tp = 1.0;
+  const double alpha = 2.0 / (maxeig - mineig);
+  const double beta = (maxeig + mineig) / (maxeig - mineig);
+  const double csy = beta / alpha;
+  double omega = 2.0 / csy;
+  krylov_bnd(x, i2, j2, periodic);
+  orc_sqprod(al, ab, ac, ar, at, x, s, i2, j2);
+  krylov_bnd(s, i2, j2, periodic);
+  for (int j = 1; j <= j2; ++j)
+    for (int i = 1; i <= i2; ++i)
+      r[IDX2(i2 + 2, i + 1, j + 1)] = (double)b[IDX2(i2, i, j)] - s[IDX2(i2 + 2, i + 1, j + 1)];
+  memcpy(work1, r, n * 8);
+  krylov_bnd(work1, i2, j2, periodic);
+  const double c0 = 1.0 / csy;
+  for (long k = 0; k < n; ++k) q[k] = c0 * work1[k];
+  for (long k = 0; k < n; ++k) x[k] = x[k] + q[k];
+  orc_sqprod(al, ab, ac, ar, at, x, s, i2, j2);
+  krylov_bnd(s, i2, j2, periodic);
+  for (int j = 1; j <= j2; ++j)
+    for (int i = 1; i <= i2; ++i)
+      r[IDX2(i2 + 2, i + 1, j + 1)] = (double)b[IDX2(i2, i, j)] - s[IDX2(i2 + 2, i + 1, j + 1)];
+  while (sqrt(tp) > tol && iter < max_iter) {
+    omega = 1.0 / (csy - omega / (4.0 * alpha * alpha));
+    memcpy(work1, r, n * 8);
+    krylov_bnd(work1, i2, j2, periodic);
+    const double cq = csy * omega - 1.0;
+    for (long k = 0; k < n; ++k) q[k] = omega * work1[k] + cq * q[k];
+    for (long k = 0; k < n; ++k) x[k] = x[k] + q[k];
+    orc_sqprod(al, ab, ac, ar, at, x, s, i2, j2);
+    for (int j = 1; j <= j2; ++j)
+      for (int i = 1; i <= i2; ++i)
+        r[IDX2(i2 + 2, i + 1, j + 1)] = (double)b[IDX2(i2, i, j)] - s[IDX2(i2 + 2, i + 1, j + 1)];
+    krylov_bnd(r, i2, j2, periodic);
+    tp = orc_inprod(r, r, i2, j2);
+    iter = iter + 1;
+  }
+  krylov_bnd(x, i2, j2, periodic);
+  *res_out = sqrt(tp);
+  free(r); free(s); free(q); free(work1);
+  return iter;
+}
+
+/* Number of eigenvalues of the symmetric tridiagonal matrix (d, e) below x,
+ * by the signs of the LDL^T pivots (Sturm count); e[i] couples rows i-1, i. */
+static int sturm_below(const double *d, const double *e, int m, double x)
+{
+  int cnt = 0;
+  double q = 1.0;
+  for (int i = 0; i < m; ++i) {
+    const double off = i > 0 ? e[i] * e[i] / q : 0.0;
+    q = (d[i] - x) - off;
+    if (q == 0.0) q = -1e-300;
+    if (q < 0.0) ++cnt;
+  }
+  return cnt;
+}
+
+/* Smallest eigenvalue of the symmetric tridiagonal matrix by bisection
+ * between its Gershgorin bounds, to the last bit. */
+double orc_tridiag_min_eig(const double *d, const double *e, int m)
+{
+  double lo = d[0], hi = d[0];
+  for (int i = 0; i < m; ++i) {
+    const double rad = (i > 0 ? fabs(e[i]) : 0.0) + (i + 1 < m ? fabs(e[i + 1]) : 0.0);
+    if (d[i] - rad < lo) lo = d[i] - rad;
+    if (d[i] + rad > hi) hi = d[i] + rad;
+  }
+  for (int it = 0; it < 2200; ++it) {
+    const double mid = 0.5 * lo + 0.5 * hi;
+    if (mid <= lo || mid >= hi) break;
+    if (sturm_below(d, e, m, mid) >= 1) hi = mid;
+    else lo = mid;
+  }
+  return hi;
+}
+
+/* lanczos (:618-694): M steps of two-sided Lanczos from V = W = 1/||1||,
+ * then the eigenvalue estimates.  alpha has m entries, beta and sigma m+1
+ * (index k holds the reference's element k+1). */
+void orc_lanczos(const float *al, const float *ab, const float *ac, const float *ar,
+                 const float *at, int i2, int j2, int m, double *alpha, double *beta,
+                 double *sigma, double *mineig, double *maxeig)
+{
+  const long n = (long)(i2 + 2) * (j2 + 2);
+  double *v = malloc(n * 8), *w = malloc(n * 8), *vp = calloc(n, 8), *wp = calloc(n, 8);
+  double *s = calloc(n, 8), *q = calloc(n, 8), *work = calloc(n, 8);
+  double tp;
+  for (long k = 0; k < n; ++k) v[k] = 1.0;
+  for (int k = 0; k < m; ++k) alpha[k] = 0.0;
+  for (int k = 0; k <= m; ++k) beta[k] = sigma[k] = 0.0;
+  tp = orc_inprod(v, v, i2, j2);
+  const double c = 1.0 / sqrt(tp);
+  for (long k = 0; k < n; ++k) v[k] = c * v[k];
+  memcpy(w, v, n * 8);
+  for (int it = 1; it <= m; ++it) {
+    orc_sqprod(al, ab, ac, ar, at, v, s, i2, j2);
+    tp = orc_inprod(s, w, i2, j2);
+    alpha[it - 1] = tp;
+    for (long k = 0; k < n; ++k) work[k] = beta[it - 1] * vp[k];
+    memcpy(vp, v, n * 8);
+    for (long k = 0; k < n; ++k) v[k] = s[k] - alpha[it - 1] * v[k] - work[k];
+    orc_sqprod_t(al, ab, ac, ar, at, w, q, i2, j2);
+    for (long k = 0; k < n; ++k) work[k] = sigma[it - 1] * wp[k];
+    memcpy(wp, w, n * 8);
+    for (long k = 0; k < n; ++k) w[k] = q[k] - alpha[it - 1] * w[k] - work[k];
+    tp = orc_inprod(v, w, i2, j2);
+    sigma[it] = sqrt(fabs(tp));
+    beta[it] = copysign(sigma[it], tp);
+    for (long k = 0; k < n; ++k) v[k] = v[k] / sigma[it];
+    for (long k = 0; k < n; ++k) w[k] = w[k] / beta[it];
+  }
+  orc_sqprod(al, ab, ac, ar, at, v, s, i2, j2);
+  tp = orc_inprod(s, w, i2, j2);
+  alpha[m - 1] = tp;
+  /* FIGI: D = ALPHA, E(I) = SQRT(SIGMA(I)*SIGMA(I)) for I = 2..M, E(1) = 0 */
+  double *d = malloc(m * 8), *e = malloc(m * 8);
+  for (int i = 0; i < m; ++i) {
+    d[i] = alpha[i];
+    e[i] = i > 0 ? sqrt(sigma[i] * sigma[i]) : 0.0;
+  }
+  double mx = d[0] + e[0];
+  for (int i = 1; i < m; ++i) {
+    const double g = d[i] + e[i] + e[i - 1];
+    mx = g > mx ? g : mx;
+  }
+  *maxeig = mx;
+  *mineig = orc_tridiag_min_eig(d, e, m);
+  free(d); free(e);
+  free(v); free(w); free(vp); free(wp); free(s); free(q); free(work);
+}
