@@ -252,6 +252,63 @@ int sevp_profile_end(sevp_plan plan, double *total_ms, long *launches);
  * those spans since sevp_profile_begin. */
 int sevp_profile_spans(sevp_plan plan, double *total_ms, long *launches);
 
+/* -------------------------------------------------------- Krylov solvers
+ * The reference's cross-check solvers (MAIN/template/GLOBAL/src/sevp.f90:
+ * 443-694), used by FS_SOLVER to compare against the SEVP solution
+ * (MAIN/template/GLOBAL/src/fs_GLOBAL.f90:1853,1968,2024-2026): BiCGSTAB
+ * (p_bicgstab_le), Chebyshev iteration (p_csi) and two-sided Lanczos
+ * eigenvalue bounds (lanczos).  Same vectors as the reference: b (I2, J2)
+ * float, x (I2+2, J2+2) double with its halo rows as Dirichlet data.  The
+ * unpreconditioned reference's CL..CT arguments have no counterpart. */
+#define SEVP_KRYLOV_BICGSTAB 0
+#define SEVP_KRYLOV_PCSI 1
+
+typedef struct {
+  int periodic;   /* FLAG_PERIODIC_WE: halo columns are periodic copies (solver_boundary_update) */
+  int ref_order;  /* 1: inner products summed sequentially in p_inprod's order, so every iterate
+                     is bitwise the reference's (one thread per sum: small grids); 0: fixed-order
+                     parallel sums, deterministic */
+  int max_iter;   /* 1000 in the reference */
+  int restart;    /* BiCGSTAB restart length, 100 in the reference */
+  double tol;     /* stop when ||r||_2 <= tol: the reference's REAL(4) 1e-8 (BiCGSTAB), 1e-7 (P-CSI) */
+  double rtol;    /* BiCGSTAB inner loop: ||r|| / ||r_restart|| <= rtol, REAL(4) 1e-8 */
+} sevp_krylov_opts;
+
+typedef struct {
+  int iterations;   /* the reference's ITER */
+  int converged;    /* residual <= tol */
+  double residual;  /* SQRT(TP) = ||b - A x||_2 over the interior at exit */
+  double seconds;
+} sevp_krylov_stats;
+
+/* Fill opts with the reference's settings for solver (SEVP_KRYLOV_*). */
+int sevp_krylov_defaults(sevp_krylov_opts *opts, int solver);
+
+/* On a plan's operator; b_dev, x_dev device pointers; opts may be NULL. */
+int sevp_bicgstab_dev(sevp_plan plan, const float *b_dev, double *x_dev,
+                      const sevp_krylov_opts *opts, sevp_krylov_stats *stats);
+int sevp_pcsi_dev(sevp_plan plan, double mineig, double maxeig, const float *b_dev, double *x_dev,
+                  const sevp_krylov_opts *opts, sevp_krylov_stats *stats);
+/* m Lanczos steps (the reference: 10); alpha (m) and sigma (m+1) may be NULL.
+ * mineig is the smallest eigenvalue of FIGI's symmetric tridiagonal matrix
+ * (the reference calls EISPACK RATQR for it), maxeig the reference's
+ * row-sum bound. */
+int sevp_lanczos_dev(sevp_plan plan, int m, const sevp_krylov_opts *opts, double *mineig,
+                     double *maxeig, double *alpha, double *sigma);
+
+/* One-to-one, host pointers (P_BICGSTAB_LE, P_CSI, LANCZOS of
+ * MAIN/template/GLOBAL/src/sevp.f90:443,553,618); each call uploads the
+ * operator. */
+int sevp_p_bicgstab_le(const float *al, const float *ab, const float *ac, const float *ar,
+                       const float *at, const float *b, double *x, int i2, int j2,
+                       const sevp_krylov_opts *opts, sevp_krylov_stats *stats);
+int sevp_p_csi(const float *al, const float *ab, const float *ac, const float *ar, const float *at,
+               double mineig, double maxeig, const float *b, double *x, int i2, int j2,
+               const sevp_krylov_opts *opts, sevp_krylov_stats *stats);
+int sevp_lanczos(const float *al, const float *ab, const float *ac, const float *ar,
+                 const float *at, int i2, int j2, int m, const sevp_krylov_opts *opts,
+                 double *mineig, double *maxeig, double *alpha, double *sigma);
+
 /* ------------------------------------------------------------- EVP file
  * The operator and influence matrices as one big-endian Fortran unformatted
  * sequential record (OPEN(..., CONVERT='BIG_ENDIAN', FORM='unformatted')).

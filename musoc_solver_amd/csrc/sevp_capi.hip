@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <memory>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -207,6 +208,20 @@ struct PassGraph {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
 };
 
+// Vectors and scalars of the Krylov solvers (BiCGSTAB needs eight vectors).
+struct KryWork {
+  DBuf<double> vec, part, dot, omega;
+  DBuf<KryState> st;
+  int ensure(int i2, int j2) {
+    const size_t n = (size_t)(i2 + 2) * (j2 + 2);
+    if (vec.n < 8 * n) CHK(vec.alloc(8 * n));
+    if (!part.p) CHK(part.alloc(2 * (size_t)KRY_MAXB));
+    if (!dot.p) CHK(dot.alloc(1));
+    if (!st.p) CHK(st.alloc(1));
+    return SEVP_OK;
+  }
+};
+
 struct sevp_plan_s {
   KProf prof;
   std::deque<PassGraph> graphs;  // deque: PassGraph addresses stay valid as graphs are added
@@ -239,6 +254,7 @@ struct sevp_plan_s {
   DBuf<double> b1s;                      // MATINV scratch (blocked path)
   DBuf<int> zp;
   DBuf<float> s14;                       // step 14: per-workgroup maxima + the maximum
+  std::unique_ptr<struct KryWork> kry;   // Krylov solvers' vectors and scalars, on first use
   const float *al() const { return coef.p; }
   const float *ab() const { return coef.p + (size_t)d.i2 * d.j2; }
   const float *ac() const { return coef.p + 2 * (size_t)d.i2 * d.j2; }
@@ -1471,3 +1487,327 @@ extern "C" int sevp_matinv(double *b, int n) {
 }
 
 
+
+// ------------------------------------------------------------ Krylov solvers
+// Drivers of the device BiCGSTAB, P-CSI and Lanczos (sevp_krylov.hip) with
+// the control flow of MAIN/template/GLOBAL/src/sevp.f90:443-694; the
+// scalar recurrences of BiCGSTAB and P-CSI run on the device, the host
+// checks the stop flag between batches of iterations.
+
+static double kry_now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static int kry_check_opts(const sevp_krylov_opts &o) {
+  if (o.max_iter < 0 || o.restart < 1) return fail(SEVP_E_ARG, "max_iter must be >= 0, restart >= 1");
+  if (!(o.tol >= 0.0) || !(o.rtol >= 0.0 && o.rtol < 1.0))
+    return fail(SEVP_E_ARG, "tol must be >= 0 and rtol in [0, 1)");
+  return SEVP_OK;
+}
+
+static int run_bicgstab(const KryOp &op, KryWork &wk, const float *b, double *x,
+                        const sevp_krylov_opts &o, sevp_krylov_stats *stats, hipStream_t s) {
+  CHK(kry_check_opts(o));
+  const double t0 = kry_now();
+  const long n = (long)(op.i2 + 2) * (op.j2 + 2);
+  CHK(wk.ensure(op.i2, op.j2));
+  double *base = wk.vec.p;
+  KryVecs q{};
+  q.b = b;
+  q.x = x;
+  q.r = base;
+  q.rh = base + n;
+  q.p[0] = base + 2 * n;
+  q.p[1] = base + 3 * n;
+  q.v[0] = base + 4 * n;
+  q.v[1] = base + 5 * n;
+  q.s = base + 6 * n;
+  q.t = base + 7 * n;
+  HIPC(kry_fill(base, 8 * n, 0.0, s));
+  KryState h{};
+  h.tp = 1.0;
+  h.tol = o.tol;
+  h.rtol = o.rtol;
+  h.restart = o.restart;
+  h.max_iter = o.max_iter;
+  HIPC(hipMemcpyAsync(wk.st.p, &h, sizeof h, hipMemcpyHostToDevice, s));
+  if (op.periodic) HIPC(kry_bnd(x, op.i2, op.j2, s));
+  const int fast = !o.ref_order;
+  bool early = false;
+  while (std::sqrt(h.tp) > o.tol && h.iter < o.max_iter) {
+    HIPC(kry_bicg_restart(op, q, wk.st.p, wk.part.p, fast, s));
+    // the device stops by the next multiple of `restart` at the latest
+    const int limit = o.restart - h.iter % o.restart;
+    int launched = 0;
+    do {
+      if (launched >= limit)
+        return fail(SEVP_E_HIP, "BiCGSTAB: the device did not reach its stop condition");
+      int batch = limit - launched;
+      if (batch > 32) batch = 32;
+      for (int k = 0; k < batch; ++k) HIPC(kry_bicg_iter(op, q, wk.st.p, wk.part.p, fast, s));
+      launched += batch;
+      HIPC(hipMemcpyAsync(&h, wk.st.p, sizeof h, hipMemcpyDeviceToHost, s));
+      HIPC(hipStreamSynchronize(s));
+    } while (!h.stop);
+    if (h.stop == 2) {  // converged at the restart: the reference returns at once
+      early = true;
+      break;
+    }
+  }
+  if (!early && op.periodic) HIPC(kry_bnd(x, op.i2, op.j2, s));
+  HIPC(hipStreamSynchronize(s));
+  if (stats) {
+    stats->iterations = h.iter;
+    stats->residual = std::sqrt(h.tp);
+    stats->converged = stats->residual <= o.tol;
+    stats->seconds = kry_now() - t0;
+  }
+  return SEVP_OK;
+}
+
+static int run_pcsi(const KryOp &op, KryWork &wk, const float *b, double *x, double mineig,
+                    double maxeig, const sevp_krylov_opts &o, sevp_krylov_stats *stats,
+                    hipStream_t s) {
+  CHK(kry_check_opts(o));
+  if (!(maxeig != mineig)) return fail(SEVP_E_ARG, "P-CSI needs mineig != maxeig");
+  const double t0 = kry_now();
+  const long n = (long)(op.i2 + 2) * (op.j2 + 2);
+  CHK(wk.ensure(op.i2, op.j2));
+  // the OMEGA recurrence depends on no data: evaluated here, in the
+  // reference's order (p_csi, :580-601)
+  const double alpha = 2.0 / (maxeig - mineig);
+  const double beta = (maxeig + mineig) / (maxeig - mineig);
+  const double csy = beta / alpha;
+  double omega = 2.0 / csy;
+  const int mi = o.max_iter;
+  std::vector<double> tab(2 * (size_t)(mi > 0 ? mi : 1));
+  for (int k = 0; k < mi; ++k) {
+    omega = 1.0 / (csy - omega / (4.0 * alpha * alpha));
+    tab[k] = omega;
+    tab[mi + k] = csy * omega - 1.0;
+  }
+  const double c0 = 1.0 / csy;
+  if (wk.omega.n < tab.size()) CHK(wk.omega.alloc(tab.size()));
+  HIPC(hipMemcpyAsync(wk.omega.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  double *r = wk.vec.p, *q = wk.vec.p + n;
+  HIPC(kry_fill(wk.vec.p, 2 * n, 0.0, s));
+  KryState h{};
+  h.tp = 1.0;
+  h.tol = o.tol;
+  h.max_iter = mi;
+  h.restart = 1;
+  h.stop = (std::sqrt(h.tp) > o.tol && 0 < mi) ? 0 : 1;
+  HIPC(hipMemcpyAsync(wk.st.p, &h, sizeof h, hipMemcpyHostToDevice, s));
+  const int fast = !o.ref_order;
+  if (op.periodic) HIPC(kry_bnd(x, op.i2, op.j2, s));
+  HIPC(kry_pcsi_r(op, b, x, r, wk.st.p, wk.part.p, 0, fast, s));
+  HIPC(kry_pcsi_qx(op.i2, op.j2, op.periodic, r, q, x, wk.omega.p, wk.omega.p + mi, c0, 1, wk.st.p, s));
+  HIPC(kry_pcsi_r(op, b, x, r, wk.st.p, wk.part.p, 0, fast, s));
+  int launched = 0;
+  while (!h.stop) {
+    if (launched >= mi) return fail(SEVP_E_HIP, "P-CSI: the device did not reach its stop condition");
+    int batch = mi - launched;
+    if (batch > 32) batch = 32;
+    launched += batch;
+    for (int k = 0; k < batch; ++k) {
+      HIPC(kry_pcsi_qx(op.i2, op.j2, op.periodic, r, q, x, wk.omega.p, wk.omega.p + mi, c0, 0,
+                       wk.st.p, s));
+      HIPC(kry_pcsi_r(op, b, x, r, wk.st.p, wk.part.p, 1, fast, s));
+    }
+    HIPC(hipMemcpyAsync(&h, wk.st.p, sizeof h, hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+  }
+  if (op.periodic) HIPC(kry_bnd(x, op.i2, op.j2, s));
+  HIPC(hipStreamSynchronize(s));
+  if (stats) {
+    stats->iterations = h.iter;
+    stats->residual = std::sqrt(h.tp);
+    stats->converged = stats->residual <= o.tol;
+    stats->seconds = kry_now() - t0;
+  }
+  return SEVP_OK;
+}
+
+static int kry_read_dot(KryWork &wk, int i2, int j2, const double *a, const double *b, int fast,
+                        hipStream_t s, double &out) {
+  HIPC(kry_dot(i2, j2, a, b, wk.dot.p, wk.part.p, &wk.st.p->count, fast, s));
+  HIPC(hipMemcpyAsync(&out, wk.dot.p, sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPC(hipStreamSynchronize(s));
+  return SEVP_OK;
+}
+
+static int run_lanczos(const KryOp &op, KryWork &wk, int m, int fast, double *mineig,
+                       double *maxeig, double *alpha_out, double *sigma_out, hipStream_t s) {
+  if (m < 1) return fail(SEVP_E_ARG, "Lanczos needs m >= 1");
+  const int i2 = op.i2, j2 = op.j2;
+  const long n = (long)(i2 + 2) * (j2 + 2);
+  CHK(wk.ensure(i2, j2));
+  double *v = wk.vec.p, *vp = v + n, *w = v + 2 * n, *wp = v + 3 * n, *sv = v + 4 * n, *qv = v + 5 * n;
+  HIPC(kry_fill(wk.vec.p, 6 * n, 0.0, s));
+  KryState h{};
+  HIPC(hipMemcpyAsync(wk.st.p, &h, sizeof h, hipMemcpyHostToDevice, s));
+  std::vector<double> alpha(m, 0.0), beta(m + 1, 0.0), sigma(m + 1, 0.0);
+  double tp = 0.0;
+  HIPC(kry_fill(v, n, 1.0, s));
+  CHK(kry_read_dot(wk, i2, j2, v, v, fast, s, tp));
+  const double c = 1.0 / std::sqrt(tp);
+  HIPC(kry_fill(v, n, c * 1.0, s));
+  HIPC(kry_fill(w, n, c * 1.0, s));
+  for (int it = 1; it <= m; ++it) {
+    HIPC(kry_spmv(op, v, sv, 0, s));
+    CHK(kry_read_dot(wk, i2, j2, sv, w, fast, s, tp));
+    alpha[it - 1] = tp;
+    HIPC(kry_lincomb(vp, sv, alpha[it - 1], v, beta[it - 1], vp, n, s));
+    std::swap(v, vp);
+    HIPC(kry_spmv(op, w, qv, 1, s));
+    HIPC(kry_lincomb(wp, qv, alpha[it - 1], w, sigma[it - 1], wp, n, s));
+    std::swap(w, wp);
+    CHK(kry_read_dot(wk, i2, j2, v, w, fast, s, tp));
+    sigma[it] = std::sqrt(std::fabs(tp));
+    beta[it] = std::copysign(sigma[it], tp);
+    HIPC(kry_scale(v, sigma[it], w, beta[it], n, s));
+  }
+  HIPC(kry_spmv(op, v, sv, 0, s));
+  CHK(kry_read_dot(wk, i2, j2, sv, w, fast, s, tp));
+  alpha[m - 1] = tp;
+  // FIGI's symmetric tridiagonal: D = ALPHA, E(I) = SQRT(SIGMA(I)*SIGMA(I));
+  // E(1), which FIGI leaves unset and the bound below reads, is zero
+  std::vector<double> d(m), e(m);
+  for (int i = 0; i < m; ++i) {
+    d[i] = alpha[i];
+    e[i] = i > 0 ? std::sqrt(sigma[i] * sigma[i]) : 0.0;
+  }
+  double mx = d[0] + e[0];
+  for (int i = 1; i < m; ++i) {
+    const double g = d[i] + e[i] + e[i - 1];
+    mx = g > mx ? g : mx;
+  }
+  if (maxeig) *maxeig = mx;
+  if (mineig) *mineig = kry_tridiag_min_eig(d.data(), e.data(), m);
+  if (alpha_out) std::memcpy(alpha_out, alpha.data(), sizeof(double) * m);
+  if (sigma_out) std::memcpy(sigma_out, sigma.data(), sizeof(double) * (m + 1));
+  return SEVP_OK;
+}
+
+static KryOp plan_kry_op(sevp_plan p, int periodic) {
+  return KryOp{p->al(), p->ab(), p->ac(), p->ar(), p->at(), p->d.i2, p->d.j2, periodic ? 1 : 0};
+}
+
+static int kry_plan_work(sevp_plan p) {
+  if (!p->kry) p->kry.reset(new KryWork());
+  return p->kry->ensure(p->d.i2, p->d.j2);
+}
+
+// A temporary device copy of host coefficients for the one-to-one calls.
+struct KryHostOp {
+  DBuf<float> coef;
+  KryOp op{};
+  int load(const float *al, const float *ab, const float *ac, const float *ar, const float *at,
+           int i2, int j2, int periodic) {
+    if (!al || !ab || !ac || !ar || !at || i2 < 2 || j2 < 2) return fail(SEVP_E_ARG, "bad operator");
+    const size_t m = (size_t)i2 * j2;
+    CHK(coef.alloc(5 * m));
+    const float *src[5] = {al, ab, ac, ar, at};
+    for (int k = 0; k < 5; ++k)
+      HIPC(hipMemcpy(coef.p + k * m, src[k], m * sizeof(float), hipMemcpyHostToDevice));
+    op = KryOp{coef.p, coef.p + m, coef.p + 2 * m, coef.p + 3 * m, coef.p + 4 * m, i2, j2, periodic ? 1 : 0};
+    return SEVP_OK;
+  }
+};
+
+extern "C" {
+
+int sevp_krylov_defaults(sevp_krylov_opts *o, int solver) {
+  if (!o) return fail(SEVP_E_ARG, "null argument");
+  o->periodic = 1;
+  o->ref_order = 0;
+  o->max_iter = 1000;
+  o->restart = 100;
+  o->tol = solver == SEVP_KRYLOV_PCSI ? (double)1e-7f : (double)1e-8f;
+  o->rtol = (double)1e-8f;
+  return SEVP_OK;
+}
+
+int sevp_bicgstab_dev(sevp_plan p, const float *b, double *x, const sevp_krylov_opts *opts,
+                      sevp_krylov_stats *stats) {
+  if (!p || !b || !x) return fail(SEVP_E_ARG, "null argument");
+  sevp_krylov_opts o;
+  sevp_krylov_defaults(&o, SEVP_KRYLOV_BICGSTAB);
+  if (opts) o = *opts;
+  CHK(kry_plan_work(p));
+  return run_bicgstab(plan_kry_op(p, o.periodic), *p->kry, b, x, o, stats, p->stream);
+}
+
+int sevp_pcsi_dev(sevp_plan p, double mineig, double maxeig, const float *b, double *x,
+                  const sevp_krylov_opts *opts, sevp_krylov_stats *stats) {
+  if (!p || !b || !x) return fail(SEVP_E_ARG, "null argument");
+  sevp_krylov_opts o;
+  sevp_krylov_defaults(&o, SEVP_KRYLOV_PCSI);
+  if (opts) o = *opts;
+  CHK(kry_plan_work(p));
+  return run_pcsi(plan_kry_op(p, o.periodic), *p->kry, b, x, mineig, maxeig, o, stats, p->stream);
+}
+
+int sevp_lanczos_dev(sevp_plan p, int m, const sevp_krylov_opts *opts, double *mineig,
+                     double *maxeig, double *alpha, double *sigma) {
+  if (!p) return fail(SEVP_E_ARG, "null argument");
+  CHK(kry_plan_work(p));
+  return run_lanczos(plan_kry_op(p, 0), *p->kry, m, opts ? !opts->ref_order : 1, mineig, maxeig,
+                     alpha, sigma, p->stream);
+}
+
+int sevp_p_bicgstab_le(const float *al, const float *ab, const float *ac, const float *ar,
+                       const float *at, const float *b, double *x, int i2, int j2,
+                       const sevp_krylov_opts *opts, sevp_krylov_stats *stats) {
+  if (!b || !x) return fail(SEVP_E_ARG, "null argument");
+  sevp_krylov_opts o;
+  sevp_krylov_defaults(&o, SEVP_KRYLOV_BICGSTAB);
+  if (opts) o = *opts;
+  KryHostOp h;
+  CHK(h.load(al, ab, ac, ar, at, i2, j2, o.periodic));
+  KryWork wk;
+  DBuf<float> bd;
+  DBuf<double> xd;
+  const size_t nx = (size_t)(i2 + 2) * (j2 + 2);
+  CHK(bd.alloc((size_t)i2 * j2));
+  CHK(xd.alloc(nx));
+  HIPC(hipMemcpy(bd.p, b, bd.bytes(), hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(xd.p, x, xd.bytes(), hipMemcpyHostToDevice));
+  CHK(run_bicgstab(h.op, wk, bd.p, xd.p, o, stats, aux_stream()));
+  HIPC(hipMemcpy(x, xd.p, xd.bytes(), hipMemcpyDeviceToHost));
+  return SEVP_OK;
+}
+
+int sevp_p_csi(const float *al, const float *ab, const float *ac, const float *ar, const float *at,
+               double mineig, double maxeig, const float *b, double *x, int i2, int j2,
+               const sevp_krylov_opts *opts, sevp_krylov_stats *stats) {
+  if (!b || !x) return fail(SEVP_E_ARG, "null argument");
+  sevp_krylov_opts o;
+  sevp_krylov_defaults(&o, SEVP_KRYLOV_PCSI);
+  if (opts) o = *opts;
+  KryHostOp h;
+  CHK(h.load(al, ab, ac, ar, at, i2, j2, o.periodic));
+  KryWork wk;
+  DBuf<float> bd;
+  DBuf<double> xd;
+  const size_t nx = (size_t)(i2 + 2) * (j2 + 2);
+  CHK(bd.alloc((size_t)i2 * j2));
+  CHK(xd.alloc(nx));
+  HIPC(hipMemcpy(bd.p, b, bd.bytes(), hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(xd.p, x, xd.bytes(), hipMemcpyHostToDevice));
+  CHK(run_pcsi(h.op, wk, bd.p, xd.p, mineig, maxeig, o, stats, aux_stream()));
+  HIPC(hipMemcpy(x, xd.p, xd.bytes(), hipMemcpyDeviceToHost));
+  return SEVP_OK;
+}
+
+int sevp_lanczos(const float *al, const float *ab, const float *ac, const float *ar,
+                 const float *at, int i2, int j2, int m, const sevp_krylov_opts *opts,
+                 double *mineig, double *maxeig, double *alpha, double *sigma) {
+  KryHostOp h;
+  CHK(h.load(al, ab, ac, ar, at, i2, j2, 0));
+  KryWork wk;
+  return run_lanczos(h.op, wk, m, opts ? !opts->ref_order : 1, mineig, maxeig, alpha, sigma,
+                     aux_stream());
+}
+
+}  // extern "C"

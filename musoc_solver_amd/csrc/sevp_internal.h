@@ -217,4 +217,46 @@ hipError_t launch_scale_cols(const double *r, const float *cy, int cyrow, int i2
                              hipStream_t st);
 hipError_t launch_halo_fix(double *x, long x_stride, int i2, int j2, int nrhs, hipStream_t st);
 
+// ------------------------------------------------ Krylov solvers (sevp_krylov.hip)
+constexpr int KRY_MAXB = 1024;  // most workgroups of a Krylov kernel (partial sums per dot)
+
+struct KryOp {
+  const float *al, *ab, *ac, *ar, *at;  // (i2, j2)
+  int i2, j2, periodic;
+};
+
+// The vectors of one solve, each (i2+2, j2+2) FP64; p and v alternate.
+struct KryVecs {
+  const float *b;  // (i2, j2) right-hand side
+  double *x, *r, *rh, *p[2], *v[2], *s, *t;
+};
+
+// Scalars of a BiCGSTAB or P-CSI solve, resident on the device.
+struct KryState {
+  double rho, alpha, w, beta, rhn, tp, res, tol, rtol;
+  int iter, stop, restart, max_iter;  // stop: 1 inner loop ended, 2 converged at a restart
+  unsigned count;                     // finishing-workgroup counter, back to 0 after each use
+};
+
+int kry_workgroups(int i2, int j2);
+hipError_t kry_bicg_restart(const KryOp &op, const KryVecs &q, KryState *st, double *part, int fast,
+                            hipStream_t s);
+hipError_t kry_bicg_iter(const KryOp &op, const KryVecs &q, KryState *st, double *part, int fast,
+                         hipStream_t s);
+hipError_t kry_bnd(double *x, int i2, int j2, hipStream_t s);
+hipError_t kry_fill(double *a, long n, double v, hipStream_t s);
+hipError_t kry_pcsi_qx(int i2, int j2, int periodic, const double *r, double *q, double *x,
+                       const double *omega, const double *cq, double c0, int first,
+                       const KryState *st, hipStream_t s);
+hipError_t kry_pcsi_r(const KryOp &op, const float *b, const double *x, double *r, KryState *st,
+                      double *part, int with_dot, int fast, hipStream_t s);
+hipError_t kry_spmv(const KryOp &op, const double *v, double *out, int transposed, hipStream_t s);
+hipError_t kry_lincomb(double *out, const double *sv, double alpha, const double *v, double beta,
+                       const double *vp, long n, hipStream_t s);
+hipError_t kry_scale(double *v, double sig, double *w, double bet, long n, hipStream_t s);
+hipError_t kry_dot(int i2, int j2, const double *a, const double *b, double *out, double *part,
+                   unsigned *count, int fast, hipStream_t s);
+// smallest eigenvalue of the symmetric tridiagonal (d, e[1..m-1]), bisection
+double kry_tridiag_min_eig(const double *d, const double *e, int m);
+
 }  // namespace sevp

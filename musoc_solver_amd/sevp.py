@@ -106,6 +106,16 @@ def lib():
         L.sevp_plan_create_cached.argtypes = ([ctypes.POINTER(_Desc)] + [VP] * 5
                                               + [ctypes.c_char_p, ctypes.POINTER(VP),
                                                  ctypes.POINTER(CI)])
+        KO, KS = ctypes.POINTER(KrylovOpts), ctypes.POINTER(KrylovStats)
+        DP = ctypes.POINTER(ctypes.c_double)
+        CD = ctypes.c_double
+        L.sevp_krylov_defaults.argtypes = [KO, CI]
+        L.sevp_bicgstab_dev.argtypes = [VP, VP, VP, KO, KS]
+        L.sevp_pcsi_dev.argtypes = [VP, CD, CD, VP, VP, KO, KS]
+        L.sevp_lanczos_dev.argtypes = [VP, CI, KO, DP, DP, VP, VP]
+        L.sevp_p_bicgstab_le.argtypes = [VP] * 7 + [CI, CI, KO, KS]
+        L.sevp_p_csi.argtypes = [VP] * 5 + [CD, CD, VP, VP, CI, CI, KO, KS]
+        L.sevp_lanczos.argtypes = [VP] * 5 + [CI, CI, CI, KO, DP, DP, VP, VP]
         L.sevp_evp_write.argtypes = [ctypes.c_char_p, ctypes.POINTER(_Desc)] + [VP] * 8 + [ctypes.c_long]
         L.sevp_evp_read.argtypes = ([ctypes.c_char_p, ctypes.POINTER(_Desc)] + [VP] * 9
                                     + [CI, ctypes.POINTER(CI)])
@@ -170,6 +180,33 @@ class Step14Grid(ctypes.Structure):
 
 class Step14State(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in ("u", "v", "w", "p0", "scr", "s", "x")]
+
+
+class KrylovOpts(ctypes.Structure):
+    """sevp_krylov_opts (include/sevp.h)."""
+    _fields_ = [("periodic", ctypes.c_int), ("ref_order", ctypes.c_int), ("max_iter", ctypes.c_int),
+                ("restart", ctypes.c_int), ("tol", ctypes.c_double), ("rtol", ctypes.c_double)]
+
+
+class KrylovStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int),
+                ("residual", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+
+KRYLOV_BICGSTAB = 0
+KRYLOV_PCSI = 1
+
+
+def krylov_opts(solver=KRYLOV_BICGSTAB, **kw) -> KrylovOpts:
+    """The reference's settings for ``solver``, with fields overridden by
+    keyword (periodic, ref_order, max_iter, restart, tol, rtol)."""
+    o = KrylovOpts()
+    _check(lib().sevp_krylov_defaults(ctypes.byref(o), solver))
+    for k, v in kw.items():
+        if v is None:
+            continue
+        setattr(o, k, int(v) if k in ("periodic", "ref_order", "max_iter", "restart") else float(v))
+    return o
 
 
 class SevpPlan:
@@ -266,6 +303,36 @@ class SevpPlan:
         plan = cls(rec["coefs"], rec["ie"], form=form, nbir=nbir, **kw)
         plan.load_influence(rec["rinv"], rec["rinv1"])
         return plan
+
+    # --------------------------------------------------- Krylov solvers
+    def bicgstab_dev(self, b, x, **opts) -> KrylovStats:
+        """p_bicgstab_le on device arrays: b (J2, I2) float32 and x
+        (J2+2, I2+2) float64 torch tensors (the Fortran arrays, transposed
+        views); x is updated in place."""
+        self._follow_torch_stream()
+        st = KrylovStats()
+        o = krylov_opts(KRYLOV_BICGSTAB, **opts)
+        _check(lib().sevp_bicgstab_dev(self._h, _p(b), _p(x), ctypes.byref(o), ctypes.byref(st)))
+        return st
+
+    def pcsi_dev(self, mineig, maxeig, b, x, **opts) -> KrylovStats:
+        """p_csi on device arrays (as bicgstab_dev)."""
+        self._follow_torch_stream()
+        st = KrylovStats()
+        o = krylov_opts(KRYLOV_PCSI, **opts)
+        _check(lib().sevp_pcsi_dev(self._h, float(mineig), float(maxeig), _p(b), _p(x),
+                                   ctypes.byref(o), ctypes.byref(st)))
+        return st
+
+    def lanczos_dev(self, m=10, ref_order=False):
+        """lanczos: (mineig, maxeig, alpha[m], sigma[m+1])."""
+        self._follow_torch_stream()
+        o = krylov_opts(ref_order=ref_order)
+        mn, mx = ctypes.c_double(), ctypes.c_double()
+        alpha, sigma = np.zeros(m), np.zeros(m + 1)
+        _check(lib().sevp_lanczos_dev(self._h, m, ctypes.byref(o), ctypes.byref(mn),
+                                      ctypes.byref(mx), _p(alpha), _p(sigma)))
+        return mn.value, mx.value, alpha, sigma
 
     def set_precision(self, precision):
         _check(lib().sevp_set_precision(self._h, precision))
@@ -481,3 +548,41 @@ def operator_hash(coefs, ie, form=FORM_BIR, nbir=0) -> int:
     _check(lib().sevp_operator_hash(ctypes.byref(d), _p(al), _p(ab), _p(ac), _p(ar), _p(at),
                                     ctypes.byref(out)))
     return out.value
+
+
+# ------------------------------------------------------------ Krylov solvers
+
+def p_bicgstab_le(coefs, b, x, **opts) -> KrylovStats:
+    """P_BICGSTAB_LE (MAIN/template/GLOBAL/src/sevp.f90:443-552) on host
+    arrays: b (I2, J2) float32, x (I2+2, J2+2) float64 updated in place."""
+    al, ab, ac, ar, at = [_f32(np.asfortranarray(c, dtype=np.float32)) for c in coefs]
+    i2, j2 = al.shape
+    st = KrylovStats()
+    o = krylov_opts(KRYLOV_BICGSTAB, **opts)
+    _check(lib().sevp_p_bicgstab_le(_p(al), _p(ab), _p(ac), _p(ar), _p(at), _p(_f32(b)),
+                                    _p(_f64(x)), i2, j2, ctypes.byref(o), ctypes.byref(st)))
+    return st
+
+
+def p_csi(coefs, mineig, maxeig, b, x, **opts) -> KrylovStats:
+    """P_CSI (MAIN/template/GLOBAL/src/sevp.f90:553-616) on host arrays."""
+    al, ab, ac, ar, at = [_f32(np.asfortranarray(c, dtype=np.float32)) for c in coefs]
+    i2, j2 = al.shape
+    st = KrylovStats()
+    o = krylov_opts(KRYLOV_PCSI, **opts)
+    _check(lib().sevp_p_csi(_p(al), _p(ab), _p(ac), _p(ar), _p(at), float(mineig), float(maxeig),
+                            _p(_f32(b)), _p(_f64(x)), i2, j2, ctypes.byref(o), ctypes.byref(st)))
+    return st
+
+
+def lanczos(coefs, m=10, ref_order=False):
+    """LANCZOS (MAIN/template/GLOBAL/src/sevp.f90:618-694): (mineig, maxeig,
+    alpha[m], sigma[m+1])."""
+    al, ab, ac, ar, at = [_f32(np.asfortranarray(c, dtype=np.float32)) for c in coefs]
+    i2, j2 = al.shape
+    o = krylov_opts(ref_order=ref_order)
+    mn, mx = ctypes.c_double(), ctypes.c_double()
+    alpha, sigma = np.zeros(m), np.zeros(m + 1)
+    _check(lib().sevp_lanczos(_p(al), _p(ab), _p(ac), _p(ar), _p(at), i2, j2, m, ctypes.byref(o),
+                              ctypes.byref(mn), ctypes.byref(mx), _p(alpha), _p(sigma)))
+    return mn.value, mx.value, alpha, sigma
