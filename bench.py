@@ -72,6 +72,10 @@ def parse():
                          "the timed steps, launched eagerly with events from the kernel's own "
                          "timestamps (after), or inside the timed steps through event nodes in "
                          "the captured pass, which also count the dispatch gap (timed)")
+    ap.add_argument("--krylov", type=int, default=1,
+                    help="also time the device BiCGSTAB cross-check on the first RHS (0: skip)")
+    ap.add_argument("--krylov-only", action="store_true",
+                    help="only the BiCGSTAB leg (for kernel profiles of its iteration)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device (reference kernel for HBM counters)")
     return ap.parse_args()
@@ -180,6 +184,12 @@ def main():
 
     legs, setup, xs = {}, {}, {}
     order = [l for l in a.legs.split(",") if l]
+    if a.krylov_only:  # BiCGSTAB needs the operator only, no influence matrices
+        ie = synth.reference_ie(j2, synth.nb0_for_height(j2, a.n0))
+        plan = S.SevpPlan(case.coefs, ie, form=S.FORM_FULL_PERIODIC)
+        print(json.dumps({"krylov": krylov_leg(plan, s, None, a)}), flush=True)
+        plan.close()
+        return
     if a.skip_plain and "plain" in order:
         order.remove("plain")
     plan_keep = None
@@ -213,6 +223,7 @@ def main():
     main_name = "mixed" if "mixed" in legs else "plain"
     mixed_main = main_name == "mixed"
     roof = roofline(main_leg, main_name, nrhs, i2, a)
+    krylov = krylov_leg(plan_keep[0], s, xs.get(main_name), a) if a.krylov else None
     cpu = None
     if a.cpu_rhs > 0:
         cpu = cpu_baseline(plan_keep[0], case, plan_keep[1], s_np, a)
@@ -241,6 +252,7 @@ def main():
         "setup_s": t_setup,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "krylov": krylov,
     }
     print(json.dumps(out), flush=True)
     plan_keep[0].close()
@@ -278,6 +290,40 @@ def roofline(leg, name, nrhs, i2, a):
               "kernel": "gemm_part_kernel" if name == "mixed" else "gemm_seq_kernel",
               "algorithmic_bytes_per_launch": nbytes, "algorithmic_flops_per_launch": flops,
               "avg_launch_us": leg["corr_avg_us"], "rhs_per_product": m})
+    return r
+
+
+def krylov_leg(plan, s, x_sevp, a):
+    """The device BiCGSTAB (p_bicgstab_le, fast-order sums) on the first
+    right-hand side from x = 0, to the reference's stop ||r||_2 <= 1e-8
+    (REAL(4)), timed after one untimed run.  One iteration is three fused
+    kernels; its algorithmic HBM traffic is 17 passes over the
+    (I2+2) x (J2+2) FP64 vectors (PV: r, p, v, rh read, p, v written; ST: r,
+    v read, s, t written; XR: x, p, s, t, rh read, x, r written) plus the
+    five FP32 coefficient arrays read by the two 5-point products:
+    136 (I2+2)(J2+2) + 40 I2 J2 bytes.  Compared with the SEVP solution
+    as FS_SOLVER compares them (max difference over max |x|)."""
+    import torch
+    i2, j2 = a.i2, a.j2
+    b = s[0]
+    res = None
+    for _ in range(2):
+        x = torch.zeros((j2 + 2, i2 + 2), dtype=torch.float64, device=s.device)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = plan.bicgstab_dev(b, x)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+    nbytes = 136.0 * (i2 + 2) * (j2 + 2) + 40.0 * i2 * j2
+    per_it = dt / max(st.iterations, 1)
+    r = {"solver": "BiCGSTAB (p_bicgstab_le), fixed-order sums", "iterations": st.iterations,
+         "residual": st.residual, "converged": bool(st.converged), "ms": 1e3 * dt,
+         "us_per_iteration": 1e6 * per_it, "bytes_per_iteration": nbytes,
+         "hbm_gbs": nbytes / per_it / 1e9, "hbm_frac": nbytes / per_it / PEAK_HBM}
+    if x_sevp is not None:
+        xe = x_sevp[0]
+        r["max_diff_vs_sevp"] = float((x - xe)[1:-1, 1:-1].abs().max() / xe[1:-1, 1:-1].abs().max())
+    log(f"krylov: {r}")
     return r
 
 

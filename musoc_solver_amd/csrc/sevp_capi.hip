@@ -214,6 +214,7 @@ struct KryWork {
   DBuf<KryState> st;
   int ensure(int i2, int j2) {
     const size_t n = (size_t)(i2 + 2) * (j2 + 2);
+    if (n > ((size_t)1 << 30)) return fail(SEVP_E_ARG, "Krylov solvers take at most 2^30 grid points");
     if (vec.n < 8 * n) CHK(vec.alloc(8 * n));
     if (!part.p) CHK(part.alloc(2 * (size_t)KRY_MAXB));
     if (!dot.p) CHK(dot.alloc(1));
@@ -1630,7 +1631,7 @@ static int run_pcsi(const KryOp &op, KryWork &wk, const float *b, double *x, dou
 
 static int kry_read_dot(KryWork &wk, int i2, int j2, const double *a, const double *b, int fast,
                         hipStream_t s, double &out) {
-  HIPC(kry_dot(i2, j2, a, b, wk.dot.p, wk.part.p, &wk.st.p->count, fast, s));
+  HIPC(kry_dot(i2, j2, a, b, wk.dot.p, wk.part.p, fast, s));
   HIPC(hipMemcpyAsync(&out, wk.dot.p, sizeof(double), hipMemcpyDeviceToHost, s));
   HIPC(hipStreamSynchronize(s));
   return SEVP_OK;

@@ -218,7 +218,7 @@ hipError_t launch_scale_cols(const double *r, const float *cy, int cyrow, int i2
 hipError_t launch_halo_fix(double *x, long x_stride, int i2, int j2, int nrhs, hipStream_t st);
 
 // ------------------------------------------------ Krylov solvers (sevp_krylov.hip)
-constexpr int KRY_MAXB = 1024;  // most workgroups of a Krylov kernel (partial sums per dot)
+constexpr int KRY_MAXB = 8192;  // most workgroups of a Krylov kernel (partial sums per dot)
 
 struct KryOp {
   const float *al, *ab, *ac, *ar, *at;  // (i2, j2)
@@ -235,7 +235,6 @@ struct KryVecs {
 struct KryState {
   double rho, alpha, w, beta, rhn, tp, res, tol, rtol;
   int iter, stop, restart, max_iter;  // stop: 1 inner loop ended, 2 converged at a restart
-  unsigned count;                     // finishing-workgroup counter, back to 0 after each use
 };
 
 int kry_workgroups(int i2, int j2);
@@ -254,8 +253,8 @@ hipError_t kry_spmv(const KryOp &op, const double *v, double *out, int transpose
 hipError_t kry_lincomb(double *out, const double *sv, double alpha, const double *v, double beta,
                        const double *vp, long n, hipStream_t s);
 hipError_t kry_scale(double *v, double sig, double *w, double bet, long n, hipStream_t s);
-hipError_t kry_dot(int i2, int j2, const double *a, const double *b, double *out, double *part,
-                   unsigned *count, int fast, hipStream_t s);
+hipError_t kry_dot(int i2, int j2, const double *a, const double *b, double *out, double *part, int fast,
+                   hipStream_t s);
 // smallest eigenvalue of the symmetric tridiagonal (d, e[1..m-1]), bisection
 double kry_tridiag_min_eig(const double *d, const double *e, int m);
 

@@ -10,14 +10,15 @@
 // which gives the same bits as computing the interior element and copying.
 //
 // Inner products are the only place where the order of summation is free.
-// Two modes:
-//   fast      each workgroup sums its contiguous chunk in a fixed order, the
-//             last workgroup to finish (device-scope fence + counter) sums
-//             the partials in a fixed order and advances the scalar
-//             recurrences on the device; deterministic from run to run.
-//   ref_order one thread sums the interior sequentially, j outer and i
-//             inner, as p_inprod does: the whole iteration is then bitwise
-//             the reference's (slow: for parity checks on small grids).
+// Each kernel that needs one stores per-workgroup partial sums (its chunk in
+// a fixed order); a one-workgroup finishing kernel launched after it adds
+// them in a fixed order and advances the scalar recurrences on the device,
+// so results are deterministic from run to run.  (A last-workgroup-finishes
+// scheme inside the kernel needs a device-scope release per workgroup, an L2
+// writeback on this GPU; measured, those cost more than the extra launch.)
+// With ref_order the finishing kernel instead sums the interior with one
+// thread, j outer and i inner, as p_inprod does: every iterate is then
+// bitwise the reference's (slow: for parity checks on small grids).
 // BiCGSTAB's scalars (RHO, ALPHA, W, BETA) and its stop flag live in device
 // memory, so iterations are enqueued without a host round trip; the host
 // only checks the flag between batches of iterations.
@@ -38,7 +39,9 @@ __device__ inline int kry_chunk_id() {
   return (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;
 }
 
-// Sum over the workgroup in a fixed order; the result is valid in thread 0.
+// Sum over a workgroup of NT threads in a fixed order; the result is valid
+// in thread 0.
+template <int NT = KT>
 __device__ inline double kry_block_sum(double v, double *lds) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
   const int wave = threadIdx.x >> 6;
@@ -46,36 +49,21 @@ __device__ inline double kry_block_sum(double v, double *lds) {
   __syncthreads();
   double s = 0.0;
   if (threadIdx.x == 0)
-    for (int k = 0; k < KT / 64; ++k) s += lds[k];
+    for (int k = 0; k < NT / 64; ++k) s += lds[k];
   __syncthreads();
   return s;
 }
 
-// Publishes this workgroup's partial sums; returns true in the last
-// workgroup to finish, whose thread 0 then holds the totals in out[].
+// Stores this workgroup's sums (thread 0) as partials of chunk
+// kry_chunk_id(); the finishing kernel that follows on the stream adds them.
 template <int ND>
-__device__ bool kry_finish(const double (&v)[ND], double *part, unsigned *count, double (&out)[ND]) {
+__device__ void kry_partials(const double (&v)[ND], double *part) {
   __shared__ double lds[KT / 64];
-  __shared__ int last;
   const int cid = kry_chunk_id();
   for (int d = 0; d < ND; ++d) {
     const double s = kry_block_sum(v[d], lds);
     if (threadIdx.x == 0) part[(long)d * KRY_MAXB + cid] = s;
   }
-  if (threadIdx.x == 0) {
-    __threadfence();
-    last = atomicAdd(count, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  for (int d = 0; d < ND; ++d) {
-    double a = 0.0;
-    for (int k = threadIdx.x; k < (int)gridDim.x; k += KT) a += part[(long)d * KRY_MAXB + k];
-    out[d] = kry_block_sum(a, lds);
-  }
-  if (threadIdx.x == 0) *count = 0u;
-  return threadIdx.x == 0;
 }
 
 // ----------------------------------------------------- scalar recurrences
@@ -108,72 +96,103 @@ __device__ inline void fin_pcsi(KryState *st, double rr) {
 
 // --------------------------------------------------------------- helpers
 struct Cells {
-  long n, chunk;
-  int I0, I2, J2;
+  int n, chunk, I0, I2, J2;
   __device__ Cells(int i2, int j2) : I0(i2 + 2), I2(i2), J2(j2) {
-    n = (long)I0 * (j2 + 2);
-    const long per = (n + gridDim.x - 1) / gridDim.x;
+    n = I0 * (j2 + 2);
+    const int per = (n + (int)gridDim.x - 1) / (int)gridDim.x;
     chunk = (per + KT - 1) / KT * KT;
   }
-  __device__ long begin() const { return (long)kry_chunk_id() * chunk + threadIdx.x; }
-  __device__ long end() const {
-    const long e = ((long)kry_chunk_id() + 1) * chunk;
+  __device__ int begin() const { return kry_chunk_id() * chunk + (int)threadIdx.x; }
+  __device__ int end() const {
+    const int e = (kry_chunk_id() + 1) * chunk;
     return e < n ? e : n;
   }
 };
 
-// The element whose stencil value a halo or interior element holds: the
-// element itself (interior), its periodic mirror (halo column of an
-// interior row, periodic), or none (-1: the value is zero).
-__device__ inline long stencil_site(int i, int j, int I0, int I2, int J2, int periodic) {
+// One thread's elements c = begin, begin + KT, ... of its chunk, with the
+// column i and row j (0-based, halos included) stepped along without a
+// division per element.
+struct Walk {
+  int c, e, i, j, I0;
+  __device__ explicit Walk(const Cells &g) : c(g.begin()), e(g.end()), I0(g.I0) {
+    i = c % I0;
+    j = c / I0;
+  }
+  __device__ bool ok() const { return c < e; }
+  __device__ void next() {
+    c += KT;
+    i += KT;
+    while (i >= I0) {
+      i -= I0;
+      ++j;
+    }
+  }
+};
+
+// The column whose 5-point value element (i, j) holds: i itself (interior),
+// its periodic mirror (halo column of an interior row, periodic), or -1 (the
+// value is zero).
+__device__ inline int site_col(int i, int j, int I2, int J2, int periodic) {
   if (j < 1 || j > J2) return -1;
-  if (i >= 1 && i <= I2) return i + (long)j * I0;
+  if (i >= 1 && i <= I2) return i;
   if (!periodic) return -1;
-  return (i == 0 ? I2 : 1) + (long)j * I0;
+  return i == 0 ? I2 : 1;
 }
 
 // p_sqprod at site cc (coefficients of interior element k) of the field f.
 template <class F>
-__device__ inline double sqprod_at(const KryOp &op, long cc, long k, int I0, F f) {
+__device__ inline double sqprod_at(const KryOp &op, int cc, int k, int I0, F f) {
   return (double)op.ab[k] * f(cc - I0) + (double)op.al[k] * f(cc - 1) +
          (double)op.ac[k] * f(cc) + (double)op.ar[k] * f(cc + 1) + (double)op.at[k] * f(cc + I0);
 }
 
-__device__ inline long coef_index(long cc, int I0, int I2) {
-  const int i = (int)(cc % I0), j = (int)(cc / I0);
-  return (i - 1) + (long)(j - 1) * I2;
+// ------------------------------------------------------------- BiCGSTAB
+// Loads are issued unconditionally (an element without a 5-point value reads
+// the valid site (1,1) and selects zero), through restrict-qualified
+// pointers, so that a thread's operands are all in flight before its stores.
+
+struct Site {
+  int cc, k;  // element and coefficient index of the 5-point site
+  bool on;    // whether the element holds a 5-point value
+  __device__ Site(int i, int j, int I0, int I2, int J2, int periodic) {
+    const int si = site_col(i, j, I2, J2, periodic);
+    on = si >= 0;
+    const int sc = on ? si : 1, sj = on ? j : 1;
+    cc = sc + sj * I0;
+    k = (sc - 1) + (sj - 1) * I2;
+  }
+};
+
+__device__ inline bool interior(int i, int j, const Cells &g) {
+  return i >= 1 && i <= g.I2 && j >= 1 && j <= g.J2;
 }
 
-// ------------------------------------------------------------- BiCGSTAB
 // R = B - A X, RH = R, (R, R) (the start of each restart cycle).
-__global__ __launch_bounds__(KT) void bicg_restart_kernel(KryOp op, const float *b, const double *x,
-                                                          double *r, double *rh, KryState *st,
+__global__ __launch_bounds__(KT) void bicg_restart_kernel(KryOp op, const float *__restrict__ b,
+                                                          const double *__restrict__ x,
+                                                          double *__restrict__ r,
+                                                          double *__restrict__ rh, KryState *st,
                                                           double *part, int fast) {
   const Cells g(op.i2, op.j2);
   double acc[1] = {0.0};
-  for (long c = g.begin(); c < g.end(); c += KT) {
-    const int i = (int)(c % g.I0), j = (int)(c / g.I0);
-    const long cc = stencil_site(i, j, g.I0, g.I2, g.J2, op.periodic);
-    double v = 0.0;
-    if (cc >= 0) {
-      const long k = coef_index(cc, g.I0, g.I2);
-      const double ax = sqprod_at(op, cc, k, g.I0, [&](long e) { return x[e]; });
-      v = (double)b[k] - ax;
-    }
+  for (Walk it(g); it.ok(); it.next()) {
+    const int c = it.c, i = it.i, j = it.j;
+    const Site t(i, j, g.I0, g.I2, g.J2, op.periodic);
+    const double bk = (double)b[t.k];
+    const double ax = sqprod_at(op, t.cc, t.k, g.I0, [&](int e) { return x[e]; });
+    const double v = t.on ? bk - ax : 0.0;
     r[c] = v;
     rh[c] = v;
-    if (i >= 1 && i <= g.I2 && j >= 1 && j <= g.J2) acc[0] += v * v;
+    if (interior(i, j, g)) acc[0] += v * v;
   }
-  if (!fast) return;
-  double out[1];
-  if (kry_finish(acc, part, &st->count, out)) fin_restart(st, out[0]);
+  if (fast) kry_partials(acc, part);
 }
 
 // The current P and V are p[iter & 1], v[iter & 1]: each iteration writes
 // the other pair, so that no workgroup reads a neighbour another one is
 // rewriting, and the parity follows the device's own iteration count (the
-// host may enqueue iterations past the stop).  Every workgroup reads
-// st->iter before the last one to finish advances it.
+// host may enqueue iterations past the stop); the XR finishing kernel
+// advances it after the iteration's last kernel.
 
 // P = R + BETA*(P - W*V); V = A P; (RH, V).
 __global__ __launch_bounds__(KT) void bicg_pv_kernel(KryOp op, KryVecs q, KryState *st, double *part,
@@ -181,23 +200,26 @@ __global__ __launch_bounds__(KT) void bicg_pv_kernel(KryOp op, KryVecs q, KrySta
   if (st->stop) return;
   const double beta = st->beta, w = st->w;
   const int cur = st->iter & 1;
-  const double *r = q.r, *rh = q.rh, *pin = q.p[cur], *vin = q.v[cur];
-  double *pout = q.p[cur ^ 1], *vout = q.v[cur ^ 1];
+  const double *__restrict__ r = q.r;
+  const double *__restrict__ rh = q.rh;
+  const double *__restrict__ pin = q.p[cur];
+  const double *__restrict__ vin = q.v[cur];
+  double *__restrict__ pout = q.p[cur ^ 1];
+  double *__restrict__ vout = q.v[cur ^ 1];
   const Cells g(op.i2, op.j2);
-  auto pn = [&](long e) { return r[e] + beta * (pin[e] - w * vin[e]); };
+  auto pn = [&](int e) { return r[e] + beta * (pin[e] - w * vin[e]); };
   double acc[1] = {0.0};
-  for (long c = g.begin(); c < g.end(); c += KT) {
-    const int i = (int)(c % g.I0), j = (int)(c / g.I0);
-    pout[c] = pn(c);
-    const long cc = stencil_site(i, j, g.I0, g.I2, g.J2, op.periodic);
-    double v = 0.0;
-    if (cc >= 0) v = sqprod_at(op, cc, coef_index(cc, g.I0, g.I2), g.I0, pn);
+  for (Walk it(g); it.ok(); it.next()) {
+    const int c = it.c, i = it.i, j = it.j;
+    const Site t(i, j, g.I0, g.I2, g.J2, op.periodic);
+    const double pc = pn(c), rhc = rh[c];
+    const double av = sqprod_at(op, t.cc, t.k, g.I0, pn);
+    const double v = t.on ? av : 0.0;
+    pout[c] = pc;
     vout[c] = v;
-    if (i >= 1 && i <= g.I2 && j >= 1 && j <= g.J2) acc[0] += rh[c] * v;
+    if (interior(i, j, g)) acc[0] += rhc * v;
   }
-  if (!fast) return;
-  double out[1];
-  if (kry_finish(acc, part, &st->count, out)) fin_pv(st, out[0]);
+  if (fast) kry_partials(acc, part);
 }
 
 // S = R - ALPHA*V; T = A S; (T, T), (T, S).
@@ -205,27 +227,27 @@ __global__ __launch_bounds__(KT) void bicg_st_kernel(KryOp op, KryVecs q, KrySta
                                                      int fast) {
   if (st->stop) return;
   const double alpha = st->alpha;
-  const double *r = q.r, *v = q.v[(st->iter & 1) ^ 1];
-  double *s = q.s, *t = q.t;
+  const double *__restrict__ r = q.r;
+  const double *__restrict__ v = q.v[(st->iter & 1) ^ 1];
+  double *__restrict__ s = q.s;
+  double *__restrict__ t = q.t;
   const Cells g(op.i2, op.j2);
-  auto sn = [&](long e) { return r[e] - alpha * v[e]; };
+  auto sn = [&](int e) { return r[e] - alpha * v[e]; };
   double acc[2] = {0.0, 0.0};
-  for (long c = g.begin(); c < g.end(); c += KT) {
-    const int i = (int)(c % g.I0), j = (int)(c / g.I0);
+  for (Walk it(g); it.ok(); it.next()) {
+    const int c = it.c, i = it.i, j = it.j;
+    const Site z(i, j, g.I0, g.I2, g.J2, op.periodic);
     const double sv = sn(c);
+    const double asv = sqprod_at(op, z.cc, z.k, g.I0, sn);
+    const double tv = z.on ? asv : 0.0;
     s[c] = sv;
-    const long cc = stencil_site(i, j, g.I0, g.I2, g.J2, op.periodic);
-    double tv = 0.0;
-    if (cc >= 0) tv = sqprod_at(op, cc, coef_index(cc, g.I0, g.I2), g.I0, sn);
     t[c] = tv;
-    if (i >= 1 && i <= g.I2 && j >= 1 && j <= g.J2) {
+    if (interior(i, j, g)) {
       acc[0] += tv * tv;
       acc[1] += tv * sv;
     }
   }
-  if (!fast) return;
-  double out[2];
-  if (kry_finish(acc, part, &st->count, out)) fin_st(st, out[0], out[1]);
+  if (fast) kry_partials(acc, part);
 }
 
 // X = X + ALPHA*P + W*S; R = S - W*T; (R, R), (R, RH).
@@ -233,23 +255,27 @@ __global__ __launch_bounds__(KT) void bicg_xr_kernel(KryOp op, KryVecs q, KrySta
                                                      int fast) {
   if (st->stop) return;
   const double alpha = st->alpha, w = st->w;
-  const double *p = q.p[(st->iter & 1) ^ 1], *s = q.s, *t = q.t, *rh = q.rh;
-  double *x = q.x, *r = q.r;
+  const double *__restrict__ p = q.p[(st->iter & 1) ^ 1];
+  const double *__restrict__ s = q.s;
+  const double *__restrict__ t = q.t;
+  const double *__restrict__ rh = q.rh;
+  double *__restrict__ x = q.x;
+  double *__restrict__ r = q.r;
   const Cells g(op.i2, op.j2);
   double acc[2] = {0.0, 0.0};
-  for (long c = g.begin(); c < g.end(); c += KT) {
-    const int i = (int)(c % g.I0), j = (int)(c / g.I0);
-    x[c] = x[c] + alpha * p[c] + w * s[c];
-    const double rv = s[c] - w * t[c];
+  for (Walk it(g); it.ok(); it.next()) {
+    const int c = it.c, i = it.i, j = it.j;
+    const double xc = x[c], pc = p[c], sc = s[c], tc = t[c], rhc = rh[c];
+    const double xn = xc + alpha * pc + w * sc;
+    const double rv = sc - w * tc;
+    x[c] = xn;
     r[c] = rv;
-    if (i >= 1 && i <= g.I2 && j >= 1 && j <= g.J2) {
+    if (interior(i, j, g)) {
       acc[0] += rv * rv;
-      acc[1] += rv * rh[c];
+      acc[1] += rv * rhc;
     }
   }
-  if (!fast) return;
-  double out[2];
-  if (kry_finish(acc, part, &st->count, out)) fin_xr(st, out[0], out[1]);
+  if (fast) kry_partials(acc, part);
 }
 
 // p_inprod, sequentially: one thread, j outer, i inner.
@@ -261,28 +287,55 @@ __device__ inline double seq_dot(const double *a, const double *b, int i2, int j
   return tp;
 }
 
-// The reference-order counterparts of the finishing sums.  which: 0
-// restart, 1 PV, 2 ST, 3 XR, 4 P-CSI.
-__global__ void bicg_seq_kernel(int which, int i2, int j2, KryVecs q, KryState *st) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (which == 0) {
-    fin_restart(st, seq_dot(q.r, q.r, i2, j2));
-    return;
+// The sums a finishing kernel adds: the partials of the kernel before it in a
+// fixed order (fast), or p_inprod's sequential sum over the vectors
+// (reference order).  FT threads, each loading its (at most KRY_MAXB / FT)
+// partials at once and then adding them in order.
+constexpr int FT = 1024;
+constexpr int FU = KRY_MAXB / FT;
+__device__ inline double fin_sum(int d, int nb, const double *part, double *lds) {
+  double v[FU];
+#pragma unroll
+  for (int u = 0; u < FU; ++u) {
+    const int k = threadIdx.x + u * FT;
+    v[u] = k < nb ? part[(long)d * KRY_MAXB + k] : 0.0;
   }
-  if (st->stop) return;
-  const int nxt = (st->iter & 1) ^ 1;
-  if (which == 1) {
-    fin_pv(st, seq_dot(q.rh, q.v[nxt], i2, j2));
-  } else if (which == 2) {
-    const double tt = seq_dot(q.t, q.t, i2, j2);
-    const double ts = seq_dot(q.t, q.s, i2, j2);
-    fin_st(st, tt, ts);
-  } else if (which == 3) {
-    const double rr = seq_dot(q.r, q.r, i2, j2);
-    const double rrh = seq_dot(q.r, q.rh, i2, j2);
-    fin_xr(st, rr, rrh);
-  } else if (which == 4) {
-    fin_pcsi(st, seq_dot(q.r, q.r, i2, j2));
+  double a = 0.0;
+#pragma unroll
+  for (int u = 0; u < FU; ++u) a += v[u];
+  return kry_block_sum<FT>(a, lds);
+}
+
+// Finishes the inner products of the kernel before it and advances the
+// scalar recurrences.  which: 0 restart, 1 PV, 2 ST, 3 XR, 4 P-CSI.  One
+// workgroup of KT threads.
+__global__ __launch_bounds__(FT) void kry_fin_kernel(int which, int i2, int j2, KryVecs q,
+                                                     KryState *st, const double *part, int nb,
+                                                     int fast) {
+  __shared__ double lds[FT / 64];
+  if (which != 0 && st->stop) return;
+  double d0 = 0.0, d1 = 0.0;
+  const int nd = (which == 2 || which == 3) ? 2 : 1;
+  if (fast) {
+    d0 = fin_sum(0, nb, part, lds);
+    if (nd == 2) d1 = fin_sum(1, nb, part, lds);
+  } else if (threadIdx.x == 0) {
+    const int nxt = (st->iter & 1) ^ 1;
+    switch (which) {
+      case 0: d0 = seq_dot(q.r, q.r, i2, j2); break;
+      case 1: d0 = seq_dot(q.rh, q.v[nxt], i2, j2); break;
+      case 2: d0 = seq_dot(q.t, q.t, i2, j2); d1 = seq_dot(q.t, q.s, i2, j2); break;
+      case 3: d0 = seq_dot(q.r, q.r, i2, j2); d1 = seq_dot(q.r, q.rh, i2, j2); break;
+      default: d0 = seq_dot(q.r, q.r, i2, j2); break;
+    }
+  }
+  if (threadIdx.x != 0) return;
+  switch (which) {
+    case 0: fin_restart(st, d0); break;
+    case 1: fin_pv(st, d0); break;
+    case 2: fin_st(st, d0, d1); break;
+    case 3: fin_xr(st, d0, d1); break;
+    default: fin_pcsi(st, d0); break;
   }
 }
 
@@ -303,47 +356,44 @@ __global__ void kry_fill_kernel(double *a, long n, double v) {
 // ----------------------------------------------------------------- P-CSI
 // WORK1 = R with the periodic halo; Q = OMEGA*WORK1 + (CSY*OMEGA-1)*Q (or
 // Q = (1/CSY)*WORK1 on the first step); X = X + Q.
-__global__ __launch_bounds__(KT) void pcsi_qx_kernel(int i2, int j2, int periodic, const double *r,
-                                                     double *q, double *x, const double *omega,
-                                                     const double *cq, double c0, int first,
-                                                     const KryState *st) {
+__global__ __launch_bounds__(KT) void pcsi_qx_kernel(int i2, int j2, int periodic,
+                                                     const double *__restrict__ r,
+                                                     double *__restrict__ q, double *__restrict__ x,
+                                                     const double *omega, const double *cq, double c0,
+                                                     int first, const KryState *st) {
   if (!first && st->stop) return;
   const Cells g(i2, j2);
   const double om = first ? 0.0 : omega[st->iter];
   const double cf = first ? 0.0 : cq[st->iter];
-  for (long c = g.begin(); c < g.end(); c += KT) {
-    const int i = (int)(c % g.I0), j = (int)(c / g.I0);
-    long src = c;
-    if (periodic && (i == 0 || i == g.I0 - 1) && j >= 1 && j <= g.J2)
-      src = (i == 0 ? g.I2 : 1) + (long)j * g.I0;
-    const double w1 = r[src];
-    const double qv = first ? c0 * w1 : om * w1 + cf * q[c];
+  for (Walk it(g); it.ok(); it.next()) {
+    const int c = it.c, i = it.i, j = it.j;
+    int src = c;
+    if (periodic && (i == 0 || i == g.I0 - 1) && j >= 1 && j <= g.J2) src = (i == 0 ? g.I2 : 1) + j * g.I0;
+    const double w1 = r[src], qc = q[c], xc = x[c];
+    const double qv = first ? c0 * w1 : om * w1 + cf * qc;
     q[c] = qv;
-    x[c] = x[c] + qv;
+    x[c] = xc + qv;
   }
 }
 
 // R = B - A X (interior, periodic halo columns), optionally (R, R).
-__global__ __launch_bounds__(KT) void pcsi_r_kernel(KryOp op, const float *b, const double *x,
-                                                    double *r, KryState *st, double *part,
-                                                    int with_dot, int fast) {
+__global__ __launch_bounds__(KT) void pcsi_r_kernel(KryOp op, const float *__restrict__ b,
+                                                    const double *__restrict__ x,
+                                                    double *__restrict__ r, KryState *st,
+                                                    double *part, int with_dot, int fast) {
   if (with_dot && st->stop) return;
   const Cells g(op.i2, op.j2);
   double acc[1] = {0.0};
-  for (long c = g.begin(); c < g.end(); c += KT) {
-    const int i = (int)(c % g.I0), j = (int)(c / g.I0);
-    const long cc = stencil_site(i, j, g.I0, g.I2, g.J2, op.periodic);
-    double v = 0.0;
-    if (cc >= 0) {
-      const long k = coef_index(cc, g.I0, g.I2);
-      v = (double)b[k] - sqprod_at(op, cc, k, g.I0, [&](long e) { return x[e]; });
-    }
+  for (Walk it(g); it.ok(); it.next()) {
+    const int c = it.c, i = it.i, j = it.j;
+    const Site t(i, j, g.I0, g.I2, g.J2, op.periodic);
+    const double bk = (double)b[t.k];
+    const double ax = sqprod_at(op, t.cc, t.k, g.I0, [&](int e) { return x[e]; });
+    const double v = t.on ? bk - ax : 0.0;
     r[c] = v;
-    if (i >= 1 && i <= g.I2 && j >= 1 && j <= g.J2) acc[0] += v * v;
+    if (interior(i, j, g)) acc[0] += v * v;
   }
-  if (!with_dot || !fast) return;
-  double out[1];
-  if (kry_finish(acc, part, &st->count, out)) fin_pcsi(st, out[0]);
+  if (with_dot && fast) kry_partials(acc, part);
 }
 
 // --------------------------------------------------------------- Lanczos
@@ -353,12 +403,12 @@ __global__ __launch_bounds__(KT) void kry_spmv_kernel(KryOp op, const double *v,
                                                       int transposed) {
   const Cells g(op.i2, op.j2);
   const int I0 = g.I0, I2 = g.I2, J2 = g.J2;
-  for (long c = g.begin(); c < g.end(); c += KT) {
-    const int i = (int)(c % I0), j = (int)(c / I0);
+  for (Walk it(g); it.ok(); it.next()) {
+    const int c = it.c, i = it.i, j = it.j;
     if (i < 1 || i > I2 || j < 1 || j > J2) continue;
-    const long k = (i - 1) + (long)(j - 1) * I2;
+    const int k = (i - 1) + (j - 1) * I2;
     if (!transposed) {
-      s[c] = sqprod_at(op, c, k, I0, [&](long e) { return v[e]; });
+      s[c] = sqprod_at(op, c, k, I0, [&](int e) { return v[e]; });
     } else {
       // zero-padded TAT(I+1,J), TAR(I,J+1), TAL(I+2,J+1), TAB(I+1,J+2)
       const double tat = j >= 2 ? (double)op.at[k - I2] : 0.0;
@@ -387,22 +437,30 @@ __global__ void kry_scale_kernel(double *v, double sig, double *w, double bet, l
   w[k] = w[k] / bet;
 }
 
-// One inner product over the interior into *out (device).
-__global__ __launch_bounds__(KT) void kry_dot_kernel(int i2, int j2, const double *a, const double *b,
-                                                     double *out, double *part, unsigned *count,
-                                                     int fast) {
-  if (!fast) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *out = seq_dot(a, b, i2, j2);
-    return;
-  }
+// One inner product over the interior: partials (fast), then *out from
+// kry_dot_fin_kernel; in reference order the finishing kernel sums alone.
+__global__ __launch_bounds__(KT) void kry_dot_kernel(int i2, int j2, const double *__restrict__ a,
+                                                     const double *__restrict__ b, double *part) {
   const Cells g(i2, j2);
   double acc[1] = {0.0};
-  for (long c = g.begin(); c < g.end(); c += KT) {
-    const int i = (int)(c % g.I0), j = (int)(c / g.I0);
-    if (i >= 1 && i <= g.I2 && j >= 1 && j <= g.J2) acc[0] += a[c] * b[c];
+  for (Walk it(g); it.ok(); it.next()) {
+    const int c = it.c, i = it.i, j = it.j;
+    const double ab = a[c] * b[c];
+    if (interior(i, j, g)) acc[0] += ab;
   }
-  double o[1];
-  if (kry_finish(acc, part, count, o)) *out = o[0];
+  kry_partials(acc, part);
+}
+
+__global__ __launch_bounds__(FT) void kry_dot_fin_kernel(int i2, int j2, const double *a,
+                                                         const double *b, const double *part, int nb,
+                                                         int fast, double *out) {
+  __shared__ double lds[FT / 64];
+  if (fast) {
+    const double d = fin_sum(0, nb, part, lds);
+    if (threadIdx.x == 0) *out = d;
+  } else if (threadIdx.x == 0) {
+    *out = seq_dot(a, b, i2, j2);
+  }
 }
 
 // -------------------------------------------------------------- launchers
@@ -418,21 +476,22 @@ static dim3 kgrid(const KryOp &op) { return dim3((unsigned)kry_workgroups(op.i2,
 
 hipError_t kry_bicg_restart(const KryOp &op, const KryVecs &q, KryState *st, double *part, int fast,
                             hipStream_t s) {
-  hipLaunchKernelGGL(bicg_restart_kernel, kgrid(op), dim3(KT), 0, s, op, q.b, q.x, q.r, q.rh, st, part,
-                     fast);
-  if (!fast) hipLaunchKernelGGL(bicg_seq_kernel, dim3(1), dim3(64), 0, s, 0, op.i2, op.j2, q, st);
+  const dim3 g = kgrid(op);
+  hipLaunchKernelGGL(bicg_restart_kernel, g, dim3(KT), 0, s, op, q.b, q.x, q.r, q.rh, st, part, fast);
+  hipLaunchKernelGGL(kry_fin_kernel, dim3(1), dim3(FT), 0, s, 0, op.i2, op.j2, q, st, part, (int)g.x, fast);
   return hipGetLastError();
 }
 
 hipError_t kry_bicg_iter(const KryOp &op, const KryVecs &q, KryState *st, double *part, int fast,
                          hipStream_t s) {
-  const dim3 g = kgrid(op), blk(KT), one(1), w64(64);
+  const dim3 g = kgrid(op), blk(KT), one(1), fin(FT);
+  const int nb = (int)g.x;
   hipLaunchKernelGGL(bicg_pv_kernel, g, blk, 0, s, op, q, st, part, fast);
-  if (!fast) hipLaunchKernelGGL(bicg_seq_kernel, one, w64, 0, s, 1, op.i2, op.j2, q, st);
+  hipLaunchKernelGGL(kry_fin_kernel, one, fin, 0, s, 1, op.i2, op.j2, q, st, part, nb, fast);
   hipLaunchKernelGGL(bicg_st_kernel, g, blk, 0, s, op, q, st, part, fast);
-  if (!fast) hipLaunchKernelGGL(bicg_seq_kernel, one, w64, 0, s, 2, op.i2, op.j2, q, st);
+  hipLaunchKernelGGL(kry_fin_kernel, one, fin, 0, s, 2, op.i2, op.j2, q, st, part, nb, fast);
   hipLaunchKernelGGL(bicg_xr_kernel, g, blk, 0, s, op, q, st, part, fast);
-  if (!fast) hipLaunchKernelGGL(bicg_seq_kernel, one, w64, 0, s, 3, op.i2, op.j2, q, st);
+  hipLaunchKernelGGL(kry_fin_kernel, one, fin, 0, s, 3, op.i2, op.j2, q, st, part, nb, fast);
   return hipGetLastError();
 }
 
@@ -456,11 +515,13 @@ hipError_t kry_pcsi_qx(int i2, int j2, int periodic, const double *r, double *q,
 
 hipError_t kry_pcsi_r(const KryOp &op, const float *b, const double *x, double *r, KryState *st,
                       double *part, int with_dot, int fast, hipStream_t s) {
-  hipLaunchKernelGGL(pcsi_r_kernel, kgrid(op), dim3(KT), 0, s, op, b, x, r, st, part, with_dot, fast);
-  if (with_dot && !fast) {
+  const dim3 g = kgrid(op);
+  hipLaunchKernelGGL(pcsi_r_kernel, g, dim3(KT), 0, s, op, b, x, r, st, part, with_dot, fast);
+  if (with_dot) {
     KryVecs q{};
     q.r = r;
-    hipLaunchKernelGGL(bicg_seq_kernel, dim3(1), dim3(64), 0, s, 4, op.i2, op.j2, q, st);
+    hipLaunchKernelGGL(kry_fin_kernel, dim3(1), dim3(FT), 0, s, 4, op.i2, op.j2, q, st, part, (int)g.x,
+                       fast);
   }
   return hipGetLastError();
 }
@@ -483,10 +544,11 @@ hipError_t kry_scale(double *v, double sig, double *w, double bet, long n, hipSt
   return hipGetLastError();
 }
 
-hipError_t kry_dot(int i2, int j2, const double *a, const double *b, double *out, double *part,
-                   unsigned *count, int fast, hipStream_t s) {
-  hipLaunchKernelGGL(kry_dot_kernel, fast ? dim3((unsigned)kry_workgroups(i2, j2)) : dim3(1), dim3(KT),
-                     0, s, i2, j2, a, b, out, part, count, fast);
+hipError_t kry_dot(int i2, int j2, const double *a, const double *b, double *out, double *part, int fast,
+                   hipStream_t s) {
+  const int nb = kry_workgroups(i2, j2);
+  if (fast) hipLaunchKernelGGL(kry_dot_kernel, dim3((unsigned)nb), dim3(KT), 0, s, i2, j2, a, b, part);
+  hipLaunchKernelGGL(kry_dot_fin_kernel, dim3(1), dim3(FT), 0, s, i2, j2, a, b, part, nb, fast, out);
   return hipGetLastError();
 }
 
