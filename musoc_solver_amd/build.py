@@ -104,8 +104,8 @@ def build_fortran_driver(verbose: bool = False) -> str | None:
         return None
     tmp = DRIVER_EXE + ".tmp"
     rpath = os.path.relpath(LIBDIR, os.path.dirname(DRIVER_EXE))
-    cmd = [fc, "-O2", DRIVER_SRC, "-o", tmp, f"-L{LIBDIR}", "-lsevp_fortran", "-lsevp_hip",
-           f"-Wl,-rpath,$ORIGIN/{rpath}"]
+    cmd = [fc, "-O2", f"-I{LIBDIR}", DRIVER_SRC, "-o", tmp, f"-L{LIBDIR}", "-lsevp_fortran",
+           "-lsevp_hip", f"-Wl,-rpath,$ORIGIN/{rpath}"]
     if verbose:
         print(" ".join(cmd))
     _run(cmd, cwd=os.path.dirname(DRIVER_EXE))

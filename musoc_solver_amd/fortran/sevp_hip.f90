@@ -43,10 +43,29 @@ module sevp_hip
     type(c_ptr) :: u, v, w, p0, scr, s, x
   end type
 
+  ! Krylov cross-check solvers (sevp_krylov_opts / sevp_krylov_stats).
+  type, bind(c), public :: sevp_krylov_opts
+    integer(c_int) :: periodic, ref_order, max_iter, restart
+    real(c_double) :: tol, rtol
+  end type
+
+  type, bind(c), public :: sevp_krylov_stats
+    integer(c_int) :: iterations, converged
+    real(c_double) :: residual, seconds
+  end type
+
+  ! Settings of the drop-in P_BICGSTAB_LE, P_CSI and LANCZOS, whose argument
+  ! lists cannot carry them: the reference's compile-time FLAG_PERIODIC_WE,
+  ! and whether inner products are summed in the reference's sequential
+  ! order (bitwise the reference, slow) or in the fast fixed parallel order.
+  integer(c_int), public :: sevp_krylov_periodic = 1
+  integer(c_int), public :: sevp_krylov_ref_order = 0
+
   public :: c_sevp_repbir, c_sevp_rep1, c_sevp_rep, c_sevp_repbir_period, c_sevp_pre, &
             c_sevp_pre1, c_sevp_matinv, sevp_plan_create, sevp_plan_destroy, &
             sevp_build_influence, sevp_load_influence, sevp_get_influence, sevp_solve, &
-            sevp_reset_seam, sevp_step14_dev, sevp_evp_write, sevp_evp_read, sevp_check
+            sevp_reset_seam, sevp_step14_dev, sevp_evp_write, sevp_evp_read, sevp_check, &
+            sevp_krylov_defaults, c_sevp_p_bicgstab_le, c_sevp_p_csi, c_sevp_lanczos
 
   interface
     integer(c_int) function c_sevp_repbir(ax, ay, bb, cx, cy, rinv, rinv1, dum0, dum1, dum2, &
@@ -199,6 +218,43 @@ module sevp_hip
       integer(c_int), intent(out) :: had_oac
     end function
 
+    integer(c_int) function sevp_krylov_defaults(opts, solver) bind(c, name="sevp_krylov_defaults")
+      import :: c_int, sevp_krylov_opts
+      type(sevp_krylov_opts), intent(out) :: opts
+      integer(c_int), value :: solver
+    end function
+
+    integer(c_int) function c_sevp_p_bicgstab_le(al, ab, ac, ar, at, b, x, i2, j2, opts, stats) &
+        bind(c, name="sevp_p_bicgstab_le")
+      import :: c_int, c_float, c_double, sevp_krylov_opts, sevp_krylov_stats
+      real(c_float), intent(in) :: al(*), ab(*), ac(*), ar(*), at(*), b(*)
+      real(c_double), intent(inout) :: x(*)
+      integer(c_int), value :: i2, j2
+      type(sevp_krylov_opts), intent(in) :: opts
+      type(sevp_krylov_stats), intent(out) :: stats
+    end function
+
+    integer(c_int) function c_sevp_p_csi(al, ab, ac, ar, at, mineig, maxeig, b, x, i2, j2, opts, &
+                                         stats) bind(c, name="sevp_p_csi")
+      import :: c_int, c_float, c_double, sevp_krylov_opts, sevp_krylov_stats
+      real(c_float), intent(in) :: al(*), ab(*), ac(*), ar(*), at(*), b(*)
+      real(c_double), value :: mineig, maxeig
+      real(c_double), intent(inout) :: x(*)
+      integer(c_int), value :: i2, j2
+      type(sevp_krylov_opts), intent(in) :: opts
+      type(sevp_krylov_stats), intent(out) :: stats
+    end function
+
+    integer(c_int) function c_sevp_lanczos(al, ab, ac, ar, at, i2, j2, m, opts, mineig, maxeig, &
+                                           alpha, sigma) bind(c, name="sevp_lanczos")
+      import :: c_int, c_float, c_double, c_ptr, sevp_krylov_opts
+      real(c_float), intent(in) :: al(*), ab(*), ac(*), ar(*), at(*)
+      integer(c_int), value :: i2, j2, m
+      type(sevp_krylov_opts), intent(in) :: opts
+      real(c_double), intent(out) :: mineig, maxeig
+      type(c_ptr), value :: alpha, sigma
+    end function
+
     type(c_ptr) function sevp_last_error_c() bind(c, name="sevp_last_error")
       import :: c_ptr
     end function
@@ -301,4 +357,57 @@ subroutine matinv(b, n)
   integer :: n
   real(8) :: b(*)
   call sevp_check(c_sevp_matinv(b, n), 'MATINV')
+end subroutine
+
+! ---------------------------------------------------------------------------
+! The Krylov cross-check solvers with the reference's names and argument
+! lists (MAIN/template/GLOBAL/src/sevp.f90:443,553,618).  The preconditioner
+! arguments CL..CT are accepted and unused, as in the reference, whose
+! PRECONDITIONING is a copy.
+
+subroutine p_bicgstab_le(al, ab, ac, ar, at, b, x, cl, cb, cc, cr, ct, i2, j2)
+  use iso_c_binding
+  use sevp_hip
+  implicit none
+  integer, intent(in) :: i2, j2
+  real, intent(in) :: al(i2, j2), ab(i2, j2), ac(i2, j2), ar(i2, j2), at(i2, j2), b(i2, j2)
+  real, intent(in) :: cl(i2, j2), cb(i2, j2), cc(i2, j2), cr(i2, j2), ct(i2, j2)
+  real(8), intent(inout) :: x(i2 + 2, j2 + 2)
+  type(sevp_krylov_opts) :: o
+  type(sevp_krylov_stats) :: st
+  call sevp_check(sevp_krylov_defaults(o, 0_c_int), 'P_BICGSTAB_LE')
+  o%periodic = sevp_krylov_periodic
+  o%ref_order = sevp_krylov_ref_order
+  call sevp_check(c_sevp_p_bicgstab_le(al, ab, ac, ar, at, b, x, i2, j2, o, st), 'P_BICGSTAB_LE')
+end subroutine
+
+subroutine p_csi(al, ab, ac, ar, at, mineig, maxeig, b, x, cl, cb, cc, cr, ct, i2, j2)
+  use iso_c_binding
+  use sevp_hip
+  implicit none
+  integer, intent(in) :: i2, j2
+  real(8), intent(in) :: mineig, maxeig
+  real, intent(in) :: al(i2, j2), ab(i2, j2), ac(i2, j2), ar(i2, j2), at(i2, j2), b(i2, j2)
+  real, intent(in) :: cl(i2, j2), cb(i2, j2), cc(i2, j2), cr(i2, j2), ct(i2, j2)
+  real(8), intent(inout) :: x(i2 + 2, j2 + 2)
+  type(sevp_krylov_opts) :: o
+  type(sevp_krylov_stats) :: st
+  call sevp_check(sevp_krylov_defaults(o, 1_c_int), 'P_CSI')
+  o%periodic = sevp_krylov_periodic
+  o%ref_order = sevp_krylov_ref_order
+  call sevp_check(c_sevp_p_csi(al, ab, ac, ar, at, mineig, maxeig, b, x, i2, j2, o, st), 'P_CSI')
+end subroutine
+
+subroutine lanczos(al, ab, ac, ar, at, mineig, maxeig, i2, j2)
+  use iso_c_binding
+  use sevp_hip
+  implicit none
+  integer, intent(in) :: i2, j2
+  real, intent(in) :: al(i2, j2), ab(i2, j2), ac(i2, j2), ar(i2, j2), at(i2, j2)
+  real(8), intent(out) :: mineig, maxeig
+  type(sevp_krylov_opts) :: o
+  call sevp_check(sevp_krylov_defaults(o, 0_c_int), 'LANCZOS')
+  o%ref_order = sevp_krylov_ref_order
+  call sevp_check(c_sevp_lanczos(al, ab, ac, ar, at, i2, j2, 10_c_int, o, mineig, maxeig, &
+                                 c_null_ptr, c_null_ptr), 'LANCZOS')
 end subroutine

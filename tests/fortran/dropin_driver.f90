@@ -7,9 +7,12 @@
 !     PRE on AL(IL,1)... for each strip, the seam strip on gathered copies;
 !   - FS_SOLVER's sweep loop (MAIN/template/GLOBAL/src/fs_GLOBAL.f90:
 !     1904-1966): REPBIR on X(IL,1) for each strip, the seam strip on XX;
-!   - PRE1 and REP1 on the full width (FL_EVP_STP = 0).
-! Writes the influence matrices and both solutions for comparison.
+!   - PRE1 and REP1 on the full width (FL_EVP_STP = 0);
+!   - FS_SOLVER's cross-checks (fs_GLOBAL.f90:1853,2024-2026): P_BICGSTAB_LE,
+!     LANCZOS and P_CSI, with the inner products in the reference's order.
+! Writes the influence matrices and the solutions for comparison.
 program dropin_driver
+  use sevp_hip, only: sevp_krylov_ref_order
   implicit none
   integer :: i2, j2, nbir, nb0, nsweep, ibir, inc, i0, j0, j1, n, il, i, j, it, zp
   integer, allocatable :: ie(:)
@@ -18,6 +21,8 @@ program dropin_driver
   real(8), allocatable :: rinv(:,:,:), rinv1(:,:,:), x(:,:), xx(:,:), h(:,:), dum0(:,:)
   real(8), allocatable :: dum1(:), dum2(:)
   real(8), allocatable :: rf(:,:,:), rf1(:,:,:), xf(:,:), hf(:,:), df0(:,:), df1(:), df2(:)
+  real(8), allocatable :: xk(:,:), xc(:,:)
+  real(8) :: emin, emax
   character(len=1024) :: fin, fout
 
   call get_command_argument(1, fin)
@@ -100,8 +105,17 @@ program dropin_driver
   xf = 0.d0
   call rep1(al, ab, ac, ar, at, rf, rf1, df0, df1, df2, s, hf, xf, ie, i0, i2, nb0)
 
+  ! the Krylov cross-check solvers, summing in the reference's order
+  sevp_krylov_ref_order = 1
+  allocate (xk(i0, j0), xc(i0, j0))
+  xk = 0.d0
+  call p_bicgstab_le(al, ab, ac, ar, at, s, xk, al, ab, ac, ar, at, i2, j2)
+  call lanczos(al, ab, ac, ar, at, emin, emax, i2, j2)
+  xc = 0.d0
+  call p_csi(al, ab, ac, ar, at, emin, emax, s, xc, al, ab, ac, ar, at, i2, j2)
+
   open (11, file=trim(fout), access='stream', form='unformatted', status='replace')
-  write (11) rinv, rinv1, x, rf, rf1, xf
+  write (11) rinv, rinv1, x, rf, rf1, xf, xk, emin, emax, xc
   close (11)
   zp = 0
   print '(A)', 'dropin_driver done'

@@ -1,6 +1,7 @@
 """The Fortran drop-in layer: a Fortran program that calls PRE, REPBIR, PRE1
 and REP1 the way the reference's PREP and FS_SOLVER do (array-element
-offsets, seam gathers, periodic halo refresh) gets the oracle's results
+offsets, seam gathers, periodic halo refresh), and FS_SOLVER's Krylov
+cross-checks P_BICGSTAB_LE, LANCZOS and P_CSI, gets the oracle's results
 bitwise, through ISO_C_BINDING into the HIP library."""
 import os
 import subprocess
@@ -35,7 +36,8 @@ def test_fortran_dropin_driver_bitwise(oracle, device, tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     nb0, ibir = len(ie), 2 * i2 // nbir
     sizes = [ibir * ibir * nb0 * nbir, ibir * ibir * (nb0 - 1) * nbir, (i2 + 2) * (j2 + 2),
-             i2 * i2 * nb0, i2 * i2 * (nb0 - 1), (i2 + 2) * (j2 + 2)]
+             i2 * i2 * nb0, i2 * i2 * (nb0 - 1), (i2 + 2) * (j2 + 2), (i2 + 2) * (j2 + 2), 2,
+             (i2 + 2) * (j2 + 2)]
     out = np.fromfile(fout, np.float64)
     assert out.size == sum(sizes)
     parts = np.split(out, np.cumsum(sizes)[:-1])
@@ -45,6 +47,9 @@ def test_fortran_dropin_driver_bitwise(oracle, device, tmp_path):
     rf = parts[3].reshape((i2, i2, nb0), order="F")
     rf1 = parts[4].reshape((i2, i2, nb0 - 1), order="F")
     xf = parts[5].reshape((i2 + 2, j2 + 2), order="F")
+    xk = parts[6].reshape((i2 + 2, j2 + 2), order="F")
+    emin, emax = parts[7]
+    xc = parts[8].reshape((i2 + 2, j2 + 2), order="F")
     r_o, r1_o, _ = oracle.bir_prep(case.coefs, ie, nbir)
     assert fields_equal(rinv, r_o) and fields_equal(rinv1, r1_o[:, :nb0 - 1])
     xo = x0.copy(order="F")
@@ -56,3 +61,12 @@ def test_fortran_dropin_driver_bitwise(oracle, device, tmp_path):
     xfo = np.zeros((i2 + 2, j2 + 2), order="F")
     oracle.rep1(case.coefs, f_o, f1_o, s, xfo, ie)
     assert fields_equal(xf, xfo)
+    # FS_SOLVER's cross-checks: P_BICGSTAB_LE, LANCZOS, P_CSI (reference order)
+    xko = np.zeros((i2 + 2, j2 + 2), order="F")
+    oracle.bicgstab(case.coefs, s, xko, periodic=True)
+    assert fields_equal(xk, xko)
+    mn, mx, _, _, _ = oracle.lanczos(case.coefs, 10)
+    assert fields_equal(np.array([emin, emax]), np.array([mn, mx]))
+    xco = np.zeros((i2 + 2, j2 + 2), order="F")
+    oracle.pcsi(case.coefs, mn, mx, s, xco, periodic=True)
+    assert fields_equal(xc, xco)
