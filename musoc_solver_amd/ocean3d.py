@@ -24,7 +24,8 @@ import dataclasses
 
 import numpy as np
 
-from .synth import GRAVITY, make_grid, make_mask
+from .metgen import metgen_sphere
+from .synth import EARTH_RADIUS, GRAVITY, make_grid, make_mask
 
 F32 = np.float32
 
@@ -101,21 +102,34 @@ def assemble_operator_3d(odx, ody, ocs, csv, odyv, odz, iu, iv, i2, j2, nbir=1,
 
 def make_ocean3d(i2: int, j2: int, k1: int, seed: int, lat_max: float = 60.0,
                  land_frac: float = 0.3, periodic: bool = False, free_surface: bool = True,
-                 dz=None, dt: float = 1800.0, nbir: int = 1) -> Ocean3D:
+                 dz=None, dt: float = 1800.0, nbir: int = 1, metric: str = "latlon",
+                 dxmnut: float | None = None, y0deg: float | None = None) -> Ocean3D:
+    """A layered ocean.  ``metric``: "latlon" (uniform latitude spacing over
+    +-lat_max) or "metgen" (PREP's METGEN spherical grid with DYDX = 1,
+    square cells, from y0deg north, zonal spacing dxmnut minutes; its R0 in
+    metres here, so the metrics agree with GRAVITY and dz in SI units)."""
     i0, j0 = i2 + 2, j2 + 2
     i1, j1 = i0 - 1, j0 - 1
-    g = make_grid(i2, j2, "latlon", lat_max)
-    # metrics (halo rows copy their neighbours)
-    odx = np.empty(j0, F32)
-    ody = np.empty(j0, F32)
-    ocs = np.empty(j0, F32)
-    odx[1:j1] = 1.0 / g.dx
-    ody[1:j1] = 1.0 / g.dy
-    ocs[1:j1] = 1.0 / np.cos(g.lat_c)
-    for a in (odx, ody, ocs):
-        a[0], a[j0 - 1] = a[1], a[j1 - 1]
-    csv = np.cos(g.lat_v).astype(F32)               # (J1,): face J between X rows J and J+1
-    odyv = (1.0 / g.dyv).astype(F32)
+    if metric == "metgen":
+        mg = metgen_sphere(i2, j2, dxmnut if dxmnut is not None else 21600.0 / i2,
+                           y0deg if y0deg is not None else -lat_max, r0=EARTH_RADIUS)
+        odx, ody, ocs = mg["odx"].copy(), mg["ody"].copy(), mg["ocs"].copy()
+        csv, odyv = mg["csv"].copy(), mg["odyv"].copy()
+    elif metric == "latlon":
+        g = make_grid(i2, j2, "latlon", lat_max)
+        # metrics (halo rows copy their neighbours)
+        odx = np.empty(j0, F32)
+        ody = np.empty(j0, F32)
+        ocs = np.empty(j0, F32)
+        odx[1:j1] = 1.0 / g.dx
+        ody[1:j1] = 1.0 / g.dy
+        ocs[1:j1] = 1.0 / np.cos(g.lat_c)
+        for a in (odx, ody, ocs):
+            a[0], a[j0 - 1] = a[1], a[j1 - 1]
+        csv = np.cos(g.lat_v).astype(F32)           # (J1,): face J between X rows J and J+1
+        odyv = (1.0 / g.dyv).astype(F32)
+    else:
+        raise ValueError(f"unknown metric {metric!r}")
     if dz is None:
         dz = [50.0, 150.0, 300.0, 500.0, 1000.0, 1500.0, 2000.0][:k1]
     dz = np.asarray(dz, np.float64)

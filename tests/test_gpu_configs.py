@@ -139,3 +139,40 @@ def test_regional_basins(oracle, device, i2, j2, seed, land):
     for r in (0, nrhs - 1):
         assert oracle.rel_residual(case.coefs, s[:, :, r], np.asfortranarray(x[:, :, r])) <= 1.5e-10
     plan.close()
+
+
+def test_metgen_global_operator(oracle, device):
+    """A global operator on METGEN's square-cell grid (1 degree, +-85 deg,
+    layered ocean, periodic), block rows from the autotuner: FP64 bitwise
+    against the oracle's PRE1 + REP1, and the mixed-precision refinement to
+    1e-10."""
+    import torch
+    from musoc_solver_amd import metgen, ocean3d
+    i2 = 360
+    j2 = metgen.rows_to_latitude(60.0, -85.0, 85.0)
+    print(f"[metgen] {i2}x{j2}", flush=True)
+    oc = ocean3d.make_ocean3d(i2, j2, 5, 8100, metric="metgen", dxmnut=60.0, y0deg=-85.0,
+                              periodic=True, nbir=2, land_frac=0.3)
+    ie = autotune.adaptive_ie(oc.coefs, max_growth=1e3, hmax=8, hmin=2, periodic=True)
+    nrhs = 4
+    mask = np.ones((i2, j2), bool)
+    s = synth.make_rhs(mask, 8101, nrhs)
+    plan = S.SevpPlan(oc.coefs, ie, form=S.FORM_FULL_PERIODIC, nrhs_max=nrhs)
+    plan.build_influence()
+    rinv, rinv1 = plan.get_influence()
+    r_o, r1_o, _ = oracle.pre1(oc.coefs, ie)
+    assert fields_equal(rinv, r_o)
+    assert fields_equal(rinv1[:, :, :len(ie) - 1], r1_o[:, :, :len(ie) - 1])
+    xg = np.zeros((i2 + 2, j2 + 2, nrhs), order="F")
+    plan.solve(s, xg)
+    for r in (0, nrhs - 1):
+        xo = np.zeros((i2 + 2, j2 + 2), order="F")
+        oracle.rep1(oc.coefs, r_o, r1_o, s[:, :, r], xo, ie)
+        assert fields_equal(xg[:, :, r], xo), f"rhs {r}"
+    plan.set_precision(S.PREC_FP32)
+    s_d = torch.from_numpy(np.ascontiguousarray(s.transpose(2, 1, 0))).cuda()
+    x_d = torch.zeros((nrhs, j2 + 2, i2 + 2), dtype=torch.float64, device="cuda")
+    rel, st, ok = plan.solve_refined_dev(s_d, x_d, tol=1e-10, max_iter=10)
+    torch.cuda.synchronize()
+    assert ok and rel.max() <= 1e-10, (rel.max(), st.iterations)
+    plan.close()
