@@ -20,6 +20,8 @@ Legs
              FP64  -> "value"
   plain      the reference operation order in FP64 (bitwise equal to the
              reference REP1), refined the same way
+  fast64     correction products on the FP64 matrix cores (FP64 arithmetic in
+             a blocked summation order), refined the same way
   cpu        the CPU oracle (C restatement, one core) on a bounded sample of
              the same workload, refined to the same tolerance
 
@@ -64,9 +66,9 @@ def parse():
     ap.add_argument("--cpu-rhs", type=int, default=16, help="RHS in the CPU baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=16, help="host threads of the CPU baseline")
     ap.add_argument("--skip-plain", action="store_true")
-    ap.add_argument("--legs", default="mixed,plain", help="comma list of GPU legs to time")
-    ap.add_argument("--rhs-groups", default="2,2",
-                    help="RHS groups whose passes run concurrently, mixed,plain legs")
+    ap.add_argument("--legs", default="mixed,plain,fast64", help="comma list of GPU legs to time")
+    ap.add_argument("--rhs-groups", default="2,2,2",
+                    help="RHS groups whose passes run concurrently, mixed,plain,fast64 legs")
     ap.add_argument("--kernel-timing", choices=("after", "timed"), default="after",
                     help="time the correction kernel with HIP events in one extra solve after "
                          "the timed steps, launched eagerly with events from the kernel's own "
@@ -118,7 +120,7 @@ def main():
     s_np = synth.make_rhs(case.mask, a.seed + 1, nrhs)
     s = torch.from_numpy(np.ascontiguousarray(s_np.transpose(2, 1, 0))).to(dev)
     x = torch.zeros((nrhs, j2 + 2, i2 + 2), dtype=torch.float64, device=dev)
-    n0_of = {"plain": a.n0, "mixed": a.n0_mixed or a.n0}
+    n0_of = {"plain": a.n0, "mixed": a.n0_mixed or a.n0, "fast64": a.n0}
 
     def make_plan(n0, prec):
         ie = synth.reference_ie(j2, synth.nb0_for_height(j2, n0))
@@ -132,7 +134,8 @@ def main():
         return plan, ie, dt
 
     def plan_groups(name):
-        return dict(zip(("mixed", "plain"), (int(v) for v in a.rhs_groups.split(",")))).get(name, 1)
+        return dict(zip(("mixed", "plain", "fast64"),
+                        (int(v) for v in a.rhs_groups.split(",")))).get(name, 1)
 
     def run_leg(plan, name):
         for _ in range(a.warmup):
@@ -192,21 +195,26 @@ def main():
         return
     if a.skip_plain and "plain" in order:
         order.remove("plain")
-    plan_keep = None
+    # one plan per block height; the legs switch its correction arithmetic
+    prec_of = {"mixed": S.PREC_FP32, "plain": S.PREC_FP64, "fast64": S.PREC_FP64_MFMA}
+    plans = {}
     for name in order:
-        prec = {"mixed": S.PREC_FP32, "plain": S.PREC_FP64}[name]
-        plan, ie, dt = make_plan(n0_of[name], prec)
+        n0 = n0_of[name]
+        if n0 not in plans:
+            plan, ie, dt = make_plan(n0, S.PREC_FP64)
+            plans[n0] = (plan, ie)
+            setup[name] = dt
+        plan = plans[n0][0]
+        t0 = time.perf_counter()
+        plan.set_precision(prec_of[name])
+        torch.cuda.synchronize()
+        setup[name] = setup.get(name, 0.0) + time.perf_counter() - t0
         plan.set_rhs_groups(plan_groups(name))
-        setup[name] = dt
         legs[name] = run_leg(plan, name)
         xs[name] = x.clone()
-        if name == "plain" or plan_keep is None:
-            if plan_keep is not None:
-                plan_keep[0].close()
-            plan_keep = (plan, ie)
-        else:
-            plan.close()
-        torch.cuda.empty_cache()
+    plan_keep = plans[n0_of[order[-1]]] if order else None
+    if plan_keep is not None and "plain" in order:
+        plan_keep = plans[n0_of["plain"]]
     t_setup = setup
 
     # error of the mixed solution against the plain FP64 solution (the plain
@@ -218,6 +226,13 @@ def main():
         n = xp[:, 1:-1, 1:-1].reshape(nrhs, -1).norm(dim=1)
         err = float((d / n).max())
         log(f"mixed vs plain FP64: max rel error {err:.3e}")
+    err64 = None
+    if "fast64" in legs and "plain" in legs:
+        xf, xp = xs["fast64"], xs["plain"]
+        d = (xf - xp)[:, 1:-1, 1:-1].reshape(nrhs, -1).norm(dim=1)
+        n = xp[:, 1:-1, 1:-1].reshape(nrhs, -1).norm(dim=1)
+        err64 = float((d / n).max())
+        log(f"fast64 vs plain FP64: max rel error {err64:.3e}")
 
     main_leg = legs.get("mixed") or legs.get("plain")
     main_name = "mixed" if "mixed" in legs else "plain"
@@ -248,14 +263,20 @@ def main():
         "legs": legs,
         "mixed_over_plain": (legs["mixed"]["solves_per_s"] / legs["plain"]["solves_per_s"]
                              if "mixed" in legs and "plain" in legs else None),
+        "fast64_over_plain": (legs["fast64"]["solves_per_s"] / legs["plain"]["solves_per_s"]
+                              if "fast64" in legs and "plain" in legs else None),
         "mixed_vs_fp64_max_rel_err": err,
+        "fast64_vs_fp64_max_rel_err": err64,
+        "roofline_fast64": (roofline(legs["fast64"], "fast64", nrhs, i2, a)
+                            if "fast64" in legs else None),
         "setup_s": t_setup,
         "roofline": roof,
         "cpu_baseline": cpu,
         "krylov": krylov,
     }
     print(json.dumps(out), flush=True)
-    plan_keep[0].close()
+    for plan, _ in plans.values():
+        plan.close()
 
 
 def roofline(leg, name, nrhs, i2, a):
@@ -287,7 +308,8 @@ def roofline(leg, name, nrhs, i2, a):
         r = {"bound": "mfma", "achieved": flops / t / 1e12, "peak": peak_f / 1e12,
              "unit": "TFLOP/s", "frac": flops / t / peak_f}
     r.update({"traffic": traffic,
-              "kernel": "gemm_part_kernel" if name == "mixed" else "gemm_seq_kernel",
+              "kernel": {"mixed": "gemm_part_kernel", "fast64": "gemm_part64_kernel"}.get(
+                  name, "gemm_seq_kernel"),
               "algorithmic_bytes_per_launch": nbytes, "algorithmic_flops_per_launch": flops,
               "avg_launch_us": leg["corr_avg_us"], "rhs_per_product": m})
     return r

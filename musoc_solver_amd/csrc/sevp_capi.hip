@@ -246,10 +246,13 @@ struct sevp_plan_s {
   DBuf<double> rinv, rinv1;      // reference layout
   DBuf<float> rinv32, rinv1_32;  // mixed precision copies
   DBuf<float> w32;               // mixed: W_b = RINV_b diag(CY(:,IE(b-1)-1)) RINV1_(b-1), b >= 2
+  DBuf<double> rinv64t, rinv1_64t, w64t;  // FP64 MFMA: the same three in FP64 lane tiles
+  bool has32 = false, has64 = false;      // the copies above match the influence matrices
   // solve scratch
   DBuf<double> e, dum0, hfree, hcpl, g;  // full forms
   DBuf<float> e32;                       // mixed: FP32 copy of the error rows
   DBuf<float> kpart;                     // mixed: split-K partial products (S, nrhs, N)
+  DBuf<double> kpart64;                  // FP64 MFMA: split-K partial products
   DBuf<double> xx, bdum0;                // BIR
   DBuf<double> r64, dx, part;            // refinement
   DBuf<double> b1s;                      // MATINV scratch (blocked path)
@@ -340,9 +343,11 @@ struct FullCtx {
   const double *rinv, *rinv1;
   const float *rinv32, *rinv1_32;  // non-null selects the FP32 MFMA corrections
   const float *w32;                // mixed: coupling operators W_b (I2 x I2, b = 2..NB0)
+  const double *rinv64t, *rinv1_64t, *w64t;  // FP64 MFMA: the same in FP64 lane tiles
   double *e, *dum0, *hfree, *hcpl, *g;
   float *e32;        // FP32 copy of e (mixed precision), else null
   float *kpart;      // mixed: split-K partial products
+  double *kpart64;   // FP64 MFMA: split-K partial products (selects that arithmetic)
   KProf *prof;
 };
 
@@ -356,18 +361,38 @@ struct FullCtx {
 // that row in the launch.
 static int run_mixed_pass(const FullCtx &c, const float *f32, const double *f64, long f_stride,
                           double *x, long x_stride, int nrhs, hipStream_t st) {
-  const long ww = lane_tiles_floats(c.i2, c.i2);  // one FP32 matrix in lane tiles
+  // FP64 MFMA products (kpart64) take the FP64 error rows e and FP64 lane
+  // tiles; FP32 ones the FP32 rows e32 and FP32 tiles
+  const bool dbl = c.kpart64 != nullptr;
+  const long ww = lane_tiles_floats(c.i2, c.i2);  // one matrix in lane tiles (elements)
   const int np = (c.i2 + 15) / 16 * 16;           // W's columns start at np in [RINV | W]
   int ks = 0;
   const int ns1 = gemm_part_slices(c.i2, c.i2, &ks);
   const int ns2 = gemm_part_slices(2 * np, c.i2, &ks);
   auto part_of = [&](int n_prod) {
     PartRef r{};
-    r.p = c.kpart;
+    r.p = dbl ? nullptr : c.kpart;
+    r.pd = dbl ? c.kpart64 : nullptr;
     r.ld = n_prod;
     r.sstride = (long)nrhs * n_prod;
     r.ns = (n_prod == 2 * np) ? ns2 : ns1;
     return r;
+  };
+  // block nb's product: forward (RINV1_nb, N = I2) or backward ([RINV_nb | W_nb])
+  auto product64 = [&](bool fwd, int nb, int n_prod) -> int {
+    GemmPart64Args g{};
+    g.a = c.e;
+    g.b = fwd ? c.rinv1_64t + (long)(nb - 1) * ww : c.rinv64t + (long)(nb - 1) * ww;
+    g.b2 = (!fwd && nb != 1) ? c.w64t + (long)(nb - 2) * ww : nullptr;
+    g.nsplit = np;
+    g.M = nrhs; g.N = n_prod; g.K = c.i2;
+    g.part = c.kpart64;
+    g.tspan = c.prof ? c.prof->next_span(SEVP_KCLASS_CORR, gemm_part64_workgroups(g)) : nullptr;
+    const auto ev = c.prof ? c.prof->start_ext(SEVP_KCLASS_CORR)
+                           : std::pair<hipEvent_t, hipEvent_t>{nullptr, nullptr};
+    HIPC(launch_gemm_part64(g, st, ev.first, ev.second));
+    if (c.prof) c.prof->stop_ext(SEVP_KCLASS_CORR, ev.first != nullptr);
+    return SEVP_OK;
   };
   auto product = [&](const float *b, const float *b2, int n_prod) -> int {
     GemmPartArgs g{};
@@ -392,8 +417,8 @@ static int run_mixed_pass(const FullCtx &c, const float *f32, const double *f64,
     m.cld = c.i2;
     m.f = f32; m.f64 = f64; m.f_stride = f_stride;
     m.x = x; m.x_stride = x_stride;
-    m.e = nullptr;
-    m.e32 = c.e32;
+    m.e = dbl ? c.e : nullptr;
+    m.e32 = dbl ? nullptr : c.e32;
     m.i2 = c.i2; m.j2 = c.j2; m.periodic = c.periodic;
     m.js = js; m.jf = ieb - 2; m.ieb = ieb;
     m.resid = (nb != c.nb0);
@@ -405,7 +430,10 @@ static int run_mixed_pass(const FullCtx &c, const float *f32, const double *f64,
     if (c.prof) c.prof->start(SEVP_KCLASS_MARCH);
     HIPC(launch_march(MODE_PART, m, nrhs, st));
     if (c.prof) c.prof->stop(SEVP_KCLASS_MARCH);
-    if (nb != c.nb0) CHK(product(c.rinv1_32 + (long)(nb - 1) * ww, nullptr, c.i2));
+    if (nb != c.nb0) {
+      if (dbl) CHK(product64(true, nb, c.i2));
+      else CHK(product(c.rinv1_32 + (long)(nb - 1) * ww, nullptr, c.i2));
+    }
     js = ieb;
   }
   auto resid_args = [&](int nb) {
@@ -413,7 +441,8 @@ static int run_mixed_pass(const FullCtx &c, const float *f32, const double *f64,
     r.ax = c.ax; r.ay = c.ay; r.bb = c.bb; r.cx = c.cx; r.cy = c.cy;
     r.f = f32; r.f64 = f64; r.f_stride = f_stride;
     r.x = x; r.x_stride = x_stride;
-    r.e32 = c.e32;
+    r.e = dbl ? c.e : nullptr;
+    r.e32 = dbl ? nullptr : c.e32;
     r.i2 = c.i2; r.j2 = c.j2; r.periodic = c.periodic; r.nb = nb; r.ieb = c.ie[nb - 1];
     r.nrhs = nrhs;
     return r;
@@ -429,8 +458,9 @@ static int run_mixed_pass(const FullCtx &c, const float *f32, const double *f64,
     js = (nb != 1) ? c.ie[nb - 2] : 1;
     // [RINV_b | W_b]: free-row correction and, for b > 1, the coupling row
     const int n_prod = (nb != 1) ? 2 * np : c.i2;
-    CHK(product(c.rinv32 + (long)(nb - 1) * ww, nb != 1 ? c.w32 + (long)(nb - 2) * ww : nullptr,
-                n_prod));
+    if (dbl) CHK(product64(false, nb, n_prod));
+    else CHK(product(c.rinv32 + (long)(nb - 1) * ww, nb != 1 ? c.w32 + (long)(nb - 2) * ww : nullptr,
+                     n_prod));
     MarchArgs m{};
     m.ax = c.ax; m.ay = c.ay; m.bb = c.bb; m.cx = c.cx; m.cy = c.cy;
     m.cld = c.i2;
@@ -456,7 +486,7 @@ static int run_mixed_pass(const FullCtx &c, const float *f32, const double *f64,
 
 static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, long f_stride,
                          double *x, long x_stride, int nrhs, hipStream_t st) {
-  if (c.kpart) return run_mixed_pass(c, f32, f64, f_stride, x, x_stride, nrhs, st);
+  if (c.kpart || c.kpart64) return run_mixed_pass(c, f32, f64, f_stride, x, x_stride, nrhs, st);
   const long ww = (long)c.i2 * c.i2;
   const int i0 = c.i2 + 2, j0 = c.j2 + 2;
   (void)j0;
@@ -678,7 +708,7 @@ int sevp_plan_create(const sevp_desc *desc, const float *al, const float *ab, co
   if (d.nrhs_max < 1) return fail(SEVP_E_ARG, "nrhs_max must be >= 1");
   if (d.form != SEVP_FORM_BIR && d.form != SEVP_FORM_FULL_PERIODIC && d.form != SEVP_FORM_FULL)
     return fail(SEVP_E_ARG, "unknown form");
-  if (d.precision < SEVP_PREC_FP64 || d.precision > SEVP_PREC_BF16)
+  if (d.precision < SEVP_PREC_FP64 || d.precision > SEVP_PREC_FP64_MFMA)
     return fail(SEVP_E_ARG, "unknown precision");
   CHK(check_ie(d.ie, d.nb0, d.j2));
   auto *p = new sevp_plan_s();
@@ -798,16 +828,27 @@ size_t sevp_plan_device_bytes(sevp_plan p) {
   return p->coef.bytes() + p->ie_d.bytes() + p->rinv.bytes() + p->rinv1.bytes() +
          p->rinv32.bytes() + p->rinv1_32.bytes() + p->w32.bytes() + p->e.bytes() + p->dum0.bytes() +
          p->hfree.bytes() + p->hcpl.bytes() + p->g.bytes() + p->e32.bytes() +
-         p->kpart.bytes() + p->xx.bytes() +
+         p->kpart.bytes() + p->xx.bytes() + p->rinv64t.bytes() + p->rinv1_64t.bytes() +
+         p->w64t.bytes() + p->kpart64.bytes() +
          p->bdum0.bytes() + p->s14.bytes() + p->r64.bytes() + p->dx.bytes() + p->part.bytes() + p->b1s.bytes() +
          p->zp.bytes();
 }
 
+// The operand copies of the current precision's products: FP32 lane tiles
+// (SEVP_PREC_FP32) or FP64 lane tiles (SEVP_PREC_FP64_MFMA) of RINV, RINV1
+// and the coupling operators W.
 static int make_low_precision_copies(sevp_plan p) {
   if (p->d.precision == SEVP_PREC_FP64 || p->d.form == SEVP_FORM_BIR) return SEVP_OK;
+  const bool dbl = p->d.precision == SEVP_PREC_FP64_MFMA;
+  if (dbl ? p->has64 : p->has32) return SEVP_OK;
   const int nb1 = p->d.nb0 > 1 ? p->d.nb0 - 1 : 1;
-  HIPC(launch_cvt_lane_tiles(p->rinv.p, p->rinv32.p, p->d.i2, p->d.i2, p->d.nb0, p->stream));
-  HIPC(launch_cvt_lane_tiles(p->rinv1.p, p->rinv1_32.p, p->d.i2, p->d.i2, nb1, p->stream));
+  if (dbl) {
+    HIPC(launch_cvt_lane_tiles64(p->rinv.p, p->rinv64t.p, p->d.i2, p->d.i2, p->d.nb0, p->stream));
+    HIPC(launch_cvt_lane_tiles64(p->rinv1.p, p->rinv1_64t.p, p->d.i2, p->d.i2, nb1, p->stream));
+  } else {
+    HIPC(launch_cvt_lane_tiles(p->rinv.p, p->rinv32.p, p->d.i2, p->d.i2, p->d.nb0, p->stream));
+    HIPC(launch_cvt_lane_tiles(p->rinv1.p, p->rinv1_32.p, p->d.i2, p->d.i2, nb1, p->stream));
+  }
   const long tw = lane_tiles_floats(p->d.i2, p->d.i2);
   // W_b = RINV_b diag(CY(:, IE(b-1)-1)) RINV1_(b-1): the coupling row of the
   // backward pass as one product with the error row (FP64, then FP32)
@@ -827,10 +868,12 @@ static int make_low_precision_copies(sevp_plan p) {
       gm.ep = EP_COLMAJ;
       gm.out = w.p; gm.out_ld = i2;
       HIPC(launch_gemm(gm, p->stream));
-      HIPC(launch_cvt_lane_tiles(w.p, p->w32.p + (long)(nb - 2) * tw, i2, i2, 1, p->stream));
+      if (dbl) HIPC(launch_cvt_lane_tiles64(w.p, p->w64t.p + (long)(nb - 2) * tw, i2, i2, 1, p->stream));
+      else HIPC(launch_cvt_lane_tiles(w.p, p->w32.p + (long)(nb - 2) * tw, i2, i2, 1, p->stream));
     }
     HIPC(hipStreamSynchronize(p->stream));
   }
+  (dbl ? p->has64 : p->has32) = true;
   return SEVP_OK;
 }
 
@@ -857,6 +900,7 @@ int sevp_build_influence(sevp_plan p, int *zero_pivots) {
     c.map = ColMap{1, 0, 0};
   }
   CHK(run_pre(c, p->stream));
+  p->has32 = p->has64 = false;
   CHK(make_low_precision_copies(p));
   int z = 0;
   HIPC(hipMemcpyAsync(&z, p->zp.p, sizeof(int), hipMemcpyDeviceToHost, p->stream));
@@ -870,6 +914,7 @@ int sevp_load_influence(sevp_plan p, const double *rinv, const double *rinv1) {
   if (!p || !rinv || !rinv1) return fail(SEVP_E_ARG, "null argument");
   HIPC(hipMemcpy(p->rinv.p, rinv, p->rinv.bytes(), hipMemcpyHostToDevice));
   if (p->d.nb0 > 1) HIPC(hipMemcpy(p->rinv1.p, rinv1, p->rinv1.bytes(), hipMemcpyHostToDevice));
+  p->has32 = p->has64 = false;
   CHK(make_low_precision_copies(p));
   HIPC(hipStreamSynchronize(p->stream));
   p->have_inv = true;
@@ -891,7 +936,9 @@ int sevp_reset_seam(sevp_plan p) {
   return SEVP_OK;
 }
 
-static FullCtx full_ctx(sevp_plan p, bool mixed) {
+static FullCtx full_ctx(sevp_plan p) {
+  const bool mixed = p->d.precision == SEVP_PREC_FP32;
+  const bool mfma64 = p->d.precision == SEVP_PREC_FP64_MFMA;
   FullCtx c{};
   c.ax = p->al(); c.ay = p->ab(); c.bb = p->ac(); c.cx = p->ar(); c.cy = p->at();
   c.i2 = p->d.i2; c.j2 = p->d.j2;
@@ -907,6 +954,12 @@ static FullCtx full_ctx(sevp_plan p, bool mixed) {
   c.e = p->e.p; c.dum0 = p->dum0.p; c.hfree = p->hfree.p; c.hcpl = p->hcpl.p; c.g = p->g.p;
   c.e32 = mixed ? p->e32.p : nullptr;
   c.kpart = mixed ? p->kpart.p : nullptr;
+  if (mfma64) {
+    c.rinv64t = p->rinv64t.p;
+    c.rinv1_64t = p->rinv1_64t.p;
+    c.w64t = p->w64t.p;
+    c.kpart64 = p->kpart64.p;
+  }
   c.prof = &p->prof;
   return c;
 }
@@ -923,24 +976,34 @@ static FullCtx group_ctx(const FullCtx &c0, int r0) {
   c.g += r0 * i2;
   c.dum0 += r0 * dstride;
   if (c.e32) c.e32 += r0 * i2;
-  if (c.kpart) c.kpart += r0 * (long)gemm_part_slices((int)i2, (int)i2, nullptr) * 2 * ((i2 + 15) / 16 * 16);
+  const long part_rhs = (long)gemm_part_slices((int)i2, (int)i2, nullptr) * 2 * ((i2 + 15) / 16 * 16);
+  if (c.kpart) c.kpart += r0 * part_rhs;
+  if (c.kpart64) c.kpart64 += r0 * part_rhs;
   return c;
 }
 
+// Buffers of the split-K products of the plan's precision (FP32 or FP64
+// MFMA), allocated on first use.
 static int alloc_mixed(sevp_plan p) {
-  if (p->e32.p) return SEVP_OK;
+  const bool dbl = p->d.precision == SEVP_PREC_FP64_MFMA;
+  if (dbl ? p->kpart64.p != nullptr : p->e32.p != nullptr) return SEVP_OK;
   const int nr = p->d.nrhs_max;
-  if (nr > 64) return fail(SEVP_E_ARG, "mixed precision solves at most 64 RHS per call");
-  if (p->d.i2 % 4 != 0) return fail(SEVP_E_ARG, "mixed precision needs I2 divisible by 4");
-  CHK(p->e32.alloc((size_t)nr * p->d.i2));
-  {
-    const int i2 = p->d.i2, np = (i2 + 15) / 16 * 16;
-    const size_t n1 = (size_t)gemm_part_slices(i2, i2, nullptr) * i2;
-    const size_t n2 = (size_t)gemm_part_slices(2 * np, i2, nullptr) * 2 * np;
-    CHK(p->kpart.alloc(std::max(n1, n2) * nr));
-  }
+  if (nr > 64) return fail(SEVP_E_ARG, "MFMA corrections solve at most 64 RHS per call");
+  if (p->d.i2 % 4 != 0) return fail(SEVP_E_ARG, "MFMA corrections need I2 divisible by 4");
+  const int i2 = p->d.i2, np = (i2 + 15) / 16 * 16;
+  const size_t n1 = (size_t)gemm_part_slices(i2, i2, nullptr) * i2;
+  const size_t n2 = (size_t)gemm_part_slices(2 * np, i2, nullptr) * 2 * np;
   const size_t tw = (size_t)lane_tiles_floats(p->d.i2, p->d.i2);
   const size_t nb1 = p->d.nb0 > 1 ? p->d.nb0 - 1 : 1;
+  if (dbl) {
+    CHK(p->kpart64.alloc(std::max(n1, n2) * nr));
+    CHK(p->rinv64t.alloc(tw * p->d.nb0));
+    CHK(p->rinv1_64t.alloc(tw * nb1));
+    CHK(p->w64t.alloc(tw * nb1));
+    return SEVP_OK;
+  }
+  CHK(p->e32.alloc((size_t)nr * p->d.i2));
+  CHK(p->kpart.alloc(std::max(n1, n2) * nr));
   CHK(p->rinv32.alloc(tw * p->d.nb0));
   CHK(p->rinv1_32.alloc(tw * nb1));
   CHK(p->w32.alloc(tw * nb1));
@@ -949,15 +1012,18 @@ static int alloc_mixed(sevp_plan p) {
 
 int sevp_set_precision(sevp_plan p, int precision) {
   if (!p) return fail(SEVP_E_ARG, "null plan");
-  if (precision < SEVP_PREC_FP64 || precision > SEVP_PREC_BF16)
+  if (precision < SEVP_PREC_FP64 || precision > SEVP_PREC_FP64_MFMA)
     return fail(SEVP_E_ARG, "unknown precision");
   if (p->d.form == SEVP_FORM_BIR && precision != SEVP_PREC_FP64)
     return fail(SEVP_E_ARG, "BIR form runs the plain FP64 arithmetic only");
   if (precision == SEVP_PREC_BF16) return fail(SEVP_E_ARG, "BF16 corrections are not built");
   const int old = p->d.precision;
   p->d.precision = precision;
-  if (precision != SEVP_PREC_FP64 && old == SEVP_PREC_FP64) {
-    CHK(alloc_mixed(p));
+  if (precision != SEVP_PREC_FP64) {
+    if (int rc = alloc_mixed(p)) {
+      p->d.precision = old;
+      return rc;
+    }
     if (p->have_inv) CHK(make_low_precision_copies(p));
     HIPC(hipStreamSynchronize(p->stream));
   }
@@ -1035,7 +1101,7 @@ int sevp_solve_dev(sevp_plan p, const float *s, double *x, int nrhs, void *strea
     HIPC(launch_bir_sweeps(a, nrhs, st));
     return SEVP_OK;
   }
-  FullCtx c = full_ctx(p, p->d.precision != SEVP_PREC_FP64);
+  FullCtx c = full_ctx(p);
   return run_full_pass(c, s, nullptr, s_stride, x, x_stride, nrhs, st);
 }
 
@@ -1244,7 +1310,7 @@ int sevp_solve_refined_dev(sevp_plan p, const float *s, double *x, int nrhs, dou
   auto t0 = std::chrono::steady_clock::now();
   const int i2 = p->d.i2, j2 = p->d.j2;
   const long ns = (long)i2 * j2, nx = (long)(i2 + 2) * (j2 + 2);
-  FullCtx c = full_ctx(p, p->d.precision != SEVP_PREC_FP64);
+  FullCtx c = full_ctx(p);
   // RHS groups: ng right-hand sides each (the last may be smaller)
   const int ngroups = std::min(p->rhs_groups, nrhs);
   const int ng = (nrhs + ngroups - 1) / ngroups;

@@ -96,7 +96,7 @@ __device__ void march_body(const MarchArgs &p, int tile, int bidx, int strip, in
     if (MODE == MODE_PART) {
       vs[k] = x[f2(i0, xcol(k), js)];
       vc[k] = x[f2(i0, xcol(k), js + 1)];
-    } else if (MODE == MODE_HOM && !p.corr.p) {
+    } else if (MODE == MODE_HOM && !p.corr.on()) {
       vs[k] = (h || !p.hcpl) ? 0.0 : p.hcpl[(long)bidx * p.i2 + pc[k] - 1];
       vc[k] = h ? 0.0 : p.hfree[(long)bidx * p.i2 + pc[k] - 1];
     }
@@ -149,7 +149,7 @@ __device__ void march_body(const MarchArgs &p, int tile, int bidx, int strip, in
     for (int k = 0; k < K; ++k) xnorth[k] = x[f2(i0, pc[k] + 1, ieb + 1)];
   }
   // split-K corrections, read after the row loads above are in flight
-  if (MODE != MODE_PRE && p.corr.p) {
+  if (MODE != MODE_PRE && p.corr.on()) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const bool h = halo_w[k] || halo_e[k];
@@ -361,7 +361,7 @@ __global__ __launch_bounds__(64) void hom_resid_kernel(MarchArgs m, ResidArgs r,
     return;
   }
   if (!has_resid) return;
-  if (r.corr.p) apply_resid_body(r, (b - nm) * 64 + threadIdx.x);
+  if (r.corr.on()) apply_resid_body(r, (b - nm) * 64 + threadIdx.x);
   else resid_body(r, (b - nm) * 64 + threadIdx.x);
 }
 

@@ -49,14 +49,27 @@ struct ColMap {
 // GEMM: value(m, n) = sum over slices s of P[s][m][n], added in FP64 in slice
 // order (so every reader of one element computes the same double).
 struct PartRef {
-  const float *p;  // null: no correction
-  long sstride;    // elements per slice (M * ld)
-  int ns;          // number of slices
-  int ld;          // N of the product (elements per right-hand side)
+  const float *p;   // FP32 slice sums; null with pd null: no correction
+  const double *pd; // FP64 slice sums (the FP64 MFMA products), else null
+  long sstride;     // elements per slice (M * ld)
+  int ns;           // number of slices
+  int ld;           // N of the product (elements per right-hand side)
   // eight loads in flight per group; the additions stay in slice order
   __device__ double sum(int m, int n) const {
-    const float *q = p + (long)m * ld + n;
     double acc = 0.0;
+    if (pd) {
+      const double *q = pd + (long)m * ld + n;
+      for (int s0 = 0; s0 < ns; s0 += 8) {
+        double v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = q[(long)min(s0 + i, ns - 1) * sstride];
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+          if (s0 + i < ns) acc += v[i];
+      }
+      return acc;
+    }
+    const float *q = p + (long)m * ld + n;
     for (int s0 = 0; s0 < ns; s0 += 8) {
       float v[8];
 #pragma unroll
@@ -67,6 +80,7 @@ struct PartRef {
     }
     return acc;
   }
+  __device__ bool on() const { return p || pd; }
 };
 
 // Full-width (or strip) march over one block, batched over independent
@@ -164,6 +178,22 @@ struct GemmPartArgs {
   float *part;      // (S, M, N)
   unsigned long long *tspan;  // profiling: kernel span slot (null: off)
 };
+// The FP64 counterpart: A the FP64 error rows, B FP64 lane tiles
+// (launch_cvt_lane_tiles64), FP64 slice sums.
+struct GemmPart64Args {
+  const double *a;
+  const double *b;
+  const double *b2;
+  int nsplit;
+  int M, N, K;
+  double *part;  // (S, M, N)
+  unsigned long long *tspan;
+};
+hipError_t launch_gemm_part64(const GemmPart64Args &a, hipStream_t st, hipEvent_t ev0 = nullptr,
+                              hipEvent_t ev1 = nullptr);
+long gemm_part64_workgroups(const GemmPart64Args &a);
+hipError_t launch_cvt_lane_tiles64(const double *src, double *dst, int K, int N, long nmat,
+                                   hipStream_t st);
 // Slice length and number of slices used for an (N, K) product.
 int gemm_part_slices(int N, int K, int *kslice);
 // ev0 / ev1 (optional): events recorded with the kernel's own start and end

@@ -188,13 +188,143 @@ hipError_t launch_gemm_part(const GemmPartArgs &a, hipStream_t st, hipEvent_t ev
   return hipGetLastError();
 }
 
+// ------------------------------------------- FP64 split-K partial products
+//
+// The same decomposition on the FP64 matrix cores (v_mfma_f64_16x16x4f64,
+// whose A, B and D lane layout is the FP32 instruction's): A (the FP64 error
+// rows) staged in LDS with a row pitch of KS+2 doubles (an odd number of
+// 16-byte units), B from FP64 lane tiles (a 2 KiB block per chunk of 16 k,
+// 32 bytes per lane), FP64 slice sums.  The products are exact FP64
+// arithmetic in a blocked order: FP64 accuracy, not the reference's order.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f64x4 load4d(const double *base, int k, int klen) {
+  const bool ok = k < klen;
+  const f64x4 v = *reinterpret_cast<const f64x4 *>(base + (ok ? k : 0));
+  return ok ? v : f64x4{0.0, 0.0, 0.0, 0.0};
+}
+
+template <int KS, int MB>
+__global__ __launch_bounds__(64 * GP_WAVES) void gemm_part64_kernel(GemmPart64Args p) {
+  constexpr int LDA = KS + 2;
+  constexpr int KQ = KS / 4;
+  constexpr int NCH = KS / 16;
+  constexpr int ROWS = 16 * MB;
+  constexpr int ALD = (ROWS * KQ + 64 * GP_WAVES - 1) / (64 * GP_WAVES);
+  __shared__ double sa[ROWS * LDA];
+  span_begin(p.tspan);
+  const int s = blockIdx.y;
+  const int k_lo = s * KS;
+  const int klen = min(p.K - k_lo, KS);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n0 = (blockIdx.x * GP_WAVES + wave) * 16;
+  const int col = lane & 15, q = lane >> 4;
+  const int n = n0 + col;
+  const bool nok = n < p.N;
+  const int nc = nok ? n0 : ((p.N - 1) / 16) * 16;
+  f64x4 at[ALD];
+#pragma unroll
+  for (int j = 0; j < ALD; ++j) {
+    const int idx = threadIdx.x + j * 64 * GP_WAVES;
+    const int m = idx / KQ, k = (idx % KQ) * 4;
+    const bool ok = idx < ROWS * KQ && m < p.M;
+    at[j] = load4d(p.a + (long)(ok ? m : 0) * p.K + k_lo, k, ok ? klen : 0);
+  }
+  const int nchk = (p.K + 15) / 16;
+  const int nt = nc / 16;
+  const bool second = p.b2 && nt >= p.nsplit / 16;
+  const f64x4 *btile = reinterpret_cast<const f64x4 *>(second ? p.b2 : p.b) +
+                       (long)(second ? nt - p.nsplit / 16 : nt) * nchk * 64;
+  const int ch0 = k_lo / 16;
+  const f64x4 zero = f64x4{0.0, 0.0, 0.0, 0.0};
+  f64x4 bq[NCH];
+#pragma unroll
+  for (int i = 0; i < NCH; ++i) {
+    const bool ok = ch0 + i < nchk;
+    const f64x4 v = btile[(long)(ok ? ch0 + i : 0) * 64 + lane];
+    bq[i] = ok ? v : zero;
+  }
+#pragma unroll
+  for (int j = 0; j < ALD; ++j) {
+    const int idx = threadIdx.x + j * 64 * GP_WAVES;
+    if (idx < ROWS * KQ) *reinterpret_cast<f64x4 *>(&sa[(idx / KQ) * LDA + (idx % KQ) * 4]) = at[j];
+  }
+  __syncthreads();
+  if (n0 < p.N) {
+    f64x4 acc[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) acc[mb] = zero;
+    const double *arow = sa + col * LDA + 4 * q;
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+      f64x4 av[MB];
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) av[mb] = *reinterpret_cast<const f64x4 *>(arow + mb * 16 * LDA + 16 * ch);
+      const f64x4 bv = nok ? bq[ch] : zero;
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+          acc[mb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mb][t], bv[t], acc[mb], 0, 0, 0);
+    }
+    double *out = p.part + (long)s * p.M * p.N + n;
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int m = mb * 16 + 4 * q + i;
+        if (nok && m < p.M) out[(long)m * p.N] = acc[mb][i];
+      }
+  }
+  span_end(p.tspan);
+}
+
+template <int KS, int MB>
+static void gp64_launch(const GemmPart64Args &a, dim3 grid, hipStream_t st, hipEvent_t ev0,
+                        hipEvent_t ev1) {
+  const dim3 block(64 * GP_WAVES);
+  if (ev0) hipExtLaunchKernelGGL(gemm_part64_kernel<KS, MB>, grid, block, 0, st, ev0, ev1, 0, a);
+  else hipLaunchKernelGGL((gemm_part64_kernel<KS, MB>), grid, block, 0, st, a);
+}
+
+template <int KS>
+static void gp64_launch_mb(const GemmPart64Args &a, dim3 grid, hipStream_t st, hipEvent_t ev0,
+                           hipEvent_t ev1) {
+  const int mb = (a.M + 15) / 16;
+  if (mb == 1) gp64_launch<KS, 1>(a, grid, st, ev0, ev1);
+  else if (mb == 2) gp64_launch<KS, 2>(a, grid, st, ev0, ev1);
+  else gp64_launch<KS, 4>(a, grid, st, ev0, ev1);
+}
+
+long gemm_part64_workgroups(const GemmPart64Args &a) {
+  return (long)((a.N + 16 * GP_WAVES - 1) / (16 * GP_WAVES)) * gemm_part_slices(a.N, a.K, nullptr);
+}
+
+hipError_t launch_gemm_part64(const GemmPart64Args &a, hipStream_t st, hipEvent_t ev0,
+                              hipEvent_t ev1) {
+  if (a.M > MX_MAXM || a.M < 1 || a.N < 1 || a.K < 1 || a.K % 4 != 0) return hipErrorInvalidValue;
+  int ks = 0;
+  const int ns = gemm_part_slices(a.N, a.K, &ks);
+  const int groups = (a.N + 16 * GP_WAVES - 1) / (16 * GP_WAVES);
+  const dim3 grid(groups, ns);
+  switch (ks) {
+    case 32: gp64_launch_mb<32>(a, grid, st, ev0, ev1); break;
+    case 48: gp64_launch_mb<48>(a, grid, st, ev0, ev1); break;
+    case 64: gp64_launch_mb<64>(a, grid, st, ev0, ev1); break;
+    case 96: gp64_launch_mb<96>(a, grid, st, ev0, ev1); break;
+    default: gp64_launch_mb<128>(a, grid, st, ev0, ev1); break;
+  }
+  return hipGetLastError();
+}
+
 // FP32 operand layout of the products ("lane tiles"): an (K x N) column-major
 // matrix becomes tiles of 16 columns; tile t holds ceil(K/16) chunks of 16 k,
 // each a 1 KiB block in the order the 64 lanes of a wavefront load it, lane
 // (q, c) = (l / 16, l % 16) holding k = 16 ch + 4q .. 4q + 3 of column
 // 16 t + c as one float4.  One chunk is one coalesced 1 KiB load, and the
 // chunks of a tile are consecutive; entries past K or N are zero.
-__global__ void cvt_lane_tiles_kernel(const double *src, float *dst, int K, int N, long nmat,
+template <class T>
+__global__ void cvt_lane_tiles_kernel(const double *src, T *dst, int K, int N, long nmat,
                                       long src_stride, long dst_stride) {
   const int nchk = (K + 15) / 16, ntile = (N + 15) / 16;
   const long per = (long)ntile * nchk * 256;
@@ -206,7 +336,7 @@ __global__ void cvt_lane_tiles_kernel(const double *src, float *dst, int K, int 
   const int ch = (int)(blk % nchk), tile = (int)(blk / nchk);
   const int k = 16 * ch + 4 * (l >> 4) + t, n = 16 * tile + (l & 15);
   dst[mat * dst_stride + r] =
-      (k < K && n < N) ? (float)src[mat * src_stride + (long)k + (long)n * K] : 0.f;
+      (k < K && n < N) ? (T)src[mat * src_stride + (long)k + (long)n * K] : (T)0;
 }
 
 long lane_tiles_floats(int K, int N) { return (long)((N + 15) / 16) * ((K + 15) / 16) * 256; }
@@ -214,8 +344,16 @@ long lane_tiles_floats(int K, int N) { return (long)((N + 15) / 16) * ((K + 15) 
 hipError_t launch_cvt_lane_tiles(const double *src, float *dst, int K, int N, long nmat,
                                  hipStream_t st) {
   const long per = lane_tiles_floats(K, N), n = per * nmat;
-  hipLaunchKernelGGL(cvt_lane_tiles_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                     src, dst, K, N, nmat, (long)K * N, per);
+  hipLaunchKernelGGL(cvt_lane_tiles_kernel<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     st, src, dst, K, N, nmat, (long)K * N, per);
+  return hipGetLastError();
+}
+
+hipError_t launch_cvt_lane_tiles64(const double *src, double *dst, int K, int N, long nmat,
+                                   hipStream_t st) {
+  const long per = lane_tiles_floats(K, N), n = per * nmat;
+  hipLaunchKernelGGL(cvt_lane_tiles_kernel<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     st, src, dst, K, N, nmat, (long)K * N, per);
   return hipGetLastError();
 }
 

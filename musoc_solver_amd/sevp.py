@@ -31,6 +31,7 @@ FORM_FULL = 2
 PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_BF16 = 2
+PREC_FP64_MFMA = 3
 
 SEVP_OK = 0
 SEVP_E_NOTCONV = -5

@@ -206,7 +206,7 @@ def _device_case(i2, j2, n0, nrhs, precision, seed=21):
     return plan, coefs, ie, s, s_d, x_d
 
 
-@pytest.mark.parametrize("precision", [S.PREC_FP64, S.PREC_FP32])
+@pytest.mark.parametrize("precision", [S.PREC_FP64, S.PREC_FP32, S.PREC_FP64_MFMA])
 def test_refined_solve_reaches_tolerance(oracle, device, precision):
     import torch
     i2, j2, n0, nrhs = 96, 48, 3, 8
@@ -289,7 +289,7 @@ def test_bir_seam_fix_bitwise(oracle, device):
     plan.close()
 
 
-@pytest.mark.parametrize("precision", [S.PREC_FP64, S.PREC_FP32])
+@pytest.mark.parametrize("precision", [S.PREC_FP64, S.PREC_FP32, S.PREC_FP64_MFMA])
 def test_graph_replays_match_eager_launches(oracle, device, precision):
     """The refinement pass replayed as a captured HIP graph gives bitwise the
     result of launching its kernels one by one, on every replay (each solve
@@ -326,7 +326,7 @@ def test_graph_replays_match_eager_launches(oracle, device, precision):
     plan.close()
 
 
-@pytest.mark.parametrize("precision", [S.PREC_FP64, S.PREC_FP32])
+@pytest.mark.parametrize("precision", [S.PREC_FP64, S.PREC_FP32, S.PREC_FP64_MFMA])
 def test_rhs_groups_match_one_group(oracle, device, precision):
     """Refinement passes of RHS groups on concurrent streams give bitwise the
     result of one group: every right-hand side's pass is independent of the
@@ -351,7 +351,7 @@ def test_rhs_groups_match_one_group(oracle, device, precision):
     plan.close()
 
 
-@pytest.mark.parametrize("precision", [S.PREC_FP64, S.PREC_FP32])
+@pytest.mark.parametrize("precision", [S.PREC_FP64, S.PREC_FP32, S.PREC_FP64_MFMA])
 def test_self_timed_products(oracle, device, precision):
     """Profiled eager solves time every correction product from the GPU clock
     and with HIP events; timing changes no result."""
@@ -372,4 +372,42 @@ def test_self_timed_products(oracle, device, precision):
     assert torch.equal(x_d, x0)
     assert span_n == ev_n > 0
     assert 0.0 < span_ms <= ev_ms
+    plan.close()
+
+
+def test_fp64_mfma_one_pass_and_precision_switching(oracle, device):
+    """FP64 corrections on the matrix cores: one refinement pass reaches
+    1e-10 (the FP32 products need three), the solution agrees with the plain
+    FP64 (reference-order) solve to FP64 rounding, and switching a plan
+    between the three arithmetics keeps each one's results."""
+    import torch
+    i2, j2, n0, nrhs = 192, 96, 2, 8
+    print("[fp64-mfma] one pass", flush=True)
+    plan, coefs, ie, s, s_d, x_d = _device_case(i2, j2, n0, nrhs, S.PREC_FP64)
+    rel, st, ok = plan.solve_refined_dev(s_d, x_d, tol=1e-10, max_iter=4)
+    assert ok and st.iterations == 1
+    x_plain = x_d.clone()
+    results = {}
+    for prec in (S.PREC_FP64_MFMA, S.PREC_FP32, S.PREC_FP64_MFMA, S.PREC_FP32):
+        plan.set_precision(prec)
+        x_d.zero_()
+        rel, st, ok = plan.solve_refined_dev(s_d, x_d, tol=1e-10, max_iter=8)
+        torch.cuda.synchronize()
+        assert ok and rel.max() <= 1e-10, (prec, rel.max())
+        if prec in results:
+            assert torch.equal(results[prec][0], x_d) and results[prec][1] == st.iterations
+        results[prec] = (x_d.clone(), st.iterations)
+    x64, it64 = results[S.PREC_FP64_MFMA]
+    assert it64 == 1 and results[S.PREC_FP32][1] >= 2
+    d = (x64 - x_plain)[:, 1:-1, 1:-1].reshape(nrhs, -1).norm(dim=1)
+    n = x_plain[:, 1:-1, 1:-1].reshape(nrhs, -1).norm(dim=1)
+    assert float((d / n).max()) <= 1e-11, float((d / n).max())
+    # and against the oracle's REP1 (FP64, the reference's order)
+    r_o, r1_o, _ = oracle.pre1(coefs, ie)
+    x = x64.cpu().numpy().transpose(2, 1, 0)
+    for r in (0, nrhs - 1):
+        xo = np.zeros((i2 + 2, j2 + 2), order="F")
+        oracle.rep1(coefs, r_o, r1_o, s[:, :, r], xo, ie)
+        err = np.linalg.norm(x[1:-1, 1:-1, r] - xo[1:-1, 1:-1]) / np.linalg.norm(xo[1:-1, 1:-1])
+        assert err <= 1e-11, err
     plan.close()
