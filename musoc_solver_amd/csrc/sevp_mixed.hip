@@ -191,7 +191,8 @@ hipError_t launch_gemm_part(const GemmPartArgs &a, hipStream_t st, hipEvent_t ev
 // ------------------------------------------- FP64 split-K partial products
 //
 // The same decomposition on the FP64 matrix cores (v_mfma_f64_16x16x4f64,
-// whose A, B and D lane layout is the FP32 instruction's): A (the FP64 error
+// whose A and B lane layout is the FP32 instruction's; D differs, see the
+// store below): A (the FP64 error
 // rows) staged in LDS with a row pitch of KS+2 doubles (an odd number of
 // 16-byte units), B from FP64 lane tiles (a 2 KiB block per chunk of 16 k,
 // 32 bytes per lane), FP64 slice sums.  The products are exact FP64
@@ -267,12 +268,14 @@ __global__ __launch_bounds__(64 * GP_WAVES) void gemm_part64_kernel(GemmPart64Ar
         for (int mb = 0; mb < MB; ++mb)
           acc[mb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mb][t], bv[t], acc[mb], 0, 0, 0);
     }
+    // the FP64 instruction's D layout: lane group q holds rows q, q+4, q+8,
+    // q+12 of its column (the FP32 one holds 4q .. 4q+3)
     double *out = p.part + (long)s * p.M * p.N + n;
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int m = mb * 16 + 4 * q + i;
+        const int m = mb * 16 + q + 4 * i;
         if (nok && m < p.M) out[(long)m * p.N] = acc[mb][i];
       }
   }
