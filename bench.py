@@ -234,9 +234,8 @@ def main():
         err64 = float((d / n).max())
         log(f"fast64 vs plain FP64: max rel error {err64:.3e}")
 
-    main_leg = legs.get("mixed") or legs.get("plain")
-    main_name = "mixed" if "mixed" in legs else "plain"
-    mixed_main = main_name == "mixed"
+    main_name = next(n for n in ("mixed", "plain", "fast64") if n in legs)
+    main_leg = legs[main_name]
     roof = roofline(main_leg, main_name, nrhs, i2, a)
     krylov = krylov_leg(plan_keep[0], s, xs.get(main_name), a) if a.krylov else None
     cpu = None
@@ -254,7 +253,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64+f32mfma" if mixed_main else "f64",
+        "dtype": {"mixed": "f64+f32mfma", "fast64": "f64mfma"}.get(main_name, "f64"),
         "data": "synthetic",
         "config": {"workload": f"SEVP {i2}x{j2} (0.25 deg zonal, +-{a.lat_max:g} deg), REP1 form, "
                                f"{nrhs} RHS per batch, each to {a.tol:g} rel resid",
