@@ -62,117 +62,154 @@ __device__ __forceinline__ double dot_col(double e, const double (&r)[WMAX], int
 #define LD_C(p, j) ((double)(p)[c.ccol + (long)((j) - 1) * c.cld])
 #define LD_F(j) ((double)c.f[c.fcol + (long)((j) - 1) * c.fld])
 #define XV(i, j) c.xv[(long)((i) - 1) + (long)((j) - 1) * c.ldx]
+#define LDC32(p, j) ((p)[c.ccol + (long)((j) - 1) * c.cld])
 
 // REPBIR on one strip for the calling wavefront.
-template <int WMAX>
-__device__ void repbir_wave(const StripCtx &c) {
+//
+// A row step is a short dependent FP64 chain (four products and differences
+// and a division, ~50 ns) while a load takes ~0.5 us, so each march reads its
+// row operands from a ring of D rows loaded D rows ahead.  The forward pass
+// consumes rows 1 .. jlast strictly in order (each a march row of its block
+// or the block's closure row), so one ring runs across the block boundaries;
+// the next block's RINV1 columns and first guess row are requested at each
+// closure, a whole block before they are needed.  The backward H march fills
+// a ring per block, after the block's restore of row IE (which its last row
+// reads: same lane, so program order makes the store visible).
+template <int WMAX, int D>
+__device__ __forceinline__ void repbir_wave(const StripCtx &c) {
   const int lane = threadIdx.x;
   const int w = c.w;
   const bool act = lane < w;
   const int li = act ? lane + 2 : 2;  // this lane's X column (1-based, strip view)
   const int wi = w + 2;               // east halo column
   const int n = act ? lane : 0;
+  const int nblk = c.nblk;
   double rc[WMAX], rc2[WMAX];
 
   // ---------------- forward ----------------
-  int js = 1;
+  const int ieb_last = c.ie[nblk - 1];
+  const int js_last = nblk > 1 ? c.ie[nblk - 2] : 1;
+  const int jlast = (ieb_last - 2 >= js_last) ? ieb_last - 2 : js_last - 1;
+  float fa[D], fb[D], fc[D], fd[D], fe[D], ff_[D];
+  double fhw[D], fhe[D];
+#define FFETCH(k, jj)                                                                  \
+  do {                                                                                 \
+    const int j_ = (jj);                                                               \
+    fa[k] = LDC32(c.ax, j_); fb[k] = LDC32(c.ay, j_); fc[k] = LDC32(c.bb, j_);         \
+    fd[k] = LDC32(c.cx, j_); fe[k] = LDC32(c.cy, j_);                                  \
+    ff_[k] = c.f[c.fcol + (long)(j_ - 1) * c.fld];                                     \
+    fhw[k] = XV(1, j_ + 1); fhe[k] = XV(wi, j_ + 1);                                   \
+  } while (0)
+#pragma unroll
+  for (int k = 0; k < D; ++k) FFETCH(k, min(1 + k, max(jlast, 1)));
+  int nb = 1, ieb = c.ie[0];
+  if (nblk > 1) load_col<WMAX>(rc, c.rinv1, w, n);
   double xs = XV(li, 1), xc = XV(li, 2);
-  for (int nb = 1; nb <= c.nblk; ++nb) {
-    const int ieb = c.ie[nb - 1];
-    const int jf = ieb - 2;
-    const bool last = (nb == c.nblk);
-    if (!last) load_col<WMAX>(rc, c.rinv1 + (long)(nb - 1) * w * w, w, n);
-    // row js's operands
-    double a = LD_C(c.ax, js), b = LD_C(c.ay, js), cc = LD_C(c.bb, js),
-           d = LD_C(c.cx, js), e = LD_C(c.cy, js), ff = LD_F(js);
-    double hw = XV(1, js + 1), he = XV(wi, js + 1);
-    for (int j = js; j <= jf; ++j) {
-      // prefetch row j+1 (j+1 <= jf+1 = closure row, always a valid row)
-      double na = LD_C(c.ax, j + 1), nbv = LD_C(c.ay, j + 1), nc = LD_C(c.bb, j + 1),
-             nd = LD_C(c.cx, j + 1), ne = LD_C(c.cy, j + 1), nf = LD_F(j + 1);
-      double nhw = XV(1, j + 2), nhe = XV(wi, j + 2);
-      double xw = wave_shr1(xc, hw);
-      double xe = wave_shl1(xc, he);
-      if (lane == w - 1) xe = he;
-      double xn = (ff - a * xw - b * xs - cc * xc - d * xe) / e;
-      if (act && j + 2 < ieb) XV(li, j + 2) = xn;
-      xs = xc;
-      xc = xn;
-      a = na; b = nbv; cc = nc; d = nd; e = ne; ff = nf; hw = nhw; he = nhe;
-    }
-    if (last) {
-      if (act) XV(li, ieb) = xc;
-      break;
-    }
-    // closure residual at row ieb-1 (operands already in a..ff, hw, he)
-    double xnn = XV(li, ieb + 1);
-    double xw = wave_shr1(xc, hw);
-    double xe = wave_shl1(xc, he);
-    if (lane == w - 1) xe = he;
-    double r = ff - a * xw - b * xs - cc * xc - d * xe - e * xnn;
-    double dd = dot_col<WMAX>(r, rc, w);
-    if (act) c.dum0[n + (long)(nb - 1) * w] = xc;
-    double xcor = xc - dd;
-    if (act) XV(li, ieb) = xcor;
-    xs = xcor;
-    xc = xnn;
-    js = ieb;
-  }
-
-  // ---------------- backward ----------------
-  for (int nbs = 1; nbs <= c.nblk; ++nbs) {
-    const int nb = c.nblk - nbs + 1;
-    const int ieb = c.ie[nb - 1];
-    js = (nb != 1) ? c.ie[nb - 2] : 1;
-    const int jf = ieb - 2;
-    load_col<WMAX>(rc, c.rinv + (long)(nb - 1) * w * w, w, n);
-    if (nb != 1) load_col<WMAX>(rc2, c.rinv1 + (long)(nb - 2) * w * w, w, n);
-    double xrow;  // X(li, ieb) after the restore
-    if (nb != c.nblk) {
-      xrow = c.dum0[n + (long)(nb - 1) * w];
-      if (act) XV(li, ieb) = xrow;
-    } else {
-      xrow = XV(li, ieb);
-    }
-    const int jr = ieb - 1;
-    double xsr = XV(li, jr), xnn = XV(li, ieb + 1);
-    double hw = XV(1, ieb), he = XV(wi, ieb);
-    double xw = wave_shr1(xrow, hw);
-    double xe = wave_shl1(xrow, he);
-    if (lane == w - 1) xe = he;
-    double r = LD_F(jr) - LD_C(c.ax, jr) * xw - LD_C(c.ay, jr) * xsr - LD_C(c.bb, jr) * xrow -
-               LD_C(c.cx, jr) * xe - LD_C(c.cy, jr) * xnn;
-    double cv = dot_col<WMAX>(r, rc, w);
-    double xf = XV(li, js + 1);
-    if (act) XV(li, js + 1) = xf + cv;
-    double hs = 0.0;
-    if (nb != 1) {
-      double g = cv * LD_C(c.cy, js - 1);
-      hs = dot_col<WMAX>(g, rc2, w);
-    }
-    double hc = cv;
-    if (jf >= js) {
-      double a = LD_C(c.ax, js), b = LD_C(c.ay, js), cc = LD_C(c.bb, js),
-             d = LD_C(c.cx, js), e = LD_C(c.cy, js);
-      double xo = XV(li, js + 2);
-      for (int j = js; j <= jf; ++j) {
-        double na = 0, nbv = 0, nc = 0, nd = 0, ne = 1, nxo = 0;
-        if (j < jf) {
-          na = LD_C(c.ax, j + 1); nbv = LD_C(c.ay, j + 1); nc = LD_C(c.bb, j + 1);
-          nd = LD_C(c.cx, j + 1); ne = LD_C(c.cy, j + 1); nxo = XV(li, j + 3);
+  double xnn = (nblk > 1) ? XV(li, ieb + 1) : 0.0;
+  for (int j0 = 1; j0 <= jlast; j0 += D) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const int j = j0 + k;
+      if (j <= jlast) {
+        const double a = fa[k], b = fb[k], cc = fc[k], d = fd[k], e = fe[k], ff = ff_[k];
+        const double hw = fhw[k], he = fhe[k];
+        FFETCH(k, min(j + D, jlast));
+        double xw = wave_shr1(xc, hw);
+        double xe = wave_shl1(xc, he);
+        if (lane == w - 1) xe = he;
+        if (j == ieb - 1) {
+          // closure residual of block nb at row ieb-1, correction of row ieb
+          const double r = ff - a * xw - b * xs - cc * xc - d * xe - e * xnn;
+          const double dd = dot_col<WMAX>(r, rc, w);
+          if (act) c.dum0[n + (long)(nb - 1) * w] = xc;
+          const double xcor = xc - dd;
+          if (act) XV(li, ieb) = xcor;
+          xs = xcor;
+          xc = xnn;
+          ++nb;
+          ieb = c.ie[nb - 1];
+          if (nb < nblk) {
+            load_col<WMAX>(rc, c.rinv1 + (long)(nb - 1) * w * w, w, n);
+            xnn = XV(li, ieb + 1);
+          }
+        } else {
+          const double xn = (ff - a * xw - b * xs - cc * xc - d * xe) / e;
+          if (act && j + 2 < ieb) XV(li, j + 2) = xn;
+          xs = xc;
+          xc = xn;
         }
-        double hwv = wave_shr1(hc, 0.0);
-        double hev = wave_shl1(hc, 0.0);
-        if (lane == w - 1) hev = 0.0;
-        double hn = (-(a * hwv) - b * hs - cc * hc - d * hev) / e;
-        if (act) XV(li, j + 2) = xo + hn;
-        hs = hc;
-        hc = hn;
-        a = na; b = nbv; cc = nc; d = nd; e = ne; xo = nxo;
       }
     }
   }
+  if (act) XV(li, ieb) = xc;  // the last block's row IE(NBLK)
+#undef FFETCH
+
+  // ---------------- backward ----------------
+  float ba[D], bb_[D], bc[D], bd[D], be[D];
+  double bxo[D];
+#define BFETCH(k, jj)                                                                  \
+  do {                                                                                 \
+    const int j_ = (jj);                                                               \
+    ba[k] = LDC32(c.ax, j_); bb_[k] = LDC32(c.ay, j_); bc[k] = LDC32(c.bb, j_);        \
+    bd[k] = LDC32(c.cx, j_); be[k] = LDC32(c.cy, j_); bxo[k] = XV(li, j_ + 2);         \
+  } while (0)
+  for (int nbs = 1; nbs <= nblk; ++nbs) {
+    const int nbk = nblk - nbs + 1;
+    const int iebk = c.ie[nbk - 1];
+    const int js = (nbk != 1) ? c.ie[nbk - 2] : 1;
+    const int jf = iebk - 2;
+    load_col<WMAX>(rc, c.rinv + (long)(nbk - 1) * w * w, w, n);
+    if (nbk != 1) load_col<WMAX>(rc2, c.rinv1 + (long)(nbk - 2) * w * w, w, n);
+    double xrow;  // X(li, ieb) after the restore
+    if (nbk != nblk) {
+      xrow = c.dum0[n + (long)(nbk - 1) * w];
+      if (act) XV(li, iebk) = xrow;
+    } else {
+      xrow = XV(li, iebk);
+    }
+    // the H march's operands, requested with the closure's
+    if (jf >= js) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) BFETCH(k, min(js + k, jf));
+    }
+    const int jr = iebk - 1;
+    const double xsr = XV(li, jr), xnn2 = XV(li, iebk + 1);
+    const double hw = XV(1, iebk), he = XV(wi, iebk);
+    double xw = wave_shr1(xrow, hw);
+    double xe = wave_shl1(xrow, he);
+    if (lane == w - 1) xe = he;
+    const double r = LD_F(jr) - LD_C(c.ax, jr) * xw - LD_C(c.ay, jr) * xsr -
+                     LD_C(c.bb, jr) * xrow - LD_C(c.cx, jr) * xe - LD_C(c.cy, jr) * xnn2;
+    const double cv = dot_col<WMAX>(r, rc, w);
+    const double xf = XV(li, js + 1);
+    if (act) XV(li, js + 1) = xf + cv;
+    double hs = 0.0;
+    if (nbk != 1) {
+      const double g = cv * LD_C(c.cy, js - 1);
+      hs = dot_col<WMAX>(g, rc2, w);
+    }
+    double hc = cv;
+    for (int j0 = js; j0 <= jf; j0 += D) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        const int j = j0 + k;
+        if (j <= jf) {
+          const double a = ba[k], b = bb_[k], cc = bc[k], d = bd[k], e = be[k], xo = bxo[k];
+          BFETCH(k, min(j + D, jf));
+          double hwv = wave_shr1(hc, 0.0);
+          double hev = wave_shl1(hc, 0.0);
+          if (lane == w - 1) hev = 0.0;
+          const double hn = (-(a * hwv) - b * hs - cc * hc - d * hev) / e;
+          if (act) XV(li, j + 2) = xo + hn;
+          hs = hc;
+          hc = hn;
+        }
+      }
+    }
+  }
+#undef BFETCH
 }
+#undef LDC32
 #undef XV
 
 // Whole FS_SOLVER BIR loop (MAIN/template/GLOBAL/src/fs_GLOBAL.f90:1902-1966)
@@ -208,7 +245,7 @@ __global__ __launch_bounds__(64) void bir_sweeps_kernel(BirArgs p) {
         c.fcol = c.ccol;
         c.xv = x + f2(i0, il, 1);
         c.ldx = i0;
-        repbir_wave<WMAX>(c);
+        repbir_wave<WMAX, (WMAX <= 32 ? 8 : 4)>(c);
         __threadfence_block();
       } else {
         // seam strip: halo columns of XX for rows 1..J2 as the reference does
@@ -223,7 +260,7 @@ __global__ __launch_bounds__(64) void bir_sweeps_kernel(BirArgs p) {
         c.fcol = c.ccol;
         c.xv = xx;
         c.ldx = m0;
-        repbir_wave<WMAX>(c);
+        repbir_wave<WMAX, (WMAX <= 32 ? 8 : 4)>(c);
         __threadfence_block();
         for (long t = lane; t < (long)w * (j1 - 1); t += SEVP_WAVE) {
           const int i = (int)(t % w) + 1, j = (int)(t / w) + 2;
@@ -265,7 +302,7 @@ __global__ __launch_bounds__(64) void repbir_kernel(RepbirArgs p) {
   c.ie = p.ie;
   c.nblk = p.nblk;
   c.w = p.m2;
-  repbir_wave<WMAX>(c);
+  repbir_wave<WMAX, (WMAX <= 32 ? 8 : 4)>(c);
 }
 
 hipError_t launch_bir_sweeps(const BirArgs &a, int nrhs, hipStream_t st) {
