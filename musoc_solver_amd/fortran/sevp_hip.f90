@@ -20,7 +20,8 @@ module sevp_hip
 
   integer(c_int), parameter, public :: SEVP_FORM_FULL_PERIODIC = 0, SEVP_FORM_BIR = 1, &
                                        SEVP_FORM_FULL = 2
-  integer(c_int), parameter, public :: SEVP_PREC_FP64 = 0, SEVP_PREC_FP32 = 1, SEVP_PREC_BF16 = 2
+  integer(c_int), parameter, public :: SEVP_PREC_FP64 = 0, SEVP_PREC_FP32 = 1, SEVP_PREC_BF16 = 2, &
+                                       SEVP_PREC_FP64_MFMA = 3
 
   type, bind(c), public :: sevp_desc
     integer(c_int) :: i2, j2, nb0
