@@ -78,6 +78,10 @@ def parse():
                     help="also time the device BiCGSTAB cross-check on the first RHS (0: skip)")
     ap.add_argument("--krylov-only", action="store_true",
                     help="only the BiCGSTAB leg (for kernel profiles of its iteration)")
+    ap.add_argument("--basins", type=int, default=0,
+                    help="regional basins refined together (sevp_batch) beside one by one (0: skip)")
+    ap.add_argument("--basin-rhs", type=int, default=8, help="right-hand sides per basin")
+    ap.add_argument("--basins-only", action="store_true", help="run the basins leg alone")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device (reference kernel for HBM counters)")
     return ap.parse_args()
@@ -185,6 +189,9 @@ def main():
         log(f"{name}: {r}")
         return r
 
+    if a.basins_only:
+        print(json.dumps({"basins": basins_leg(a)}), flush=True)
+        return
     legs, setup, xs = {}, {}, {}
     order = [l for l in a.legs.split(",") if l]
     if a.krylov_only:  # BiCGSTAB needs the operator only, no influence matrices
@@ -272,6 +279,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "krylov": krylov,
+        "basins": basins_leg(a) if a.basins > 0 else None,
     }
     print(json.dumps(out), flush=True)
     for plan, _ in plans.values():
@@ -311,6 +319,80 @@ def roofline(leg, name, nrhs, i2, a):
                   name, "gemm_seq_kernel"),
               "algorithmic_bytes_per_launch": nbytes, "algorithmic_flops_per_launch": flops,
               "avg_launch_us": leg["corr_avg_us"], "rhs_per_product": m})
+    return r
+
+
+def basins_leg(a):
+    """Regional basins (sizes log-uniform in 64x64 .. 512x384, 20-45% land,
+    IE from the growth autotuner) with FP32 MFMA corrections and FP64
+    refinement to the tolerance: all basins refined together by one batch
+    (one grouped launch per pass step, sevp_batch_solve_refined_dev) against
+    the same basins refined one plan at a time; the batch's solutions must
+    equal the plans' bit for bit."""
+    import torch
+    from musoc_solver_amd import autotune, synth
+    from musoc_solver_amd import sevp as S
+    nrhs = a.basin_rhs
+    t0 = time.perf_counter()
+    plans, ss, nb0 = [], [], []
+    for i2, j2, land, seed in synth.basin_specs(a.basins, a.seed + 4000):
+        case = synth.make_case(i2, j2, seed, metric="latlon", lat_max=60.0, land_frac=land,
+                               h_land=a.h_land)
+        ie = autotune.adaptive_ie(case.coefs, max_growth=1e3, hmax=8, hmin=2, periodic=False)
+        p = S.SevpPlan(case.coefs, ie, form=S.FORM_FULL, nrhs_max=nrhs, precision=S.PREC_FP32)
+        p.build_influence()
+        s_np = synth.make_rhs(case.mask, seed + 1, nrhs)
+        ss.append(torch.from_numpy(np.ascontiguousarray(s_np.transpose(2, 1, 0))).cuda())
+        plans.append(p)
+        nb0.append(len(ie))
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    log(f"basins: {len(plans)} plans built in {setup:.1f} s (NB0 max {max(nb0)}, sum {sum(nb0)}, "
+        f"{sum(p.device_bytes for p in plans) / 2**30:.2f} GiB on device)")
+    zeros = lambda: [torch.zeros((nrhs, p.j2 + 2, p.i2 + 2), dtype=torch.float64, device="cuda")
+                     for p in plans]
+    reps = max(1, min(a.steps, 3))
+    # one plan after another
+    x1 = zeros()
+    worst1, passes1 = 0.0, []
+    for rep in range(reps + 1):  # the first repetition warms up (graph capture)
+        if rep == 1:
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+        passes1 = []
+        for p, s, x in zip(plans, ss, x1):
+            x.zero_()
+            rel, st, ok = p.solve_refined_dev(s, x, tol=a.tol, max_iter=a.max_iter)
+            passes1.append(st.iterations)
+            worst1 = max(worst1, float(rel.max()))
+    torch.cuda.synchronize()
+    dt1 = (time.perf_counter() - t1) / reps
+    # all basins together
+    xb = zeros()
+    with S.SevpBatch(plans) as batch:
+        for rep in range(reps + 1):
+            if rep == 1:
+                torch.cuda.synchronize()
+                t2 = time.perf_counter()
+            for x in xb:
+                x.zero_()
+            rel, passes, st, ok = batch.solve_refined_dev(ss, xb, tol=a.tol, max_iter=a.max_iter)
+        torch.cuda.synchronize()
+        dt2 = (time.perf_counter() - t2) / reps
+    same = all(bool(torch.equal(u, v)) for u, v in zip(x1, xb))
+    worst = max(float(r.max()) for r in rel)
+    n = len(plans) * nrhs
+    r = {"basins": len(plans), "rhs_per_basin": nrhs, "precision": "f64+f32mfma",
+         "batched_solves_per_s": n / dt2, "one_by_one_solves_per_s": n / dt1,
+         "speedup": dt1 / dt2, "ms_batched": 1e3 * dt2, "ms_one_by_one": 1e3 * dt1,
+         "passes_max": int(np.max(passes)), "passes_one_by_one": int(np.max(passes1)),
+         "nb0_max": int(max(nb0)), "nb0_sum": int(sum(nb0)),
+         "launches_per_pass_batched": 4 * int(max(nb0)) + 2,
+         "max_rel_resid": worst, "converged": bool(ok), "bitwise_equal_to_plans": same,
+         "setup_s": setup}
+    log(f"basins: {r}")
+    for p in plans:
+        p.close()
     return r
 
 

@@ -122,6 +122,31 @@ int sevp_solve_dev(sevp_plan plan, const float *s_dev, double *x_dev, int nrhs, 
 int sevp_solve_refined_dev(sevp_plan plan, const float *s_dev, double *x_dev, int nrhs,
                            double tol, int max_iter, double *rel_resid, sevp_stats *stats);
 
+/* Batched problems: many independent full-width plans (regional basins, each
+ * with its own grid, operator, IE and right-hand sides) refined together.
+ * There is no counterpart in the reference, which solves one domain per call
+ * (REP, MAIN/template/GLOBAL/src/sevp.f90:335-438, or REP1 :11-107); each
+ * problem's result is bitwise that of sevp_solve_refined_dev on its plan.
+ * Every step of a pass is one launch over all problems and the pass is one
+ * HIP graph, so the batch takes as many dependent launches as its tallest
+ * problem, not as all of them together.
+ * Requirements: full forms, influence matrices present, one precision for
+ * all plans (SEVP_PREC_FP32 or SEVP_PREC_FP64_MFMA), I2 divisible by 4, and
+ * one product slice length (all I2 <= 1536, or all I2 > 1536).  The
+ * batch uses the plans' scratch arrays: do not run a plan on its own while a
+ * batched solve that holds it runs.  The plans must outlive the batch. */
+typedef struct sevp_batch_s *sevp_batch;
+int sevp_batch_create(const sevp_plan *plans, int nplans, sevp_batch *out);
+int sevp_batch_destroy(sevp_batch batch);
+/* s[i], x[i]: device arrays of problem i, as for sevp_solve_refined_dev with
+ * nrhs[i] right-hand sides.  rel_resid (host, sum of nrhs[i] doubles in
+ * problem order) and passes (host, nplans ints) may be NULL; stats gets the
+ * most passes and the largest residual.  SEVP_E_NOTCONV if any problem
+ * missed the tolerance. */
+int sevp_batch_solve_refined_dev(sevp_batch batch, const float *const *s_dev,
+                                 double *const *x_dev, const int *nrhs, double tol, int max_iter,
+                                 double *rel_resid, int *passes, sevp_stats *stats);
+
 /* Relative residual ||s - A x||_2 / ||s||_2 per RHS in FP64, with the
  * 5-point product of p_sqprod (MAIN/template/GLOBAL/src/sevp.f90:696-713). */
 int sevp_residual_dev(sevp_plan plan, const float *s_dev, const double *x_dev, int nrhs,

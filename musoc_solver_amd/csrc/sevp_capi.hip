@@ -351,7 +351,7 @@ struct FullCtx {
   KProf *prof;
 };
 
-// One REP1 / REP pass with the corrections as split-K FP32 MFMA products.
+// One REP1 / REP pass with the corrections as split-K MFMA products.
 // The forward march of block b reads the previous block's correction of row
 // IE(b-1) straight from the partial products (the X array keeps the
 // uncorrected value, which is what the backward pass restores anyway); the
@@ -359,11 +359,31 @@ struct FullCtx {
 // same way, and the launch that carries it adds the free-row correction to
 // X(JS+1) and forms the closure residual of block b-1, the only reader of
 // that row in the launch.
-static int run_mixed_pass(const FullCtx &c, const float *f32, const double *f64, long f_stride,
-                          double *x, long x_stride, int nrhs, hipStream_t st) {
+//
+// mixed_pass_ops lists the launches' arguments, in order: for b = 1..NB0 the
+// forward march of block b and, below the last block, its product with
+// RINV1_b; the closure residual of the last block; for b = NB0..1 the
+// product with [RINV_b | W_b] and the backward march of b with the residual
+// of b-1; and, periodic, the halo fix.  run_mixed_pass launches them for one
+// problem, the batched solve one grouped launch per step for many.
+struct MixedPassOps {
+  bool dbl = false;                     // FP64 MFMA products (else FP32)
+  std::vector<MarchArgs> fmarch;        // [b-1], b = 1..NB0
+  std::vector<GemmPartArgs> fprod;      // [b-1], b = 1..NB0-1 (FP32 products)
+  std::vector<GemmPart64Args> fprod64;  // (FP64 products)
+  ResidArgs mid{};
+  std::vector<GemmPartArgs> bprod;      // [b-1], b = 1..NB0
+  std::vector<GemmPart64Args> bprod64;
+  std::vector<MarchArgs> bmarch;        // [b-1]
+  std::vector<ResidArgs> bresid;        // [b-1]
+};
+
+static void mixed_pass_ops(const FullCtx &c, const float *f32, const double *f64, long f_stride,
+                           double *x, long x_stride, int nrhs, MixedPassOps &o) {
   // FP64 MFMA products (kpart64) take the FP64 error rows e and FP64 lane
   // tiles; FP32 ones the FP32 rows e32 and FP32 tiles
   const bool dbl = c.kpart64 != nullptr;
+  o.dbl = dbl;
   const long ww = lane_tiles_floats(c.i2, c.i2);  // one matrix in lane tiles (elements)
   const int np = (c.i2 + 15) / 16 * 16;           // W's columns start at np in [RINV | W]
   int ks = 0;
@@ -379,35 +399,26 @@ static int run_mixed_pass(const FullCtx &c, const float *f32, const double *f64,
     return r;
   };
   // block nb's product: forward (RINV1_nb, N = I2) or backward ([RINV_nb | W_nb])
-  auto product64 = [&](bool fwd, int nb, int n_prod) -> int {
-    GemmPart64Args g{};
-    g.a = c.e;
-    g.b = fwd ? c.rinv1_64t + (long)(nb - 1) * ww : c.rinv64t + (long)(nb - 1) * ww;
-    g.b2 = (!fwd && nb != 1) ? c.w64t + (long)(nb - 2) * ww : nullptr;
-    g.nsplit = np;
-    g.M = nrhs; g.N = n_prod; g.K = c.i2;
-    g.part = c.kpart64;
-    g.tspan = c.prof ? c.prof->next_span(SEVP_KCLASS_CORR, gemm_part64_workgroups(g)) : nullptr;
-    const auto ev = c.prof ? c.prof->start_ext(SEVP_KCLASS_CORR)
-                           : std::pair<hipEvent_t, hipEvent_t>{nullptr, nullptr};
-    HIPC(launch_gemm_part64(g, st, ev.first, ev.second));
-    if (c.prof) c.prof->stop_ext(SEVP_KCLASS_CORR, ev.first != nullptr);
-    return SEVP_OK;
-  };
-  auto product = [&](const float *b, const float *b2, int n_prod) -> int {
-    GemmPartArgs g{};
-    g.a = c.e32;
-    g.b = b;
-    g.b2 = b2;
-    g.nsplit = np;
-    g.M = nrhs; g.N = n_prod; g.K = c.i2;
-    g.part = c.kpart;
-    g.tspan = c.prof ? c.prof->next_span(SEVP_KCLASS_CORR, gemm_part_workgroups(g)) : nullptr;
-    const auto ev = c.prof ? c.prof->start_ext(SEVP_KCLASS_CORR)
-                           : std::pair<hipEvent_t, hipEvent_t>{nullptr, nullptr};
-    HIPC(launch_gemm_part(g, st, ev.first, ev.second));
-    if (c.prof) c.prof->stop_ext(SEVP_KCLASS_CORR, ev.first != nullptr);
-    return SEVP_OK;
+  auto product = [&](bool fwd, int nb, int n_prod) {
+    if (dbl) {
+      GemmPart64Args g{};
+      g.a = c.e;
+      g.b = fwd ? c.rinv1_64t + (long)(nb - 1) * ww : c.rinv64t + (long)(nb - 1) * ww;
+      g.b2 = (!fwd && nb != 1) ? c.w64t + (long)(nb - 2) * ww : nullptr;
+      g.nsplit = np;
+      g.M = nrhs; g.N = n_prod; g.K = c.i2;
+      g.part = c.kpart64;
+      (fwd ? o.fprod64 : o.bprod64).push_back(g);
+    } else {
+      GemmPartArgs g{};
+      g.a = c.e32;
+      g.b = fwd ? c.rinv1_32 + (long)(nb - 1) * ww : c.rinv32 + (long)(nb - 1) * ww;
+      g.b2 = (!fwd && nb != 1) ? c.w32 + (long)(nb - 2) * ww : nullptr;
+      g.nsplit = np;
+      g.M = nrhs; g.N = n_prod; g.K = c.i2;
+      g.part = c.kpart;
+      (fwd ? o.fprod : o.bprod).push_back(g);
+    }
   };
   int js = 1;
   for (int nb = 1; nb <= c.nb0; ++nb) {
@@ -427,13 +438,8 @@ static int run_mixed_pass(const FullCtx &c, const float *f32, const double *f64,
     m.w = c.i2;
     m.map = ColMap{1, 0, 0};
     m.nrhs = nrhs;
-    if (c.prof) c.prof->start(SEVP_KCLASS_MARCH);
-    HIPC(launch_march(MODE_PART, m, nrhs, st));
-    if (c.prof) c.prof->stop(SEVP_KCLASS_MARCH);
-    if (nb != c.nb0) {
-      if (dbl) CHK(product64(true, nb, c.i2));
-      else CHK(product(c.rinv1_32 + (long)(nb - 1) * ww, nullptr, c.i2));
-    }
+    o.fmarch.push_back(m);
+    if (nb != c.nb0) product(true, nb, c.i2);
     js = ieb;
   }
   auto resid_args = [&](int nb) {
@@ -447,20 +453,18 @@ static int run_mixed_pass(const FullCtx &c, const float *f32, const double *f64,
     r.nrhs = nrhs;
     return r;
   };
-  {
-    const ResidArgs r = resid_args(c.nb0);
-    if (c.prof) c.prof->start(SEVP_KCLASS_RESID);
-    HIPC(launch_resid(r, st));
-    if (c.prof) c.prof->stop(SEVP_KCLASS_RESID);
-  }
-  for (int nb = c.nb0; nb >= 1; --nb) {
+  o.mid = resid_args(c.nb0);
+  o.bmarch.resize(c.nb0);
+  o.bresid.resize(c.nb0);
+  if (dbl) o.bprod64.reserve(c.nb0);
+  else o.bprod.reserve(c.nb0);
+  // backward products are listed by block (index b-1), built from b = 1 up
+  for (int nb = 1; nb <= c.nb0; ++nb) {
     const int ieb = c.ie[nb - 1];
     js = (nb != 1) ? c.ie[nb - 2] : 1;
     // [RINV_b | W_b]: free-row correction and, for b > 1, the coupling row
     const int n_prod = (nb != 1) ? 2 * np : c.i2;
-    if (dbl) CHK(product64(false, nb, n_prod));
-    else CHK(product(c.rinv32 + (long)(nb - 1) * ww, nb != 1 ? c.w32 + (long)(nb - 2) * ww : nullptr,
-                     n_prod));
+    product(false, nb, n_prod);
     MarchArgs m{};
     m.ax = c.ax; m.ay = c.ay; m.bb = c.bb; m.cx = c.cx; m.cy = c.cy;
     m.cld = c.i2;
@@ -476,8 +480,43 @@ static int run_mixed_pass(const FullCtx &c, const float *f32, const double *f64,
     rb.corr = part_of(n_prod);
     rb.apply_row = js + 1;
     rb.do_resid = (nb != 1);
+    o.bmarch[nb - 1] = m;
+    o.bresid[nb - 1] = rb;
+  }
+}
+
+static int run_mixed_pass(const FullCtx &c, const float *f32, const double *f64, long f_stride,
+                          double *x, long x_stride, int nrhs, hipStream_t st) {
+  MixedPassOps o;
+  mixed_pass_ops(c, f32, f64, f_stride, x, x_stride, nrhs, o);
+  auto product = [&](bool fwd, int nb) -> int {
+    const auto ev = c.prof ? c.prof->start_ext(SEVP_KCLASS_CORR)
+                           : std::pair<hipEvent_t, hipEvent_t>{nullptr, nullptr};
+    if (o.dbl) {
+      GemmPart64Args g = fwd ? o.fprod64[nb - 1] : o.bprod64[nb - 1];
+      g.tspan = c.prof ? c.prof->next_span(SEVP_KCLASS_CORR, gemm_part64_workgroups(g)) : nullptr;
+      HIPC(launch_gemm_part64(g, st, ev.first, ev.second));
+    } else {
+      GemmPartArgs g = fwd ? o.fprod[nb - 1] : o.bprod[nb - 1];
+      g.tspan = c.prof ? c.prof->next_span(SEVP_KCLASS_CORR, gemm_part_workgroups(g)) : nullptr;
+      HIPC(launch_gemm_part(g, st, ev.first, ev.second));
+    }
+    if (c.prof) c.prof->stop_ext(SEVP_KCLASS_CORR, ev.first != nullptr);
+    return SEVP_OK;
+  };
+  for (int nb = 1; nb <= c.nb0; ++nb) {
     if (c.prof) c.prof->start(SEVP_KCLASS_MARCH);
-    HIPC(launch_hom_resid(m, &rb, st));
+    HIPC(launch_march(MODE_PART, o.fmarch[nb - 1], nrhs, st));
+    if (c.prof) c.prof->stop(SEVP_KCLASS_MARCH);
+    if (nb != c.nb0) CHK(product(true, nb));
+  }
+  if (c.prof) c.prof->start(SEVP_KCLASS_RESID);
+  HIPC(launch_resid(o.mid, st));
+  if (c.prof) c.prof->stop(SEVP_KCLASS_RESID);
+  for (int nb = c.nb0; nb >= 1; --nb) {
+    CHK(product(false, nb));
+    if (c.prof) c.prof->start(SEVP_KCLASS_MARCH);
+    HIPC(launch_hom_resid(o.bmarch[nb - 1], &o.bresid[nb - 1], st));
     if (c.prof) c.prof->stop(SEVP_KCLASS_MARCH);
   }
   if (c.periodic) HIPC(launch_halo_fix(x, x_stride, c.i2, c.j2, nrhs, st));
@@ -1368,6 +1407,439 @@ int sevp_solve_refined_dev(sevp_plan p, const float *s, double *x, int nrhs, dou
     stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
   if (worst > tol) return fail(SEVP_E_NOTCONV, "refinement did not reach the tolerance");
+  return SEVP_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ batched problems
+// Many independent full-width problems (regional basins, ensemble members on
+// different operators) refined together.  Every step of the mixed pass is one
+// grouped launch over all problems: forward step t marches block t of every
+// problem with at least t blocks, then forms their products; backward step t
+// takes block NB0 - t + 1 of each.  A problem's entries carry exactly the
+// arguments of its own launches (mixed_pass_ops), so its results are bitwise
+// those of sevp_solve_refined_dev on its plan.  The pass is one captured
+// graph of 4 * max(NB0) + 2 launches; each refinement step reads every
+// problem's residual norms with one host synchronisation, and a problem
+// that has converged keeps its X (its correction is computed and not added).
+enum { BL_FILL, BL_MARCH, BL_PROD, BL_PROD64, BL_RESID, BL_HOM, BL_HALO };
+struct BatchLaunch {
+  int kind;
+  size_t tab, start;  // byte offsets of the entry table and workgroup starts
+  int n;
+  long nwg;
+};
+
+struct sevp_batch_s {
+  std::vector<sevp_plan> plans;
+  hipStream_t stream = nullptr, cap = nullptr;
+  int ks = 0;                 // the product slice length every problem uses
+  std::vector<int> key;       // nrhs per problem of the captured pass
+  int kmarch = 1, mb = 1;     // lane width of the grouped marches, 16-row blocks of products
+  std::vector<BatchLaunch> launches;
+  DBuf<unsigned char> tables;
+  hipGraphExec_t exec = nullptr;
+  DBuf<double> norm_part;     // residual-norm partials, NCHUNK * 2 per RHS per problem
+  DBuf<unsigned char> xtab;   // per call: X += DX entries (all, then periodic) and starts
+  DBuf<int> act;              // per problem: still refining
+  int *act_h = nullptr;       // pinned
+  double *norm_h = nullptr;   // pinned
+  size_t norm_h_n = 0;
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+    key.clear();
+  }
+  ~sevp_batch_s() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    drop_graph();
+    if (act_h) (void)hipHostFree(act_h);
+    if (norm_h) (void)hipHostFree(norm_h);
+    if (cap) (void)hipStreamDestroy(cap);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+// Host image of the batch's tables: entry arrays and their starts, each at a
+// 256-byte boundary.
+struct TableImage {
+  std::vector<unsigned char> bytes;
+  template <class T>
+  size_t put(const T *v, size_t n) {
+    const size_t off = (bytes.size() + 255) / 256 * 256;
+    bytes.resize(off + n * sizeof(T));
+    std::memcpy(bytes.data() + off, v, n * sizeof(T));
+    return off;
+  }
+};
+
+template <class T>
+static void add_launch(TableImage &img, std::vector<BatchLaunch> &out, int kind,
+                       const std::vector<T> &ents, const std::vector<long> &nwg) {
+  if (ents.empty()) return;
+  std::vector<int> start(ents.size());
+  long total = 0;
+  for (size_t i = 0; i < ents.size(); ++i) {
+    start[i] = (int)total;
+    total += nwg[i];
+  }
+  BatchLaunch l;
+  l.kind = kind;
+  l.tab = img.put(ents.data(), ents.size());
+  l.start = img.put(start.data(), start.size());
+  l.n = (int)ents.size();
+  l.nwg = total;
+  out.push_back(l);
+}
+
+static long cdiv(long a, long b) { return (a + b - 1) / b; }
+
+static int batch_launch(sevp_batch b, const BatchLaunch &l, hipStream_t st) {
+  const unsigned char *base = b->tables.p;
+  const int *start = reinterpret_cast<const int *>(base + l.start);
+  switch (l.kind) {
+    case BL_FILL:
+      HIPC(launch_batch_fill(reinterpret_cast<const FillEnt *>(base + l.tab), start, l.n, l.nwg, st));
+      break;
+    case BL_MARCH:
+      HIPC(launch_batch_march(b->kmarch, reinterpret_cast<const MarchEnt *>(base + l.tab), start, l.n,
+                              l.nwg, st));
+      break;
+    case BL_PROD:
+      HIPC(launch_batch_gemm_part(b->ks, b->mb, reinterpret_cast<const GemmPartArgs *>(base + l.tab),
+                                  start, l.n, l.nwg, st));
+      break;
+    case BL_PROD64:
+      HIPC(launch_batch_gemm_part64(b->ks, b->mb, reinterpret_cast<const GemmPart64Args *>(base + l.tab),
+                                    start, l.n, l.nwg, st));
+      break;
+    case BL_RESID:
+      HIPC(launch_batch_resid(reinterpret_cast<const ResidArgs *>(base + l.tab), start, l.n, l.nwg, st));
+      break;
+    case BL_HOM:
+      HIPC(launch_batch_hom_resid(b->kmarch, reinterpret_cast<const HomResidEnt *>(base + l.tab), start,
+                                  l.n, l.nwg, st));
+      break;
+    case BL_HALO:
+      HIPC(launch_batch_axpy(reinterpret_cast<const AxpyEnt *>(base + l.tab), start, l.n, l.nwg,
+                             nullptr, true, st));
+      break;
+    default:
+      return fail(SEVP_E_STATE, "unknown batch launch");
+  }
+  return SEVP_OK;
+}
+
+// The grouped launches of one pass for nrhs[i] right-hand sides of problem i,
+// and the graph that runs them.
+static int batch_build(sevp_batch b, const int *nrhs) {
+  b->drop_graph();
+  b->launches.clear();
+  const int np = (int)b->plans.size();
+  std::vector<MixedPassOps> ops(np);
+  int maxnb = 0, maxsteps = 0, maxm = 1;
+  bool dbl = false;
+  for (int i = 0; i < np; ++i) {
+    sevp_plan p = b->plans[i];
+    const long ns = (long)p->d.i2 * p->d.j2, nx = (long)(p->d.i2 + 2) * (p->d.j2 + 2);
+    mixed_pass_ops(full_ctx(p), nullptr, p->r64.p, ns, p->dx.p, nx, nrhs[i], ops[i]);
+    dbl = ops[i].dbl;
+    maxnb = std::max(maxnb, p->d.nb0);
+    maxm = std::max(maxm, nrhs[i]);
+    for (const auto &m : ops[i].fmarch) maxsteps = std::max(maxsteps, m.jf - m.js + 1);
+    for (const auto &m : ops[i].bmarch) maxsteps = std::max(maxsteps, m.jf - m.js + 1);
+  }
+  // the narrowest lane width whose tile is positive for every block
+  const int halo = std::max(maxsteps, 0) + 1;
+  b->kmarch = (64 - 2 * halo > 0) ? 1 : (128 - 2 * halo > 0) ? 2 : (256 - 2 * halo > 0) ? 4 : -1;
+  if (b->kmarch < 0) return fail(SEVP_E_ARG, "a block is too tall for the grouped march (more than 126 rows)");
+  b->mb = maxm <= 16 ? 1 : maxm <= 32 ? 2 : 4;
+  TableImage img;
+  {
+    std::vector<FillEnt> f;
+    std::vector<long> w;
+    for (int i = 0; i < np; ++i) {
+      sevp_plan p = b->plans[i];
+      const long nx = (long)(p->d.i2 + 2) * (p->d.j2 + 2);
+      f.push_back(FillEnt{reinterpret_cast<unsigned int *>(p->dx.p), 2 * nx * nrhs[i]});
+      w.push_back(cdiv(f.back().n, 256));
+    }
+    add_launch(img, b->launches, BL_FILL, f, w);
+  }
+  for (int t = 1; t <= maxnb; ++t) {
+    std::vector<MarchEnt> me;
+    std::vector<long> mw;
+    std::vector<GemmPartArgs> g32;
+    std::vector<GemmPart64Args> g64;
+    std::vector<long> gw;
+    for (int i = 0; i < np; ++i) {
+      const int nb0 = b->plans[i]->d.nb0;
+      if (t > nb0) continue;
+      MarchEnt e;
+      e.m = ops[i].fmarch[t - 1];
+      e.ntiles = march_tiles(e.m, b->kmarch);
+      if (e.ntiles <= 0) return fail(SEVP_E_STATE, "grouped march geometry");
+      me.push_back(e);
+      mw.push_back((long)e.ntiles * nrhs[i]);
+      if (t == nb0) continue;
+      if (dbl) {
+        g64.push_back(ops[i].fprod64[t - 1]);
+        gw.push_back(gemm_part64_workgroups(g64.back()));
+      } else {
+        g32.push_back(ops[i].fprod[t - 1]);
+        gw.push_back(gemm_part_workgroups(g32.back()));
+      }
+    }
+    add_launch(img, b->launches, BL_MARCH, me, mw);
+    if (dbl) add_launch(img, b->launches, BL_PROD64, g64, gw);
+    else add_launch(img, b->launches, BL_PROD, g32, gw);
+  }
+  {
+    std::vector<ResidArgs> r;
+    std::vector<long> w;
+    for (int i = 0; i < np; ++i) {
+      r.push_back(ops[i].mid);
+      w.push_back(cdiv((long)nrhs[i] * b->plans[i]->d.i2, 256));
+    }
+    add_launch(img, b->launches, BL_RESID, r, w);
+  }
+  for (int t = 1; t <= maxnb; ++t) {
+    std::vector<HomResidEnt> he;
+    std::vector<long> hw;
+    std::vector<GemmPartArgs> g32;
+    std::vector<GemmPart64Args> g64;
+    std::vector<long> gw;
+    for (int i = 0; i < np; ++i) {
+      const int nb0 = b->plans[i]->d.nb0;
+      if (t > nb0) continue;
+      const int nb = nb0 - t + 1;
+      if (dbl) {
+        g64.push_back(ops[i].bprod64[nb - 1]);
+        gw.push_back(gemm_part64_workgroups(g64.back()));
+      } else {
+        g32.push_back(ops[i].bprod[nb - 1]);
+        gw.push_back(gemm_part_workgroups(g32.back()));
+      }
+      HomResidEnt e;
+      const long n = hom_resid_prepare(ops[i].bmarch[nb - 1], &ops[i].bresid[nb - 1], b->kmarch, e);
+      if (n < 0) return fail(SEVP_E_STATE, "grouped march geometry");
+      if (n == 0) continue;
+      he.push_back(e);
+      hw.push_back(n);
+    }
+    if (dbl) add_launch(img, b->launches, BL_PROD64, g64, gw);
+    else add_launch(img, b->launches, BL_PROD, g32, gw);
+    add_launch(img, b->launches, BL_HOM, he, hw);
+  }
+  {
+    std::vector<AxpyEnt> h;
+    std::vector<long> w;
+    for (int i = 0; i < np; ++i) {
+      sevp_plan p = b->plans[i];
+      if (p->d.form != SEVP_FORM_FULL_PERIODIC) continue;
+      const long nx = (long)(p->d.i2 + 2) * (p->d.j2 + 2);
+      h.push_back(AxpyEnt{p->dx.p, nx, nullptr, 0, p->d.i2, p->d.j2, nrhs[i], i});
+      w.push_back(cdiv((long)nrhs[i] * p->d.j2, 256));
+    }
+    add_launch(img, b->launches, BL_HALO, h, w);
+  }
+  CHK(b->tables.alloc(img.bytes.size()));
+  HIPC(hipMemcpy(b->tables.p, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice));
+  // capture the pass
+  HIPC(hipStreamSynchronize(b->stream));
+  HIPC(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+  int rc = SEVP_OK;
+  for (const auto &l : b->launches)
+    if ((rc = batch_launch(b, l, b->cap)) != SEVP_OK) break;
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(b->cap, &graph);
+  if (rc != SEVP_OK || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc != SEVP_OK) return rc;
+    HIPC(e);
+  }
+  const hipError_t ei = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HIPC(ei);
+  b->key.assign(nrhs, nrhs + np);
+  return SEVP_OK;
+}
+
+extern "C" {
+
+int sevp_batch_create(const sevp_plan *plans, int nplans, sevp_batch *out) {
+  if (!plans || !out || nplans < 1) return fail(SEVP_E_ARG, "null argument or no plans");
+  *out = nullptr;
+  int ks = 0, prec = -1;
+  for (int i = 0; i < nplans; ++i) {
+    sevp_plan p = plans[i];
+    if (!p) return fail(SEVP_E_ARG, "null plan");
+    if (p->d.form == SEVP_FORM_BIR) return fail(SEVP_E_ARG, "batched solves need full-width forms");
+    if (p->d.precision != SEVP_PREC_FP32 && p->d.precision != SEVP_PREC_FP64_MFMA)
+      return fail(SEVP_E_ARG, "batched solves need FP32 or FP64_MFMA corrections");
+    if (prec >= 0 && p->d.precision != prec) return fail(SEVP_E_ARG, "plans of a batch must share the precision");
+    prec = p->d.precision;
+    if (!p->have_inv) return fail(SEVP_E_STATE, "influence matrices not built or loaded");
+    if (p->d.i2 % 4 != 0) return fail(SEVP_E_ARG, "batched solves need I2 divisible by 4");
+    int k = 0;
+    gemm_part_slices(p->d.i2, p->d.i2, &k);
+    if (ks && k != ks)
+      return fail(SEVP_E_ARG, "plans of a batch must share the product slice length "
+                              "(all I2 <= 1536 or all I2 > 1536)");
+    ks = k;
+    for (int j = 0; j < i; ++j)
+      if (plans[j] == p) return fail(SEVP_E_ARG, "a plan appears twice in the batch");
+  }
+  auto *b = new sevp_batch_s();
+  b->plans.assign(plans, plans + nplans);
+  b->ks = ks;
+  auto bail = [&](int code) {
+    delete b;
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamDefault) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(SEVP_E_HIP, "hipStreamCreate failed"));
+  if (int rc = b->act.alloc(nplans)) return bail(rc);
+  if (hipHostMalloc((void **)&b->act_h, sizeof(int) * nplans, hipHostMallocDefault) != hipSuccess) {
+    b->act_h = nullptr;
+    return bail(fail(SEVP_E_NOMEM, "hipHostMalloc failed"));
+  }
+  *out = b;
+  return SEVP_OK;
+}
+
+int sevp_batch_destroy(sevp_batch b) {
+  delete b;
+  return SEVP_OK;
+}
+
+int sevp_batch_solve_refined_dev(sevp_batch b, const float *const *s, double *const *x,
+                                 const int *nrhs, double tol, int max_iter, double *rel_out,
+                                 int *passes, sevp_stats *stats) {
+  if (!b || !s || !x || !nrhs) return fail(SEVP_E_ARG, "null argument");
+  const int np = (int)b->plans.size();
+  size_t total_rhs = 0;
+  for (int i = 0; i < np; ++i) {
+    sevp_plan p = b->plans[i];
+    if (!s[i] || !x[i]) return fail(SEVP_E_ARG, "null right-hand side or solution array");
+    if (nrhs[i] < 1 || nrhs[i] > p->d.nrhs_max || nrhs[i] > 64) return fail(SEVP_E_ARG, "nrhs out of range");
+    if (!p->have_inv) return fail(SEVP_E_STATE, "influence matrices not built or loaded");
+    if (p->d.precision != b->plans[0]->d.precision) return fail(SEVP_E_STATE, "plan precision changed");
+    total_rhs += nrhs[i];
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  // the plans' own queued work (their scratch arrays are shared with the batch)
+  for (sevp_plan p : b->plans) {
+    HIPC(hipStreamSynchronize(p->stream));
+    for (auto st : p->gstreams) HIPC(hipStreamSynchronize(st));
+  }
+  if (b->key.size() != (size_t)np || !std::equal(b->key.begin(), b->key.end(), nrhs)) CHK(batch_build(b, nrhs));
+  // per call: X += DX entries of every problem, then the periodic ones
+  std::vector<AxpyEnt> ax, hx;
+  std::vector<int> as, hs;
+  long anwg = 0, hnwg = 0;
+  for (int i = 0; i < np; ++i) {
+    sevp_plan p = b->plans[i];
+    const long nx = (long)(p->d.i2 + 2) * (p->d.j2 + 2);
+    const AxpyEnt e{x[i], nx, p->dx.p, nx, p->d.i2, p->d.j2, nrhs[i], i};
+    ax.push_back(e);
+    as.push_back((int)anwg);
+    anwg += cdiv((long)nrhs[i] * p->d.i2 * p->d.j2, 256);
+    if (p->d.form == SEVP_FORM_FULL_PERIODIC) {
+      hx.push_back(e);
+      hs.push_back((int)hnwg);
+      hnwg += cdiv((long)nrhs[i] * p->d.j2, 256);
+    }
+  }
+  if (anwg > 0x7fffffffL) return fail(SEVP_E_ARG, "batch too large");
+  TableImage img;
+  const size_t o_ax = img.put(ax.data(), ax.size()), o_as = img.put(as.data(), as.size());
+  size_t o_hx = 0, o_hs = 0;
+  if (!hx.empty()) {
+    o_hx = img.put(hx.data(), hx.size());
+    o_hs = img.put(hs.data(), hs.size());
+  }
+  if (b->xtab.n < img.bytes.size()) CHK(b->xtab.alloc(img.bytes.size()));
+  HIPC(hipMemcpy(b->xtab.p, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice));
+  const size_t nn = total_rhs * NCHUNK * 2;
+  if (b->norm_part.n < nn) CHK(b->norm_part.alloc(nn));
+  if (b->norm_h_n < nn) {
+    if (b->norm_h) (void)hipHostFree(b->norm_h);
+    b->norm_h = nullptr;
+    b->norm_h_n = 0;
+    if (hipHostMalloc((void **)&b->norm_h, nn * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+      b->norm_h = nullptr;
+      return fail(SEVP_E_NOMEM, "hipHostMalloc failed");
+    }
+    b->norm_h_n = nn;
+  }
+  std::vector<int> it(np, 0);
+  std::vector<double> rel(total_rhs, 0.0), worst(np, 0.0);
+  for (;;) {
+    // every problem's residual and its norms (sevp_solve_refined_dev's
+    // kernel and summation order), one synchronisation
+    size_t off = 0;
+    for (int i = 0; i < np; ++i) {
+      sevp_plan p = b->plans[i];
+      const int i2 = p->d.i2, j2 = p->d.j2;
+      const long ns = (long)i2 * j2, nx = (long)(i2 + 2) * (j2 + 2);
+      hipLaunchKernelGGL(resid_norm_kernel, dim3(NCHUNK, nrhs[i]), dim3(256), 0, b->stream, p->al(),
+                         p->ab(), p->ac(), p->ar(), p->at(), s[i], nullptr, ns, x[i], nx, p->r64.p,
+                         i2, j2, NCHUNK, b->norm_part.p + off);
+      HIPC(hipGetLastError());
+      off += (size_t)nrhs[i] * NCHUNK * 2;
+    }
+    HIPC(hipMemcpyAsync(b->norm_h, b->norm_part.p, nn * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIPC(hipStreamSynchronize(b->stream));
+    bool any = false;
+    size_t r0 = 0;
+    off = 0;
+    for (int i = 0; i < np; ++i) {
+      worst[i] = 0.0;
+      for (int r = 0; r < nrhs[i]; ++r) {
+        double a = 0, c = 0;
+        for (int k = 0; k < NCHUNK; ++k) {
+          a += b->norm_h[off + ((size_t)r * NCHUNK + k) * 2];
+          c += b->norm_h[off + ((size_t)r * NCHUNK + k) * 2 + 1];
+        }
+        rel[r0 + r] = c > 0 ? std::sqrt(a / c) : std::sqrt(a);
+        worst[i] = std::max(worst[i], rel[r0 + r]);
+      }
+      r0 += nrhs[i];
+      off += (size_t)nrhs[i] * NCHUNK * 2;
+      b->act_h[i] = (worst[i] > tol && it[i] < max_iter) ? 1 : 0;
+      any = any || b->act_h[i];
+    }
+    if (!any) break;
+    HIPC(hipMemcpyAsync(b->act.p, b->act_h, sizeof(int) * np, hipMemcpyHostToDevice, b->stream));
+    HIPC(hipGraphLaunch(b->exec, b->stream));
+    const int *act = b->act.p;
+    HIPC(launch_batch_axpy(reinterpret_cast<const AxpyEnt *>(b->xtab.p + o_ax),
+                           reinterpret_cast<const int *>(b->xtab.p + o_as), (int)ax.size(), anwg, act,
+                           false, b->stream));
+    if (!hx.empty())
+      HIPC(launch_batch_axpy(reinterpret_cast<const AxpyEnt *>(b->xtab.p + o_hx),
+                             reinterpret_cast<const int *>(b->xtab.p + o_hs), (int)hx.size(), hnwg,
+                             act, true, b->stream));
+    for (int i = 0; i < np; ++i) it[i] += b->act_h[i];
+  }
+  double w = 0.0;
+  int itmax = 0;
+  for (int i = 0; i < np; ++i) {
+    w = std::max(w, worst[i]);
+    itmax = std::max(itmax, it[i]);
+    if (passes) passes[i] = it[i];
+  }
+  if (rel_out) std::memcpy(rel_out, rel.data(), sizeof(double) * total_rhs);
+  if (stats) {
+    stats->iterations = itmax;
+    stats->max_rel_resid = w;
+    stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  for (int i = 0; i < np; ++i)
+    if (worst[i] > tol) return fail(SEVP_E_NOTCONV, "refinement did not reach the tolerance");
   return SEVP_OK;
 }
 

@@ -56,3 +56,24 @@ __device__ __forceinline__ void span_end(unsigned long long *t) {
   __syncthreads();
   if (threadIdx.x == 0) t[2 * span_wg() + 1] = __builtin_amdgcn_s_memrealtime();
 }
+
+// Grouped launches: one launch carries the same step of many independent
+// problems.  Entry e of a launch owns workgroups start[e] .. start[e+1]-1
+// (start[0] = 0, increasing); a workgroup's entry is the number of starts at
+// or below its index, minus one, counted by wavefront ballots 64 starts at a
+// time (one round of vector loads for up to 256 entries, then bit counts).
+__device__ __forceinline__ int group_entry(const int *start, int n, int b) {
+  const int lane = threadIdx.x & 63;
+  int cnt = 0;
+  for (int base = 0; base < n; base += 256) {
+    int v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = base + 64 * u + lane;
+      v[u] = i < n ? start[i] : 0x7fffffff;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) cnt += __popcll(__ballot(v[u] <= b));
+  }
+  return cnt - 1;
+}

@@ -352,9 +352,8 @@ __global__ void resid_kernel(ResidArgs p) {
 // block b-1 touch disjoint rows and read nothing the other writes: one
 // launch runs both (march tiles first, then 64-column residual groups).
 template <int K>
-__global__ __launch_bounds__(64) void hom_resid_kernel(MarchArgs m, ResidArgs r, int ntiles,
-                                                       int has_resid) {
-  const long b = blockIdx.x;
+__device__ __forceinline__ void hom_resid_body(const MarchArgs &m, const ResidArgs &r, int ntiles,
+                                               int has_resid, long b) {
   const long nm = (long)ntiles * m.nrhs;
   if (b < nm) {
     march_body<K, MODE_HOM>(m, (int)(b % ntiles), (int)(b / ntiles), 0, 1);
@@ -365,25 +364,58 @@ __global__ __launch_bounds__(64) void hom_resid_kernel(MarchArgs m, ResidArgs r,
   else resid_body(r, (b - nm) * 64 + threadIdx.x);
 }
 
-hipError_t launch_hom_resid(const MarchArgs &a0, const ResidArgs *r, hipStream_t st) {
-  MarchArgs a = a0;
-  int K, tile, halo, ntiles;
-  march_shape(a.w, a.jf - a.js + 1, K, tile, halo, ntiles);
-  if (K < 0) return hipErrorInvalidValue;
+template <int K>
+__global__ __launch_bounds__(64) void hom_resid_kernel(MarchArgs m, ResidArgs r, int ntiles,
+                                                       int has_resid) {
+  hom_resid_body<K>(m, r, ntiles, has_resid, blockIdx.x);
+}
+
+int march_tiles(MarchArgs &a, int K) {
+  const int steps = a.jf - a.js + 1;
+  int k, tile, halo, ntiles;
+  march_shape(a.w, steps, k, tile, halo, ntiles);
+  if (k < 0) return 0;
+  if (K > 0) {  // a given lane width (any width whose tile is positive gives the same values)
+    tile = 64 * K - 2 * halo;
+    if (tile <= 0) return 0;
+    ntiles = (a.w + tile - 1) / tile;
+  }
   a.tile_out = tile;
   a.halo = halo;
-  const long nm = (long)ntiles * a.nrhs;
+  return ntiles;
+}
+
+int march_lanes(int w, int steps) {
+  int k, tile, halo, ntiles;
+  march_shape(w, steps, k, tile, halo, ntiles);
+  return k;
+}
+
+long hom_resid_prepare(const MarchArgs &a0, const ResidArgs *r, int K, HomResidEnt &ent) {
+  ent.m = a0;
+  const int ntiles = march_tiles(ent.m, K);
+  if (ntiles <= 0) return -1;
+  ent.ntiles = ntiles;
+  const long nm = (long)ntiles * a0.nrhs;
   const long nr = r ? ((long)r->nrhs * r->i2 + 63) / 64 : 0;
-  const bool march = a.jf >= a.js;
-  const long nmb = march ? nm : 0;
-  ResidArgs rr = r ? *r : ResidArgs{};
-  dim3 grid((unsigned)(nmb + nr));
-  if (nmb + nr == 0) return hipSuccess;
-  MarchArgs am = a;
-  if (!march) am.nrhs = 0;  // no march blocks: every block is a residual group
-  if (K == 1) hipLaunchKernelGGL((hom_resid_kernel<1>), grid, dim3(64), 0, st, am, rr, ntiles, r != nullptr);
-  else if (K == 2) hipLaunchKernelGGL((hom_resid_kernel<2>), grid, dim3(64), 0, st, am, rr, ntiles, r != nullptr);
-  else hipLaunchKernelGGL((hom_resid_kernel<4>), grid, dim3(64), 0, st, am, rr, ntiles, r != nullptr);
+  const bool march = a0.jf >= a0.js;
+  ent.r = r ? *r : ResidArgs{};
+  ent.has_resid = r != nullptr;
+  if (!march) ent.m.nrhs = 0;  // no march blocks: every block is a residual group
+  return (march ? nm : 0) + nr;
+}
+
+hipError_t launch_hom_resid(const MarchArgs &a0, const ResidArgs *r, hipStream_t st) {
+  const int K = march_lanes(a0.w, a0.jf - a0.js + 1);
+  if (K < 0) return hipErrorInvalidValue;
+  HomResidEnt e;
+  const long nwg = hom_resid_prepare(a0, r, K, e);
+  if (nwg < 0) return hipErrorInvalidValue;
+  if (nwg == 0) return hipSuccess;
+  const dim3 grid((unsigned)nwg);
+  if (K == 1) hipLaunchKernelGGL((hom_resid_kernel<1>), grid, dim3(64), 0, st, e.m, e.r, e.ntiles, e.has_resid);
+  else if (K == 2) hipLaunchKernelGGL((hom_resid_kernel<2>), grid, dim3(64), 0, st, e.m, e.r, e.ntiles, e.has_resid);
+  else hipLaunchKernelGGL((hom_resid_kernel<4>), grid, dim3(64), 0, st, e.m, e.r, e.ntiles, e.has_resid);
   return hipGetLastError();
 }
 
@@ -651,6 +683,126 @@ __global__ void halo_fix_kernel(double *x, long xs, int i2, int j2, int nrhs) {
 hipError_t launch_halo_fix(double *x, long xs, int i2, int j2, int nrhs, hipStream_t st) {
   const long n = (long)nrhs * j2;
   hipLaunchKernelGGL(halo_fix_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, xs, i2, j2, nrhs);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Grouped launches of the mixed pass: one launch runs the same step of many
+// independent problems (group_entry finds a workgroup's problem), each entry
+// with exactly the workgroups and arguments of its own launch, so every value
+// written is the one the problem's own launch writes.
+template <int K>
+__global__ __launch_bounds__(64) void batch_march_kernel(const MarchEnt *__restrict__ tab,
+                                                         const int *__restrict__ start, int n) {
+  const int b = blockIdx.x;
+  const int e = group_entry(start, n, b);
+  const MarchEnt &t = tab[e];
+  const int l = b - start[e];
+  march_body<K, MODE_PART>(t.m, l % t.ntiles, l / t.ntiles, 0, 1);
+}
+
+template <int K>
+__global__ __launch_bounds__(64) void batch_hom_resid_kernel(const HomResidEnt *__restrict__ tab,
+                                                             const int *__restrict__ start, int n) {
+  const int b = blockIdx.x;
+  const int e = group_entry(start, n, b);
+  const HomResidEnt &t = tab[e];
+  hom_resid_body<K>(t.m, t.r, t.ntiles, t.has_resid, b - start[e]);
+}
+
+__global__ __launch_bounds__(256) void batch_resid_kernel(const ResidArgs *__restrict__ tab,
+                                                          const int *__restrict__ start, int n) {
+  const int b = blockIdx.x;
+  const int e = group_entry(start, n, b);
+  resid_body(tab[e], (long)(b - start[e]) * 256 + threadIdx.x);
+}
+
+// Zero fill of n 32-bit words per entry (the correction arrays of a pass).
+__global__ __launch_bounds__(256) void batch_fill_kernel(const FillEnt *__restrict__ tab,
+                                                         const int *__restrict__ start, int n) {
+  const int b = blockIdx.x;
+  const int e = group_entry(start, n, b);
+  const long t = (long)(b - start[e]) * 256 + threadIdx.x;
+  if (t < tab[e].n) tab[e].p[t] = 0u;
+}
+
+// X += D over the interior of entries whose problem is still refining
+// (act[problem] != 0, or every entry when act is null), then, in a second
+// launch, the periodic halo columns.
+__global__ __launch_bounds__(256) void batch_axpy_kernel(const AxpyEnt *__restrict__ tab,
+                                                         const int *__restrict__ start, int n,
+                                                         const int *__restrict__ act) {
+  const int b = blockIdx.x;
+  const int e = group_entry(start, n, b);
+  const AxpyEnt &a = tab[e];
+  if (act && !act[a.problem]) return;
+  const long t = (long)(b - start[e]) * 256 + threadIdx.x;
+  const long np = (long)a.i2 * a.j2;
+  if (t >= np * a.nrhs) return;
+  const int r = (int)(t / np);
+  const long q = t % np;
+  const int i = (int)(q % a.i2) + 2, j = (int)(q / a.i2) + 2;
+  const long o = f2(a.i2 + 2, i, j);
+  a.x[(long)r * a.x_stride + o] += a.d[(long)r * a.d_stride + o];
+}
+
+__global__ __launch_bounds__(256) void batch_halo_kernel(const AxpyEnt *__restrict__ tab,
+                                                         const int *__restrict__ start, int n,
+                                                         const int *__restrict__ act) {
+  const int b = blockIdx.x;
+  const int e = group_entry(start, n, b);
+  const AxpyEnt &a = tab[e];
+  if (act && !act[a.problem]) return;
+  const long t = (long)(b - start[e]) * 256 + threadIdx.x;
+  if (t >= (long)a.nrhs * a.j2) return;
+  const int r = (int)(t / a.j2), j = (int)(t % a.j2) + 2;
+  const int i0 = a.i2 + 2;
+  double *xr = a.x + (long)r * a.x_stride;
+  xr[f2(i0, 1, j)] = xr[f2(i0, i0 - 1, j)];
+  xr[f2(i0, i0, j)] = xr[f2(i0, 2, j)];
+}
+
+static bool batch_grid_ok(int n, long nwg) { return n >= 1 && nwg >= 1 && nwg <= 0x7fffffffL; }
+
+hipError_t launch_batch_march(int K, const MarchEnt *tab, const int *start, int n, long nwg,
+                              hipStream_t st) {
+  if (!batch_grid_ok(n, nwg)) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)nwg);
+  if (K == 1) hipLaunchKernelGGL((batch_march_kernel<1>), grid, dim3(64), 0, st, tab, start, n);
+  else if (K == 2) hipLaunchKernelGGL((batch_march_kernel<2>), grid, dim3(64), 0, st, tab, start, n);
+  else if (K == 4) hipLaunchKernelGGL((batch_march_kernel<4>), grid, dim3(64), 0, st, tab, start, n);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_hom_resid(int K, const HomResidEnt *tab, const int *start, int n, long nwg,
+                                  hipStream_t st) {
+  if (!batch_grid_ok(n, nwg)) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)nwg);
+  if (K == 1) hipLaunchKernelGGL((batch_hom_resid_kernel<1>), grid, dim3(64), 0, st, tab, start, n);
+  else if (K == 2) hipLaunchKernelGGL((batch_hom_resid_kernel<2>), grid, dim3(64), 0, st, tab, start, n);
+  else if (K == 4) hipLaunchKernelGGL((batch_hom_resid_kernel<4>), grid, dim3(64), 0, st, tab, start, n);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_resid(const ResidArgs *tab, const int *start, int n, long nwg, hipStream_t st) {
+  if (!batch_grid_ok(n, nwg)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(batch_resid_kernel, dim3((unsigned)nwg), dim3(256), 0, st, tab, start, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_fill(const FillEnt *tab, const int *start, int n, long nwg, hipStream_t st) {
+  if (!batch_grid_ok(n, nwg)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(batch_fill_kernel, dim3((unsigned)nwg), dim3(256), 0, st, tab, start, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_axpy(const AxpyEnt *tab, const int *start, int n, long nwg, const int *act,
+                             bool halo, hipStream_t st) {
+  if (!batch_grid_ok(n, nwg)) return hipErrorInvalidValue;
+  if (halo) hipLaunchKernelGGL(batch_halo_kernel, dim3((unsigned)nwg), dim3(256), 0, st, tab, start, n, act);
+  else hipLaunchKernelGGL(batch_axpy_kernel, dim3((unsigned)nwg), dim3(256), 0, st, tab, start, n, act);
   return hipGetLastError();
 }
 

@@ -247,6 +247,54 @@ hipError_t launch_scale_cols(const double *r, const float *cy, int cyrow, int i2
                              hipStream_t st);
 hipError_t launch_halo_fix(double *x, long x_stride, int i2, int j2, int nrhs, hipStream_t st);
 
+// ------------------------------- grouped launches (many problems per launch)
+// A grouped launch takes a table of n entries and the first workgroup of
+// each (start[0] = 0, increasing, every entry at least one workgroup);
+// nwg is the total.  Each entry runs with the workgroups and arguments its
+// own launch would have.
+struct MarchEnt {     // forward march (MODE_PART): ntiles * m.nrhs workgroups of 64
+  MarchArgs m;
+  int ntiles;
+};
+struct HomResidEnt {  // backward march + residual: hom_resid_prepare's count, 64 each
+  MarchArgs m;
+  ResidArgs r;
+  int ntiles, has_resid;
+};
+struct FillEnt {      // zero n words: ceil(n / 256) workgroups of 256
+  unsigned int *p;
+  long n;
+};
+struct AxpyEnt {      // x += d over the interior (or the periodic halo fix), problem `problem`
+  double *x;
+  long x_stride;
+  const double *d;
+  long d_stride;
+  int i2, j2, nrhs, problem;
+};
+// Tiles of a march with K columns per lane (K <= 0: the launch's own choice);
+// sets tile_out and halo, 0 when K is too narrow for the block.
+int march_tiles(MarchArgs &a, int K);
+int march_lanes(int w, int steps);  // the single launch's K (-1: none fits)
+// Fills ent for the backward step (launch_hom_resid's arguments at lane
+// width K); returns the workgroup count (-1: K too narrow).
+long hom_resid_prepare(const MarchArgs &a, const ResidArgs *r, int K, HomResidEnt &ent);
+hipError_t launch_batch_march(int K, const MarchEnt *tab, const int *start, int n, long nwg,
+                              hipStream_t st);
+hipError_t launch_batch_hom_resid(int K, const HomResidEnt *tab, const int *start, int n, long nwg,
+                                  hipStream_t st);
+hipError_t launch_batch_resid(const ResidArgs *tab, const int *start, int n, long nwg, hipStream_t st);
+hipError_t launch_batch_fill(const FillEnt *tab, const int *start, int n, long nwg, hipStream_t st);
+// act[problem] != 0 selects the entries applied (act null: all); halo: the
+// periodic halo fix instead
+hipError_t launch_batch_axpy(const AxpyEnt *tab, const int *start, int n, long nwg, const int *act,
+                             bool halo, hipStream_t st);
+// ks: the shared slice length; mb: 16-row blocks covering the largest M (1, 2 or 4)
+hipError_t launch_batch_gemm_part(int ks, int mb, const GemmPartArgs *tab, const int *start, int n,
+                                  long nwg, hipStream_t st);
+hipError_t launch_batch_gemm_part64(int ks, int mb, const GemmPart64Args *tab, const int *start,
+                                    int n, long nwg, hipStream_t st);
+
 // ------------------------------------------------ Krylov solvers (sevp_krylov.hip)
 constexpr int KRY_MAXB = 8192;  // most workgroups of a Krylov kernel (partial sums per dot)
 

@@ -15,6 +15,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "sevp_dev.h"
 #include "sevp_internal.h"
@@ -53,7 +54,7 @@ __device__ __forceinline__ f32x4 load4m(const float *base, int k, int klen) {
 // MB (1, 2 or 4) 16-row accumulator blocks cover the right-hand sides; LDS
 // rows past M are zero.  Requires K % 4 == 0 (float4 runs of k are aligned).
 template <int KS, int MB>
-__global__ __launch_bounds__(64 * GP_WAVES) void gemm_part_kernel(GemmPartArgs p) {
+__device__ __forceinline__ void gemm_part_body(const GemmPartArgs &p, int bx, int s) {
   constexpr int LDA = KS + 4;
   constexpr int KQ = KS / 4;
   constexpr int NCH = KS / 16;
@@ -61,11 +62,10 @@ __global__ __launch_bounds__(64 * GP_WAVES) void gemm_part_kernel(GemmPartArgs p
   constexpr int ALD = (ROWS * KQ + 64 * GP_WAVES - 1) / (64 * GP_WAVES);
   __shared__ float sa[ROWS * LDA];
   span_begin(p.tspan);
-  const int s = blockIdx.y;
   const int k_lo = s * KS;
   const int klen = min(p.K - k_lo, KS);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int n0 = (blockIdx.x * GP_WAVES + wave) * 16;
+  const int n0 = (bx * GP_WAVES + wave) * 16;
   const int col = lane & 15, q = lane >> 4;
   const int n = n0 + col;
   const bool nok = n < p.N;
@@ -132,18 +132,28 @@ __global__ __launch_bounds__(64 * GP_WAVES) void gemm_part_kernel(GemmPartArgs p
   span_end(p.tspan);
 }
 
+template <int KS, int MB>
+__global__ __launch_bounds__(64 * GP_WAVES) void gemm_part_kernel(GemmPartArgs p) {
+  gemm_part_body<KS, MB>(p, blockIdx.x, blockIdx.y);
+}
+
 // Slice lengths with a compiled kernel, and the one used for a product: the
-// shortest (most workgroups) that leaves at most 16 partial products per
-// output, since every consumer reads all of them.  For 64 x 1440 x N this is
-// 96 (15 slices; 345 workgroups at N = 1440, 675 at N = 2880).
+// shortest (most workgroups) of at least 96 that leaves at most 16 partial
+// products per output, since every consumer reads all of them.  For
+// 64 x 1440 x N this is 96 (15 slices; 345 workgroups at N = 1440, 675 at
+// N = 2880).  Shorter slices give a narrow product more workgroups but
+// little work each: grouped launches over many basins (I2 <= 512) ran 17%
+// faster per pass with 96 than with 32, and a single product of that size
+// is latency-bound either way.
 static const int kGpSlices[] = {32, 48, 64, 96, 128};
 constexpr int GP_MAX_SLICES = 16;
+constexpr int GP_MIN_SLICE = 96;
 
 int gemm_part_slices(int N, int K, int *kslice) {
   (void)N;
   int ks = kGpSlices[4];
   for (int c : kGpSlices)
-    if ((K + c - 1) / c <= GP_MAX_SLICES) {
+    if (c >= GP_MIN_SLICE && (K + c - 1) / c <= GP_MAX_SLICES) {
       ks = c;
       break;
     }
@@ -206,7 +216,7 @@ __device__ __forceinline__ f64x4 load4d(const double *base, int k, int klen) {
 }
 
 template <int KS, int MB>
-__global__ __launch_bounds__(64 * GP_WAVES) void gemm_part64_kernel(GemmPart64Args p) {
+__device__ __forceinline__ void gemm_part64_body(const GemmPart64Args &p, int bx, int s) {
   constexpr int LDA = KS + 2;
   constexpr int KQ = KS / 4;
   constexpr int NCH = KS / 16;
@@ -214,11 +224,10 @@ __global__ __launch_bounds__(64 * GP_WAVES) void gemm_part64_kernel(GemmPart64Ar
   constexpr int ALD = (ROWS * KQ + 64 * GP_WAVES - 1) / (64 * GP_WAVES);
   __shared__ double sa[ROWS * LDA];
   span_begin(p.tspan);
-  const int s = blockIdx.y;
   const int k_lo = s * KS;
   const int klen = min(p.K - k_lo, KS);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int n0 = (blockIdx.x * GP_WAVES + wave) * 16;
+  const int n0 = (bx * GP_WAVES + wave) * 16;
   const int col = lane & 15, q = lane >> 4;
   const int n = n0 + col;
   const bool nok = n < p.N;
@@ -280,6 +289,11 @@ __global__ __launch_bounds__(64 * GP_WAVES) void gemm_part64_kernel(GemmPart64Ar
       }
   }
   span_end(p.tspan);
+}
+
+template <int KS, int MB>
+__global__ __launch_bounds__(64 * GP_WAVES) void gemm_part64_kernel(GemmPart64Args p) {
+  gemm_part64_body<KS, MB>(p, blockIdx.x, blockIdx.y);
 }
 
 template <int KS, int MB>
@@ -368,6 +382,76 @@ __global__ void cvt_f32_kernel(const double *src, float *dst, long n) {
 hipError_t launch_cvt_f32(const double *src, float *dst, long n, hipStream_t st) {
   hipLaunchKernelGGL(cvt_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, n);
   return hipGetLastError();
+}
+
+// Grouped products (one launch for the same block step of many problems):
+// entry e is one problem's product with gemm_part_workgroups(e) workgroups,
+// n groups fastest, exactly the workgroups of its own launch.  Every entry of
+// a launch shares the slice length KS (a problem's own choice, so each
+// output is summed as in its own launch) and MB covers the largest M (rows
+// past an entry's M are zero and never stored).
+template <int KS, int MB>
+__global__ __launch_bounds__(64 * GP_WAVES) void batch_gemm_part_kernel(const GemmPartArgs *tab,
+                                                                      const int *start, int n) {
+  const int b = blockIdx.x;
+  const int e = group_entry(start, n, b);
+  const GemmPartArgs &p = tab[e];
+  const int groups = (p.N + 16 * GP_WAVES - 1) / (16 * GP_WAVES);
+  const int l = b - start[e];
+  gemm_part_body<KS, MB>(p, l % groups, l / groups);
+}
+
+template <int KS, int MB>
+__global__ __launch_bounds__(64 * GP_WAVES) void batch_gemm_part64_kernel(const GemmPart64Args *tab,
+                                                                        const int *start, int n) {
+  const int b = blockIdx.x;
+  const int e = group_entry(start, n, b);
+  const GemmPart64Args &p = tab[e];
+  const int groups = (p.N + 16 * GP_WAVES - 1) / (16 * GP_WAVES);
+  const int l = b - start[e];
+  gemm_part64_body<KS, MB>(p, l % groups, l / groups);
+}
+
+template <class T, int KS, int MB>
+static void bgp_launch(const T *tab, const int *start, int n, long nwg, hipStream_t st) {
+  const dim3 grid((unsigned)nwg), block(64 * GP_WAVES);
+  if constexpr (std::is_same<T, GemmPartArgs>::value)
+    hipLaunchKernelGGL((batch_gemm_part_kernel<KS, MB>), grid, block, 0, st, tab, start, n);
+  else
+    hipLaunchKernelGGL((batch_gemm_part64_kernel<KS, MB>), grid, block, 0, st, tab, start, n);
+}
+
+template <class T, int KS>
+static void bgp_launch_mb(int mb, const T *tab, const int *start, int n, long nwg, hipStream_t st) {
+  if (mb == 1) bgp_launch<T, KS, 1>(tab, start, n, nwg, st);
+  else if (mb == 2) bgp_launch<T, KS, 2>(tab, start, n, nwg, st);
+  else bgp_launch<T, KS, 4>(tab, start, n, nwg, st);
+}
+
+template <class T>
+static hipError_t bgp_dispatch(int ks, int mb, const T *tab, const int *start, int n, long nwg,
+                               hipStream_t st) {
+  if (n < 1 || nwg < 1 || nwg > 0x7fffffffL || (mb != 1 && mb != 2 && mb != 4))
+    return hipErrorInvalidValue;
+  switch (ks) {
+    case 32: bgp_launch_mb<T, 32>(mb, tab, start, n, nwg, st); break;
+    case 48: bgp_launch_mb<T, 48>(mb, tab, start, n, nwg, st); break;
+    case 64: bgp_launch_mb<T, 64>(mb, tab, start, n, nwg, st); break;
+    case 96: bgp_launch_mb<T, 96>(mb, tab, start, n, nwg, st); break;
+    case 128: bgp_launch_mb<T, 128>(mb, tab, start, n, nwg, st); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_gemm_part(int ks, int mb, const GemmPartArgs *tab, const int *start, int n,
+                                  long nwg, hipStream_t st) {
+  return bgp_dispatch(ks, mb, tab, start, n, nwg, st);
+}
+
+hipError_t launch_batch_gemm_part64(int ks, int mb, const GemmPart64Args *tab, const int *start,
+                                    int n, long nwg, hipStream_t st) {
+  return bgp_dispatch(ks, mb, tab, start, n, nwg, st);
 }
 
 }  // namespace sevp

@@ -124,6 +124,10 @@ def lib():
                                        ctypes.POINTER(ctypes.c_long)]
         L.sevp_profile_spans.argtypes = [VP, ctypes.POINTER(ctypes.c_double),
                                          ctypes.POINTER(ctypes.c_long)]
+        L.sevp_batch_create.argtypes = [VP, CI, ctypes.POINTER(VP)]
+        L.sevp_batch_destroy.argtypes = [VP]
+        L.sevp_batch_solve_refined_dev.argtypes = [VP, VP, VP, VP, ctypes.c_double, CI, VP, VP,
+                                                   ctypes.POINTER(SevpStats)]
         _lib = L
     return _lib
 
@@ -421,6 +425,68 @@ class SevpPlan:
         rel = np.zeros(nrhs)
         _check(lib().sevp_residual_dev(self._h, _p(s), _p(x), nrhs, _p(rel)))
         return rel
+
+
+class SevpBatch:
+    """Independent full-width plans (regional basins) refined together
+    (sevp_batch_*): each step of a pass is one launch over every problem,
+    and each problem's result is bitwise its plan's solve_refined_dev.  The
+    plans must stay open while the batch is used."""
+
+    def __init__(self, plans):
+        self.plans = list(plans)
+        arr = (VP * len(self.plans))(*[p._h.value if isinstance(p._h, VP) else p._h
+                                       for p in self.plans])
+        h = VP()
+        _check(lib().sevp_batch_create(arr, len(self.plans), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sevp_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        self.close()
+
+    def solve_refined_dev(self, s, x, tol=1e-10, max_iter=20):
+        """s[i] float32 (nrhs_i, J2, I2), x[i] float64 (nrhs_i, J2+2, I2+2),
+        torch device tensors of problem i.  Returns (per-problem relative
+        residuals, passes per problem, SevpStats, converged)."""
+        import torch
+        n = len(self.plans)
+        if len(s) != n or len(x) != n:
+            raise SevpError(SEVP_E_ARG, "one right-hand side and one solution array per plan")
+        for p, si, xi in zip(self.plans, s, x):
+            ok = (si.is_cuda and xi.is_cuda and si.dtype == torch.float32
+                  and xi.dtype == torch.float64 and si.is_contiguous() and xi.is_contiguous()
+                  and tuple(si.shape[1:]) == (p.j2, p.i2)
+                  and tuple(xi.shape) == (si.shape[0], p.j2 + 2, p.i2 + 2))
+            if not ok:
+                raise SevpError(SEVP_E_ARG, f"arrays of the {p.i2}x{p.j2} problem: s float32 "
+                                "(nrhs, J2, I2) and x float64 (nrhs, J2+2, I2+2), contiguous")
+        torch.cuda.synchronize()  # the batch's stream follows the null stream only
+        nrhs = (CI * n)(*[int(t.shape[0]) for t in s])
+        sp = (VP * n)(*[t.data_ptr() for t in s])
+        xp = (VP * n)(*[t.data_ptr() for t in x])
+        rel = np.zeros(sum(nrhs))
+        passes = np.zeros(n, np.int32)
+        st = SevpStats()
+        rc = lib().sevp_batch_solve_refined_dev(self._h, sp, xp, nrhs, tol, max_iter, _p(rel),
+                                                _p(passes), ctypes.byref(st))
+        if rc not in (SEVP_OK, SEVP_E_NOTCONV):
+            _check(rc)
+        out, o = [], 0
+        for k in nrhs:
+            out.append(rel[o:o + k])
+            o += k
+        return out, passes, st, rc == SEVP_OK
 
 
 # ------------------------------------------------- one-to-one reference calls

@@ -217,3 +217,20 @@ def make_case(i2: int, j2: int, seed: int, metric: str = "latlon",
     depth = make_depth(mask, seed + 1, h_land=h_land)
     al, ab, ac, ar, at = make_operator(depth, grid, dt)
     return Case(al, ab, ac, ar, at, mask, grid)
+
+
+def basin_specs(n: int, seed: int, i2_range=(64, 512), j2_range=(64, 384),
+                land_range=(0.20, 0.45)) -> list:
+    """Regional basins for a batched workload: (i2, j2, land fraction, seed)
+    per basin, sizes log-uniform in i2_range x j2_range (I2 rounded to a
+    multiple of 16, J2 to a multiple of 8), land fraction uniform in
+    land_range, one seed per basin for its mask, depth and right-hand sides."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        i2 = int(np.exp(rng.uniform(np.log(i2_range[0]), np.log(i2_range[1]))) / 16 + 0.5) * 16
+        j2 = int(np.exp(rng.uniform(np.log(j2_range[0]), np.log(j2_range[1]))) / 8 + 0.5) * 8
+        i2 = min(max(i2, i2_range[0]), i2_range[1])
+        j2 = min(max(j2, j2_range[0]), j2_range[1])
+        out.append((i2, j2, float(rng.uniform(*land_range)), seed + 10 * k))
+    return out
