@@ -81,6 +81,8 @@ def parse():
     ap.add_argument("--basins", type=int, default=0,
                     help="regional basins refined together (sevp_batch) beside one by one (0: skip)")
     ap.add_argument("--basin-rhs", type=int, default=8, help="right-hand sides per basin")
+    ap.add_argument("--basin-prec", choices=("mixed", "fast64"), default="mixed",
+                    help="basin corrections: FP32 MFMA (mixed) or FP64 MFMA (fast64)")
     ap.add_argument("--basins-only", action="store_true", help="run the basins leg alone")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device (reference kernel for HBM counters)")
@@ -325,7 +327,8 @@ def roofline(leg, name, nrhs, i2, a):
 def basins_leg(a):
     """Regional basins (sizes log-uniform in 64x64 .. 512x384, 20-45% land,
     IE from the growth autotuner) with FP32 MFMA corrections and FP64
-    refinement to the tolerance: all basins refined together by one batch
+    refinement to the tolerance (or FP64 MFMA corrections with --basin-prec
+    fast64): all basins refined together by one batch
     (one grouped launch per pass step, sevp_batch_solve_refined_dev) against
     the same basins refined one plan at a time; the batch's solutions must
     equal the plans' bit for bit."""
@@ -339,7 +342,8 @@ def basins_leg(a):
         case = synth.make_case(i2, j2, seed, metric="latlon", lat_max=60.0, land_frac=land,
                                h_land=a.h_land)
         ie = autotune.adaptive_ie(case.coefs, max_growth=1e3, hmax=8, hmin=2, periodic=False)
-        p = S.SevpPlan(case.coefs, ie, form=S.FORM_FULL, nrhs_max=nrhs, precision=S.PREC_FP32)
+        prec = S.PREC_FP32 if a.basin_prec == "mixed" else S.PREC_FP64_MFMA
+        p = S.SevpPlan(case.coefs, ie, form=S.FORM_FULL, nrhs_max=nrhs, precision=prec)
         p.build_influence()
         s_np = synth.make_rhs(case.mask, seed + 1, nrhs)
         ss.append(torch.from_numpy(np.ascontiguousarray(s_np.transpose(2, 1, 0))).cuda())
@@ -382,7 +386,8 @@ def basins_leg(a):
     same = all(bool(torch.equal(u, v)) for u, v in zip(x1, xb))
     worst = max(float(r.max()) for r in rel)
     n = len(plans) * nrhs
-    r = {"basins": len(plans), "rhs_per_basin": nrhs, "precision": "f64+f32mfma",
+    r = {"basins": len(plans), "rhs_per_basin": nrhs,
+         "precision": "f64+f32mfma" if a.basin_prec == "mixed" else "f64mfma",
          "batched_solves_per_s": n / dt2, "one_by_one_solves_per_s": n / dt1,
          "speedup": dt1 / dt2, "ms_batched": 1e3 * dt2, "ms_one_by_one": 1e3 * dt1,
          "passes_max": int(np.max(passes)), "passes_one_by_one": int(np.max(passes1)),

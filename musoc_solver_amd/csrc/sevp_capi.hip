@@ -630,15 +630,13 @@ static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, 
 // ------------------------------------------------------------- residual
 // part[(r*nchunk + chunk)*2 + {0,1}] = partial sums of (s - A x)^2 and s^2.
 namespace sevp {
-__global__ __launch_bounds__(256) void resid_norm_kernel(const float *al, const float *ab,
-                                                         const float *ac, const float *ar,
-                                                         const float *at, const float *s,
-                                                         const double *s64, long s_stride,
-                                                         const double *x, long x_stride,
-                                                         double *rout, int i2, int j2, int nchunk,
-                                                         double *part) {
+__device__ __forceinline__ void resid_norm_body(const float *al, const float *ab, const float *ac,
+                                                const float *ar, const float *at, const float *s,
+                                                const double *s64, long s_stride, const double *x,
+                                                long x_stride, double *rout, int i2, int j2,
+                                                int nchunk, double *part, int r, int ch) {
   __shared__ double red[2][256];
-  const int r = blockIdx.y, ch = blockIdx.x, t = threadIdx.x;
+  const int t = threadIdx.x;
   const long n = (long)i2 * j2;
   const long per = (n + nchunk - 1) / nchunk;
   const long lo = ch * per, hi = min(n, lo + per);
@@ -670,6 +668,37 @@ __global__ __launch_bounds__(256) void resid_norm_kernel(const float *al, const 
     part[((long)r * nchunk + ch) * 2] = red[0][0];
     part[((long)r * nchunk + ch) * 2 + 1] = red[1][0];
   }
+}
+
+__global__ __launch_bounds__(256) void resid_norm_kernel(const float *al, const float *ab,
+                                                         const float *ac, const float *ar,
+                                                         const float *at, const float *s,
+                                                         const double *s64, long s_stride,
+                                                         const double *x, long x_stride,
+                                                         double *rout, int i2, int j2, int nchunk,
+                                                         double *part) {
+  resid_norm_body(al, ab, ac, ar, at, s, s64, s_stride, x, x_stride, rout, i2, j2, nchunk, part,
+                  blockIdx.y, blockIdx.x);
+}
+
+// The residual norms of many problems in one launch: entry e runs the
+// nchunk x nrhs workgroups of its own resid_norm_kernel launch.
+struct NormEnt {
+  const float *al, *ab, *ac, *ar, *at, *s;
+  const double *x;
+  long s_stride, x_stride;
+  double *rout, *part;
+  int i2, j2, nrhs, nchunk;
+};
+
+__global__ __launch_bounds__(256) void batch_resid_norm_kernel(const NormEnt *__restrict__ tab,
+                                                               const int *__restrict__ start, int n) {
+  const int b = blockIdx.x;
+  const int e = group_entry(start, n, b);
+  const NormEnt &q = tab[e];
+  const int l = b - start[e];
+  resid_norm_body(q.al, q.ab, q.ac, q.ar, q.at, q.s, nullptr, q.s_stride, q.x, q.x_stride, q.rout,
+                  q.i2, q.j2, q.nchunk, q.part, l / q.nchunk, l % q.nchunk);
 }
 
 __global__ void zero_kernel(unsigned int *p, long n) {
@@ -1737,12 +1766,24 @@ int sevp_batch_solve_refined_dev(sevp_batch b, const float *const *s, double *co
   }
   if (b->key.size() != (size_t)np || !std::equal(b->key.begin(), b->key.end(), nrhs)) CHK(batch_build(b, nrhs));
   // per call: X += DX entries of every problem, then the periodic ones
+  const size_t nn = total_rhs * NCHUNK * 2;
+  if (b->norm_part.n < nn) CHK(b->norm_part.alloc(nn));
   std::vector<AxpyEnt> ax, hx;
-  std::vector<int> as, hs;
-  long anwg = 0, hnwg = 0;
+  std::vector<NormEnt> nx_;
+  std::vector<int> as, hs, ns_;
+  long anwg = 0, hnwg = 0, nnwg = 0;
+  size_t poff = 0;
   for (int i = 0; i < np; ++i) {
     sevp_plan p = b->plans[i];
     const long nx = (long)(p->d.i2 + 2) * (p->d.j2 + 2);
+    // every problem's residual and its norms (sevp_solve_refined_dev's
+    // kernel, chunks and summation order)
+    NormEnt q{p->al(), p->ab(), p->ac(), p->ar(), p->at(), s[i], x[i], (long)p->d.i2 * p->d.j2, nx,
+              p->r64.p, b->norm_part.p + poff, p->d.i2, p->d.j2, nrhs[i], NCHUNK};
+    nx_.push_back(q);
+    ns_.push_back((int)nnwg);
+    nnwg += (long)NCHUNK * nrhs[i];
+    poff += (size_t)nrhs[i] * NCHUNK * 2;
     const AxpyEnt e{x[i], nx, p->dx.p, nx, p->d.i2, p->d.j2, nrhs[i], i};
     ax.push_back(e);
     as.push_back((int)anwg);
@@ -1753,9 +1794,10 @@ int sevp_batch_solve_refined_dev(sevp_batch b, const float *const *s, double *co
       hnwg += cdiv((long)nrhs[i] * p->d.j2, 256);
     }
   }
-  if (anwg > 0x7fffffffL) return fail(SEVP_E_ARG, "batch too large");
+  if (anwg > 0x7fffffffL || nnwg > 0x7fffffffL) return fail(SEVP_E_ARG, "batch too large");
   TableImage img;
   const size_t o_ax = img.put(ax.data(), ax.size()), o_as = img.put(as.data(), as.size());
+  const size_t o_nx = img.put(nx_.data(), nx_.size()), o_ns = img.put(ns_.data(), ns_.size());
   size_t o_hx = 0, o_hs = 0;
   if (!hx.empty()) {
     o_hx = img.put(hx.data(), hx.size());
@@ -1763,8 +1805,6 @@ int sevp_batch_solve_refined_dev(sevp_batch b, const float *const *s, double *co
   }
   if (b->xtab.n < img.bytes.size()) CHK(b->xtab.alloc(img.bytes.size()));
   HIPC(hipMemcpy(b->xtab.p, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice));
-  const size_t nn = total_rhs * NCHUNK * 2;
-  if (b->norm_part.n < nn) CHK(b->norm_part.alloc(nn));
   if (b->norm_h_n < nn) {
     if (b->norm_h) (void)hipHostFree(b->norm_h);
     b->norm_h = nullptr;
@@ -1778,24 +1818,15 @@ int sevp_batch_solve_refined_dev(sevp_batch b, const float *const *s, double *co
   std::vector<int> it(np, 0);
   std::vector<double> rel(total_rhs, 0.0), worst(np, 0.0);
   for (;;) {
-    // every problem's residual and its norms (sevp_solve_refined_dev's
-    // kernel and summation order), one synchronisation
-    size_t off = 0;
-    for (int i = 0; i < np; ++i) {
-      sevp_plan p = b->plans[i];
-      const int i2 = p->d.i2, j2 = p->d.j2;
-      const long ns = (long)i2 * j2, nx = (long)(i2 + 2) * (j2 + 2);
-      hipLaunchKernelGGL(resid_norm_kernel, dim3(NCHUNK, nrhs[i]), dim3(256), 0, b->stream, p->al(),
-                         p->ab(), p->ac(), p->ar(), p->at(), s[i], nullptr, ns, x[i], nx, p->r64.p,
-                         i2, j2, NCHUNK, b->norm_part.p + off);
-      HIPC(hipGetLastError());
-      off += (size_t)nrhs[i] * NCHUNK * 2;
-    }
+    // every problem's residual and norm partials in one launch, one synchronisation
+    hipLaunchKernelGGL(batch_resid_norm_kernel, dim3((unsigned)nnwg), dim3(256), 0, b->stream,
+                       reinterpret_cast<const NormEnt *>(b->xtab.p + o_nx),
+                       reinterpret_cast<const int *>(b->xtab.p + o_ns), np);
+    HIPC(hipGetLastError());
     HIPC(hipMemcpyAsync(b->norm_h, b->norm_part.p, nn * sizeof(double), hipMemcpyDeviceToHost, b->stream));
     HIPC(hipStreamSynchronize(b->stream));
     bool any = false;
-    size_t r0 = 0;
-    off = 0;
+    size_t r0 = 0, off = 0;
     for (int i = 0; i < np; ++i) {
       worst[i] = 0.0;
       for (int r = 0; r < nrhs[i]; ++r) {
