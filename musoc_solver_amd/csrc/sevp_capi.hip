@@ -1583,7 +1583,7 @@ static int batch_build(sevp_batch b, const int *nrhs) {
   const int halo = std::max(maxsteps, 0) + 1;
   b->kmarch = (64 - 2 * halo > 0) ? 1 : (128 - 2 * halo > 0) ? 2 : (256 - 2 * halo > 0) ? 4 : -1;
   if (b->kmarch < 0) return fail(SEVP_E_ARG, "a block is too tall for the grouped march (more than 126 rows)");
-  b->mb = maxm <= 16 ? 1 : maxm <= 32 ? 2 : 4;
+  b->mb = maxm <= 16 ? 1 : 2;  // 32-row groups beyond 32 rows, as in a single launch
   TableImage img;
   {
     std::vector<FillEnt> f;

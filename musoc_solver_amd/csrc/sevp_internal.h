@@ -289,7 +289,7 @@ hipError_t launch_batch_fill(const FillEnt *tab, const int *start, int n, long n
 // periodic halo fix instead
 hipError_t launch_batch_axpy(const AxpyEnt *tab, const int *start, int n, long nwg, const int *act,
                              bool halo, hipStream_t st);
-// ks: the shared slice length; mb: 16-row blocks covering the largest M (1, 2 or 4)
+// ks: the shared slice length; mb: 1 when every M <= 16, else 2 (32-row groups)
 hipError_t launch_batch_gemm_part(int ks, int mb, const GemmPartArgs *tab, const int *start, int n,
                                   long nwg, hipStream_t st);
 hipError_t launch_batch_gemm_part64(int ks, int mb, const GemmPart64Args *tab, const int *start,

@@ -51,10 +51,11 @@ __device__ __forceinline__ f32x4 load4m(const float *base, int k, int klen) {
 // loads, the LDS staging and the 4 x KS/4 MFMA steps are straight-line code.
 // Every global load of the workgroup is issued before the first wait: the A
 // slice (to LDS), then the B chunks of the slice into registers.
-// MB (1, 2 or 4) 16-row accumulator blocks cover the right-hand sides; LDS
-// rows past M are zero.  Requires K % 4 == 0 (float4 runs of k are aligned).
+// MB (1 or 2) 16-row accumulator blocks cover right-hand sides m0 ..
+// m0 + 16 MB - 1; LDS rows past M are zero.  Requires K % 4 == 0 (float4
+// runs of k are aligned).
 template <int KS, int MB>
-__device__ __forceinline__ void gemm_part_body(const GemmPartArgs &p, int bx, int s) {
+__device__ __forceinline__ void gemm_part_body(const GemmPartArgs &p, int bx, int s, int m0) {
   constexpr int LDA = KS + 4;
   constexpr int KQ = KS / 4;
   constexpr int NCH = KS / 16;
@@ -76,7 +77,7 @@ __device__ __forceinline__ void gemm_part_body(const GemmPartArgs &p, int bx, in
 #pragma unroll
   for (int j = 0; j < ALD; ++j) {
     const int idx = threadIdx.x + j * 64 * GP_WAVES;
-    const int m = idx / KQ, k = (idx % KQ) * 4;
+    const int m = m0 + idx / KQ, k = (idx % KQ) * 4;
     const bool ok = idx < ROWS * KQ && m < p.M;
     at[j] = load4m(p.a + (long)(ok ? m : 0) * p.K + k_lo, k, ok ? klen : 0);
   }
@@ -125,7 +126,7 @@ __device__ __forceinline__ void gemm_part_body(const GemmPartArgs &p, int bx, in
   for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int m = mb * 16 + 4 * q + i;
+      const int m = m0 + mb * 16 + 4 * q + i;
       if (nok && m < p.M) out[(long)m * p.N] = acc[mb][i];
     }
   }
@@ -134,7 +135,7 @@ __device__ __forceinline__ void gemm_part_body(const GemmPartArgs &p, int bx, in
 
 template <int KS, int MB>
 __global__ __launch_bounds__(64 * GP_WAVES) void gemm_part_kernel(GemmPartArgs p) {
-  gemm_part_body<KS, MB>(p, blockIdx.x, blockIdx.y);
+  gemm_part_body<KS, MB>(p, blockIdx.x, blockIdx.y, blockIdx.z * 16 * MB);
 }
 
 // Slice lengths with a compiled kernel, and the one used for a product: the
@@ -169,17 +170,23 @@ static void gp_launch(const GemmPartArgs &a, dim3 grid, hipStream_t st, hipEvent
   else hipLaunchKernelGGL((gemm_part_kernel<KS, MB>), grid, block, 0, st, a);
 }
 
+// Right-hand sides per workgroup: up to 32 (two 16-row MFMA blocks); more
+// rows go to further workgroups (grid z), which read the same B tiles while
+// they are in the cache and leave each wavefront's MFMA chain as short as
+// at 32.  Each output row's MFMA sequence does not depend on the split.
+static int gp_row_blocks(int M) { return M <= 16 ? 1 : 2; }
+static int gp_row_groups(int M) { return (M + 31) / 32; }
+
 template <int KS>
 static void gp_launch_mb(const GemmPartArgs &a, dim3 grid, hipStream_t st, hipEvent_t ev0,
                          hipEvent_t ev1) {
-  const int mb = (a.M + 15) / 16;
-  if (mb == 1) gp_launch<KS, 1>(a, grid, st, ev0, ev1);
-  else if (mb == 2) gp_launch<KS, 2>(a, grid, st, ev0, ev1);
-  else gp_launch<KS, 4>(a, grid, st, ev0, ev1);
+  if (gp_row_blocks(a.M) == 1) gp_launch<KS, 1>(a, grid, st, ev0, ev1);
+  else gp_launch<KS, 2>(a, grid, st, ev0, ev1);
 }
 
 long gemm_part_workgroups(const GemmPartArgs &a) {
-  return (long)((a.N + 16 * GP_WAVES - 1) / (16 * GP_WAVES)) * gemm_part_slices(a.N, a.K, nullptr);
+  return (long)((a.N + 16 * GP_WAVES - 1) / (16 * GP_WAVES)) * gemm_part_slices(a.N, a.K, nullptr) *
+         gp_row_groups(a.M);
 }
 
 hipError_t launch_gemm_part(const GemmPartArgs &a, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
@@ -187,7 +194,7 @@ hipError_t launch_gemm_part(const GemmPartArgs &a, hipStream_t st, hipEvent_t ev
   int ks = 0;
   const int ns = gemm_part_slices(a.N, a.K, &ks);
   const int groups = (a.N + 16 * GP_WAVES - 1) / (16 * GP_WAVES);
-  const dim3 grid(groups, ns);
+  const dim3 grid(groups, ns, gp_row_groups(a.M));
   switch (ks) {
     case 32: gp_launch_mb<32>(a, grid, st, ev0, ev1); break;
     case 48: gp_launch_mb<48>(a, grid, st, ev0, ev1); break;
@@ -216,7 +223,7 @@ __device__ __forceinline__ f64x4 load4d(const double *base, int k, int klen) {
 }
 
 template <int KS, int MB>
-__device__ __forceinline__ void gemm_part64_body(const GemmPart64Args &p, int bx, int s) {
+__device__ __forceinline__ void gemm_part64_body(const GemmPart64Args &p, int bx, int s, int m0) {
   constexpr int LDA = KS + 2;
   constexpr int KQ = KS / 4;
   constexpr int NCH = KS / 16;
@@ -236,7 +243,7 @@ __device__ __forceinline__ void gemm_part64_body(const GemmPart64Args &p, int bx
 #pragma unroll
   for (int j = 0; j < ALD; ++j) {
     const int idx = threadIdx.x + j * 64 * GP_WAVES;
-    const int m = idx / KQ, k = (idx % KQ) * 4;
+    const int m = m0 + idx / KQ, k = (idx % KQ) * 4;
     const bool ok = idx < ROWS * KQ && m < p.M;
     at[j] = load4d(p.a + (long)(ok ? m : 0) * p.K + k_lo, k, ok ? klen : 0);
   }
@@ -284,7 +291,7 @@ __device__ __forceinline__ void gemm_part64_body(const GemmPart64Args &p, int bx
     for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int m = mb * 16 + q + 4 * i;
+        const int m = m0 + mb * 16 + q + 4 * i;
         if (nok && m < p.M) out[(long)m * p.N] = acc[mb][i];
       }
   }
@@ -293,7 +300,7 @@ __device__ __forceinline__ void gemm_part64_body(const GemmPart64Args &p, int bx
 
 template <int KS, int MB>
 __global__ __launch_bounds__(64 * GP_WAVES) void gemm_part64_kernel(GemmPart64Args p) {
-  gemm_part64_body<KS, MB>(p, blockIdx.x, blockIdx.y);
+  gemm_part64_body<KS, MB>(p, blockIdx.x, blockIdx.y, blockIdx.z * 16 * MB);
 }
 
 template <int KS, int MB>
@@ -307,14 +314,13 @@ static void gp64_launch(const GemmPart64Args &a, dim3 grid, hipStream_t st, hipE
 template <int KS>
 static void gp64_launch_mb(const GemmPart64Args &a, dim3 grid, hipStream_t st, hipEvent_t ev0,
                            hipEvent_t ev1) {
-  const int mb = (a.M + 15) / 16;
-  if (mb == 1) gp64_launch<KS, 1>(a, grid, st, ev0, ev1);
-  else if (mb == 2) gp64_launch<KS, 2>(a, grid, st, ev0, ev1);
-  else gp64_launch<KS, 4>(a, grid, st, ev0, ev1);
+  if (gp_row_blocks(a.M) == 1) gp64_launch<KS, 1>(a, grid, st, ev0, ev1);
+  else gp64_launch<KS, 2>(a, grid, st, ev0, ev1);
 }
 
 long gemm_part64_workgroups(const GemmPart64Args &a) {
-  return (long)((a.N + 16 * GP_WAVES - 1) / (16 * GP_WAVES)) * gemm_part_slices(a.N, a.K, nullptr);
+  return (long)((a.N + 16 * GP_WAVES - 1) / (16 * GP_WAVES)) * gemm_part_slices(a.N, a.K, nullptr) *
+         gp_row_groups(a.M);
 }
 
 hipError_t launch_gemm_part64(const GemmPart64Args &a, hipStream_t st, hipEvent_t ev0,
@@ -323,7 +329,7 @@ hipError_t launch_gemm_part64(const GemmPart64Args &a, hipStream_t st, hipEvent_
   int ks = 0;
   const int ns = gemm_part_slices(a.N, a.K, &ks);
   const int groups = (a.N + 16 * GP_WAVES - 1) / (16 * GP_WAVES);
-  const dim3 grid(groups, ns);
+  const dim3 grid(groups, ns, gp_row_groups(a.M));
   switch (ks) {
     case 32: gp64_launch_mb<32>(a, grid, st, ev0, ev1); break;
     case 48: gp64_launch_mb<48>(a, grid, st, ev0, ev1); break;
@@ -386,10 +392,11 @@ hipError_t launch_cvt_f32(const double *src, float *dst, long n, hipStream_t st)
 
 // Grouped products (one launch for the same block step of many problems):
 // entry e is one problem's product with gemm_part_workgroups(e) workgroups,
-// n groups fastest, exactly the workgroups of its own launch.  Every entry of
-// a launch shares the slice length KS (a problem's own choice, so each
-// output is summed as in its own launch) and MB covers the largest M (rows
-// past an entry's M are zero and never stored).
+// n groups fastest, then slices, then 32-row groups: exactly the workgroups
+// of its own launch.  Every entry of a launch shares the slice length KS (a
+// problem's own choice, so each output is summed as in its own launch); MB
+// is 2 when any entry has more than 16 rows (rows past an entry's M are zero
+// and never stored, and each row's MFMA sequence does not depend on MB).
 template <int KS, int MB>
 __global__ __launch_bounds__(64 * GP_WAVES) void batch_gemm_part_kernel(const GemmPartArgs *tab,
                                                                       const int *start, int n) {
@@ -397,8 +404,9 @@ __global__ __launch_bounds__(64 * GP_WAVES) void batch_gemm_part_kernel(const Ge
   const int e = group_entry(start, n, b);
   const GemmPartArgs &p = tab[e];
   const int groups = (p.N + 16 * GP_WAVES - 1) / (16 * GP_WAVES);
+  const int gs = groups * ((p.K + KS - 1) / KS);
   const int l = b - start[e];
-  gemm_part_body<KS, MB>(p, l % groups, l / groups);
+  gemm_part_body<KS, MB>(p, l % groups, (l % gs) / groups, (l / gs) * 32);
 }
 
 template <int KS, int MB>
@@ -408,8 +416,9 @@ __global__ __launch_bounds__(64 * GP_WAVES) void batch_gemm_part64_kernel(const 
   const int e = group_entry(start, n, b);
   const GemmPart64Args &p = tab[e];
   const int groups = (p.N + 16 * GP_WAVES - 1) / (16 * GP_WAVES);
+  const int gs = groups * ((p.K + KS - 1) / KS);
   const int l = b - start[e];
-  gemm_part64_body<KS, MB>(p, l % groups, l / groups);
+  gemm_part64_body<KS, MB>(p, l % groups, (l % gs) / groups, (l / gs) * 32);
 }
 
 template <class T, int KS, int MB>
@@ -424,14 +433,13 @@ static void bgp_launch(const T *tab, const int *start, int n, long nwg, hipStrea
 template <class T, int KS>
 static void bgp_launch_mb(int mb, const T *tab, const int *start, int n, long nwg, hipStream_t st) {
   if (mb == 1) bgp_launch<T, KS, 1>(tab, start, n, nwg, st);
-  else if (mb == 2) bgp_launch<T, KS, 2>(tab, start, n, nwg, st);
-  else bgp_launch<T, KS, 4>(tab, start, n, nwg, st);
+  else bgp_launch<T, KS, 2>(tab, start, n, nwg, st);
 }
 
 template <class T>
 static hipError_t bgp_dispatch(int ks, int mb, const T *tab, const int *start, int n, long nwg,
                                hipStream_t st) {
-  if (n < 1 || nwg < 1 || nwg > 0x7fffffffL || (mb != 1 && mb != 2 && mb != 4))
+  if (n < 1 || nwg < 1 || nwg > 0x7fffffffL || (mb != 1 && mb != 2))
     return hipErrorInvalidValue;
   switch (ks) {
     case 32: bgp_launch_mb<T, 32>(mb, tab, start, n, nwg, st); break;
