@@ -1465,6 +1465,7 @@ struct sevp_batch_s {
   hipStream_t stream = nullptr, cap = nullptr;
   int ks = 0;                 // the product slice length every problem uses
   std::vector<int> key;       // nrhs per problem of the captured pass
+  int key_prec = -1;          // the plans' correction precision of the captured pass
   int kmarch = 1, mb = 1;     // lane width of the grouped marches, 16-row blocks of products
   std::vector<BatchLaunch> launches;
   DBuf<unsigned char> tables;
@@ -1479,6 +1480,7 @@ struct sevp_batch_s {
     if (exec) (void)hipGraphExecDestroy(exec);
     exec = nullptr;
     key.clear();
+    key_prec = -1;
   }
   ~sevp_batch_s() {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -1692,6 +1694,7 @@ static int batch_build(sevp_batch b, const int *nrhs) {
   (void)hipGraphDestroy(graph);
   HIPC(ei);
   b->key.assign(nrhs, nrhs + np);
+  b->key_prec = b->plans[0]->d.precision;
   return SEVP_OK;
 }
 
@@ -1755,7 +1758,10 @@ int sevp_batch_solve_refined_dev(sevp_batch b, const float *const *s, double *co
     if (!s[i] || !x[i]) return fail(SEVP_E_ARG, "null right-hand side or solution array");
     if (nrhs[i] < 1 || nrhs[i] > p->d.nrhs_max || nrhs[i] > 64) return fail(SEVP_E_ARG, "nrhs out of range");
     if (!p->have_inv) return fail(SEVP_E_STATE, "influence matrices not built or loaded");
-    if (p->d.precision != b->plans[0]->d.precision) return fail(SEVP_E_STATE, "plan precision changed");
+    if (p->d.precision != b->plans[0]->d.precision)
+      return fail(SEVP_E_STATE, "the plans of a batch no longer share one precision");
+    if (p->d.precision != SEVP_PREC_FP32 && p->d.precision != SEVP_PREC_FP64_MFMA)
+      return fail(SEVP_E_STATE, "batched solves need FP32 or FP64_MFMA corrections");
     total_rhs += nrhs[i];
   }
   auto t0 = std::chrono::steady_clock::now();
@@ -1764,7 +1770,10 @@ int sevp_batch_solve_refined_dev(sevp_batch b, const float *const *s, double *co
     HIPC(hipStreamSynchronize(p->stream));
     for (auto st : p->gstreams) HIPC(hipStreamSynchronize(st));
   }
-  if (b->key.size() != (size_t)np || !std::equal(b->key.begin(), b->key.end(), nrhs)) CHK(batch_build(b, nrhs));
+  // the pass is rebuilt when the RHS counts or the plans' (common) precision change
+  if (b->key.size() != (size_t)np || !std::equal(b->key.begin(), b->key.end(), nrhs) ||
+      b->key_prec != b->plans[0]->d.precision)
+    CHK(batch_build(b, nrhs));
   // per call: X += DX entries of every problem, then the periodic ones
   const size_t nn = total_rhs * NCHUNK * 2;
   if (b->norm_part.n < nn) CHK(b->norm_part.alloc(nn));

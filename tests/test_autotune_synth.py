@@ -49,3 +49,21 @@ def test_synthetic_operator_has_the_reference_form():
     assert 0.25 < land < 0.35
     s = synth.make_rhs(case.mask, 6, 2)
     assert abs(s[:, :, 0][case.mask].mean()) < 1e-6 and (s[:, :, 0][~case.mask] == 0).all()
+
+
+def test_basin_specs_cover_the_regional_size_range():
+    """256 basins: sizes log-uniform in 64x64 .. 512x384 (I2 a multiple of 16,
+    as the batched products need I2 % 4 == 0), land 20-45%, distinct seeds,
+    reproducible from the seed."""
+    sp = synth.basin_specs(256, 7000)
+    assert sp == synth.basin_specs(256, 7000) and sp != synth.basin_specs(256, 7001)
+    i2 = np.array([s[0] for s in sp])
+    j2 = np.array([s[1] for s in sp])
+    land = np.array([s[2] for s in sp])
+    assert (i2 % 16 == 0).all() and i2.min() >= 64 and i2.max() <= 512
+    assert (j2 % 8 == 0).all() and j2.min() >= 64 and j2.max() <= 384
+    assert ((land >= 0.20) & (land <= 0.45)).all()
+    assert len({s[3] for s in sp}) == 256
+    # log-uniform: the geometric mean sits near the range's, both ends are reached
+    assert abs(np.log(i2).mean() - np.log(np.sqrt(64 * 512))) < 0.15
+    assert i2.min() <= 80 and i2.max() >= 448

@@ -86,6 +86,39 @@ def test_batch_equals_plan_solves(device, precision):
             p.close()
 
 
+def test_batch_follows_a_precision_switch(device):
+    """All plans switched from FP32 to FP64 MFMA corrections after the batch
+    ran: the next batched solve rebuilds its pass and matches the plans'
+    FP64 MFMA solves; a batch whose plans disagree is refused."""
+    import torch
+    basins = make_basins(S.PREC_FP32)[:3]
+    try:
+        with S.SevpBatch([p for p, _ in basins]) as batch:
+            s_b = [dev(p, s, 2)[0] for p, s in basins]
+            x_b = [dev(p, s, 2)[1] for p, s in basins]
+            batch.solve_refined_dev(s_b, x_b)
+            for p, _ in basins:
+                p.set_precision(S.PREC_FP64_MFMA)
+            ref = []
+            for (p, s), sd in zip(basins, s_b):
+                x = torch.zeros((2, p.j2 + 2, p.i2 + 2), dtype=torch.float64, device="cuda")
+                rel, st, ok = p.solve_refined_dev(sd, x, tol=1e-10, max_iter=12)
+                assert ok
+                ref.append((x.cpu().numpy(), st.iterations))
+            x_b = [torch.zeros_like(x) for x in x_b]
+            rel, passes, st, ok = batch.solve_refined_dev(s_b, x_b, tol=1e-10, max_iter=12)
+            torch.cuda.synchronize()
+            assert ok and list(passes) == [r[1] for r in ref]
+            for x, r in zip(x_b, ref):
+                assert fields_equal(x.cpu().numpy(), r[0])
+            basins[0][0].set_precision(S.PREC_FP32)
+            with pytest.raises(S.SevpError, match="no longer share"):
+                batch.solve_refined_dev(s_b, x_b)
+    finally:
+        for p, _ in basins:
+            p.close()
+
+
 def test_batch_rejects_mismatched_plans(device):
     case = synth.make_case(64, 32, 6200, metric="latlon", lat_max=50.0, h_land=50.0)
     ie = synth.reference_ie(32, 16)
