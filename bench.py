@@ -78,7 +78,7 @@ def parse():
                     help="also time the device BiCGSTAB cross-check on the first RHS (0: skip)")
     ap.add_argument("--krylov-only", action="store_true",
                     help="only the BiCGSTAB leg (for kernel profiles of its iteration)")
-    ap.add_argument("--basins", type=int, default=0,
+    ap.add_argument("--basins", type=int, default=256,
                     help="regional basins refined together (sevp_batch) beside one by one (0: skip)")
     ap.add_argument("--basin-rhs", type=int, default=8, help="right-hand sides per basin")
     ap.add_argument("--basin-prec", choices=("mixed", "fast64"), default="mixed",
@@ -281,7 +281,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "krylov": krylov,
-        "basins": basins_leg(a) if a.basins > 0 else None,
+        "basins": basins_or_error(a),
     }
     print(json.dumps(out), flush=True)
     for plan, _ in plans.values():
@@ -322,6 +322,18 @@ def roofline(leg, name, nrhs, i2, a):
               "algorithmic_bytes_per_launch": nbytes, "algorithmic_flops_per_launch": flops,
               "avg_launch_us": leg["corr_avg_us"], "rhs_per_product": m})
     return r
+
+
+def basins_or_error(a):
+    """The basins leg after the main legs: a failure there is reported in its
+    own field and leaves the main line intact."""
+    if a.basins <= 0:
+        return None
+    try:
+        return basins_leg(a)
+    except Exception as e:  # noqa: BLE001 - reported, not hidden
+        log(f"basins leg failed: {e!r}")
+        return {"error": repr(e)}
 
 
 def basins_leg(a):
