@@ -248,9 +248,11 @@ class SevpPlan:
         self._stream = None
 
     def close(self):
-        if getattr(self, "_h", None):
-            lib().sevp_plan_destroy(self._h)
-            self._h = None
+        # at interpreter exit the module's globals may already be gone; the
+        # process exit releases the device memory then
+        h, self._h = getattr(self, "_h", None), None
+        if h and _lib is not None:
+            _lib.sevp_plan_destroy(h)
 
     __del__ = close
 
@@ -442,9 +444,9 @@ class SevpBatch:
         self._h = h
 
     def close(self):
-        if getattr(self, "_h", None):
-            lib().sevp_batch_destroy(self._h)
-            self._h = None
+        h, self._h = getattr(self, "_h", None), None
+        if h and _lib is not None:
+            _lib.sevp_batch_destroy(h)
 
     def __enter__(self):
         return self
