@@ -84,6 +84,9 @@ def parse():
     ap.add_argument("--basin-prec", choices=("mixed", "fast64"), default="mixed",
                     help="basin corrections: FP32 MFMA (mixed) or FP64 MFMA (fast64)")
     ap.add_argument("--basins-only", action="store_true", help="run the basins leg alone")
+    ap.add_argument("--sequence", type=int, default=200,
+                    help="sequential single-RHS solves at 720x360, warm-started (0: skip)")
+    ap.add_argument("--sequence-only", action="store_true", help="run the sequence leg alone")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device (reference kernel for HBM counters)")
     return ap.parse_args()
@@ -194,6 +197,9 @@ def main():
     if a.basins_only:
         print(json.dumps({"basins": basins_leg(a)}), flush=True)
         return
+    if a.sequence_only:
+        print(json.dumps({"sequence": sequence_leg(a)}), flush=True)
+        return
     legs, setup, xs = {}, {}, {}
     order = [l for l in a.legs.split(",") if l]
     if a.krylov_only:  # BiCGSTAB needs the operator only, no influence matrices
@@ -282,6 +288,7 @@ def main():
         "cpu_baseline": cpu,
         "krylov": krylov,
         "basins": basins_or_error(a),
+        "sequence": sequence_or_error(a),
     }
     print(json.dumps(out), flush=True)
     for plan, _ in plans.values():
@@ -322,6 +329,58 @@ def roofline(leg, name, nrhs, i2, a):
               "algorithmic_bytes_per_launch": nbytes, "algorithmic_flops_per_launch": flops,
               "avg_launch_us": leg["corr_avg_us"], "rhs_per_product": m})
     return r
+
+
+def sequence_or_error(a):
+    """The sequence leg after the main legs, isolated like the basins leg."""
+    if a.sequence <= 0:
+        return None
+    try:
+        return sequence_leg(a)
+    except Exception as e:  # noqa: BLE001 - reported, not hidden
+        log(f"sequence leg failed: {e!r}")
+        return {"error": repr(e)}
+
+
+def sequence_leg(a):
+    """Sequential single-RHS solves on the 0.5 degree grid (720x360): each
+    step's forcing is N(0,1) plus a slowly drifting low-wavenumber field, and
+    each solve starts from the previous step's solution and is refined to the
+    tolerance with the REP1 form (block height 2), as a model's time loop
+    calls the solver.  Reports milliseconds per solve for each correction
+    arithmetic on one plan."""
+    import torch
+    from musoc_solver_amd import synth
+    from musoc_solver_amd import sevp as S
+    i2, j2, n = 720, 360, a.sequence
+    case = synth.make_case(i2, j2, a.seed + 2000, metric="latlon", lat_max=a.lat_max,
+                           h_land=a.h_land)
+    ie = synth.reference_ie(j2, synth.nb0_for_height(j2, 2))
+    forcing = synth.make_rhs(case.mask, a.seed + 2001, n, lowfreq=2.0)
+    sd = [torch.from_numpy(np.ascontiguousarray(forcing[:, :, k:k + 1].transpose(2, 1, 0))).cuda()
+          for k in range(n)]
+    out = {"grid": f"{i2}x{j2}", "solves": n, "block_height": 2, "nb0": len(ie), "tol": a.tol}
+    with S.SevpPlan(case.coefs, ie, form=S.FORM_FULL_PERIODIC, nrhs_max=1) as plan:
+        plan.build_influence()
+        for name, prec in (("plain", S.PREC_FP64), ("fast64", S.PREC_FP64_MFMA),
+                           ("mixed", S.PREC_FP32)):
+            plan.set_precision(prec)
+            x = torch.zeros((1, j2 + 2, i2 + 2), dtype=torch.float64, device="cuda")
+            plan.solve_refined_dev(sd[0], x, tol=a.tol, max_iter=a.max_iter)  # warm-up
+            x.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            passes, worst = [], 0.0
+            for k in range(n):
+                rel, st, ok = plan.solve_refined_dev(sd[k], x, tol=a.tol, max_iter=a.max_iter)
+                passes.append(st.iterations)
+                worst = max(worst, float(rel.max()))
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / n
+            out[name] = {"ms_per_solve": 1e3 * dt, "passes_mean": float(np.mean(passes)),
+                         "max_rel_resid": worst}
+    log(f"sequence: {out}")
+    return out
 
 
 def basins_or_error(a):
