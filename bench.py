@@ -420,6 +420,8 @@ def basins_leg(a):
         ss.append(torch.from_numpy(np.ascontiguousarray(s_np.transpose(2, 1, 0))).cuda())
         plans.append(p)
         nb0.append(len(ie))
+        if len(plans) % 32 == 0:
+            log(f"basins: {len(plans)} of {a.basins} plans built")
     torch.cuda.synchronize()
     setup = time.perf_counter() - t0
     log(f"basins: {len(plans)} plans built in {setup:.1f} s (NB0 max {max(nb0)}, sum {sum(nb0)}, "
