@@ -591,7 +591,23 @@ def cpu_baseline(plan, case, ie, s_np, a):
                       f"REP1 {'compiled from the reference source' if use_ref else 'C restatement'}, "
                       "residuals by the C restatement of p_sqprod, influence matrices from the "
                       "GPU build",
-            "single_core_s_per_solve": t_one, "rel_resid": worst}
+            "single_core_s_per_solve": t_one, "rel_resid": worst,
+            "host_cpu": host_cpu()}
+
+
+def host_cpu():
+    """Model name and logical CPU count of the host the baseline ran on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(),
+            "usable_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
 
 
 if __name__ == "__main__":
