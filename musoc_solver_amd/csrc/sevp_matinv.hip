@@ -11,7 +11,12 @@
 // element still receives its updates one at a time in pivot order:
 //
 //   forward, panel P = rows/pivots [p0, p1):
-//     panel_kernel     rows of P: steps of P in order (scale, eliminate)
+//     diag_kernel      rows of P, columns of P: steps of P in order (scale,
+//                      eliminate), recording each step's reciprocal pivot
+//                      and multipliers
+//     cols_kernel      rows of P, every other column: the same steps, one
+//                      column per thread (a column's updates need only its
+//                      own values and the recorded pivots and multipliers)
 //     (copy the final rows of P to Rq)
 //     tri_kernel       each later row: the KB steps on its P columns, which
 //                      yields its KB multipliers (the running values)
@@ -20,7 +25,7 @@
 //   backward, panel Q (increasing), rows above the panel first:
 //     (copy rows of Q to Rq: these are the forward results the steps read)
 //     tri_kernel / trail_kernel on rows [0, q0)
-//     panel_kernel     rows of Q: steps of Q above their own index
+//     diag_kernel / cols_kernel  rows of Q: steps of Q above their own index
 //
 // No operation is reassociated or fused (the file is built with
 // -ffp-contract=off), so the result is bitwise MATINV's.
@@ -33,53 +38,118 @@ constexpr int MB_KB = 32;
 
 #define BM(i, j) B[(long)(i) + (long)(j) * n]  // 0-based, column-major
 
-// Forward: rows [p0,p1) all columns; scale each pivot row then eliminate the
-// panel rows below it.  Backward: rows [q0,q1), steps i in (row, q1) using the
-// forward rows kept in Rq.
-__global__ __launch_bounds__(256) void mb_panel_kernel(double *mats, long ms, int n, int p0,
-                                                       int p1, int backward, const double *rqs,
-                                                       int *zp) {
+// Steps of one panel on its own columns [p0, p1), one workgroup per matrix
+// with the kb x kb block in LDS.  Forward: for each pivot i, scale row i
+// (recording 1/B(i,i)), then rows below i in the panel take
+// B(r,c) <- (c == i ? 0 : B(r,c)) - m(r,i) * B(i,c) with m(r,i) = B(r,i) just
+// before the step (recorded).  Backward: for each i, rows above i in the panel
+// take B(r,c) <- (c == i ? 0 : B(r,c)) - m(r,i) * Rq(i,c), Rq the panel's
+// forward rows.  pm per matrix: [0, MB_KB) reciprocal pivots, then
+// m(r, i) at MB_KB + r * MB_KB + i.
+constexpr int MB_PM = MB_KB + MB_KB * MB_KB;
+
+__global__ __launch_bounds__(256) void mb_diag_kernel(double *mats, long ms, int n, int p0, int p1,
+                                                      int backward, const double *rqs, int *zp,
+                                                      double *pms, long pm_stride) {
+  __shared__ double Dg[MB_KB][MB_KB + 1];
+  __shared__ double Rd[MB_KB][MB_KB + 1];
   __shared__ double b1[MB_KB];
   __shared__ double piv;
   double *B = mats + (long)blockIdx.x * ms;
   const double *Rq = rqs + (long)blockIdx.x * MB_KB * n;
-  const int t = threadIdx.x, nt = blockDim.x;
-  const int kb = p1 - p0;
+  double *P = pms + (long)blockIdx.x * pm_stride;
+  const int t = threadIdx.x, kb = p1 - p0;
+  for (int idx = t; idx < kb * kb; idx += 256) {
+    const int r = idx % kb, c = idx / kb;
+    Dg[r][c] = BM(p0 + r, p0 + c);
+    if (backward) Rd[r][c] = Rq[(long)r * n + p0 + c];
+  }
+  __syncthreads();
   if (!backward) {
-    for (int i = p0; i < p1; ++i) {
+    for (int i = 0; i < kb; ++i) {
       if (t == 0) {
-        if (BM(i, i) == 0.0 && zp) atomicAdd(zp, 1);
-        piv = 1.0 / BM(i, i);
-        BM(i, i) = 1.0;
+        if (Dg[i][i] == 0.0 && zp) atomicAdd(zp, 1);
+        piv = 1.0 / Dg[i][i];
+        Dg[i][i] = 1.0;
+        P[i] = piv;
       }
       __syncthreads();
       const double bp = piv;
-      for (int j = t; j < n; j += nt) BM(i, j) = BM(i, j) * bp;
-      for (int r = i + 1 + t; r < p1; r += nt) b1[r - p0] = BM(r, i);
+      if (t < kb) Dg[i][t] = Dg[i][t] * bp;
+      if (t > i && t < kb) {
+        b1[t] = Dg[t][i];
+        P[MB_KB + t * MB_KB + i] = Dg[t][i];
+      }
       __syncthreads();
-      const int rows = p1 - i - 1;
-      for (long idx = t; idx < (long)rows * n; idx += nt) {
-        const int r = i + 1 + (int)(idx % rows), j = (int)(idx / rows);
-        const double cur = (j == i) ? 0.0 : BM(r, j);
-        BM(r, j) = cur - b1[r - p0] * BM(i, j);
+      const int rows = kb - i - 1;
+      for (int idx = t; idx < rows * kb; idx += 256) {
+        const int r = i + 1 + idx % rows, c = idx / rows;
+        const double cur = (c == i) ? 0.0 : Dg[r][c];
+        Dg[r][c] = cur - b1[r] * Dg[i][c];
       }
       __syncthreads();
     }
   } else {
-    for (int i = p0 + 1; i < p1; ++i) {
-      for (int r = p0 + t; r < i; r += nt) b1[r - p0] = BM(r, i);
+    for (int i = 1; i < kb; ++i) {
+      if (t < i) {
+        b1[t] = Dg[t][i];
+        P[MB_KB + t * MB_KB + i] = Dg[t][i];
+      }
       __syncthreads();
-      const int rows = i - p0;
-      const double *Ri = Rq + (long)(i - p0) * n;
-      for (long idx = t; idx < (long)rows * n; idx += nt) {
-        const int r = p0 + (int)(idx % rows), j = (int)(idx / rows);
-        const double cur = (j == i) ? 0.0 : BM(r, j);
-        BM(r, j) = cur - b1[r - p0] * Ri[j];
+      for (int idx = t; idx < i * kb; idx += 256) {
+        const int r = idx % i, c = idx / i;
+        const double cur = (c == i) ? 0.0 : Dg[r][c];
+        Dg[r][c] = cur - b1[r] * Rd[i][c];
       }
       __syncthreads();
     }
   }
-  (void)kb;
+  for (int idx = t; idx < kb * kb; idx += 256) {
+    const int r = idx % kb, c = idx / kb;
+    BM(p0 + r, p0 + c) = Dg[r][c];
+  }
+}
+
+// The same steps on every column outside [p0, p1): one thread per column,
+// the kb panel rows in registers, updates in step order per element.
+__global__ __launch_bounds__(256) void mb_cols_kernel(double *mats, long ms, int n, int p0, int p1,
+                                                      int backward, const double *rqs,
+                                                      const double *pms, long pm_stride) {
+  __shared__ double Pm[MB_PM];
+  double *B = mats + (long)blockIdx.y * ms;
+  const double *Rq = rqs + (long)blockIdx.y * MB_KB * n;
+  const double *P = pms + (long)blockIdx.y * pm_stride;
+  const int kb = p1 - p0;
+  for (int idx = threadIdx.x; idx < MB_PM; idx += 256) Pm[idx] = P[idx];
+  __syncthreads();
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= n || (j >= p0 && j < p1)) return;
+  double x[MB_KB];
+#pragma unroll
+  for (int k = 0; k < MB_KB; ++k) x[k] = (k < kb) ? BM(p0 + k, j) : 0.0;
+  if (!backward) {
+#pragma unroll
+    for (int i = 0; i < MB_KB; ++i) {
+      if (i < kb) {
+        x[i] = x[i] * Pm[i];
+#pragma unroll
+        for (int r = i + 1; r < MB_KB; ++r)
+          if (r < kb) x[r] = x[r] - Pm[MB_KB + r * MB_KB + i] * x[i];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int i = 1; i < MB_KB; ++i) {
+      if (i < kb) {
+        const double rv = Rq[(long)i * n + j];
+#pragma unroll
+        for (int r = 0; r < i; ++r) x[r] = x[r] - Pm[MB_KB + r * MB_KB + i] * rv;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < MB_KB; ++k)
+    if (k < kb) BM(p0 + k, j) = x[k];
 }
 
 // Rq(k, j) = B(p0 + k, j), k < kb (row-major, ld n)
@@ -194,6 +264,7 @@ size_t matinv_blocked_scratch(int n, int count) {
 
 hipError_t launch_matinv_blocked(double *mats, long ms, int n, int count, int *zp,
                                  double *scratch, hipStream_t st) {
+  if (n < MB_PM / MB_KB + 1) return hipErrorInvalidValue;  // pm must fit a matrix's multiplier rows
   double *rq = scratch;
   double *mult = scratch + (size_t)count * MB_KB * n;
   auto copy_rows = [&](int p0, int kb) {
@@ -212,10 +283,18 @@ hipError_t launch_matinv_blocked(double *mats, long ms, int n, int count, int *z
     hipLaunchKernelGGL(mb_trail_kernel, grid, dim3(256), 0, st, mats, ms, n, p0, kb, r_lo, r_hi,
                        rq, mult);
   };
+  // a panel's pivots and multipliers live in the multiplier scratch, whose
+  // tri / trail use of the same panel is over before they are written
+  const long pm_stride = (long)n * MB_KB;
+  auto panel = [&](int p0, int p1, int backward) {
+    hipLaunchKernelGGL(mb_diag_kernel, dim3(count), dim3(256), 0, st, mats, ms, n, p0, p1, backward,
+                       rq, backward ? nullptr : zp, mult, pm_stride);
+    hipLaunchKernelGGL(mb_cols_kernel, dim3((n + 255) / 256, count), dim3(256), 0, st, mats, ms, n,
+                       p0, p1, backward, rq, mult, pm_stride);
+  };
   for (int p0 = 0; p0 < n; p0 += MB_KB) {
     const int p1 = min(n, p0 + MB_KB), kb = p1 - p0;
-    hipLaunchKernelGGL(mb_panel_kernel, dim3(count), dim3(256), 0, st, mats, ms, n, p0, p1, 0,
-                       rq, zp);
+    panel(p0, p1, 0);
     if (p1 < n) {
       copy_rows(p0, kb);
       tri(p0, kb, p1, n);
@@ -227,8 +306,7 @@ hipError_t launch_matinv_blocked(double *mats, long ms, int n, int count, int *z
     copy_rows(q0, kb);
     tri(q0, kb, 0, q0);
     trail(q0, kb, 0, q0);
-    hipLaunchKernelGGL(mb_panel_kernel, dim3(count), dim3(256), 0, st, mats, ms, n, q0, q1, 1,
-                       rq, zp);
+    panel(q0, q1, 1);
   }
   return hipGetLastError();
 }
