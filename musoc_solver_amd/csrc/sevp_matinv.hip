@@ -164,7 +164,8 @@ __global__ void mb_copy_rows_kernel(const double *mats, long ms, int n, int p0, 
 }
 
 // Rows [r_lo, r_hi): apply the kb steps p0.. to the panel columns, recording
-// the multipliers m(r, k) (row-major, ld MB_KB).
+// the multipliers m(r, k) at M[k * n + r] (consecutive rows adjacent, so the
+// stores of a wavefront are contiguous).
 __global__ __launch_bounds__(256) void mb_tri_kernel(double *mats, long ms, int n, int p0, int kb,
                                                      int r_lo, int r_hi, const double *rqs,
                                                      double *mult) {
@@ -186,7 +187,7 @@ __global__ __launch_bounds__(256) void mb_tri_kernel(double *mats, long ms, int 
   for (int k = 0; k < MB_KB; ++k) {
     if (k >= kb) break;
     const double b = cur[k];
-    M[(long)r * MB_KB + k] = b;
+    M[(long)k * n + r] = b;
 #pragma unroll
     for (int j = 0; j < MB_KB; ++j) {
       if (j >= kb) break;
@@ -214,9 +215,9 @@ __global__ __launch_bounds__(256) void mb_trail_kernel(double *mats, long ms, in
   if (r0 >= r_hi) return;
   const int t = threadIdx.x;
   for (int idx = t; idx < MT_R * MB_KB; idx += 256) {
-    const int k = idx % MB_KB, rr = idx / MB_KB;
+    const int rr = idx % MT_R, k = idx / MT_R;
     const int r = r0 + rr;
-    Ms[rr][k] = (r < r_hi && k < kb) ? M[(long)r * MB_KB + k] : 0.0;
+    Ms[rr][k] = (r < r_hi && k < kb) ? M[(long)k * n + r] : 0.0;
   }
   for (int idx = t; idx < MB_KB * MT_C; idx += 256) {
     const int cc = idx % MT_C, k = idx / MT_C;
