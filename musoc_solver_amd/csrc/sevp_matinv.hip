@@ -183,16 +183,19 @@ __global__ __launch_bounds__(256) void mb_tri_kernel(double *mats, long ms, int 
   double cur[MB_KB];
 #pragma unroll
   for (int j = 0; j < MB_KB; ++j) cur[j] = (j < kb) ? BM(r, p0 + j) : 0.0;
+  // guarded, not broken out of, so that the loops unroll and cur stays in registers
 #pragma unroll
   for (int k = 0; k < MB_KB; ++k) {
-    if (k >= kb) break;
-    const double b = cur[k];
-    M[(long)k * n + r] = b;
+    if (k < kb) {
+      const double b = cur[k];
+      M[(long)k * n + r] = b;
 #pragma unroll
-    for (int j = 0; j < MB_KB; ++j) {
-      if (j >= kb) break;
-      const double c = (j == k) ? 0.0 : cur[j];
-      cur[j] = c - b * D[k][j];
+      for (int j = 0; j < MB_KB; ++j) {
+        if (j < kb) {
+          const double c = (j == k) ? 0.0 : cur[j];
+          cur[j] = c - b * D[k][j];
+        }
+      }
     }
   }
 #pragma unroll
