@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 BASINS = [
     (64, 64, 1, 0.20, False, 8),
     (136, 96, 2, 0.30, False, 3),
-    (200, 152, 3, 0.45, False, 17),
+    (200, 152, 3, 0.45, False, 40),  # more than 32 RHS: a second 32-row product group
     (96, 40, 4, 0.25, True, 5),
     (48, 120, 5, 0.35, False, 8),
     (512, 48, 6, 0.30, False, 4),
@@ -33,7 +33,7 @@ def make_basins(precision):
                                land_frac=land, h_land=50.0)
         ie = autotune.adaptive_ie(case.coefs, max_growth=1e3, hmax=8, hmin=2, periodic=periodic)
         form = S.FORM_FULL_PERIODIC if periodic else S.FORM_FULL
-        plan = S.SevpPlan(case.coefs, ie, form=form, periodic=periodic, nrhs_max=32,
+        plan = S.SevpPlan(case.coefs, ie, form=form, periodic=periodic, nrhs_max=64,
                           precision=precision)
         plan.build_influence()
         s = synth.make_rhs(case.mask, 7100 + seed, nrhs)
