@@ -499,23 +499,26 @@ def repbir(ax, ay, bb, cx, cy, rinv, rinv1, dum0, dum1, dum2, f, h, x, ie, i0, i
     ``x_off`` are element offsets into the coefficient arrays and X, the
     equivalent of the Fortran caller passing AL(IL,1) and X(IL,1)."""
     cs = [_p(_f32(c), coef_off) for c in (ax, ay, bb, cx, cy)]
+    ie32 = _i32(ie)  # kept alive across the call
     _check(lib().sevp_repbir(*cs, _p(rinv), _p(rinv1), _p(dum0), _p(dum1), _p(dum2), _p(_f32(f)),
-                             _p(h), _p(_f64(x), x_off), _p(_i32(ie)), i0, i2, m0, m2, nblk))
+                             _p(h), _p(_f64(x), x_off), _p(ie32), i0, i2, m0, m2, nblk))
 
 
 def repbir_period(ax, ay, bb, cx, cy, rinv, rinv1, dum0, dum1, dum2, f, h, x, ie, i0, i2, m0, m2,
                   nblk):
     """REPBIR_PERIOD (MAIN/template/GLOBAL/src/sevp.f90:109-217), M0=I0, M2=I2."""
     cs = [_p(_f32(c)) for c in (ax, ay, bb, cx, cy)]
+    ie32 = _i32(ie)  # kept alive across the call
     _check(lib().sevp_repbir_period(*cs, _p(rinv), _p(rinv1), _p(dum0), _p(dum1), _p(dum2),
-                                    _p(_f32(f)), _p(h), _p(_f64(x)), _p(_i32(ie)), i0, i2, m0, m2,
+                                    _p(_f32(f)), _p(h), _p(_f64(x)), _p(ie32), i0, i2, m0, m2,
                                     nblk))
 
 
 def _full(fn, ax, ay, bb, cx, cy, rinv, rinv1, dum0, dum1, dum2, f, h, x, ie, i0, i2, nblk):
     cs = [_p(_f32(c)) for c in (ax, ay, bb, cx, cy)]
+    ie32 = _i32(ie)  # kept alive across the call
     _check(fn(*cs, _p(rinv), _p(rinv1), _p(dum0), _p(dum1), _p(dum2), _p(_f32(f)), _p(h),
-              _p(_f64(x)), _p(_i32(ie)), i0, i2, nblk))
+              _p(_f64(x)), _p(ie32), i0, i2, nblk))
 
 
 def rep1(*args):
@@ -531,14 +534,16 @@ def rep(*args):
 def pre(ax, ay, bb, cx, cy, rinv, rinv1, h, ie, i2, m0, m2, nblk, coef_off=0):
     """PRE (PREP/template/GLOBAL/src/prep_GLOBAL.f90:457-513); RINV, RINV1 out."""
     cs = [_p(_f32(c), coef_off) for c in (ax, ay, bb, cx, cy)]
-    _check(lib().sevp_pre(*cs, _p(_f64(rinv)), _p(_f64(rinv1)), _p(h), _p(_i32(ie)), i2, m0, m2,
+    ie32 = _i32(ie)  # kept alive across the call
+    _check(lib().sevp_pre(*cs, _p(_f64(rinv)), _p(_f64(rinv1)), _p(h), _p(ie32), i2, m0, m2,
                           nblk))
 
 
 def pre1(ax, ay, bb, cx, cy, rinv, rinv1, h, ie, i0, i2, nblk):
     """PRE1 (PREP/template/GLOBAL/src/prep_GLOBAL.f90:327-386)."""
     cs = [_p(_f32(c)) for c in (ax, ay, bb, cx, cy)]
-    _check(lib().sevp_pre1(*cs, _p(_f64(rinv)), _p(_f64(rinv1)), _p(h), _p(_i32(ie)), i0, i2,
+    ie32 = _i32(ie)  # kept alive across the call
+    _check(lib().sevp_pre1(*cs, _p(_f64(rinv)), _p(_f64(rinv1)), _p(h), _p(ie32), i0, i2,
                            nblk))
 
 

@@ -176,3 +176,43 @@ def test_metgen_global_operator(oracle, device):
     torch.cuda.synchronize()
     assert ok and rel.max() <= 1e-10, (rel.max(), st.iterations)
     plan.close()
+
+
+def test_quarter_degree_mixed_64rhs(oracle, device):
+    """The 0.25 degree configuration as the benchmark runs it: 1440x720
+    (+-80 deg), block height 2 (NB0 = 360), 64 right-hand sides in one call,
+    FP32 MFMA corrections refined in FP64 to 1e-10.  Checked per RHS against
+    the plain FP64 solve of the same plan (the reference's REP1 arithmetic,
+    bitwise the oracle at every size the oracle can run) and, for two RHS,
+    with the oracle's own p_sqprod residual."""
+    import torch
+    i2, j2, nrhs = 1440, 720, 64
+    print(f"[cfg-0.25] {i2}x{j2} nrhs={nrhs}", flush=True)
+    case = synth.make_case(i2, j2, 3000, metric="latlon", lat_max=80.0, h_land=50.0)
+    ie = synth.reference_ie(j2, synth.nb0_for_height(j2, 2))
+    s = synth.make_rhs(case.mask, 3001, nrhs)
+    plan = S.SevpPlan(case.coefs, ie, form=S.FORM_FULL_PERIODIC, nrhs_max=nrhs,
+                      precision=S.PREC_FP64)
+    assert plan.build_influence() == 0
+    s_d = torch.from_numpy(np.ascontiguousarray(s.transpose(2, 1, 0))).cuda()
+    x64 = torch.zeros((nrhs, j2 + 2, i2 + 2), dtype=torch.float64, device="cuda")
+    rel64, st64, ok64 = plan.solve_refined_dev(s_d, x64, tol=1e-10, max_iter=4)
+    assert ok64, rel64.max()
+    print(f"[cfg-0.25] plain FP64: {st64.iterations} pass(es), max rel resid {rel64.max():.2e}",
+          flush=True)
+    plan.set_precision(S.PREC_FP32)
+    x32 = torch.zeros_like(x64)
+    rel, st, ok = plan.solve_refined_dev(s_d, x32, tol=1e-10, max_iter=10)
+    torch.cuda.synchronize()
+    print(f"[cfg-0.25] mixed: {st.iterations} passes, max rel resid {rel.max():.2e}", flush=True)
+    assert ok and rel.max() <= 1e-10, (rel.max(), st.iterations)
+    d = (x32 - x64)[:, 1:-1, 1:-1].reshape(nrhs, -1).norm(dim=1)
+    n = x64[:, 1:-1, 1:-1].reshape(nrhs, -1).norm(dim=1)
+    err = (d / n).cpu().numpy()
+    print(f"[cfg-0.25] mixed vs plain FP64: max rel err {err.max():.2e}", flush=True)
+    assert err.max() <= 1e-9, err.max()
+    x = x32.cpu().numpy().transpose(2, 1, 0)
+    for r in (0, nrhs - 1):
+        ro = oracle.rel_residual(case.coefs, s[:, :, r], np.asfortranarray(x[:, :, r]))
+        assert ro <= 1.5e-10, (r, ro)
+    plan.close()
