@@ -415,6 +415,7 @@ def basins_leg(a):
         ie = autotune.adaptive_ie(case.coefs, max_growth=1e3, hmax=8, hmin=2, periodic=False)
         prec = S.PREC_FP32 if a.basin_prec == "mixed" else S.PREC_FP64_MFMA
         p = S.SevpPlan(case.coefs, ie, form=S.FORM_FULL, nrhs_max=nrhs, precision=prec)
+        p.set_pass(S.PASS_SPLITK)  # the batch's pass kernels, also for the one-by-one solves
         p.build_influence()
         s_np = synth.make_rhs(case.mask, seed + 1, nrhs)
         ss.append(torch.from_numpy(np.ascontiguousarray(s_np.transpose(2, 1, 0))).cuda())

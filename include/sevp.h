@@ -132,7 +132,9 @@ int sevp_solve_refined_dev(sevp_plan plan, const float *s_dev, double *x_dev, in
  * problem, not as all of them together.
  * Requirements: full forms, influence matrices present, one precision for
  * all plans (SEVP_PREC_FP32 or SEVP_PREC_FP64_MFMA), I2 divisible by 4, and
- * one product slice length (all I2 <= 1536, or all I2 > 1536).  The
+ * one product slice length (all I2 <= 1536, or all I2 > 1536).  The batch
+ * runs the split-K pass kernels and switches its plans to them
+ * (sevp_set_pass).  The
  * batch uses the plans' scratch arrays: do not run a plan on its own while a
  * batched solve that holds it runs.  The plans must outlive the batch. */
 typedef struct sevp_batch_s *sevp_batch;
@@ -208,6 +210,46 @@ int sevp_set_graphs(sevp_plan plan, int on);
  * (default 1).  The result is bitwise the same for any grouping of FP64
  * passes; mixed-precision passes are deterministic for a given grouping. */
 int sevp_set_rhs_groups(sevp_plan plan, int groups);
+
+/* Kernels of a full-width pass with MFMA corrections (SEVP_PREC_FP32,
+ * SEVP_PREC_FP64_MFMA).
+ *   SEVP_PASS_SPLITK  four launches per block: march, split-K product into
+ *                     slice partials, product, march; the consumers add the
+ *                     partials.  Operands in lane tiles.  Any block height;
+ *                     the batched solves use it.
+ *   SEVP_PASS_FUSED   one launch per block and direction: each workgroup
+ *                     sums whole dot products and marches the next block (or
+ *                     forms the next closure residual) in its epilogue.
+ *                     Column-major operands.  Block heights <= 4, I2
+ *                     divisible by 4.  The default where it applies.
+ * Both give the same FP64 marching and closure residuals; their products
+ * add the same terms in different orders, so results differ at rounding
+ * level.  Each is deterministic.  A products-only plan keeps the layout it
+ * was built with.  No counterpart in the reference (REP1's corrections are
+ * sequential loops, MAIN/template/GLOBAL/src/sevp.f90:11-107). */
+#define SEVP_PASS_SPLITK 0
+#define SEVP_PASS_FUSED 1
+int sevp_set_pass(sevp_plan plan, int kind);
+int sevp_get_pass(sevp_plan plan, int *kind);
+
+/* Where the influence matrices live (set before sevp_build_influence or
+ * sevp_load_influence; changing it drops the matrices built so far).
+ *   SEVP_STORE_FP64      RINV / RINV1 in FP64 on the device (the default),
+ *                        plus the copies the correction arithmetic needs.
+ *   SEVP_STORE_PRODUCTS  only the FP32 product operands (RINV, RINV1 and the
+ *                        coupling operators, 3 * NB0 * I2^2 * 4 bytes): the
+ *                        FP64 matrices of one block at a time are built (or
+ *                        uploaded) and rounded as soon as they are complete.
+ *                        Full forms with SEVP_PREC_FP32 only; sevp_get_influence
+ *                        and switching to another precision are refused.  For
+ *                        grids whose FP64 set does not fit in HBM (3600 x 1800
+ *                        with 2-row blocks: 186 GB of FP64 matrices, 140 GB of
+ *                        FP32 operands).
+ * No counterpart in the reference, which keeps RINV / RINV1 as REAL(8)
+ * module arrays (MAIN/template/GLOBAL/src/grid_var_GLOBAL.f90:147-155). */
+#define SEVP_STORE_FP64 0
+#define SEVP_STORE_PRODUCTS 1
+int sevp_set_storage(sevp_plan plan, int storage);
 
 /* Switch the arithmetic of the correction products of an existing
  * full-width plan (SEVP_PREC_*); FP32 copies of the influence matrices are

@@ -22,7 +22,7 @@ FSRC = os.path.join(PKG, "fortran")
 LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libsevp_hip.so")
 FLIB = os.path.join(LIBDIR, "libsevp_fortran.so")
-HIP_SOURCES = ["sevp_bir.hip", "sevp_full.hip", "sevp_matinv.hip", "sevp_mixed.hip", "sevp_caller.hip",
+HIP_SOURCES = ["sevp_bir.hip", "sevp_full.hip", "sevp_matinv.hip", "sevp_mixed.hip", "sevp_fused.hip", "sevp_caller.hip",
                "sevp_krylov.hip", "sevp_capi.hip", "sevp_evp.cpp"]
 ARCH = "gfx950"
 

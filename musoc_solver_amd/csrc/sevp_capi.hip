@@ -241,6 +241,9 @@ struct sevp_plan_s {
   std::vector<hipStream_t> gstreams;
   std::vector<hipEvent_t> gevents;
   bool have_inv = false;
+  // SEVP_STORE_FP64: RINV / RINV1 kept in FP64 (allocated on first build or
+  // load); SEVP_STORE_PRODUCTS: only the FP32 product operands are kept
+  int storage = SEVP_STORE_FP64;
   DBuf<float> coef;     // 5 * i2*j2 (AL, AB, AC, AR, AT)
   DBuf<int> ie_d;
   DBuf<double> rinv, rinv1;      // reference layout
@@ -248,6 +251,15 @@ struct sevp_plan_s {
   DBuf<float> w32;               // mixed: W_b = RINV_b diag(CY(:,IE(b-1)-1)) RINV1_(b-1), b >= 2
   DBuf<double> rinv64t, rinv1_64t, w64t;  // FP64 MFMA: the same three in FP64 lane tiles
   bool has32 = false, has64 = false;      // the copies above match the influence matrices
+  // the fused pass (one launch per block step) reads column-major operands:
+  // FP32 copies of RINV, RINV1 and W, or (FP64 MFMA) RINV and RINV1
+  // themselves and W in FP64
+  int pass_kind = SEVP_PASS_SPLITK;
+  DBuf<float> rinv32c, rinv1_32c, w32c;
+  DBuf<double> w64c;
+  bool has32c = false, has64c = false;
+  DBuf<double> fe, fh, fhc;  // fused pass: error rows and c / cpl rows, two parities each
+  DBuf<float> fe32;
   // solve scratch
   DBuf<double> e, dum0, hfree, hcpl, g;  // full forms
   DBuf<float> e32;                       // mixed: FP32 copy of the error rows
@@ -348,6 +360,14 @@ struct FullCtx {
   float *e32;        // FP32 copy of e (mixed precision), else null
   float *kpart;      // mixed: split-K partial products
   double *kpart64;   // FP64 MFMA: split-K partial products (selects that arithmetic)
+  // fused pass (one launch per block step): column-major operands and the
+  // two-parity scratch rows (each parity half holds fhalf elements)
+  bool fused, fused64;
+  const float *rinv32c, *rinv1_32c, *w32c;
+  const double *w64c;
+  double *fe, *fh, *fhc;
+  float *fe32;
+  long fhalf;
   KProf *prof;
 };
 
@@ -523,8 +543,122 @@ static int run_mixed_pass(const FullCtx &c, const float *f32, const double *f64,
   return SEVP_OK;
 }
 
+// One REP1 / REP pass with one launch per block step (sevp_fused.hip): the
+// march of block 1 with its closure residual; for b = 1..NB0-1 the product
+// e_b RINV1_b with block b+1's march and closure residual; for b = NB0..1 the
+// product e_b [RINV_b | W_b] with the free-row update of block b, the closure
+// residual of block b-1 and the homogeneous march of block b+1; the
+// homogeneous march of block 1; and, periodic, the halo fix.  The error rows
+// and the c / cpl rows alternate between two buffers by block parity, so no
+// launch reads what it writes.
+static int run_fused_pass(const FullCtx &c, const float *f32, const double *f64, long f_stride,
+                          double *x, long x_stride, int nrhs, hipStream_t st) {
+  const bool dbl = c.fused64;
+  const long ww = (long)c.i2 * c.i2;
+  double *E[2] = {c.fe, c.fe + c.fhalf};
+  float *E32[2] = {c.fe32, c.fe32 + c.fhalf};
+  double *HF[2] = {c.fh, c.fh + c.fhalf}, *HC[2] = {c.fhc, c.fhc + c.fhalf};
+  auto erow = [&](int b) -> const void * {
+    return dbl ? (const void *)E[b & 1] : (const void *)E32[b & 1];
+  };
+  auto march = [&](int mode, int b) -> int {
+    MarchArgs m{};
+    m.ax = c.ax; m.ay = c.ay; m.bb = c.bb; m.cx = c.cx; m.cy = c.cy;
+    m.cld = c.i2;
+    m.i2 = c.i2; m.j2 = c.j2; m.periodic = c.periodic;
+    m.x = x; m.x_stride = x_stride;
+    m.js = (b != 1) ? c.ie[b - 2] : 1; m.jf = c.ie[b - 1] - 2; m.ieb = c.ie[b - 1];
+    m.w = c.i2;
+    m.map = ColMap{1, 0, 0};
+    m.nrhs = nrhs;
+    m.cpl_off = -1;
+    if (mode == MODE_PART) {
+      m.f = f32; m.f64 = f64; m.f_stride = f_stride;
+      m.resid = 1;
+      m.e = dbl ? E[b & 1] : nullptr;
+      m.e32 = dbl ? nullptr : E32[b & 1];
+    } else {
+      m.hfree = HF[b & 1];
+      m.hcpl = (b != 1) ? HC[b & 1] : nullptr;
+      if (m.jf < m.js) return SEVP_OK;  // a one-row block has no homogeneous march
+    }
+    if (c.prof) c.prof->start(SEVP_KCLASS_MARCH);
+    HIPC(launch_march(mode, m, nrhs, st));
+    if (c.prof) c.prof->stop(SEVP_KCLASS_MARCH);
+    return SEVP_OK;
+  };
+  auto timed = [&](auto launch, long wgs, unsigned long long *&tspan) -> int {
+    tspan = c.prof ? c.prof->next_span(SEVP_KCLASS_CORR, wgs) : nullptr;
+    const auto ev = c.prof ? c.prof->start_ext(SEVP_KCLASS_CORR)
+                           : std::pair<hipEvent_t, hipEvent_t>{nullptr, nullptr};
+    HIPC(launch(ev.first, ev.second));
+    if (c.prof) c.prof->stop_ext(SEVP_KCLASS_CORR, ev.first != nullptr);
+    return SEVP_OK;
+  };
+  CHK(march(MODE_PART, 1));
+  for (int b = 1; b < c.nb0; ++b) {
+    FusedFwdArgs a{};
+    a.a = erow(b);
+    a.b = dbl ? (const void *)(c.rinv1 + (b - 1) * ww) : (const void *)(c.rinv1_32c + (b - 1) * ww);
+    a.M = nrhs; a.i2 = c.i2; a.periodic = c.periodic;
+    a.js = c.ie[b - 1]; a.jf = c.ie[b] - 2; a.ieb = c.ie[b];
+    a.steps = a.jf - a.js + 1;
+    a.ax = c.ax; a.ay = c.ay; a.bb = c.bb; a.cx = c.cx; a.cy = c.cy;
+    a.f = f32; a.f64 = f64; a.f_stride = f_stride;
+    a.x = x; a.x_stride = x_stride;
+    a.e_out = dbl ? E[(b + 1) & 1] : nullptr;
+    a.e32_out = dbl ? nullptr : E32[(b + 1) & 1];
+    FusedFwdArgs sz = a;
+    if (fused_fwd_prepare(sz)) return fail(SEVP_E_ARG, "block too tall for the fused pass");
+    CHK(timed([&](hipEvent_t e0, hipEvent_t e1) { return launch_fused_fwd(a, dbl, st, e0, e1); },
+              fused_fwd_workgroups(sz), a.tspan));
+  }
+  for (int b = c.nb0; b >= 1; --b) {
+    FusedBwdArgs a{};
+    a.a = erow(b);
+    a.b = dbl ? (const void *)(c.rinv + (b - 1) * ww) : (const void *)(c.rinv32c + (b - 1) * ww);
+    a.bw = (b == 1) ? nullptr
+                    : dbl ? (const void *)(c.w64c + (b - 2) * ww) : (const void *)(c.w32c + (b - 2) * ww);
+    a.M = nrhs; a.i2 = c.i2; a.periodic = c.periodic;
+    a.js = (b != 1) ? c.ie[b - 2] : 1;
+    a.do_resid = b != 1;
+    a.ax = c.ax; a.ay = c.ay; a.bb = c.bb; a.cx = c.cx; a.cy = c.cy;
+    a.f = f32; a.f64 = f64; a.f_stride = f_stride;
+    a.x = x; a.x_stride = x_stride;
+    a.e_out = dbl ? E[(b - 1) & 1] : nullptr;
+    a.e32_out = dbl ? nullptr : E32[(b - 1) & 1];
+    a.hfree_out = HF[b & 1];
+    a.hcpl_out = HC[b & 1];
+    if (b != c.nb0) {  // the homogeneous march of block b+1, from the previous launch's rows
+      MarchArgs m{};
+      m.ax = c.ax; m.ay = c.ay; m.bb = c.bb; m.cx = c.cx; m.cy = c.cy;
+      m.cld = c.i2;
+      m.i2 = c.i2; m.j2 = c.j2; m.periodic = c.periodic;
+      m.x = x; m.x_stride = x_stride;
+      m.js = c.ie[b - 1]; m.jf = c.ie[b] - 2; m.ieb = c.ie[b];
+      m.hfree = HF[(b + 1) & 1];
+      m.hcpl = HC[(b + 1) & 1];
+      m.w = c.i2;
+      m.map = ColMap{1, 0, 0};
+      m.nrhs = nrhs;
+      m.cpl_off = -1;
+      a.hom = m;
+      a.has_hom = 1;
+    }
+    FusedBwdArgs sz = a;
+    int K = 1;
+    if (fused_bwd_prepare(sz, &K)) return fail(SEVP_E_ARG, "shape refused by the fused pass");
+    CHK(timed([&](hipEvent_t e0, hipEvent_t e1) { return launch_fused_bwd(a, dbl, st, e0, e1); },
+              fused_bwd_workgroups(sz), a.tspan));
+  }
+  CHK(march(MODE_HOM, 1));
+  if (c.periodic) HIPC(launch_halo_fix(x, x_stride, c.i2, c.j2, nrhs, st));
+  return SEVP_OK;
+}
+
 static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, long f_stride,
                          double *x, long x_stride, int nrhs, hipStream_t st) {
+  if (c.fused) return run_fused_pass(c, f32, f64, f_stride, x, x_stride, nrhs, st);
   if (c.kpart || c.kpart64) return run_mixed_pass(c, f32, f64, f_stride, x, x_stride, nrhs, st);
   const long ww = (long)c.i2 * c.i2;
   const int i0 = c.i2 + 2, j0 = c.j2 + 2;
@@ -760,6 +894,8 @@ static int residual_norms(sevp_plan p, const float *s, const double *s64, long s
 
 static int alloc_mixed(sevp_plan p);
 static int make_low_precision_copies(sevp_plan p);
+static bool fused_eligible(sevp_plan p);
+static int alloc_operands(sevp_plan p);
 
 // ------------------------------------------------------------------ C ABI
 extern "C" {
@@ -808,6 +944,7 @@ int sevp_plan_create(const sevp_desc *desc, const float *al, const float *ab, co
     p->w = d.i2;
     p->nmat = 1;
   }
+  p->pass_kind = fused_eligible(p) ? SEVP_PASS_FUSED : SEVP_PASS_SPLITK;
   int rc = SEVP_OK;
   auto bail = [&](int code) {
     sevp_plan_destroy(p);
@@ -829,8 +966,6 @@ int sevp_plan_create(const sevp_desc *desc, const float *al, const float *ab, co
   if ((rc = p->ie_d.alloc(d.nb0))) return bail(rc);
   if (hipMemcpy(p->ie_d.p, p->ie.data(), d.nb0 * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
     return bail(fail(SEVP_E_HIP, "IE upload failed"));
-  if ((rc = p->rinv.alloc(p->rinv_count()))) return bail(rc);
-  if ((rc = p->rinv1.alloc(p->rinv1_count()))) return bail(rc);
   if ((rc = p->zp.alloc(1))) return bail(rc);
   if (!matinv_uses_lds(p->w))
     if ((rc = p->b1s.alloc(matinv_blocked_scratch(p->w, p->nmat)))) return bail(rc);
@@ -902,26 +1037,138 @@ size_t sevp_plan_device_bytes(sevp_plan p) {
          p->zp.bytes();
 }
 
+// FP64 influence matrices of an SEVP_STORE_FP64 plan, allocated on first use.
+static int ensure_inverses(sevp_plan p) {
+  if (!p->rinv.p) CHK(p->rinv.alloc(p->rinv_count()));
+  if (!p->rinv1.p) CHK(p->rinv1.alloc(p->rinv1_count()));
+  return SEVP_OK;
+}
+
+// SEVP_STORE_PRODUCTS: the influence matrices of one block at a time in FP64,
+// turned into the FP32 product operands (RINV_b, RINV1_b and the coupling
+// operator W_b in lane tiles) as soon as they are complete, so the plan never
+// holds the FP64 set (186 GB at 3600 columns and NB0 = 900).  Block b needs
+// only RINV1_(b-1) of the block before it (PRE1's coupling row and W_b), kept
+// in a two-slot ring.  With h_rinv / h_rinv1 (host, reference layout) the
+// blocks are uploaded instead of built.  The FP64 arithmetic of each block is
+// the one run_pre and make_low_precision_copies perform, so the FP32 operands
+// are bitwise those of an SEVP_STORE_FP64 plan.
+static int build_products(sevp_plan p, const double *h_rinv, const double *h_rinv1,
+                          int *zero_pivots) {
+  if (p->d.precision != SEVP_PREC_FP32) return fail(SEVP_E_ARG, "products-only plans are FP32");
+  CHK(alloc_mixed(p));
+  CHK(alloc_operands(p));
+  const bool fused = p->pass_kind == SEVP_PASS_FUSED;
+  const int i2 = p->d.i2, nb0 = p->d.nb0;
+  const long ww = (long)i2 * i2;
+  const long tw = lane_tiles_floats(i2, i2);
+  hipStream_t st = p->stream;
+  DBuf<double> cur, tm, r1a, r1b, sc, wt;
+  CHK(cur.alloc(ww));
+  CHK(tm.alloc(ww));
+  CHK(r1a.alloc(ww));
+  CHK(r1b.alloc(ww));
+  CHK(sc.alloc(ww));
+  CHK(wt.alloc(ww));
+  double *r1[2] = {r1a.p, r1b.p};
+  CHK(zero_async(p->zp.p, sizeof(int), st));
+  const int periodic = p->d.form == SEVP_FORM_FULL_PERIODIC;
+  int jl = 1;
+  for (int nb = 1; nb <= nb0; ++nb) {
+    const int jh = p->ie[nb - 1];
+    double *r1cur = r1[(nb - 1) & 1], *r1prev = r1[nb & 1];  // RINV1 of blocks nb and nb-1
+    if (h_rinv) {
+      HIPC(hipMemcpyAsync(cur.p, h_rinv + (long)(nb - 1) * ww, ww * sizeof(double),
+                          hipMemcpyHostToDevice, st));
+      if (nb != nb0)
+        HIPC(hipMemcpyAsync(r1cur, h_rinv1 + (long)(nb - 1) * ww, ww * sizeof(double),
+                            hipMemcpyHostToDevice, st));
+    } else {
+      MarchArgs m{};
+      m.ax = p->al(); m.ay = p->ab(); m.bb = p->ac(); m.cx = p->ar(); m.cy = p->at();
+      m.cld = i2;
+      m.i2 = i2; m.j2 = p->d.j2;
+      m.periodic = periodic;
+      m.js = jl; m.jf = jh - 2; m.ieb = jh;
+      m.w = i2;
+      m.map = ColMap{1, 0, 0};
+      m.nrhs = 1;
+      m.rinv_b = cur.p;
+      m.tmat = (nb != nb0) ? tm.p : nullptr;
+      m.rinv1_prev = (nb != 1) ? r1prev : nullptr;
+      HIPC(launch_march(MODE_PRE, m, i2, st));
+      HIPC(launch_matinv(cur.p, ww, i2, 1, p->zp.p, p->b1s.p, st));
+      if (nb != nb0) {
+        GemmArgs gm{};
+        gm.a = cur.p; gm.a_sm = 1; gm.a_sk = i2;
+        gm.b = tm.p; gm.b_sk = 1; gm.b_sn = i2;
+        gm.M = i2; gm.N = i2; gm.K = i2;
+        gm.ep = EP_NEGSUB;
+        gm.out = r1cur;
+        gm.out_ld = i2;
+        HIPC(launch_gemm(gm, st));
+      }
+    }
+    if (fused) {
+      HIPC(launch_cvt_f32(cur.p, p->rinv32c.p + (long)(nb - 1) * ww, ww, st));
+      if (nb != nb0) HIPC(launch_cvt_f32(r1cur, p->rinv1_32c.p + (long)(nb - 1) * ww, ww, st));
+    } else {
+      HIPC(launch_cvt_lane_tiles(cur.p, p->rinv32.p + (long)(nb - 1) * tw, i2, i2, 1, st));
+      if (nb != nb0)
+        HIPC(launch_cvt_lane_tiles(r1cur, p->rinv1_32.p + (long)(nb - 1) * tw, i2, i2, 1, st));
+    }
+    if (nb >= 2) {  // W_nb = RINV_nb diag(CY(:, IE(nb-1)-1)) RINV1_(nb-1)
+      HIPC(launch_scale_cols(cur.p, p->at(), jl - 1, i2, sc.p, st));
+      GemmArgs gm{};
+      gm.a = sc.p; gm.a_sm = 1; gm.a_sk = i2;
+      gm.b = r1prev; gm.b_sk = 1; gm.b_sn = i2;
+      gm.M = i2; gm.N = i2; gm.K = i2;
+      gm.ep = EP_COLMAJ;
+      gm.out = wt.p; gm.out_ld = i2;
+      HIPC(launch_gemm(gm, st));
+      if (fused) HIPC(launch_cvt_f32(wt.p, p->w32c.p + (long)(nb - 2) * ww, ww, st));
+      else HIPC(launch_cvt_lane_tiles(wt.p, p->w32.p + (long)(nb - 2) * tw, i2, i2, 1, st));
+    }
+    jl = jh;
+  }
+  int z = 0;
+  HIPC(hipMemcpyAsync(&z, p->zp.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPC(hipStreamSynchronize(st));
+  if (zero_pivots) *zero_pivots = z;
+  p->has32 = !fused;
+  p->has32c = fused;
+  p->has64 = p->has64c = false;
+  p->have_inv = true;
+  return SEVP_OK;
+}
+
 // The operand copies of the current precision's products: FP32 lane tiles
 // (SEVP_PREC_FP32) or FP64 lane tiles (SEVP_PREC_FP64_MFMA) of RINV, RINV1
 // and the coupling operators W.
 static int make_low_precision_copies(sevp_plan p) {
   if (p->d.precision == SEVP_PREC_FP64 || p->d.form == SEVP_FORM_BIR) return SEVP_OK;
+  if (p->storage == SEVP_STORE_PRODUCTS) return SEVP_OK;  // made while the blocks were built
   const bool dbl = p->d.precision == SEVP_PREC_FP64_MFMA;
-  if (dbl ? p->has64 : p->has32) return SEVP_OK;
-  const int nb1 = p->d.nb0 > 1 ? p->d.nb0 - 1 : 1;
-  if (dbl) {
-    HIPC(launch_cvt_lane_tiles64(p->rinv.p, p->rinv64t.p, p->d.i2, p->d.i2, p->d.nb0, p->stream));
-    HIPC(launch_cvt_lane_tiles64(p->rinv1.p, p->rinv1_64t.p, p->d.i2, p->d.i2, nb1, p->stream));
-  } else {
-    HIPC(launch_cvt_lane_tiles(p->rinv.p, p->rinv32.p, p->d.i2, p->d.i2, p->d.nb0, p->stream));
-    HIPC(launch_cvt_lane_tiles(p->rinv1.p, p->rinv1_32.p, p->d.i2, p->d.i2, nb1, p->stream));
-  }
-  const long tw = lane_tiles_floats(p->d.i2, p->d.i2);
-  // W_b = RINV_b diag(CY(:, IE(b-1)-1)) RINV1_(b-1): the coupling row of the
-  // backward pass as one product with the error row (FP64, then FP32)
-  const int i2 = p->d.i2;
+  const bool fused = p->pass_kind == SEVP_PASS_FUSED;
+  bool &have = fused ? (dbl ? p->has64c : p->has32c) : (dbl ? p->has64 : p->has32);
+  if (have) return SEVP_OK;
+  CHK(alloc_operands(p));
+  const int i2 = p->d.i2, nb1 = p->d.nb0 > 1 ? p->d.nb0 - 1 : 1;
   const long ww = (long)i2 * i2;
+  const long tw = lane_tiles_floats(i2, i2);
+  if (fused && !dbl) {
+    HIPC(launch_cvt_f32(p->rinv.p, p->rinv32c.p, ww * p->d.nb0, p->stream));
+    HIPC(launch_cvt_f32(p->rinv1.p, p->rinv1_32c.p, ww * nb1, p->stream));
+  } else if (!fused && dbl) {
+    HIPC(launch_cvt_lane_tiles64(p->rinv.p, p->rinv64t.p, i2, i2, p->d.nb0, p->stream));
+    HIPC(launch_cvt_lane_tiles64(p->rinv1.p, p->rinv1_64t.p, i2, i2, nb1, p->stream));
+  } else if (!fused) {
+    HIPC(launch_cvt_lane_tiles(p->rinv.p, p->rinv32.p, i2, i2, p->d.nb0, p->stream));
+    HIPC(launch_cvt_lane_tiles(p->rinv1.p, p->rinv1_32.p, i2, i2, nb1, p->stream));
+  }
+  // W_b = RINV_b diag(CY(:, IE(b-1)-1)) RINV1_(b-1): the coupling row of the
+  // backward pass as one product with the error row (FP64, then the layout
+  // and precision of the products)
   if (p->d.nb0 > 1) {
     DBuf<double> t, w;
     CHK(t.alloc(ww));
@@ -934,19 +1181,25 @@ static int make_low_precision_copies(sevp_plan p) {
       gm.b = p->rinv1.p + (long)(nb - 2) * ww; gm.b_sk = 1; gm.b_sn = i2;
       gm.M = i2; gm.N = i2; gm.K = i2;
       gm.ep = EP_COLMAJ;
-      gm.out = w.p; gm.out_ld = i2;
+      gm.out = (fused && dbl) ? p->w64c.p + (long)(nb - 2) * ww : w.p;
+      gm.out_ld = i2;
       HIPC(launch_gemm(gm, p->stream));
-      if (dbl) HIPC(launch_cvt_lane_tiles64(w.p, p->w64t.p + (long)(nb - 2) * tw, i2, i2, 1, p->stream));
-      else HIPC(launch_cvt_lane_tiles(w.p, p->w32.p + (long)(nb - 2) * tw, i2, i2, 1, p->stream));
+      if (fused && !dbl) HIPC(launch_cvt_f32(w.p, p->w32c.p + (long)(nb - 2) * ww, ww, p->stream));
+      else if (!fused && dbl)
+        HIPC(launch_cvt_lane_tiles64(w.p, p->w64t.p + (long)(nb - 2) * tw, i2, i2, 1, p->stream));
+      else if (!fused)
+        HIPC(launch_cvt_lane_tiles(w.p, p->w32.p + (long)(nb - 2) * tw, i2, i2, 1, p->stream));
     }
     HIPC(hipStreamSynchronize(p->stream));
   }
-  (dbl ? p->has64 : p->has32) = true;
+  have = true;
   return SEVP_OK;
 }
 
 int sevp_build_influence(sevp_plan p, int *zero_pivots) {
   if (!p) return fail(SEVP_E_ARG, "null plan");
+  if (p->storage == SEVP_STORE_PRODUCTS) return build_products(p, nullptr, nullptr, zero_pivots);
+  CHK(ensure_inverses(p));
   CHK(zero_async(p->zp.p, sizeof(int), p->stream));
   CHK(zero_async(p->rinv1.p, p->rinv1.bytes(), p->stream));
   PreCtx c{};
@@ -968,7 +1221,7 @@ int sevp_build_influence(sevp_plan p, int *zero_pivots) {
     c.map = ColMap{1, 0, 0};
   }
   CHK(run_pre(c, p->stream));
-  p->has32 = p->has64 = false;
+  p->has32 = p->has64 = p->has32c = p->has64c = false;
   CHK(make_low_precision_copies(p));
   int z = 0;
   HIPC(hipMemcpyAsync(&z, p->zp.p, sizeof(int), hipMemcpyDeviceToHost, p->stream));
@@ -980,9 +1233,11 @@ int sevp_build_influence(sevp_plan p, int *zero_pivots) {
 
 int sevp_load_influence(sevp_plan p, const double *rinv, const double *rinv1) {
   if (!p || !rinv || !rinv1) return fail(SEVP_E_ARG, "null argument");
+  if (p->storage == SEVP_STORE_PRODUCTS) return build_products(p, rinv, rinv1, nullptr);
+  CHK(ensure_inverses(p));
   HIPC(hipMemcpy(p->rinv.p, rinv, p->rinv.bytes(), hipMemcpyHostToDevice));
   if (p->d.nb0 > 1) HIPC(hipMemcpy(p->rinv1.p, rinv1, p->rinv1.bytes(), hipMemcpyHostToDevice));
-  p->has32 = p->has64 = false;
+  p->has32 = p->has64 = p->has32c = p->has64c = false;
   CHK(make_low_precision_copies(p));
   HIPC(hipStreamSynchronize(p->stream));
   p->have_inv = true;
@@ -992,6 +1247,8 @@ int sevp_load_influence(sevp_plan p, const double *rinv, const double *rinv1) {
 int sevp_get_influence(sevp_plan p, double *rinv, double *rinv1) {
   if (!p || !rinv || !rinv1) return fail(SEVP_E_ARG, "null argument");
   if (!p->have_inv) return fail(SEVP_E_STATE, "influence matrices not built or loaded");
+  if (p->storage == SEVP_STORE_PRODUCTS)
+    return fail(SEVP_E_STATE, "the plan keeps only the FP32 product operands (SEVP_STORE_PRODUCTS)");
   HIPC(hipStreamSynchronize(p->stream));
   HIPC(hipMemcpy(rinv, p->rinv.p, p->rinv.bytes(), hipMemcpyDeviceToHost));
   if (p->d.nb0 > 1) HIPC(hipMemcpy(rinv1, p->rinv1.p, p->rinv1.bytes(), hipMemcpyDeviceToHost));
@@ -1028,6 +1285,12 @@ static FullCtx full_ctx(sevp_plan p) {
     c.w64t = p->w64t.p;
     c.kpart64 = p->kpart64.p;
   }
+  c.fused = (mixed || mfma64) && p->pass_kind == SEVP_PASS_FUSED;
+  c.fused64 = mfma64;
+  c.rinv32c = p->rinv32c.p; c.rinv1_32c = p->rinv1_32c.p; c.w32c = p->w32c.p;
+  c.w64c = p->w64c.p;
+  c.fe = p->fe.p; c.fh = p->fh.p; c.fhc = p->fhc.p; c.fe32 = p->fe32.p;
+  c.fhalf = (long)p->d.nrhs_max * p->d.i2;
   c.prof = &p->prof;
   return c;
 }
@@ -1047,6 +1310,10 @@ static FullCtx group_ctx(const FullCtx &c0, int r0) {
   const long part_rhs = (long)gemm_part_slices((int)i2, (int)i2, nullptr) * 2 * ((i2 + 15) / 16 * 16);
   if (c.kpart) c.kpart += r0 * part_rhs;
   if (c.kpart64) c.kpart64 += r0 * part_rhs;
+  if (c.fe) c.fe += r0 * i2;
+  if (c.fh) c.fh += r0 * i2;
+  if (c.fhc) c.fhc += r0 * i2;
+  if (c.fe32) c.fe32 += r0 * i2;
   return c;
 }
 
@@ -1061,20 +1328,111 @@ static int alloc_mixed(sevp_plan p) {
   const int i2 = p->d.i2, np = (i2 + 15) / 16 * 16;
   const size_t n1 = (size_t)gemm_part_slices(i2, i2, nullptr) * i2;
   const size_t n2 = (size_t)gemm_part_slices(2 * np, i2, nullptr) * 2 * np;
-  const size_t tw = (size_t)lane_tiles_floats(p->d.i2, p->d.i2);
-  const size_t nb1 = p->d.nb0 > 1 ? p->d.nb0 - 1 : 1;
+  const size_t nf = 2 * (size_t)nr * p->d.i2;  // two parities of (nrhs, I2)
+  if (!p->fe.p) {
+    CHK(p->fe.alloc(nf));
+    CHK(p->fh.alloc(nf));
+    CHK(p->fhc.alloc(nf));
+    CHK(p->fe32.alloc(nf));
+  }
   if (dbl) {
     CHK(p->kpart64.alloc(std::max(n1, n2) * nr));
-    CHK(p->rinv64t.alloc(tw * p->d.nb0));
-    CHK(p->rinv1_64t.alloc(tw * nb1));
-    CHK(p->w64t.alloc(tw * nb1));
     return SEVP_OK;
   }
   CHK(p->e32.alloc((size_t)nr * p->d.i2));
   CHK(p->kpart.alloc(std::max(n1, n2) * nr));
-  CHK(p->rinv32.alloc(tw * p->d.nb0));
-  CHK(p->rinv1_32.alloc(tw * nb1));
-  CHK(p->w32.alloc(tw * nb1));
+  return SEVP_OK;
+}
+
+// The product operands of the plan's precision and pass kernels, allocated
+// on first use: lane tiles for the split-K pass, column-major arrays for the
+// fused pass (FP64 MFMA reads RINV / RINV1 themselves and needs only W).
+static int alloc_operands(sevp_plan p) {
+  const size_t nb1 = p->d.nb0 > 1 ? p->d.nb0 - 1 : 1;
+  const bool dbl = p->d.precision == SEVP_PREC_FP64_MFMA;
+  if (p->pass_kind == SEVP_PASS_FUSED) {
+    const size_t ww = (size_t)p->d.i2 * p->d.i2;
+    if (dbl) {
+      if (!p->w64c.p) CHK(p->w64c.alloc(ww * nb1));
+    } else if (!p->rinv32c.p) {
+      CHK(p->rinv32c.alloc(ww * p->d.nb0));
+      CHK(p->rinv1_32c.alloc(ww * nb1));
+      CHK(p->w32c.alloc(ww * nb1));
+    }
+    return SEVP_OK;
+  }
+  const size_t tw = (size_t)lane_tiles_floats(p->d.i2, p->d.i2);
+  if (dbl) {
+    if (!p->rinv64t.p) {
+      CHK(p->rinv64t.alloc(tw * p->d.nb0));
+      CHK(p->rinv1_64t.alloc(tw * nb1));
+      CHK(p->w64t.alloc(tw * nb1));
+    }
+  } else if (!p->rinv32.p) {
+    CHK(p->rinv32.alloc(tw * p->d.nb0));
+    CHK(p->rinv1_32.alloc(tw * nb1));
+    CHK(p->w32.alloc(tw * nb1));
+  }
+  return SEVP_OK;
+}
+
+// The fused pass marches a block's rows inside a 16-column product tile and
+// keeps the 16 - 2 (h - 1) columns that stay exact: blocks up to 4 rows high.
+static bool fused_eligible(sevp_plan p) {
+  if (p->d.form == SEVP_FORM_BIR || p->d.i2 % 4 != 0 || p->d.i2 < 16) return false;
+  int prev = 1;
+  for (int b = 0; b < p->d.nb0; ++b) {
+    if (p->ie[b] - prev > 4) return false;
+    prev = p->ie[b];
+  }
+  return true;
+}
+
+int sevp_set_pass(sevp_plan p, int kind) {
+  if (!p) return fail(SEVP_E_ARG, "null plan");
+  if (kind != SEVP_PASS_SPLITK && kind != SEVP_PASS_FUSED) return fail(SEVP_E_ARG, "unknown pass kind");
+  if (kind == p->pass_kind) return SEVP_OK;
+  if (p->d.form == SEVP_FORM_BIR) return fail(SEVP_E_ARG, "the BIR form has no correction products to fuse");
+  if (kind == SEVP_PASS_FUSED && !fused_eligible(p))
+    return fail(SEVP_E_ARG, "the fused pass needs block heights <= 4 and I2 divisible by 4 (>= 16)");
+  if (p->storage == SEVP_STORE_PRODUCTS && p->have_inv)
+    return fail(SEVP_E_STATE, "a products-only plan keeps the operand layout it was built with");
+  HIPC(hipStreamSynchronize(p->stream));
+  drop_graphs(p);
+  const int old = p->pass_kind;
+  p->pass_kind = kind;
+  if (p->have_inv && p->d.precision != SEVP_PREC_FP64) {
+    if (int rc = make_low_precision_copies(p)) {
+      p->pass_kind = old;
+      return rc;
+    }
+  }
+  return SEVP_OK;
+}
+
+int sevp_get_pass(sevp_plan p, int *kind) {
+  if (!p || !kind) return fail(SEVP_E_ARG, "null argument");
+  *kind = p->pass_kind;
+  return SEVP_OK;
+}
+
+int sevp_set_storage(sevp_plan p, int storage) {
+  if (!p) return fail(SEVP_E_ARG, "null plan");
+  if (storage != SEVP_STORE_FP64 && storage != SEVP_STORE_PRODUCTS)
+    return fail(SEVP_E_ARG, "unknown storage");
+  if (storage == p->storage) return SEVP_OK;
+  if (storage == SEVP_STORE_PRODUCTS) {
+    if (p->d.form == SEVP_FORM_BIR) return fail(SEVP_E_ARG, "products-only storage needs a full-width form");
+    if (p->d.precision != SEVP_PREC_FP32)
+      return fail(SEVP_E_ARG, "products-only storage needs SEVP_PREC_FP32");
+  }
+  HIPC(hipStreamSynchronize(p->stream));
+  drop_graphs(p);
+  p->rinv.release();
+  p->rinv1.release();
+  p->has32 = p->has64 = p->has32c = p->has64c = false;
+  p->have_inv = false;
+  p->storage = storage;
   return SEVP_OK;
 }
 
@@ -1085,6 +1443,8 @@ int sevp_set_precision(sevp_plan p, int precision) {
   if (p->d.form == SEVP_FORM_BIR && precision != SEVP_PREC_FP64)
     return fail(SEVP_E_ARG, "BIR form runs the plain FP64 arithmetic only");
   if (precision == SEVP_PREC_BF16) return fail(SEVP_E_ARG, "BF16 corrections are not built");
+  if (p->storage == SEVP_STORE_PRODUCTS && precision != SEVP_PREC_FP32)
+    return fail(SEVP_E_ARG, "a plan that keeps only FP32 product operands runs SEVP_PREC_FP32 only");
   const int old = p->d.precision;
   p->d.precision = precision;
   if (precision != SEVP_PREC_FP64) {
@@ -1714,6 +2074,9 @@ int sevp_batch_create(const sevp_plan *plans, int nplans, sevp_batch *out) {
     prec = p->d.precision;
     if (!p->have_inv) return fail(SEVP_E_STATE, "influence matrices not built or loaded");
     if (p->d.i2 % 4 != 0) return fail(SEVP_E_ARG, "batched solves need I2 divisible by 4");
+    // the grouped launches are the split-K pass's; the plans switch to it so
+    // that their own solves give the batch's results
+    if (p->pass_kind != SEVP_PASS_SPLITK) CHK(sevp_set_pass(p, SEVP_PASS_SPLITK));
     int k = 0;
     gemm_part_slices(p->d.i2, p->d.i2, &k);
     if (ks && k != ks)

@@ -1,0 +1,392 @@
+// One launch per block step: the correction product of a full-width SEVP
+// pass fused with what depends on it, for the MFMA corrections (FP32 with
+// FP64 refinement, or FP64).
+//
+// The split-K pass (sevp_mixed.hip) takes four launches per block and writes
+// every product as slice partials that the next launch adds up.  Here a
+// workgroup computes whole dot products: its W wavefronts split K, each keeps
+// a 16 x 16 accumulator of the MFMA instruction, and the W partial tiles are
+// added in FP64 in wavefront order through LDS, so every output is one fixed
+// sum.  The epilogue then does the work that depends only on that output's
+// column:
+//
+//   forward, block b:  d = e_b RINV1_b.  Block b+1's first march row uses the
+//     corrected south row X(IE(b)) - d (never written back: the backward pass
+//     restores the uncorrected value anyway, REP1's DUM0), its closure
+//     residual e_(b+1) needs the marched rows at the neighbouring columns, so
+//     a tile computes 16 columns of d and marches them with DPP row shifts
+//     (16 lanes = 16 columns of one right-hand side) and writes only the
+//     16 - 2*steps columns that stay exact after `steps` rows (the trapezoid
+//     of the march kernels; recomputed columns repeat the same operations on
+//     the same operands, so they agree bitwise with the tile that owns them).
+//   backward, block b: [c | cpl] = e_b [RINV_b | W_b].  X(JS+1) += c and the
+//     closure residual of block b-1, which reads that row only at its own
+//     column, are done at once; c and cpl are stored for the homogeneous
+//     march of block b, which needs c at the neighbouring columns and runs in
+//     the next launch (extra workgroups there), when every column is complete.
+//
+// A pass is 2*NB0 + 1 launches instead of 4*NB0 + 2, each influence matrix is
+// read once for all right-hand sides, and no partial products go through
+// memory.  The workgroups of one column tile (one per 16 right-hand sides)
+// are placed on the same XCD so the tile's matrix columns are fetched from
+// HBM once.
+#include <hip/hip_ext.h>
+
+#include "sevp_dev.h"
+#include "sevp_internal.h"
+#include "sevp_march.h"
+
+namespace sevp {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <class T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  typedef f32x4 type;
+};
+template <>
+struct Vec4<double> {
+  typedef f64x4 type;
+};
+
+// Lane i of each 16-lane row receives lane i-1's (row_shr:1) or lane i+1's
+// (row_shl:1) value; the row's edge lane keeps its own.  The edge lanes are
+// trapezoid halo columns whose results are never stored.
+__device__ __forceinline__ double row_shr1(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, 0x111, 0xf, 0xf, false),
+                          __builtin_amdgcn_update_dpp(lo, lo, 0x111, 0xf, 0xf, false));
+}
+__device__ __forceinline__ double row_shl1(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, 0x101, 0xf, 0xf, false),
+                          __builtin_amdgcn_update_dpp(lo, lo, 0x101, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ f32x4 mfma16(f32x4 a, f32x4 b, f32x4 c, int t) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a[t], b[t], c, 0, 0, 0);
+}
+__device__ __forceinline__ f64x4 mfma16(f64x4 a, f64x4 b, f64x4 c, int t) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b[t], c, 0, 0, 0);
+}
+
+// Accumulator element i of lane group q holds row q4(q, i) of its column:
+// 4q + i for the FP32 instruction, q + 4i for the FP64 one.
+template <class T>
+__device__ __forceinline__ int acc_row(int q, int i) {
+  return sizeof(T) == 4 ? 4 * q + i : q + 4 * i;
+}
+
+// Four consecutive k of a run, zero at or past klen (klen a multiple of 4).
+template <class T>
+__device__ __forceinline__ typename Vec4<T>::type ld4(const T *base, int k, int klen) {
+  typedef typename Vec4<T>::type V;
+  const bool ok = k < klen;
+  const V v = *reinterpret_cast<const V *>(base + (ok ? k : 0));
+  return ok ? v : V{0, 0, 0, 0};
+}
+
+// Partial tile of one wavefront: rows m0 .. m0+15 of A (ld K) times the
+// column-major B column of this lane's column (lane % 16), over the chunks
+// [c_lo, c_hi) of 16 k.  Loads are issued G chunks at a time, one group
+// ahead of the MFMA steps (2G chunks in flight).
+template <class T, int G>
+__device__ __forceinline__ typename Vec4<T>::type part_tile(const T *a, int M, int K, int m0,
+                                                            const T *bcol, bool bok, int c_lo,
+                                                            int c_hi) {
+  typedef typename Vec4<T>::type V;
+  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+  const int m = m0 + r;
+  const T *arow = a + (long)(m < M ? m : 0) * K;
+  const int alen = m < M ? K : 0;
+  const int blen = bok ? K : 0;
+  V acc = V{0, 0, 0, 0};
+  V av[G], bv[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const int k = 16 * (c_lo + g) + 4 * q;
+    const bool in = c_lo + g < c_hi;
+    av[g] = ld4<T>(arow, k, in ? alen : 0);
+    bv[g] = ld4<T>(bcol, k, in ? blen : 0);
+  }
+  for (int c0 = c_lo; c0 < c_hi; c0 += G) {
+    V an[G], bn[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {  // the next group, issued before this group's MFMA steps
+      const int k = 16 * (c0 + G + g) + 4 * q;
+      const bool in = c0 + G + g < c_hi;
+      an[g] = ld4<T>(arow, k, in ? alen : 0);
+      bn[g] = ld4<T>(bcol, k, in ? blen : 0);
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc = mfma16(av[g], bv[g], acc, t);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      av[g] = an[g];
+      bv[g] = bn[g];
+    }
+  }
+  return acc;
+}
+
+template <class T>
+struct Chunks;
+template <>
+struct Chunks<float> {
+  static constexpr int G = 6;
+};
+template <>
+struct Chunks<double> {
+  static constexpr int G = 4;
+};
+
+// The product tile of a workgroup: W wavefronts, each its share of the K
+// chunks; their partial tiles go to LDS and thread t < 256 receives the FP64
+// sum over the wavefronts, in wavefront order, of row t / 16 and column
+// t % 16 of the tile.  Every thread of the workgroup must call it.
+template <class T, int W>
+__device__ __forceinline__ double product_tile(const T *a, int M, int K, int m0, const T *b,
+                                               int bcolumn, bool bok, T *red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, q = lane >> 4;
+  const int nchk = (K + 15) / 16;
+  const int c_lo = (int)((long)nchk * wave / W), c_hi = (int)((long)nchk * (wave + 1) / W);
+  const T *bcol = b + (long)(bok ? bcolumn : 0) * K;
+  const auto acc = part_tile<T, Chunks<T>::G>(a, M, K, m0, bcol, bok, c_lo, c_hi);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) red[wave * 256 + acc_row<T>(q, i) * 16 + (lane & 15)] = acc[i];
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x < 256)
+#pragma unroll
+    for (int w = 0; w < W; ++w) s += (double)red[w * 256 + threadIdx.x];
+  return s;
+}
+
+// XCD-aware placement: workgroup id -> (column tile, 16-row block).  The
+// dispatcher deals workgroups to the 8 XCDs in turn; the row blocks of one
+// column tile get ids that land on the same XCD one after another.
+__device__ __forceinline__ bool tile_of(int wg, int ntiles, int nmb, int &tile, int &mb) {
+  const int xcd = wg & 7, slot = wg >> 3;
+  tile = (slot / nmb) * 8 + xcd;
+  mb = slot % nmb;
+  return tile < ntiles;
+}
+
+__device__ __forceinline__ double rhs_at(const float *f, const double *f64, long o) {
+  return f64 ? f64[o] : (double)f[o];
+}
+
+// ------------------------------------------------------------------ forward
+template <class T, int W>
+__global__ __launch_bounds__(64 * W) void fused_fwd_kernel(FusedFwdArgs p) {
+  __shared__ T red[W * 256];
+  span_begin(p.tspan);
+  int tile, mb;
+  if (tile_of(blockIdx.x, p.ntiles, p.nmb, tile, mb)) {
+    const int t = threadIdx.x;
+    const int col = t & 15;
+    const int m = 16 * mb + (t >> 4);
+    const int steps = p.steps;
+    const int i2 = p.i2, i0 = i2 + 2;
+    // interior column (1-based, before wrapping) of this lane's tile column
+    const int c = tile * (16 - 2 * steps) - steps + 1 + col;
+    int pc;
+    bool halo = false;
+    if (p.periodic) {
+      pc = ((c - 1) % i2 + i2) % i2 + 1;
+    } else {
+      halo = c < 1 || c > i2;  // Dirichlet columns: X holds the boundary values
+      pc = min(max(c, 1), i2);
+    }
+    const int xc = halo ? (c < 1 ? 1 : i0) : pc + 1;  // X column
+    const bool out = col >= steps && col < 16 - steps && c >= 1 && c <= i2;
+    const bool act = t < 256 && m < p.M;
+    double *x = p.x + (long)(act ? m : 0) * p.x_stride;
+    const int js = p.js, jf = p.jf, ieb = p.ieb;
+    // the epilogue's rows that do not depend on the product, loaded first
+    double vs = 0.0, vc = 0.0, xnorth = 0.0;
+    if (act) {
+      vs = x[f2(i0, xc, js)];
+      vc = x[f2(i0, xc, js + 1)];
+      xnorth = x[f2(i0, xc, ieb + 1)];
+    }
+    const double d = product_tile<T, W>((const T *)p.a, p.M, i2, 16 * mb, (const T *)p.b, pc - 1,
+                                        !halo, red);
+    if (act) {
+      if (!halo) vs = vs - d;  // the south row as block b's correction leaves it
+      const long co = pc - 1;
+      const long fo = (long)m * p.f_stride + co;
+      auto cf = [&](const float *arr, int j) -> double { return (double)arr[co + (long)(j - 1) * i2]; };
+      for (int j = js; j <= jf; ++j) {
+        const double xw = row_shr1(vc), xe = row_shl1(vc);
+        double r = (rhs_at(p.f, p.f64, fo + (long)(j - 1) * i2) - cf(p.ax, j) * xw -
+                    cf(p.ay, j) * vs - cf(p.bb, j) * vc - cf(p.cx, j) * xe) / cf(p.cy, j);
+        if (halo) r = x[f2(i0, xc, j + 2)];
+        if (out) x[f2(i0, xc, j + 2)] = r;
+        vs = vc;
+        vc = r;
+      }
+      const int jr = ieb - 1;  // closure row
+      const double xw = row_shr1(vc), xe = row_shl1(vc);
+      const double e = rhs_at(p.f, p.f64, fo + (long)(jr - 1) * i2) - cf(p.ax, jr) * xw -
+                       cf(p.ay, jr) * vs - cf(p.bb, jr) * vc - cf(p.cx, jr) * xe -
+                       cf(p.cy, jr) * xnorth;
+      if (out) {
+        if (p.e_out) p.e_out[(long)m * i2 + co] = e;
+        if (p.e32_out) p.e32_out[(long)m * i2 + co] = (float)e;
+      }
+    }
+  }
+  span_end(p.tspan);
+}
+
+// ----------------------------------------------------------------- backward
+// Workgroups [0, nprod): product tiles, first those of RINV_b (with the
+// epilogue), then, when W_b is given, those of W_b (the coupling row, only
+// stored); then the homogeneous march of block b+1, one wavefront per
+// (tile, RHS).
+template <class T, int W, int K>
+__global__ __launch_bounds__(64 * W) void fused_bwd_kernel(FusedBwdArgs p) {
+  __shared__ T red[W * 256];
+  span_begin(p.tspan);
+  const int wg = blockIdx.x;
+  const int nprod = p.bw ? 2 * p.nprod_half : p.nprod_half;
+  int tile, mb;
+  if (wg >= nprod) {
+    const long wave = (long)(wg - nprod) * W + (threadIdx.x >> 6);
+    if (wave < (long)p.hom_tiles * p.hom.nrhs)
+      march_body<K, MODE_HOM>(p.hom, (int)(wave % p.hom_tiles), (int)(wave / p.hom_tiles), 0, 1,
+                              threadIdx.x & 63);
+  } else if (tile_of(wg >= p.nprod_half ? wg - p.nprod_half : wg, p.ntiles, p.nmb, tile, mb)) {
+    const bool cpl = wg >= p.nprod_half;
+    const int t = threadIdx.x;
+    const int m = 16 * mb + (t >> 4);
+    const int i2 = p.i2, i0 = i2 + 2;
+    const int c = 16 * tile + 1 + (t & 15);
+    const bool ok = c <= i2;
+    const bool act = t < 256 && m < p.M && ok;
+    const int js = p.js;
+    double *x = p.x + (long)(act ? m : 0) * p.x_stride;
+    const bool resid = !cpl && p.do_resid && act;
+    // the epilogue's operands, loaded before the product
+    double xfree = 0.0, xsouth = 0.0, xw = 0.0, xcen = 0.0, xe = 0.0;
+    if (act && !cpl) xfree = x[f2(i0, c + 1, js + 1)];
+    if (resid) {  // closure row js-1 of block b-1; row js = IE(b-1)
+      xsouth = x[f2(i0, c + 1, js - 1)];
+      xcen = x[f2(i0, c + 1, js)];
+      if (p.periodic) {
+        xw = x[f2(i0, (c == 1 ? i2 : c - 1) + 1, js)];
+        xe = x[f2(i0, (c == i2 ? 1 : c + 1) + 1, js)];
+      } else {
+        xw = x[f2(i0, c, js)];
+        xe = x[f2(i0, c + 2, js)];
+      }
+    }
+    const double v = product_tile<T, W>((const T *)p.a, p.M, i2, 16 * mb,
+                                        (const T *)(cpl ? p.bw : p.b), c - 1, ok, red);
+    if (act) {
+      const long ro = (long)m * i2 + c - 1;
+      if (cpl) {
+        p.hcpl_out[ro] = v;
+      } else {
+        p.hfree_out[ro] = v;
+        const double north = xfree + v;
+        x[f2(i0, c + 1, js + 1)] = north;
+        if (resid) {
+          const long co = (long)(c - 1) + (long)(js - 2) * i2;  // coefficient row js-1
+          const double e = rhs_at(p.f, p.f64, (long)m * p.f_stride + co) - (double)p.ax[co] * xw -
+                           (double)p.ay[co] * xsouth - (double)p.bb[co] * xcen -
+                           (double)p.cx[co] * xe - (double)p.cy[co] * north;
+          if (p.e_out) p.e_out[ro] = e;
+          if (p.e32_out) p.e32_out[ro] = (float)e;
+        }
+      }
+    }
+  }
+  span_end(p.tspan);
+}
+
+// ---------------------------------------------------------------- launchers
+int fused_tiles_fwd(int i2, int steps) {
+  const int tout = 16 - 2 * steps;
+  return tout > 0 ? (i2 + tout - 1) / tout : 0;
+}
+
+static long placed_workgroups(int ntiles, int nmb) { return 8L * nmb * ((ntiles + 7) / 8); }
+
+long fused_fwd_workgroups(const FusedFwdArgs &a) { return placed_workgroups(a.ntiles, a.nmb); }
+
+long fused_bwd_workgroups(const FusedBwdArgs &a) {
+  const long hom_waves = (long)a.hom_tiles * (a.has_hom ? a.hom.nrhs : 0);
+  return (a.bw ? 2L : 1L) * a.nprod_half + (hom_waves + FUSED_WAVES - 1) / FUSED_WAVES;
+}
+
+// Fill the launcher-set fields (tiles, row blocks) of the arguments.
+int fused_fwd_prepare(FusedFwdArgs &a) {
+  if (a.steps < 0 || 16 - 2 * a.steps < 2 || a.M < 1 || a.M > 64 || a.i2 % 4 != 0) return -1;
+  a.ntiles = fused_tiles_fwd(a.i2, a.steps);
+  a.nmb = (a.M + 15) / 16;
+  return 0;
+}
+
+int fused_bwd_prepare(FusedBwdArgs &a, int *K) {
+  if (a.M < 1 || a.M > 64 || a.i2 % 4 != 0) return -1;
+  a.ntiles = (a.i2 + 15) / 16;
+  a.nmb = (a.M + 15) / 16;
+  a.nprod_half = (int)placed_workgroups(a.ntiles, a.nmb);
+  *K = 1;
+  a.hom_tiles = 0;
+  if (a.has_hom && a.hom.jf >= a.hom.js) {  // a one-row block has nothing to march
+    *K = march_lanes(a.hom.w, a.hom.jf - a.hom.js + 1);
+    if (*K < 0) return -1;
+    a.hom_tiles = march_tiles(a.hom, *K);
+    if (a.hom_tiles <= 0) return -1;
+  }
+  return 0;
+}
+
+hipError_t launch_fused_fwd(const FusedFwdArgs &a0, bool dbl, hipStream_t st, hipEvent_t ev0,
+                            hipEvent_t ev1) {
+  FusedFwdArgs a = a0;
+  if (fused_fwd_prepare(a)) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)fused_fwd_workgroups(a)), block(64 * FUSED_WAVES);
+  if (dbl) {
+    if (ev0) hipExtLaunchKernelGGL((fused_fwd_kernel<double, FUSED_WAVES>), grid, block, 0, st, ev0, ev1, 0, a);
+    else hipLaunchKernelGGL((fused_fwd_kernel<double, FUSED_WAVES>), grid, block, 0, st, a);
+  } else {
+    if (ev0) hipExtLaunchKernelGGL((fused_fwd_kernel<float, FUSED_WAVES>), grid, block, 0, st, ev0, ev1, 0, a);
+    else hipLaunchKernelGGL((fused_fwd_kernel<float, FUSED_WAVES>), grid, block, 0, st, a);
+  }
+  return hipGetLastError();
+}
+
+template <class T, int K>
+static void bwd_launch_k(const FusedBwdArgs &a, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+  const dim3 grid((unsigned)fused_bwd_workgroups(a)), block(64 * FUSED_WAVES);
+  if (ev0) hipExtLaunchKernelGGL((fused_bwd_kernel<T, FUSED_WAVES, K>), grid, block, 0, st, ev0, ev1, 0, a);
+  else hipLaunchKernelGGL((fused_bwd_kernel<T, FUSED_WAVES, K>), grid, block, 0, st, a);
+}
+
+template <class T>
+static void bwd_launch(const FusedBwdArgs &a, int K, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+  if (K == 1) bwd_launch_k<T, 1>(a, st, ev0, ev1);
+  else if (K == 2) bwd_launch_k<T, 2>(a, st, ev0, ev1);
+  else bwd_launch_k<T, 4>(a, st, ev0, ev1);
+}
+
+hipError_t launch_fused_bwd(const FusedBwdArgs &a0, bool dbl, hipStream_t st, hipEvent_t ev0,
+                            hipEvent_t ev1) {
+  FusedBwdArgs a = a0;
+  int K = 1;
+  if (fused_bwd_prepare(a, &K)) return hipErrorInvalidValue;
+  if (dbl) bwd_launch<double>(a, K, st, ev0, ev1);
+  else bwd_launch<float>(a, K, st, ev0, ev1);
+  return hipGetLastError();
+}
+
+}  // namespace sevp

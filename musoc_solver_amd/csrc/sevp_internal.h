@@ -202,6 +202,70 @@ hipError_t launch_gemm_part(const GemmPartArgs &a, hipStream_t st, hipEvent_t ev
                             hipEvent_t ev1 = nullptr);
 long gemm_part_workgroups(const GemmPartArgs &a);  // grid size of launch_gemm_part
 
+// One launch per block step (sevp_fused.hip): a product whose workgroups
+// sum whole dot products (W wavefronts split K, FP64 sums of their partial
+// tiles in wavefront order) with the work that depends on it in the
+// epilogue.  T = float (FP32 MFMA, A the FP32 error rows, B FP32
+// column-major) or double (FP64 MFMA).
+constexpr int FUSED_WAVES = 8;
+
+// Forward step: d = e_b RINV1_b, then block b+1's march from the corrected
+// south row X(js) - d and its closure residual e_(b+1).
+struct FusedFwdArgs {
+  const void *a;      // e_b (M, i2) row-major, T
+  const void *b;      // RINV1_b (i2, i2) column-major, T
+  int M, i2, periodic;
+  // block b+1: march rows js .. jf (steps = jf - js + 1), closure row ieb - 1
+  int js, jf, ieb, steps;
+  const float *ax, *ay, *bb, *cx, *cy;  // (i2, j2)
+  const float *f;      // right-hand sides (i2, j2) per RHS, or
+  const double *f64;   // FP64 right-hand sides (the refinement residual)
+  long f_stride;
+  double *x;           // (i2+2, j2+2) per RHS
+  long x_stride;
+  double *e_out;       // e_(b+1) (M, i2), FP64 (may be null)
+  float *e32_out;      // e_(b+1) rounded to FP32 (may be null)
+  int ntiles, nmb;     // set by the launcher
+  unsigned long long *tspan;  // profiling: kernel span slot (null: off)
+};
+
+// Backward step for block b: [c | cpl] = e_b [RINV_b | W_b]; X(js+1) += c;
+// closure residual of block b-1 (do_resid); c and cpl stored for the
+// homogeneous march of block b, which the next backward launch runs (hom).
+struct FusedBwdArgs {
+  const void *a;       // e_b (M, i2), T
+  const void *b;       // RINV_b column-major, T
+  const void *bw;      // W_b column-major, T (null for the first block)
+  int M, i2, periodic;
+  int js;              // JS of block b (IE(b-1), or 1)
+  int do_resid;        // b > 1: closure residual of block b-1 at row js-1
+  const float *ax, *ay, *bb, *cx, *cy;
+  const float *f;
+  const double *f64;
+  long f_stride;
+  double *x;
+  long x_stride;
+  double *e_out;       // e_(b-1) (M, i2)
+  float *e32_out;
+  double *hfree_out, *hcpl_out;  // c and cpl of block b (M, i2)
+  MarchArgs hom;       // MODE_HOM march of block b+1 (hfree / hcpl of the previous launch)
+  int has_hom;
+  int ntiles, nmb, nprod_half, hom_tiles;  // set by the launcher
+  unsigned long long *tspan;
+};
+int fused_tiles_fwd(int i2, int steps);
+// Set the launcher's fields (tiles, row blocks; K: the march lane width of
+// the backward launch's homogeneous march); nonzero if the shape is refused.
+int fused_fwd_prepare(FusedFwdArgs &a);
+int fused_bwd_prepare(FusedBwdArgs &a, int *K);
+long fused_fwd_workgroups(const FusedFwdArgs &a);  // after the prepare call
+long fused_bwd_workgroups(const FusedBwdArgs &a);
+// ev0 / ev1 (optional): events with the kernel's own start and end times.
+hipError_t launch_fused_fwd(const FusedFwdArgs &a, bool dbl, hipStream_t st,
+                            hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+hipError_t launch_fused_bwd(const FusedBwdArgs &a, bool dbl, hipStream_t st,
+                            hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+
 // Step 14 of FS_GLOBAL (the barotropic correction around the solve); arrays
 // have the reference shapes, Fortran order (see include/sevp.h).
 struct Step14Args {

@@ -22,6 +22,7 @@ module sevp_hip
                                        SEVP_FORM_FULL = 2
   integer(c_int), parameter, public :: SEVP_PREC_FP64 = 0, SEVP_PREC_FP32 = 1, SEVP_PREC_BF16 = 2, &
                                        SEVP_PREC_FP64_MFMA = 3
+  integer(c_int), parameter, public :: SEVP_STORE_FP64 = 0, SEVP_STORE_PRODUCTS = 1
 
   type, bind(c), public :: sevp_desc
     integer(c_int) :: i2, j2, nb0
@@ -64,7 +65,8 @@ module sevp_hip
 
   public :: c_sevp_repbir, c_sevp_rep1, c_sevp_rep, c_sevp_repbir_period, c_sevp_pre, &
             c_sevp_pre1, c_sevp_matinv, sevp_plan_create, sevp_plan_destroy, &
-            sevp_build_influence, sevp_load_influence, sevp_get_influence, sevp_solve, &
+            sevp_set_storage, sevp_build_influence, sevp_load_influence, sevp_get_influence, &
+            sevp_solve, &
             sevp_reset_seam, sevp_step14_dev, sevp_evp_write, sevp_evp_read, sevp_check, &
             sevp_krylov_defaults, c_sevp_p_bicgstab_le, c_sevp_p_csi, c_sevp_lanczos
 
@@ -146,6 +148,12 @@ module sevp_hip
     integer(c_int) function sevp_plan_destroy(plan) bind(c, name="sevp_plan_destroy")
       import :: c_int, c_ptr
       type(c_ptr), value :: plan
+    end function
+
+    integer(c_int) function sevp_set_storage(plan, storage) bind(c, name="sevp_set_storage")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: plan
+      integer(c_int), value :: storage
     end function
 
     integer(c_int) function sevp_build_influence(plan, zero_pivots) &

@@ -32,6 +32,10 @@ PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_BF16 = 2
 PREC_FP64_MFMA = 3
+STORE_FP64 = 0
+STORE_PRODUCTS = 1
+PASS_SPLITK = 0
+PASS_FUSED = 1
 
 SEVP_OK = 0
 SEVP_E_NOTCONV = -5
@@ -95,6 +99,9 @@ def lib():
         L.sevp_pre1.argtypes = [VP] * 9 + [CI] * 3
         L.sevp_matinv.argtypes = [VP, CI]
         L.sevp_set_precision.argtypes = [VP, CI]
+        L.sevp_set_storage.argtypes = [VP, CI]
+        L.sevp_set_pass.argtypes = [VP, CI]
+        L.sevp_get_pass.argtypes = [VP, ctypes.POINTER(CI)]
         L.sevp_set_stream.argtypes = [VP, VP]
         L.sevp_set_graphs.argtypes = [VP, CI]
         L.sevp_set_rhs_groups.argtypes = [VP, CI]
@@ -219,10 +226,12 @@ class SevpPlan:
     scratch for up to ``nrhs_max`` simultaneous right-hand sides."""
 
     def __init__(self, coefs, ie, form=FORM_BIR, nbir=None, nsweep=5, periodic=True,
-                 nrhs_max=1, precision=PREC_FP64, seam_fix=False, cache_dir=None):
+                 nrhs_max=1, precision=PREC_FP64, seam_fix=False, cache_dir=None,
+                 storage=STORE_FP64):
         """With ``cache_dir`` the influence matrices come from the plan cache
         (sevp_plan_create_cached): ``cache_hit`` is 1 when they were loaded,
-        0 when built and stored, -1 when built but not stored."""
+        0 when built and stored, -1 when built but not stored.  ``storage``
+        STORE_PRODUCTS keeps only the FP32 product operands (sevp_set_storage)."""
         al, ab, ac, ar, at = [_f32(np.asfortranarray(c, dtype=np.float32)) for c in coefs]
         self.coefs = (al, ab, ac, ar, at)
         self.i2, self.j2 = al.shape
@@ -246,6 +255,8 @@ class SevpPlan:
             self.cache_hit = hit.value
         self._h = h
         self._stream = None
+        if storage != STORE_FP64:
+            self.set_storage(storage)
 
     def close(self):
         # at interpreter exit the module's globals may already be gone; the
@@ -276,6 +287,21 @@ class SevpPlan:
             w2 = self.ibir * self.ibir
             return (w2, self.nb0, self.nbir), (w2, nb1, self.nbir)
         return (self.i2, self.i2, self.nb0), (self.i2, self.i2, nb1)
+
+    def set_storage(self, storage):
+        """STORE_FP64 (default) or STORE_PRODUCTS: keep only the FP32 product
+        operands (set before build_influence / load_influence)."""
+        _check(lib().sevp_set_storage(self._h, storage))
+
+    def set_pass(self, kind):
+        """PASS_SPLITK or PASS_FUSED kernels for the MFMA corrections."""
+        _check(lib().sevp_set_pass(self._h, kind))
+
+    @property
+    def pass_kind(self) -> int:
+        k = CI(0)
+        _check(lib().sevp_get_pass(self._h, ctypes.byref(k)))
+        return k.value
 
     def build_influence(self) -> int:
         z = CI(0)

@@ -35,6 +35,7 @@ def make_basins(precision):
         form = S.FORM_FULL_PERIODIC if periodic else S.FORM_FULL
         plan = S.SevpPlan(case.coefs, ie, form=form, periodic=periodic, nrhs_max=64,
                           precision=precision)
+        plan.set_pass(S.PASS_SPLITK)  # the pass kernels of the batch
         plan.build_influence()
         s = synth.make_rhs(case.mask, 7100 + seed, nrhs)
         out.append((plan, s))

@@ -216,3 +216,36 @@ def test_quarter_degree_mixed_64rhs(oracle, device):
         ro = oracle.rel_residual(case.coefs, s[:, :, r], np.asfortranarray(x[:, :, r]))
         assert ro <= 1.5e-10, (r, ro)
     plan.close()
+
+
+def test_tenth_degree_mixed_16rhs(oracle, device):
+    """The 0.1 degree configuration: 3600x1800 (+-80 deg), 16 right-hand
+    sides refined to 1e-10 with FP32 MFMA corrections, on one GPU.  With
+    2-row blocks (NB0 = 900) the FP64 influence matrices would take 186 GB,
+    so the plan keeps only the FP32 product operands (140 GB), built one block
+    at a time.  Checked with the oracle's p_sqprod residual on two RHS."""
+    import time
+    import torch
+    i2, j2, nrhs = 3600, 1800, 16
+    print(f"[cfg-0.1] {i2}x{j2} nrhs={nrhs}", flush=True)
+    case = synth.make_case(i2, j2, 5100, metric="latlon", lat_max=80.0, h_land=50.0)
+    ie = synth.reference_ie(j2, synth.nb0_for_height(j2, 2))
+    s = synth.make_rhs(case.mask, 5101, nrhs)
+    plan = S.SevpPlan(case.coefs, ie, form=S.FORM_FULL_PERIODIC, nrhs_max=nrhs,
+                      precision=S.PREC_FP32, storage=S.STORE_PRODUCTS)
+    t0 = time.perf_counter()
+    assert plan.build_influence() == 0
+    print(f"[cfg-0.1] FP32 operands of NB0={len(ie)} blocks built in "
+          f"{time.perf_counter() - t0:.1f} s, {plan.device_bytes / 2**30:.1f} GiB", flush=True)
+    s_d = torch.from_numpy(np.ascontiguousarray(s.transpose(2, 1, 0))).cuda()
+    x_d = torch.zeros((nrhs, j2 + 2, i2 + 2), dtype=torch.float64, device="cuda")
+    rel, st, ok = plan.solve_refined_dev(s_d, x_d, tol=1e-10, max_iter=10)
+    torch.cuda.synchronize()
+    print(f"[cfg-0.1] mixed: {st.iterations} passes, max rel resid {rel.max():.2e}, "
+          f"{st.seconds:.2f} s", flush=True)
+    assert ok and rel.max() <= 1e-10, (rel.max(), st.iterations)
+    for r in (0, nrhs - 1):
+        xr = np.asfortranarray(x_d[r].cpu().numpy().T)
+        ro = oracle.rel_residual(case.coefs, s[:, :, r], xr)
+        assert ro <= 1.5e-10, (r, ro)
+    plan.close()

@@ -441,3 +441,105 @@ def test_fp64_mfma_one_pass_and_precision_switching(oracle, device):
         err = np.linalg.norm(x[1:-1, 1:-1, r] - xo[1:-1, 1:-1]) / np.linalg.norm(xo[1:-1, 1:-1])
         assert err <= 1e-11, err
     plan.close()
+
+
+def test_products_only_storage(oracle, device):
+    """A plan that keeps only the FP32 product operands (built one block at a
+    time, or uploaded block by block) refines bitwise like a plan that keeps
+    the FP64 influence matrices, and refuses what needs them."""
+    import torch
+    i2, j2, n0, nrhs = 96, 48, 2, 4
+    print("[products-only] build and load", flush=True)
+    ref, coefs, ie, s, s_d, x_ref = _device_case(i2, j2, n0, nrhs, S.PREC_FP32)
+    rel0, st0, ok0 = ref.solve_refined_dev(s_d, x_ref, tol=1e-10, max_iter=8)
+    assert ok0
+    rinv, rinv1 = ref.get_influence()
+    for mode in ("build", "load"):
+        plan = S.SevpPlan(coefs, ie, form=S.FORM_FULL_PERIODIC, nrhs_max=nrhs,
+                          precision=S.PREC_FP32, storage=S.STORE_PRODUCTS)
+        if mode == "build":
+            assert plan.build_influence() == 0
+        else:
+            plan.load_influence(rinv, rinv1)
+        x_d = torch.zeros_like(x_ref)
+        rel, st, ok = plan.solve_refined_dev(s_d, x_d, tol=1e-10, max_iter=8)
+        torch.cuda.synchronize()
+        assert ok and st.iterations == st0.iterations
+        assert torch.equal(x_d, x_ref), mode
+        with pytest.raises(S.SevpError):
+            plan.get_influence()
+        with pytest.raises(S.SevpError):
+            plan.set_precision(S.PREC_FP64)
+        assert plan.device_bytes < ref.device_bytes
+        plan.close()
+    ref.close()
+
+
+FUSED_CASES = [
+    # (i2, j2, form, IE heights cycle, nrhs): Dirichlet and periodic, block
+    # heights 1..4 (the fused tile keeps 16 - 2 (h - 1) columns), I2 neither a
+    # multiple of 14 nor of 16, RHS counts across the 16-row blocks
+    (200, 60, S.FORM_FULL_PERIODIC, (2,), 5),
+    (200, 61, S.FORM_FULL, (2, 3, 1, 4), 21),
+    (96, 48, S.FORM_FULL_PERIODIC, (3, 4, 2), 3),
+    (64, 20, S.FORM_FULL, (4,), 16),
+]
+
+
+def _ie_cycle(j2, heights):
+    ie, row, k = [], 1, 0
+    while row + heights[k % len(heights)] < j2 + 1:
+        row += heights[k % len(heights)]
+        ie.append(row)
+        k += 1
+    ie.append(j2 + 1)
+    return np.array(ie, np.int32)
+
+
+@pytest.mark.parametrize("i2,j2,form,heights,nrhs", FUSED_CASES)
+def test_fused_pass_matches_reference_order(oracle, device, i2, j2, form, heights, nrhs):
+    """One pass of the fused kernels with FP64 corrections (FP64 MFMA, whole
+    dot products) against the plain FP64 pass of the same plan (the
+    reference's order, bitwise the oracle's REP1/REP): the same FP64
+    arithmetic summed in another order, so they agree to rounding; an index
+    slip in the fused epilogue would show as O(1).  The FP32 fused pass and
+    the split-K pass both refine to 1e-10."""
+    import torch
+    coefs, _, mask = make(i2, j2, 2, metric="latlon", lat_max=70.0)
+    ie = _ie_cycle(j2, heights)
+    print(f"[fused] {i2}x{j2} form={form} heights={heights} nrhs={nrhs} nb0={len(ie)}", flush=True)
+    plan = S.SevpPlan(coefs, ie, form=form, nrhs_max=nrhs, precision=S.PREC_FP64)
+    plan.build_influence()
+    s = synth.make_rhs(mask, 41, nrhs)
+    x0 = rand_x(i2, j2, nrhs, 42)
+    x_plain = x0.copy(order="F")
+    plan.solve(s, x_plain)
+    plan.set_precision(S.PREC_FP64_MFMA)
+    assert plan.pass_kind == S.PASS_FUSED
+    x_f = x0.copy(order="F")
+    plan.solve(s, x_f)
+    assert all_finite(x_f)
+    d = np.abs(x_f - x_plain).max() / np.abs(x_plain).max()
+    print(f"[fused] FP64 fused vs reference order: {d:.2e}", flush=True)
+    assert d <= 1e-10, d
+    s_d = torch.from_numpy(np.ascontiguousarray(s.transpose(2, 1, 0))).cuda()
+    sols = {}
+    for prec in (S.PREC_FP64_MFMA, S.PREC_FP32):
+        plan.set_precision(prec)
+        for kind in (S.PASS_FUSED, S.PASS_SPLITK):
+            plan.set_pass(kind)
+            x_d = torch.zeros((nrhs, j2 + 2, i2 + 2), dtype=torch.float64, device="cuda")
+            rel, st, ok = plan.solve_refined_dev(s_d, x_d, tol=1e-10, max_iter=10)
+            torch.cuda.synchronize()
+            print(f"[fused] prec={prec} pass={kind}: {st.iterations} passes, {rel.max():.2e}",
+                  flush=True)
+            assert ok and rel.max() <= 1e-10, (prec, kind, rel.max())
+            sols[(prec, kind)] = x_d.clone()
+    ref = sols[(S.PREC_FP64_MFMA, S.PASS_FUSED)]
+    for k, v in sols.items():
+        e = float((v - ref).abs().max() / ref.abs().max())
+        assert e <= 1e-8, (k, e)
+    x = sols[(S.PREC_FP32, S.PASS_FUSED)].cpu().numpy().transpose(2, 1, 0)
+    for r in (0, nrhs - 1):
+        assert oracle.rel_residual(coefs, s[:, :, r], np.asfortranarray(x[:, :, r])) <= 1.5e-10
+    plan.close()
