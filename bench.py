@@ -67,8 +67,16 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=16, help="host threads of the CPU baseline")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--legs", default="mixed,plain,fast64", help="comma list of GPU legs to time")
-    ap.add_argument("--rhs-groups", default="2,2,2",
-                    help="RHS groups whose passes run concurrently, mixed,plain,fast64 legs")
+    ap.add_argument("--rhs-groups", default="1,2,1",
+                    help="RHS groups whose passes run concurrently, mixed,plain,fast64 legs "
+                         "(the fused pass reads each influence matrix once per group)")
+    ap.add_argument("--pass-kernels", choices=("fused", "splitk"), default="fused",
+                    help="pass kernels of the MFMA legs: one launch per block step (fused) or "
+                         "the split-K products with separate marches (splitk)")
+    ap.add_argument("--tenth", type=int, default=1,
+                    help="also run the 0.1 degree leg: 3600x1800, 16 RHS, FP32 MFMA corrections "
+                         "refined to the tolerance, products-only plan (0: skip)")
+    ap.add_argument("--tenth-only", action="store_true", help="run the 0.1 degree leg alone")
     ap.add_argument("--kernel-timing", choices=("after", "timed"), default="after",
                     help="time the correction kernel with HIP events in one extra solve after "
                          "the timed steps, launched eagerly with events from the kernel's own "
@@ -200,6 +208,9 @@ def main():
     if a.sequence_only:
         print(json.dumps({"sequence": sequence_leg(a)}), flush=True)
         return
+    if a.tenth_only:
+        print(json.dumps({"tenth": tenth_leg(a)}), flush=True)
+        return
     legs, setup, xs = {}, {}, {}
     order = [l for l in a.legs.split(",") if l]
     if a.krylov_only:  # BiCGSTAB needs the operator only, no influence matrices
@@ -221,6 +232,7 @@ def main():
             setup[name] = dt
         plan = plans[n0][0]
         t0 = time.perf_counter()
+        plan.set_pass(S.PASS_FUSED if a.pass_kernels == "fused" else S.PASS_SPLITK)
         plan.set_precision(prec_of[name])
         torch.cuda.synchronize()
         setup[name] = setup.get(name, 0.0) + time.perf_counter() - t0
@@ -251,7 +263,8 @@ def main():
 
     main_name = next(n for n in ("mixed", "plain", "fast64") if n in legs)
     main_leg = legs[main_name]
-    roof = roofline(main_leg, main_name, nrhs, i2, a)
+    fused = a.pass_kernels == "fused"
+    roof = roofline(main_leg, main_name, nrhs, i2, a, fused)
     krylov = krylov_leg(plan_keep[0], s, xs.get(main_name), a) if a.krylov else None
     cpu = None
     if a.cpu_rhs > 0:
@@ -281,8 +294,10 @@ def main():
                               if "fast64" in legs and "plain" in legs else None),
         "mixed_vs_fp64_max_rel_err": err,
         "fast64_vs_fp64_max_rel_err": err64,
-        "roofline_fast64": (roofline(legs["fast64"], "fast64", nrhs, i2, a)
+        "roofline_fast64": (roofline(legs["fast64"], "fast64", nrhs, i2, a, fused)
                             if "fast64" in legs else None),
+        "roofline_plain": (roofline(legs["plain"], "plain", nrhs, i2, a, False)
+                           if "plain" in legs else None),
         "setup_s": t_setup,
         "roofline": roof,
         "cpu_baseline": cpu,
@@ -290,45 +305,183 @@ def main():
         "basins": basins_or_error(a),
         "sequence": sequence_or_error(a),
     }
-    print(json.dumps(out), flush=True)
     for plan, _ in plans.values():
         plan.close()
+    torch.cuda.empty_cache()
+    out["tenth"] = tenth_or_error(a)
+    print(json.dumps(out), flush=True)
 
 
-def roofline(leg, name, nrhs, i2, a):
-    """Roofline of the dominant kernel, the influence-matrix correction
-    product, from its average launch duration (HIP events around every launch
-    of one solve).  Per refinement pass and RHS group of m right-hand sides
-    there are NB0-1 forward products (N = I2) and NB0 backward ones (N = 2 I2
-    for the NB0-1 blocks with a coupling row, N = I2 for the bottom block),
-    each with K = I2.  Algorithmic work per product: 2 m K N flops and
-    4 (m K + K N + m N) bytes (FP32 operands and result; the FP64 plain
-    product moves 8-byte elements).  The bound is whichever of MFMA time and
-    HBM time is larger for the average product."""
+def corr_work(name, fused, m, k, nb0):
+    """Algorithmic flops and bytes per launch of the correction kernels,
+    averaged over the launches of one pass (block height 2, K = N = I2 = k,
+    m right-hand sides per launch, operand elements of esz bytes).
+
+    Split-K and plain passes: NB0-1 forward products (N = K) and NB0
+    backward ones (N = 2K with the coupling row for NB0-1 blocks, N = K for
+    the bottom block; the plain pass runs the coupling as a separate FP64
+    product, so it has 3 NB0 - 2 products of N = K), each
+    2 m K N flops and esz (m K + K N + m N) bytes.
+
+    Fused pass: the product's operands esz (m K + K N) (its result stays in
+    the workgroup) plus what the epilogue reads and writes per right-hand
+    side and column -- forward: X rows south, free and north (3 x 8 B) and
+    the marched row (8 B), two FP64 right-hand-side rows (16 B), the next
+    error row (esz) and 2 x 5 coefficient rows per column (40 B); backward:
+    the free row read and written, the two rows of the closure residual
+    (32 B), the right-hand side (8 B), the error row (esz), c and cpl (16 B),
+    one coefficient row (20 B per column), and the homogeneous march of the
+    block above (c and cpl read, one X row read and written: 32 B, 20 B of
+    coefficients per column)."""
+    esz = 4 if name == "mixed" else 8
+    if name == "plain" or not fused:
+        if name == "plain":
+            ncols = [(k, nb0 - 1), (k, nb0), (k, nb0 - 1)]
+        else:
+            ncols = [(k, nb0 - 1), (2 * k, nb0 - 1), (k, 1)]
+        launches = sum(c for _, c in ncols)
+        flops = sum(2.0 * m * k * n * c for n, c in ncols) / launches
+        nbytes = sum(esz * (m * k + k * n + m * n) * c for n, c in ncols) / launches
+        return flops, nbytes, launches
+    fwd_f = 2.0 * m * k * k
+    fwd_b = esz * (m * k + k * k) + m * k * (32 + 16 + esz) + 40 * k
+    bwd_f = 2.0 * m * k * 2 * k
+    bwd_b = esz * (m * k + 2 * k * k) + m * k * (16 + 32 + 8 + esz + 16 + 32) + 40 * k
+    bot_f = 2.0 * m * k * k
+    bot_b = esz * (m * k + k * k) + m * k * (16 + 8 + esz + 16 + 32) + 20 * k
+    launches = (nb0 - 1) + nb0
+    flops = ((nb0 - 1) * (fwd_f + bwd_f) + bot_f) / launches
+    nbytes = ((nb0 - 1) * (fwd_b + bwd_b) + bot_b) / launches
+    return flops, nbytes, launches
+
+
+def roofline(leg, name, nrhs, i2, a, fused=True):
+    """Roofline of the dominant kernels, the correction launches (the fused
+    kernels of the fused pass, the split-K products, or the plain FP64
+    products), from their average launch duration (HIP events with the
+    kernels' own start and end timestamps, one solve after the timed steps)
+    and the algorithmic work of corr_work.  The bound is whichever of the
+    MFMA (or FP64 vector, plain) time and the HBM time at peak is larger for
+    the average launch; both fractions are reported."""
     nb0, groups = leg["nb0"], leg["rhs_groups"]
     m = -(-nrhs // groups)
-    n_fwd, n_bwd2, n_bwd1 = nb0 - 1, nb0 - 1, 1
-    launches = n_fwd + n_bwd2 + n_bwd1
-    ncols = [(i2, n_fwd), (2 * i2, n_bwd2), (i2, n_bwd1)]
-    esz = 4 if name == "mixed" else 8
-    flops = sum(2.0 * m * i2 * n * c for n, c in ncols) / launches
-    nbytes = sum(esz * (m * i2 + i2 * n + m * n) * c for n, c in ncols) / launches
+    fused = fused and name != "plain"
+    flops, nbytes, _ = corr_work(name, fused, m, i2, nb0)
     t = leg["corr_avg_us"] * 1e-6
     peak_f = PEAK_FP32_MFMA if name == "mixed" else PEAK_FP64
-    hbm_bound = nbytes / PEAK_HBM > flops / peak_f
-    traffic = measured_traffic(name, m, i2)
-    if hbm_bound:
+    mfma_frac = flops / t / peak_f
+    hbm_frac = nbytes / t / PEAK_HBM
+    if nbytes / PEAK_HBM >= flops / peak_f:
         r = {"bound": "hbm", "achieved": nbytes / t / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-             "frac": nbytes / t / PEAK_HBM}
+             "frac": hbm_frac}
     else:
         r = {"bound": "mfma", "achieved": flops / t / 1e12, "peak": peak_f / 1e12,
-             "unit": "TFLOP/s", "frac": flops / t / peak_f}
-    r.update({"traffic": traffic,
-              "kernel": {"mixed": "gemm_part_kernel", "fast64": "gemm_part64_kernel"}.get(
-                  name, "gemm_seq_kernel"),
+             "unit": "TFLOP/s", "frac": mfma_frac}
+    if fused:
+        kern = "fused_fwd_kernel + fused_bwd_kernel (%s)" % ("float" if name == "mixed" else "double")
+    else:
+        kern = {"mixed": "gemm_part_kernel", "fast64": "gemm_part64_kernel"}.get(name, "gemm_seq_kernel")
+    r.update({"traffic": measured_traffic(name, m, i2, fused),
+              "kernel": kern,
+              "compute_peak": "fp32 mfma" if name == "mixed" else (
+                  "fp64 vector (separate multiply and add)" if name == "plain" else "fp64 mfma"),
+              "mfma_frac": mfma_frac, "hbm_frac": hbm_frac,
               "algorithmic_bytes_per_launch": nbytes, "algorithmic_flops_per_launch": flops,
-              "avg_launch_us": leg["corr_avg_us"], "rhs_per_product": m})
+              "avg_launch_us": leg["corr_avg_us"], "span_us": leg.get("corr_span_us"),
+              "rhs_per_launch": m,
+              "corr_ms_per_step": leg["corr_avg_us"] * 1e-3 * leg["corr_launches"],
+              "ms_per_step": leg["ms_per_step"]})
     return r
+
+
+def kernel_source_sha():
+    """Hash of the kernel sources the correction launches are built from: a
+    stored traffic record applies only to the build it was measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in ("sevp_fused.hip", "sevp_mixed.hip", "sevp_full.hip", "sevp_march.h", "sevp_dev.h",
+              "sevp_internal.h"):
+        with open(os.path.join(ROOT, "musoc_solver_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def tenth_or_error(a):
+    """The 0.1 degree leg after the main legs (their plans are closed first:
+    this one needs 143 GiB), isolated like the basins leg."""
+    if a.tenth <= 0:
+        return None
+    try:
+        return tenth_leg(a)
+    except Exception as e:  # noqa: BLE001 - reported, not hidden
+        log(f"tenth leg failed: {e!r}")
+        return {"error": repr(e)}
+
+
+def tenth_leg(a):
+    """The 0.1 degree configuration on one GPU: 3600x1800 cells (uniform
+    latitude spacing over +-80 deg, the benchmark operator's form), block
+    height 2 (NB0 = 900), 16 right-hand sides per call, FP32 MFMA corrections
+    refined in FP64 to the tolerance.  The FP64 influence matrices of this
+    layout would take 186 GB, so the plan keeps only the FP32 product
+    operands (RINV, RINV1, W: 140 GB), built one block at a time
+    (sevp_set_storage(SEVP_STORE_PRODUCTS)).  Reports solves/s, passes,
+    residuals, and the fused correction kernels' roofline (their HBM bytes
+    include the march rows of the epilogue)."""
+    import torch
+    from musoc_solver_amd import synth
+    from musoc_solver_amd import sevp as S
+    i2, j2, nrhs = 3600, 1800, 16
+    case = synth.make_case(i2, j2, a.seed + 5000, metric="latlon", lat_max=a.lat_max,
+                           h_land=a.h_land)
+    ie = synth.reference_ie(j2, synth.nb0_for_height(j2, 2))
+    s_np = synth.make_rhs(case.mask, a.seed + 5001, nrhs)
+    s = torch.from_numpy(np.ascontiguousarray(s_np.transpose(2, 1, 0))).cuda()
+    x = torch.zeros((nrhs, j2 + 2, i2 + 2), dtype=torch.float64, device="cuda")
+    plan = S.SevpPlan(case.coefs, ie, form=S.FORM_FULL_PERIODIC, nrhs_max=nrhs,
+                      precision=S.PREC_FP32, storage=S.STORE_PRODUCTS)
+    try:
+        plan.set_pass(S.PASS_FUSED if a.pass_kernels == "fused" else S.PASS_SPLITK)
+        t0 = time.perf_counter()
+        zp = plan.build_influence()
+        torch.cuda.synchronize()
+        setup = time.perf_counter() - t0
+        log(f"tenth: {i2}x{j2} FP32 operands of NB0={len(ie)} blocks built in {setup:.1f} s "
+            f"({plan.device_bytes / 2**30:.1f} GiB, zero pivots {zp})")
+        plan.solve_refined_dev(s, x, tol=a.tol, max_iter=a.max_iter)  # warm-up (graph capture)
+        steps = max(1, min(a.steps, 3))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        passes, worst, ok_all = [], 0.0, True
+        for _ in range(steps):
+            x.zero_()
+            rel, st, ok = plan.solve_refined_dev(s, x, tol=a.tol, max_iter=a.max_iter)
+            passes.append(st.iterations)
+            worst = max(worst, float(rel.max()))
+            ok_all = ok_all and ok
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        rel_chk = plan.residual_dev(s, x)  # independent FP64 residual of the returned X
+        plan.set_graphs(False)
+        plan.profile_begin(S.KCLASS_CORR)
+        x.zero_()
+        plan.solve_refined_dev(s, x, tol=a.tol, max_iter=a.max_iter)
+        span_ms, span_n = plan.profile_spans()
+        corr_ms, corr_n = plan.profile_end()
+        leg = {"solves_per_s": nrhs / dt, "ms_per_step": 1e3 * dt, "block_height": 2,
+               "nb0": len(ie), "passes": float(np.mean(passes)), "max_rel_resid": worst,
+               "residual_check_max": float(np.max(rel_chk)), "converged": ok_all,
+               "corr_avg_us": 1e3 * corr_ms / max(corr_n, 1),
+               "corr_span_us": 1e3 * span_ms / span_n if span_n else None,
+               "corr_launches": corr_n, "rhs_groups": 1, "setup_s": setup,
+               "device_gib": plan.device_bytes / 2**30, "storage": "fp32 products only",
+               "workload": f"SEVP {i2}x{j2} (0.1 deg zonal, +-{a.lat_max:g} deg), REP1 form, "
+                           f"{nrhs} RHS per call, each to {a.tol:g} rel resid"}
+        leg["roofline"] = roofline(leg, "mixed", nrhs, i2, a, a.pass_kernels == "fused")
+        log(f"tenth: {leg}")
+        return leg
+    finally:
+        plan.close()
 
 
 def sequence_or_error(a):
@@ -509,19 +662,23 @@ def krylov_leg(plan, s, x_sevp, a):
     return r
 
 
-def measured_traffic(name, m, i2):
-    """HBM bytes per launch of the correction product as measured by the
-    FETCH_SIZE / WRITE_SIZE counter runs recorded under profiles/ (scaled by
-    the same counters of a copy of known size), for this configuration; null
-    when no recorded run matches."""
+def measured_traffic(name, m, i2, fused):
+    """HBM bytes per correction launch as measured by the FETCH_SIZE /
+    WRITE_SIZE counter runs recorded under profiles/ (scaled by the same
+    counters of a copy of known size), for this configuration; null when no
+    recorded run matches or the record was measured on other kernel sources
+    (its source_sha differs from the current build's)."""
     path = os.path.join(ROOT, "profiles", "correction_product_traffic.json")
     try:
         with open(path) as f:
             rec = json.load(f)
     except (OSError, ValueError):
         return None
+    if rec.get("source_sha") != kernel_source_sha():
+        return None
     for r in rec.get("runs", []):
-        if r.get("leg") == name and r.get("rhs_per_product") == m and r.get("i2") == i2:
+        if (r.get("leg") == name and r.get("rhs_per_launch") == m and r.get("i2") == i2
+                and bool(r.get("fused")) == bool(fused)):
             return r.get("hbm_bytes_per_launch")
     return None
 

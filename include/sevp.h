@@ -194,6 +194,18 @@ int sevp_pre1(const float *ax, const float *ay, const float *bb, const float *cx
 /* MATINV, PREP/template/GLOBAL/src/prep_GLOBAL.f90:516-568. */
 int sevp_matinv(double *b, int n);
 
+/* MATINV on a device matrix, in place, with no allocation.  No counterpart
+ * in the reference: for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The result is bitwise sevp_matinv's.  b_dev: n x n FP64
+ * column-major, device.  zero_pivots_dev (device int, may be NULL) receives
+ * the zero pivots met.  Enqueued on hip_stream (NULL: the null stream); the
+ * call does not wait.  The inversion's scratch is a fixed 2 MiB device array
+ * per GPU (orders up to 4096; larger ones are refused with SEVP_E_ARG),
+ * shared by every call on that GPU: calls that overlap in time on one GPU
+ * race. */
+int sevp_matinv_dev_noalloc(double *b_dev, int n, int *zero_pivots_dev, void *hip_stream);
+
 /* Enqueue this plan's work on `hip_stream` (NULL: the plan's own stream,
  * which is ordered with the null stream).
  * Callers that produce a plan's inputs on another stream set it here so the

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <memory>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -208,6 +209,20 @@ struct PassGraph {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
 };
 
+// The whole refinement captured as one graph: the residual of the initial
+// guess, then `passes` times (the pass of every RHS group, x += dx, the
+// residual, the stop test), with the stop flag on the device (the updates
+// after it is set are skipped, so the result is the loop's).  The caller's s
+// and x are baked in.
+struct RefineGraph {
+  int nrhs = 0, prec = 0, kind = 0, groups = 0, passes = 0;
+  bool prof = false;
+  const float *s = nullptr;
+  double *x = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+};
+
 // Vectors and scalars of the Krylov solvers (BiCGSTAB needs eight vectors).
 struct KryWork {
   DBuf<double> vec, part, dot, omega;
@@ -226,6 +241,14 @@ struct KryWork {
 struct sevp_plan_s {
   KProf prof;
   std::deque<PassGraph> graphs;  // deque: PassGraph addresses stay valid as graphs are added
+  std::deque<RefineGraph> rgraphs;
+  int expect_passes[4][2] = {};   // passes the last refined solve took, by precision and kernels
+  DBuf<int> ctl;                  // captured refinement: passes, done, max_iter
+  DBuf<double> ctl_d;             // tol, relative residuals (nrhs_max), worst
+  int *ctl_h = nullptr;           // pinned: [0..2] uploaded (0, 0, max_iter), [4..5] read back,
+                                  // tol as the double at [8]
+  double *part_h = nullptr;
+  DBuf<float> s_stage;            // the right-hand sides the captured refinement reads
   hipStream_t cap_stream = nullptr;
   bool use_graphs = true;
   sevp_desc d{};
@@ -810,7 +833,8 @@ __global__ __launch_bounds__(256) void resid_norm_kernel(const float *al, const 
                                                          const double *s64, long s_stride,
                                                          const double *x, long x_stride,
                                                          double *rout, int i2, int j2, int nchunk,
-                                                         double *part) {
+                                                         double *part, const int *gate) {
+  if (gate && gate[1]) return;  // captured refinement: converged, the residual stays
   resid_norm_body(al, ab, ac, ar, at, s, s64, s_stride, x, x_stride, rout, i2, j2, nchunk, part,
                   blockIdx.y, blockIdx.x);
 }
@@ -851,6 +875,57 @@ __global__ void axpy_interior_kernel(double *x, long x_stride, const double *d, 
   const long o = f2(i2 + 2, i, j);
   x[(long)r * x_stride + o] += d[(long)r * d_stride + o];
 }
+
+// Captured refinement (one graph for all passes).  ctl[0] = passes run,
+// ctl[1] = done, ctl[2] = max_iter; tol in *tolp.  The update x += dx with
+// the periodic halo columns refreshed from the new interior (what
+// axpy_interior_kernel and halo_fix_kernel do), skipped once done.
+__global__ void refine_axpy_kernel(double *x, long x_stride, const double *d, long d_stride,
+                                   int i2, int j2, int nrhs, int periodic, const int *ctl) {
+  if (ctl[1]) return;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n = (long)i2 * j2;
+  if (t >= n * nrhs) return;
+  const int r = (int)(t / n);
+  const long q = t % n;
+  const int i = (int)(q % i2) + 2, j = (int)(q / i2) + 2;
+  const int i0 = i2 + 2;
+  const long o = f2(i0, i, j);
+  double *xr = x + (long)r * x_stride;
+  const double v = xr[o] + d[(long)r * d_stride + o];
+  xr[o] = v;
+  if (periodic) {
+    if (i == i0 - 1) xr[f2(i0, 1, j)] = v;
+    if (i == 2) xr[f2(i0, i0, j)] = v;
+  }
+}
+
+// Relative residuals from the residual kernel's partial sums, summed in the
+// order the host sums them, and the stop test of the refinement loop:
+// done = !(worst > tol) || passes >= max_iter, worst the running maximum in
+// RHS order with the host's std::max semantics.  first: the residual of the
+// initial guess (passes = 0); otherwise a pass was just added (unless done).
+__global__ void refine_finish_kernel(const double *part, int nrhs, int nchunk, double *rel, int *ctl,
+                                     const double *tolp, int first) {
+  if (!first && ctl[1]) return;
+  for (int r = threadIdx.x; r < nrhs; r += blockDim.x) {
+    double a = 0.0, b = 0.0;
+    for (int c = 0; c < nchunk; ++c) {
+      a += part[((long)r * nchunk + c) * 2];
+      b += part[((long)r * nchunk + c) * 2 + 1];
+    }
+    rel[r] = b > 0 ? sqrt(a / b) : sqrt(a);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double worst = 0.0;
+    for (int k = 0; k < nrhs; ++k) worst = (worst < rel[k]) ? rel[k] : worst;
+    const int it = first ? 0 : ctl[0] + 1;
+    ctl[0] = it;
+    ctl[1] = (!(worst > *tolp) || it >= ctl[2]) ? 1 : 0;
+    rel[nrhs] = worst;
+  }
+}
 }  // namespace sevp
 
 static constexpr int NCHUNK = 64;
@@ -872,7 +947,7 @@ static int residual_norms(sevp_plan p, const float *s, const double *s64, long s
   p->prof.start(SEVP_KCLASS_RESID);
   hipLaunchKernelGGL(resid_norm_kernel, dim3(NCHUNK, nrhs), dim3(256), 0, p->stream, p->al(),
                      p->ab(), p->ac(), p->ar(), p->at(), s, s64, s_stride, x, x_stride, rout, i2,
-                     j2, NCHUNK, p->part.p);
+                     j2, NCHUNK, p->part.p, (const int *)nullptr);
   HIPC(hipGetLastError());
   p->prof.stop(SEVP_KCLASS_RESID);
   std::vector<double> h((size_t)nrhs * NCHUNK * 2);
@@ -1003,6 +1078,14 @@ static void drop_graphs(sevp_plan p) {
     }
   }
   p->graphs.clear();
+  for (auto &g : p->rgraphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    for (auto &e : g.events) {
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
+  }
+  p->rgraphs.clear();
   p->prof.pending.clear();
 }
 
@@ -1014,6 +1097,8 @@ int sevp_plan_destroy(sevp_plan p) {
   for (auto ev : p->gevents) (void)hipEventDestroy(ev);
   if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
   if (p->own_stream) (void)hipStreamDestroy(p->own_stream);
+  if (p->ctl_h) (void)hipHostFree(p->ctl_h);
+  if (p->part_h) (void)hipHostFree(p->part_h);
   delete p;
   return SEVP_OK;
 }
@@ -1729,6 +1814,146 @@ int sevp_set_graphs(sevp_plan p, int on) {
   return SEVP_OK;
 }
 
+// Buffers of the captured refinement, allocated on first use.
+static int ensure_refine_ctl(sevp_plan p) {
+  if (p->ctl.p) return SEVP_OK;
+  CHK(p->ctl.alloc(4));
+  CHK(p->ctl_d.alloc((size_t)p->d.nrhs_max + 2));
+  HIPC(hipHostMalloc((void **)&p->ctl_h, 16 * sizeof(int), hipHostMallocDefault));
+  HIPC(hipHostMalloc((void **)&p->part_h, sizeof(double) * 2 * NCHUNK * p->d.nrhs_max,
+                     hipHostMallocDefault));
+  CHK(p->s_stage.alloc((size_t)p->d.nrhs_max * p->d.i2 * p->d.j2));
+  return SEVP_OK;
+}
+
+// The passes of RHS groups (ng right-hand sides each) on their streams,
+// forked from and joined back into `st` (which may be capturing).
+static int run_group_passes(sevp_plan p, const FullCtx &c, int nrhs, int ngroups, int ng,
+                            hipStream_t st, bool graph_groups) {
+  const long ns = (long)p->d.i2 * p->d.j2, nx = (long)(p->d.i2 + 2) * (p->d.j2 + 2);
+  if (ngroups > 1) HIPC(hipEventRecord(p->gevents[0], st));
+  for (int g = 0; g < ngroups; ++g) {
+    const int r0 = g * ng;
+    if (r0 >= nrhs) break;
+    const int count = std::min(ng, nrhs - r0);
+    hipStream_t gs = g == 0 ? st : p->gstreams[g - 1];
+    if (g > 0) HIPC(hipStreamWaitEvent(gs, p->gevents[0], 0));
+    PassGraph *pg = nullptr;
+    if (graph_groups) CHK(get_pass_graph(p, c, nrhs, r0, count, &pg));
+    if (pg) {
+      HIPC(hipGraphLaunch(pg->exec, gs));
+      if (pg->prof) p->prof.pending.push_back(&pg->events);
+    } else {
+      const FullCtx cg = group_ctx(c, r0);
+      hipStream_t saved = p->prof.st;
+      p->prof.st = gs;
+      int rc = zero_async(p->dx.p + r0 * nx, sizeof(double) * nx * count, gs);
+      if (rc == SEVP_OK)
+        rc = run_full_pass(cg, nullptr, p->r64.p + r0 * ns, ns, p->dx.p + r0 * nx, nx, count, gs);
+      p->prof.st = saved;
+      CHK(rc);
+    }
+    if (g > 0) {
+      HIPC(hipEventRecord(p->gevents[g], gs));
+      HIPC(hipStreamWaitEvent(st, p->gevents[g], 0));
+    }
+  }
+  return SEVP_OK;
+}
+
+static void launch_resid_norm(sevp_plan p, const float *s, const double *x, int nrhs, const int *gate,
+                              hipStream_t st) {
+  const int i2 = p->d.i2, j2 = p->d.j2;
+  hipLaunchKernelGGL(resid_norm_kernel, dim3(NCHUNK, nrhs), dim3(256), 0, st, p->al(), p->ab(),
+                     p->ac(), p->ar(), p->at(), s, (const double *)nullptr, (long)i2 * j2, x,
+                     (long)(i2 + 2) * (j2 + 2), p->r64.p, i2, j2, NCHUNK, p->part.p, gate);
+}
+
+// Capture the whole refinement for `passes` passes (see RefineGraph).  It
+// reads the right-hand sides from the plan's staging copy (so a caller that
+// passes a new s each time step reuses the graph) and updates the caller's x.
+static int get_refine_graph(sevp_plan p, const FullCtx &c, double *x, int nrhs, int ngroups,
+                            int ng, int passes, RefineGraph **out) {
+  const bool prof = p->prof.kclass >= 0;
+  CHK(ensure_refine_ctl(p));
+  const float *s = p->s_stage.p;
+  for (auto &g : p->rgraphs)
+    if (g.nrhs == nrhs && g.prec == p->d.precision && g.kind == p->pass_kind &&
+        g.groups == ngroups && g.passes == passes && g.prof == prof && g.x == x) {
+      *out = &g;
+      return SEVP_OK;
+    }
+  if (p->rgraphs.size() >= 8) {  // the oldest capture goes
+    RefineGraph &o = p->rgraphs.front();
+    HIPC(hipStreamSynchronize(p->stream));
+    if (o.exec) (void)hipGraphExecDestroy(o.exec);
+    for (auto &e : o.events) {
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
+    p->rgraphs.pop_front();
+  }
+  const int i2 = p->d.i2, j2 = p->d.j2;
+  const long nx = (long)(i2 + 2) * (j2 + 2);
+  RefineGraph g;
+  g.nrhs = nrhs; g.prec = p->d.precision; g.kind = p->pass_kind; g.groups = ngroups;
+  g.passes = passes; g.prof = prof; g.s = s; g.x = x;
+  hipStream_t cs = p->cap_stream;
+  HIPC(hipStreamSynchronize(p->stream));
+  HIPC(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+  hipStream_t saved = p->prof.st;
+  p->prof.st = cs;
+  std::vector<hipGraphNode_t> knodes;
+  if (prof) p->prof.capture_into = &knodes;
+  int rc = SEVP_OK;
+  const long n = (long)i2 * j2 * nrhs;
+  launch_resid_norm(p, s, x, nrhs, nullptr, cs);
+  hipLaunchKernelGGL(refine_finish_kernel, dim3(1), dim3(256), 0, cs, (const double *)p->part.p,
+                     nrhs, NCHUNK, p->ctl_d.p + 1, p->ctl.p, (const double *)p->ctl_d.p, 1);
+  for (int it = 0; it < passes && rc == SEVP_OK; ++it) {
+    rc = run_group_passes(p, c, nrhs, ngroups, ng, cs, false);
+    if (rc != SEVP_OK) break;
+    hipLaunchKernelGGL(refine_axpy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, cs, x, nx,
+                       (const double *)p->dx.p, nx, i2, j2, nrhs, c.periodic, (const int *)p->ctl.p);
+    launch_resid_norm(p, s, x, nrhs, p->ctl.p, cs);
+    hipLaunchKernelGGL(refine_finish_kernel, dim3(1), dim3(256), 0, cs, (const double *)p->part.p,
+                       nrhs, NCHUNK, p->ctl_d.p + 1, p->ctl.p, (const double *)p->ctl_d.p, 0);
+  }
+  if (rc == SEVP_OK && hipGetLastError() != hipSuccess) rc = fail(SEVP_E_HIP, "launch failed in capture");
+  p->prof.capture_into = nullptr;
+  p->prof.st = saved;
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(cs, &graph);
+  if (rc != SEVP_OK || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc != SEVP_OK) return rc;
+    HIPC(e);
+  }
+  if (prof) {
+    rc = add_timing_nodes(graph, knodes, g.events);
+    if (rc != SEVP_OK) {
+      (void)hipGraphDestroy(graph);
+      for (auto &ev : g.events) {
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+      }
+      return rc;
+    }
+  }
+  const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) {
+    for (auto &ev : g.events) {
+      (void)hipEventDestroy(ev.first);
+      (void)hipEventDestroy(ev.second);
+    }
+    HIPC(ei);
+  }
+  p->rgraphs.push_back(std::move(g));
+  *out = &p->rgraphs.back();
+  return SEVP_OK;
+}
+
 int sevp_solve_refined_dev(sevp_plan p, const float *s, double *x, int nrhs, double tol,
                            int max_iter, double *rel_out, sevp_stats *stats) {
   if (!p || !s || !x) return fail(SEVP_E_ARG, "null argument");
@@ -1743,52 +1968,61 @@ int sevp_solve_refined_dev(sevp_plan p, const float *s, double *x, int nrhs, dou
   const int ngroups = std::min(p->rhs_groups, nrhs);
   const int ng = (nrhs + ngroups - 1) / ngroups;
   CHK(ensure_group_streams(p, ngroups));
-  std::vector<PassGraph *> graph(ngroups, nullptr);
-  if (p->use_graphs)
-    for (int g = 0; g < ngroups; ++g) {
-      const int r0 = g * ng;
-      if (r0 < nrhs) CHK(get_pass_graph(p, c, nrhs, r0, std::min(ng, nrhs - r0), &graph[g]));
-    }
   std::vector<double> rel;
   int it = 0;
   double worst = -1.0;
-  for (;;) {
-    CHK(residual_norms(p, s, nullptr, ns, x, nx, p->r64.p, nrhs, rel));
+  bool done = false;
+  int &expect = p->expect_passes[p->d.precision & 3][p->pass_kind & 1];
+  if (p->use_graphs && expect > 0 && max_iter > 0) {
+    // the passes the last solve took, as one captured graph; the stop flag
+    // lives on the device and the host reads it once
+    RefineGraph *rg = nullptr;
+    CHK(get_refine_graph(p, c, x, nrhs, ngroups, ng, std::min(expect, max_iter), &rg));
+    HIPC(hipMemcpyAsync(p->s_stage.p, s, sizeof(float) * ns * nrhs, hipMemcpyDeviceToDevice,
+                        p->stream));
+    p->ctl_h[0] = 0;
+    p->ctl_h[1] = 0;
+    p->ctl_h[2] = max_iter;
+    std::memcpy(p->ctl_h + 8, &tol, sizeof(double));
+    HIPC(hipMemcpyAsync(p->ctl.p, p->ctl_h, 3 * sizeof(int), hipMemcpyHostToDevice, p->stream));
+    HIPC(hipMemcpyAsync(p->ctl_d.p, p->ctl_h + 8, sizeof(double), hipMemcpyHostToDevice, p->stream));
+    HIPC(hipGraphLaunch(rg->exec, p->stream));
+    if (rg->prof) p->prof.pending.push_back(&rg->events);
+    HIPC(hipMemcpyAsync(p->ctl_h + 4, p->ctl.p, 2 * sizeof(int), hipMemcpyDeviceToHost, p->stream));
+    HIPC(hipMemcpyAsync(p->part_h, p->part.p, sizeof(double) * 2 * NCHUNK * nrhs,
+                        hipMemcpyDeviceToHost, p->stream));
+    HIPC(hipStreamSynchronize(p->stream));
+    p->prof.harvest();
+    it = p->ctl_h[4];
+    rel.assign(nrhs, 0.0);
     worst = 0.0;
-    for (double v : rel) worst = std::max(worst, v);
-    if (!(worst > tol) || it >= max_iter) break;
-    if (ngroups > 1) HIPC(hipEventRecord(p->gevents[0], p->stream));
-    for (int g = 0; g < ngroups; ++g) {
-      const int r0 = g * ng;
-      if (r0 >= nrhs) break;
-      const int count = std::min(ng, nrhs - r0);
-      hipStream_t st = g == 0 ? p->stream : p->gstreams[g - 1];
-      if (g > 0) HIPC(hipStreamWaitEvent(st, p->gevents[0], 0));
-      if (graph[g]) {
-        HIPC(hipGraphLaunch(graph[g]->exec, st));
-        if (graph[g]->prof) p->prof.pending.push_back(&graph[g]->events);
-      } else {
-        const FullCtx cg = group_ctx(c, r0);
-        hipStream_t saved = p->prof.st;
-        p->prof.st = st;
-        CHK(zero_async(p->dx.p + r0 * nx, sizeof(double) * nx * count, st));
-        const int rc = run_full_pass(cg, nullptr, p->r64.p + r0 * ns, ns, p->dx.p + r0 * nx, nx,
-                                     count, st);
-        p->prof.st = saved;
-        CHK(rc);
+    for (int r = 0; r < nrhs; ++r) {  // as residual_norms sums them
+      double a = 0, b = 0;
+      for (int k = 0; k < NCHUNK; ++k) {
+        a += p->part_h[((size_t)r * NCHUNK + k) * 2];
+        b += p->part_h[((size_t)r * NCHUNK + k) * 2 + 1];
       }
-      if (g > 0) {
-        HIPC(hipEventRecord(p->gevents[g], st));
-        HIPC(hipStreamWaitEvent(p->stream, p->gevents[g], 0));
-      }
+      rel[r] = b > 0 ? std::sqrt(a / b) : std::sqrt(a);
+      worst = std::max(worst, rel[r]);
     }
-    const long n = ns * nrhs;
-    hipLaunchKernelGGL(axpy_interior_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                       p->stream, x, nx, p->dx.p, nx, i2, j2, nrhs);
-    HIPC(hipGetLastError());
-    if (c.periodic) HIPC(launch_halo_fix(x, nx, i2, j2, nrhs, p->stream));
-    ++it;
+    done = !(worst > tol) || it >= max_iter;
   }
+  if (!done) {
+    for (;;) {
+      CHK(residual_norms(p, s, nullptr, ns, x, nx, p->r64.p, nrhs, rel));
+      worst = 0.0;
+      for (double v : rel) worst = std::max(worst, v);
+      if (!(worst > tol) || it >= max_iter) break;
+      CHK(run_group_passes(p, c, nrhs, ngroups, ng, p->stream, p->use_graphs));
+      const long n = ns * nrhs;
+      hipLaunchKernelGGL(axpy_interior_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                         p->stream, x, nx, p->dx.p, nx, i2, j2, nrhs);
+      HIPC(hipGetLastError());
+      if (c.periodic) HIPC(launch_halo_fix(x, nx, i2, j2, nrhs, p->stream));
+      ++it;
+    }
+  }
+  if (!(worst > tol)) expect = it;
   if (rel_out) std::memcpy(rel_out, rel.data(), sizeof(double) * nrhs);
   if (stats) {
     stats->iterations = it;
@@ -2429,6 +2663,65 @@ extern "C" int sevp_matinv(double *b, int n) {
 }
 
 
+
+// ------------------------------------------- MATINV with no allocation
+// A device-pointer MATINV that allocates nothing: the blocked inversion's
+// scratch (panel rows and multipliers, 2 * 32 * n doubles) and the zero-pivot
+// counter live in `scratch`, a file-scope device array of a fixed size, one
+// copy per GPU for the life of the process.  Host code never passes the
+// array's name to a kernel (it is a host-side stand-in there): its device
+// address is looked up once per device with hipGetSymbolAddress, with that
+// device current, and that address is what the kernels get.  Calls on one
+// GPU share the array: calls that overlap in time on one GPU race.
+namespace {
+constexpr int kNoAllocMaxN = 4096;
+constexpr size_t kNoAllocCounter = 2 * 32 * (size_t)kNoAllocMaxN;  // counter slot (in doubles)
+constexpr size_t kNoAllocDoubles = kNoAllocCounter + 32;           // 2 MiB + 256 B
+constexpr int kNoAllocDevices = 64;
+std::mutex g_noalloc_mu;
+double *g_noalloc_ptr[kNoAllocDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+__global__ void copy_int_kernel(const int *src, int *dst) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) dst[0] = src[0];
+}
+
+// The calling thread's current device's copy of `scratch`.
+static int noalloc_scratch(double **out) {
+  int dev = 0;
+  HIPC(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocDevices) return fail(SEVP_E_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(SEVP_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_noalloc_ptr[dev] = (double *)ptr;
+  }
+  *out = g_noalloc_ptr[dev];
+  return SEVP_OK;
+}
+
+extern "C" int sevp_matinv_dev_noalloc(double *b_dev, int n, int *zero_pivots_dev, void *hip_stream) {
+  if (!b_dev || n < 1) return fail(SEVP_E_ARG, "bad MATINV argument");
+  if (n > kNoAllocMaxN)
+    return fail(SEVP_E_ARG, "sevp_matinv_dev_noalloc: order " + std::to_string(n) +
+                                " exceeds its fixed scratch (order <= 4096)");
+  double *base = nullptr;
+  CHK(noalloc_scratch(&base));
+  hipStream_t st = (hipStream_t)hip_stream;
+  int *zp = (int *)(base + kNoAllocCounter);
+  CHK(zero_async(zp, sizeof(int), st));  // a kernel, not a memset node
+  HIPC(launch_matinv(b_dev, (long)n * n, n, 1, zp, matinv_uses_lds(n) ? nullptr : base, st));
+  if (zero_pivots_dev) {
+    hipLaunchKernelGGL(copy_int_kernel, dim3(1), dim3(64), 0, st, (const int *)zp, zero_pivots_dev);
+    HIPC(hipGetLastError());
+  }
+  return SEVP_OK;
+}
 
 // ------------------------------------------------------------ Krylov solvers
 // Drivers of the device BiCGSTAB, P-CSI and Lanczos (sevp_krylov.hip) with

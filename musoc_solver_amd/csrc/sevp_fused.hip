@@ -15,10 +15,13 @@
 //     restores the uncorrected value anyway, REP1's DUM0), its closure
 //     residual e_(b+1) needs the marched rows at the neighbouring columns, so
 //     a tile computes 16 columns of d and marches them with DPP row shifts
-//     (16 lanes = 16 columns of one right-hand side) and writes only the
-//     16 - 2*steps columns that stay exact after `steps` rows (the trapezoid
-//     of the march kernels; recomputed columns repeat the same operations on
-//     the same operands, so they agree bitwise with the tile that owns them).
+//     (16 lanes = 16 columns of one right-hand side).  The first row
+//     operation takes the free row's neighbours from memory and is exact in
+//     every lane; each later one (march rows, then the closure) loses a lane
+//     per side, so a tile writes the 16 - 2*steps columns that stay exact
+//     (the trapezoid of the march kernels; recomputed columns repeat the
+//     same operations on the same operands, so they agree bitwise with the
+//     tile that owns them).
 //   backward, block b: [c | cpl] = e_b [RINV_b | W_b].  X(JS+1) += c and the
 //     closure residual of block b-1, which reads that row only at its own
 //     column, are done at once; c and cpl are stored for the homogeneous
@@ -204,15 +207,30 @@ __global__ __launch_bounds__(64 * W) void fused_fwd_kernel(FusedFwdArgs p) {
       pc = min(max(c, 1), i2);
     }
     const int xc = halo ? (c < 1 ? 1 : i0) : pc + 1;  // X column
+    // X columns of the west and east neighbours (wrapped, or the Dirichlet
+    // column at the edge)
+    int xcw, xce;
+    if (p.periodic) {
+      xcw = (pc == 1 ? i2 : pc - 1) + 1;
+      xce = (pc == i2 ? 1 : pc + 1) + 1;
+    } else {
+      xcw = max(min(c - 1, i2 + 1), 0) + 1;
+      xce = max(min(c + 1, i2 + 1), 0) + 1;
+    }
     const bool out = col >= steps && col < 16 - steps && c >= 1 && c <= i2;
     const bool act = t < 256 && m < p.M;
     double *x = p.x + (long)(act ? m : 0) * p.x_stride;
     const int js = p.js, jf = p.jf, ieb = p.ieb;
-    // the epilogue's rows that do not depend on the product, loaded first
-    double vs = 0.0, vc = 0.0, xnorth = 0.0;
+    // the epilogue's rows that do not depend on the product, loaded first;
+    // the free row's neighbours come from memory (that row is not written
+    // in the forward pass), so the first row operation is exact in every
+    // lane and each later one loses one lane per side
+    double vs = 0.0, vc = 0.0, xnorth = 0.0, fw = 0.0, fe = 0.0;
     if (act) {
       vs = x[f2(i0, xc, js)];
       vc = x[f2(i0, xc, js + 1)];
+      fw = x[f2(i0, xcw, js + 1)];
+      fe = x[f2(i0, xce, js + 1)];
       xnorth = x[f2(i0, xc, ieb + 1)];
     }
     const double d = product_tile<T, W>((const T *)p.a, p.M, i2, 16 * mb, (const T *)p.b, pc - 1,
@@ -223,7 +241,8 @@ __global__ __launch_bounds__(64 * W) void fused_fwd_kernel(FusedFwdArgs p) {
       const long fo = (long)m * p.f_stride + co;
       auto cf = [&](const float *arr, int j) -> double { return (double)arr[co + (long)(j - 1) * i2]; };
       for (int j = js; j <= jf; ++j) {
-        const double xw = row_shr1(vc), xe = row_shl1(vc);
+        const double sw = row_shr1(vc), se = row_shl1(vc);
+        const double xw = j == js ? fw : sw, xe = j == js ? fe : se;
         double r = (rhs_at(p.f, p.f64, fo + (long)(j - 1) * i2) - cf(p.ax, j) * xw -
                     cf(p.ay, j) * vs - cf(p.bb, j) * vc - cf(p.cx, j) * xe) / cf(p.cy, j);
         if (halo) r = x[f2(i0, xc, j + 2)];
@@ -232,7 +251,8 @@ __global__ __launch_bounds__(64 * W) void fused_fwd_kernel(FusedFwdArgs p) {
         vc = r;
       }
       const int jr = ieb - 1;  // closure row
-      const double xw = row_shr1(vc), xe = row_shl1(vc);
+      const double sw = row_shr1(vc), se = row_shl1(vc);
+      const double xw = steps == 0 ? fw : sw, xe = steps == 0 ? fe : se;
       const double e = rhs_at(p.f, p.f64, fo + (long)(jr - 1) * i2) - cf(p.ax, jr) * xw -
                        cf(p.ay, jr) * vs - cf(p.bb, jr) * vc - cf(p.cx, jr) * xe -
                        cf(p.cy, jr) * xnorth;

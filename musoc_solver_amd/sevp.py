@@ -98,6 +98,7 @@ def lib():
         L.sevp_pre.argtypes = [VP] * 9 + [CI] * 4
         L.sevp_pre1.argtypes = [VP] * 9 + [CI] * 3
         L.sevp_matinv.argtypes = [VP, CI]
+        L.sevp_matinv_dev_noalloc.argtypes = [VP, CI, VP, VP]
         L.sevp_set_precision.argtypes = [VP, CI]
         L.sevp_set_storage.argtypes = [VP, CI]
         L.sevp_set_pass.argtypes = [VP, CI]
@@ -577,6 +578,13 @@ def matinv(b):
     """MATINV (PREP/template/GLOBAL/src/prep_GLOBAL.f90:516-568), in place."""
     _check(lib().sevp_matinv(_p(_f64(b)), b.shape[0]))
     return b
+
+
+def matinv_dev_noalloc(b_dev, n, zero_pivots_dev=None, stream=None):
+    """MATINV of the n x n column-major FP64 matrix at device tensor b_dev, in
+    place, without allocating (sevp_matinv_dev_noalloc); enqueued on stream
+    (a raw HIP stream handle, None: the null stream), not waited for."""
+    _check(lib().sevp_matinv_dev_noalloc(_p(b_dev), n, _p(zero_pivots_dev), stream))
 
 
 # ---------------------------------------------------------------- EVP file
