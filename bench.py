@@ -77,6 +77,8 @@ def parse():
                     help="also run the 0.1 degree leg: 3600x1800, 16 RHS, FP32 MFMA corrections "
                          "refined to the tolerance, products-only plan (0: skip)")
     ap.add_argument("--tenth-only", action="store_true", help="run the 0.1 degree leg alone")
+    ap.add_argument("--bf16-passes", type=int, default=4,
+                    help="passes of the BF16 trial on the first RHS (0: skip)")
     ap.add_argument("--kernel-timing", choices=("after", "timed"), default="after",
                     help="time the correction kernel with HIP events in one extra solve after "
                          "the timed steps, launched eagerly with events from the kernel's own "
@@ -261,6 +263,23 @@ def main():
         err64 = float((d / n).max())
         log(f"fast64 vs plain FP64: max rel error {err64:.3e}")
 
+    # the BF16 trial: one pass at a time from x = 0 on the first right-hand
+    # side, BF16 matrices and error rows with FP32 accumulation (fused pass)
+    bf16 = None
+    if a.bf16_passes > 0 and a.pass_kernels == "fused":
+        p_b = plans[n0_of[order[0]]][0]
+        try:
+            p_b.set_precision(S.PREC_BF16)
+            xb = torch.zeros((1, j2 + 2, i2 + 2), dtype=torch.float64, device=dev)
+            hist = [1.0]
+            for _ in range(a.bf16_passes):
+                rel_b, _, _ = p_b.solve_refined_dev(s[:1], xb, tol=0.0, max_iter=1)
+                hist.append(float(rel_b[0]))
+            bf16 = {"rel_resid_after_pass": hist[1:],
+                    "contraction": [hist[k + 1] / hist[k] for k in range(len(hist) - 1)]}
+            log(f"bf16 trial: {bf16}")
+        except Exception as e:  # noqa: BLE001 - reported, not hidden
+            bf16 = {"error": repr(e)}
     main_name = next(n for n in ("mixed", "plain", "fast64") if n in legs)
     main_leg = legs[main_name]
     fused = a.pass_kernels == "fused"
@@ -300,6 +319,7 @@ def main():
                            if "plain" in legs else None),
         "setup_s": t_setup,
         "roofline": roof,
+        "bf16_trial": bf16,
         "cpu_baseline": cpu,
         "krylov": krylov,
         "basins": basins_or_error(a),

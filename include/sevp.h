@@ -48,7 +48,9 @@ extern "C" {
 /* Arithmetic of the influence-matrix corrections. */
 #define SEVP_PREC_FP64 0 /* plain: reference operation order, bitwise equal to the reference */
 #define SEVP_PREC_FP32 1 /* corrections as FP32 MFMA GEMMs; marching and residual FP64 */
-#define SEVP_PREC_BF16 2 /* corrections as BF16-input, FP32-accumulate MFMA GEMMs (not built) */
+#define SEVP_PREC_BF16 2 /* corrections as BF16-input, FP32-accumulate MFMA GEMMs (the fused pass
+                            only: BF16 matrices, FP32 error rows rounded to BF16 in registers);
+                            a trial: each refinement pass contracts the residual by ~0.2 at best */
 #define SEVP_PREC_FP64_MFMA 3 /* corrections as FP64 MFMA split-K products: FP64 arithmetic in a
                                  blocked summation order (not bitwise the reference) */
 

@@ -280,8 +280,10 @@ struct sevp_plan_s {
   int pass_kind = SEVP_PASS_SPLITK;
   DBuf<float> rinv32c, rinv1_32c, w32c;
   DBuf<double> w64c;
-  bool has32c = false, has64c = false;
+  DBuf<unsigned short> rinv16c, rinv1_16c, w16c;  // BF16 (fused pass only)
+  bool has32c = false, has64c = false, has16c = false;
   DBuf<double> fe, fh, fhc;  // fused pass: error rows and c / cpl rows, two parities each
+  DBuf<double> fd;           // fused pass: d of the blocks below one-row blocks, (NB0-1) slices
   DBuf<float> fe32;
   // solve scratch
   DBuf<double> e, dum0, hfree, hcpl, g;  // full forms
@@ -385,12 +387,15 @@ struct FullCtx {
   double *kpart64;   // FP64 MFMA: split-K partial products (selects that arithmetic)
   // fused pass (one launch per block step): column-major operands and the
   // two-parity scratch rows (each parity half holds fhalf elements)
-  bool fused, fused64;
+  bool fused;
+  int farith;  // FUSED_F32 / FUSED_F64 / FUSED_B16
   const float *rinv32c, *rinv1_32c, *w32c;
   const double *w64c;
+  const unsigned short *rinv16c, *rinv1_16c, *w16c;
   double *fe, *fh, *fhc;
   float *fe32;
   long fhalf;
+  double *fd;  // d_b, slice b-1 of fhalf elements, for blocks b followed by a one-row block
   KProf *prof;
 };
 
@@ -576,8 +581,15 @@ static int run_mixed_pass(const FullCtx &c, const float *f32, const double *f64,
 // launch reads what it writes.
 static int run_fused_pass(const FullCtx &c, const float *f32, const double *f64, long f_stride,
                           double *x, long x_stride, int nrhs, hipStream_t st) {
-  const bool dbl = c.fused64;
+  const bool dbl = c.farith == FUSED_F64;
   const long ww = (long)c.i2 * c.i2;
+  // column-major operands of block b (RINV1, RINV, W) in the product's arithmetic
+  auto opnd = [&](const double *f64, const float *f32, const unsigned short *b16,
+                  long off) -> const void * {
+    if (c.farith == FUSED_F64) return f64 + off;
+    if (c.farith == FUSED_B16) return b16 + off;
+    return f32 + off;
+  };
   double *E[2] = {c.fe, c.fe + c.fhalf};
   float *E32[2] = {c.fe32, c.fe32 + c.fhalf};
   double *HF[2] = {c.fh, c.fh + c.fhalf}, *HC[2] = {c.fhc, c.fhc + c.fhalf};
@@ -622,7 +634,7 @@ static int run_fused_pass(const FullCtx &c, const float *f32, const double *f64,
   for (int b = 1; b < c.nb0; ++b) {
     FusedFwdArgs a{};
     a.a = erow(b);
-    a.b = dbl ? (const void *)(c.rinv1 + (b - 1) * ww) : (const void *)(c.rinv1_32c + (b - 1) * ww);
+    a.b = opnd(c.rinv1, c.rinv1_32c, c.rinv1_16c, (b - 1) * ww);
     a.M = nrhs; a.i2 = c.i2; a.periodic = c.periodic;
     a.js = c.ie[b - 1]; a.jf = c.ie[b] - 2; a.ieb = c.ie[b];
     a.steps = a.jf - a.js + 1;
@@ -631,17 +643,17 @@ static int run_fused_pass(const FullCtx &c, const float *f32, const double *f64,
     a.x = x; a.x_stride = x_stride;
     a.e_out = dbl ? E[(b + 1) & 1] : nullptr;
     a.e32_out = dbl ? nullptr : E32[(b + 1) & 1];
+    if (a.steps == 0) a.d_out = c.fd + (long)(b - 1) * c.fhalf;  // block b+1 is one row high
     FusedFwdArgs sz = a;
     if (fused_fwd_prepare(sz)) return fail(SEVP_E_ARG, "block too tall for the fused pass");
-    CHK(timed([&](hipEvent_t e0, hipEvent_t e1) { return launch_fused_fwd(a, dbl, st, e0, e1); },
+    CHK(timed([&](hipEvent_t e0, hipEvent_t e1) { return launch_fused_fwd(a, c.farith, st, e0, e1); },
               fused_fwd_workgroups(sz), a.tspan));
   }
   for (int b = c.nb0; b >= 1; --b) {
     FusedBwdArgs a{};
     a.a = erow(b);
-    a.b = dbl ? (const void *)(c.rinv + (b - 1) * ww) : (const void *)(c.rinv32c + (b - 1) * ww);
-    a.bw = (b == 1) ? nullptr
-                    : dbl ? (const void *)(c.w64c + (b - 2) * ww) : (const void *)(c.w32c + (b - 2) * ww);
+    a.b = opnd(c.rinv, c.rinv32c, c.rinv16c, (b - 1) * ww);
+    a.bw = (b == 1) ? nullptr : opnd(c.w64c, c.w32c, c.w16c, (b - 2) * ww);
     a.M = nrhs; a.i2 = c.i2; a.periodic = c.periodic;
     a.js = (b != 1) ? c.ie[b - 2] : 1;
     a.do_resid = b != 1;
@@ -650,6 +662,8 @@ static int run_fused_pass(const FullCtx &c, const float *f32, const double *f64,
     a.x = x; a.x_stride = x_stride;
     a.e_out = dbl ? E[(b - 1) & 1] : nullptr;
     a.e32_out = dbl ? nullptr : E32[(b - 1) & 1];
+    if (b >= 3 && c.ie[b - 2] - c.ie[b - 3] == 1)  // block b-1 is one row high
+      a.dsouth = c.fd + (long)(b - 3) * c.fhalf;
     a.hfree_out = HF[b & 1];
     a.hcpl_out = HC[b & 1];
     if (b != c.nb0) {  // the homogeneous march of block b+1, from the previous launch's rows
@@ -671,7 +685,7 @@ static int run_fused_pass(const FullCtx &c, const float *f32, const double *f64,
     FusedBwdArgs sz = a;
     int K = 1;
     if (fused_bwd_prepare(sz, &K)) return fail(SEVP_E_ARG, "shape refused by the fused pass");
-    CHK(timed([&](hipEvent_t e0, hipEvent_t e1) { return launch_fused_bwd(a, dbl, st, e0, e1); },
+    CHK(timed([&](hipEvent_t e0, hipEvent_t e1) { return launch_fused_bwd(a, c.farith, st, e0, e1); },
               fused_bwd_workgroups(sz), a.tspan));
   }
   CHK(march(MODE_HOM, 1));
@@ -972,6 +986,29 @@ static int make_low_precision_copies(sevp_plan p);
 static bool fused_eligible(sevp_plan p);
 static int alloc_operands(sevp_plan p);
 
+// A block after the first that is one row high: its backward closure
+// residual reads the row below it as REP1's forward correction left it.
+static bool has_unit_block(sevp_plan p) {
+  for (int b = 1; b < p->d.nb0; ++b)
+    if (p->ie[b] - p->ie[b - 1] == 1) return true;
+  return false;
+}
+
+// Whether pass kernels `kind` can run corrections of `precision` on this
+// plan's block rows: the fused pass takes blocks of 1 to 4 rows; the split-K
+// pass keeps no corrected rows for the backward pass, so no one-row block may
+// follow the first; BF16 runs in the fused pass only.
+static int check_pass(sevp_plan p, int kind, int precision) {
+  if (precision == SEVP_PREC_FP64 || p->d.form == SEVP_FORM_BIR) return SEVP_OK;
+  if (kind == SEVP_PASS_FUSED && !fused_eligible(p))
+    return fail(SEVP_E_ARG, "the fused pass needs block heights <= 4 and I2 divisible by 4 (>= 16)");
+  if (kind == SEVP_PASS_SPLITK && precision == SEVP_PREC_BF16)
+    return fail(SEVP_E_ARG, "BF16 corrections run in the fused pass only");
+  if (kind == SEVP_PASS_SPLITK && has_unit_block(p))
+    return fail(SEVP_E_ARG, "the split-K pass needs blocks of at least 2 rows after the first");
+  return SEVP_OK;
+}
+
 // ------------------------------------------------------------------ C ABI
 extern "C" {
 
@@ -1061,7 +1098,7 @@ int sevp_plan_create(const sevp_desc *desc, const float *al, const float *ab, co
     if ((rc = p->g.alloc((size_t)nr * d.i2))) return bail(rc);
     if ((rc = p->r64.alloc((size_t)nr * nc))) return bail(rc);
     if ((rc = p->dx.alloc((size_t)nr * nx))) return bail(rc);
-    if (d.precision == SEVP_PREC_BF16) return bail(fail(SEVP_E_ARG, "BF16 corrections are not built"));
+    if ((rc = check_pass(p, p->pass_kind, d.precision))) return bail(rc);
     if (d.precision != SEVP_PREC_FP64)
       if ((rc = alloc_mixed(p))) return bail(rc);
   }
@@ -1119,7 +1156,10 @@ size_t sevp_plan_device_bytes(sevp_plan p) {
          p->kpart.bytes() + p->xx.bytes() + p->rinv64t.bytes() + p->rinv1_64t.bytes() +
          p->w64t.bytes() + p->kpart64.bytes() +
          p->bdum0.bytes() + p->s14.bytes() + p->r64.bytes() + p->dx.bytes() + p->part.bytes() + p->b1s.bytes() +
-         p->zp.bytes();
+         p->zp.bytes() + p->rinv32c.bytes() + p->rinv1_32c.bytes() + p->w32c.bytes() + p->w64c.bytes() +
+         p->rinv16c.bytes() + p->rinv1_16c.bytes() + p->w16c.bytes() + p->fe.bytes() + p->fh.bytes() +
+         p->fhc.bytes() + p->fe32.bytes() + p->fd.bytes() + p->ctl.bytes() + p->ctl_d.bytes() +
+         p->s_stage.bytes();
 }
 
 // FP64 influence matrices of an SEVP_STORE_FP64 plan, allocated on first use.
@@ -1234,14 +1274,20 @@ static int make_low_precision_copies(sevp_plan p) {
   if (p->d.precision == SEVP_PREC_FP64 || p->d.form == SEVP_FORM_BIR) return SEVP_OK;
   if (p->storage == SEVP_STORE_PRODUCTS) return SEVP_OK;  // made while the blocks were built
   const bool dbl = p->d.precision == SEVP_PREC_FP64_MFMA;
+  const bool b16 = p->d.precision == SEVP_PREC_BF16;
   const bool fused = p->pass_kind == SEVP_PASS_FUSED;
-  bool &have = fused ? (dbl ? p->has64c : p->has32c) : (dbl ? p->has64 : p->has32);
+  if (b16 && !fused) return fail(SEVP_E_ARG, "BF16 corrections run in the fused pass only");
+  bool &have = b16 ? p->has16c
+                   : fused ? (dbl ? p->has64c : p->has32c) : (dbl ? p->has64 : p->has32);
   if (have) return SEVP_OK;
   CHK(alloc_operands(p));
   const int i2 = p->d.i2, nb1 = p->d.nb0 > 1 ? p->d.nb0 - 1 : 1;
   const long ww = (long)i2 * i2;
   const long tw = lane_tiles_floats(i2, i2);
-  if (fused && !dbl) {
+  if (b16) {
+    HIPC(launch_cvt_bf16(p->rinv.p, p->rinv16c.p, ww * p->d.nb0, p->stream));
+    HIPC(launch_cvt_bf16(p->rinv1.p, p->rinv1_16c.p, ww * nb1, p->stream));
+  } else if (fused && !dbl) {
     HIPC(launch_cvt_f32(p->rinv.p, p->rinv32c.p, ww * p->d.nb0, p->stream));
     HIPC(launch_cvt_f32(p->rinv1.p, p->rinv1_32c.p, ww * nb1, p->stream));
   } else if (!fused && dbl) {
@@ -1269,7 +1315,8 @@ static int make_low_precision_copies(sevp_plan p) {
       gm.out = (fused && dbl) ? p->w64c.p + (long)(nb - 2) * ww : w.p;
       gm.out_ld = i2;
       HIPC(launch_gemm(gm, p->stream));
-      if (fused && !dbl) HIPC(launch_cvt_f32(w.p, p->w32c.p + (long)(nb - 2) * ww, ww, p->stream));
+      if (b16) HIPC(launch_cvt_bf16(w.p, p->w16c.p + (long)(nb - 2) * ww, ww, p->stream));
+      else if (fused && !dbl) HIPC(launch_cvt_f32(w.p, p->w32c.p + (long)(nb - 2) * ww, ww, p->stream));
       else if (!fused && dbl)
         HIPC(launch_cvt_lane_tiles64(w.p, p->w64t.p + (long)(nb - 2) * tw, i2, i2, 1, p->stream));
       else if (!fused)
@@ -1306,7 +1353,7 @@ int sevp_build_influence(sevp_plan p, int *zero_pivots) {
     c.map = ColMap{1, 0, 0};
   }
   CHK(run_pre(c, p->stream));
-  p->has32 = p->has64 = p->has32c = p->has64c = false;
+  p->has32 = p->has64 = p->has32c = p->has64c = p->has16c = false;
   CHK(make_low_precision_copies(p));
   int z = 0;
   HIPC(hipMemcpyAsync(&z, p->zp.p, sizeof(int), hipMemcpyDeviceToHost, p->stream));
@@ -1322,7 +1369,7 @@ int sevp_load_influence(sevp_plan p, const double *rinv, const double *rinv1) {
   CHK(ensure_inverses(p));
   HIPC(hipMemcpy(p->rinv.p, rinv, p->rinv.bytes(), hipMemcpyHostToDevice));
   if (p->d.nb0 > 1) HIPC(hipMemcpy(p->rinv1.p, rinv1, p->rinv1.bytes(), hipMemcpyHostToDevice));
-  p->has32 = p->has64 = p->has32c = p->has64c = false;
+  p->has32 = p->has64 = p->has32c = p->has64c = p->has16c = false;
   CHK(make_low_precision_copies(p));
   HIPC(hipStreamSynchronize(p->stream));
   p->have_inv = true;
@@ -1370,11 +1417,14 @@ static FullCtx full_ctx(sevp_plan p) {
     c.w64t = p->w64t.p;
     c.kpart64 = p->kpart64.p;
   }
-  c.fused = (mixed || mfma64) && p->pass_kind == SEVP_PASS_FUSED;
-  c.fused64 = mfma64;
+  const bool bf16 = p->d.precision == SEVP_PREC_BF16;
+  c.fused = (mixed || mfma64 || bf16) && p->pass_kind == SEVP_PASS_FUSED;
+  c.farith = mfma64 ? FUSED_F64 : (bf16 ? FUSED_B16 : FUSED_F32);
   c.rinv32c = p->rinv32c.p; c.rinv1_32c = p->rinv1_32c.p; c.w32c = p->w32c.p;
   c.w64c = p->w64c.p;
+  c.rinv16c = p->rinv16c.p; c.rinv1_16c = p->rinv1_16c.p; c.w16c = p->w16c.p;
   c.fe = p->fe.p; c.fh = p->fh.p; c.fhc = p->fhc.p; c.fe32 = p->fe32.p;
+  c.fd = p->fd.p;
   c.fhalf = (long)p->d.nrhs_max * p->d.i2;
   c.prof = &p->prof;
   return c;
@@ -1399,6 +1449,7 @@ static FullCtx group_ctx(const FullCtx &c0, int r0) {
   if (c.fh) c.fh += r0 * i2;
   if (c.fhc) c.fhc += r0 * i2;
   if (c.fe32) c.fe32 += r0 * i2;
+  if (c.fd) c.fd += r0 * i2;
   return c;
 }
 
@@ -1419,6 +1470,8 @@ static int alloc_mixed(sevp_plan p) {
     CHK(p->fh.alloc(nf));
     CHK(p->fhc.alloc(nf));
     CHK(p->fe32.alloc(nf));
+    if (has_unit_block(p))
+      CHK(p->fd.alloc((size_t)(p->d.nb0 - 1) * nr * p->d.i2));
   }
   if (dbl) {
     CHK(p->kpart64.alloc(std::max(n1, n2) * nr));
@@ -1437,7 +1490,13 @@ static int alloc_operands(sevp_plan p) {
   const bool dbl = p->d.precision == SEVP_PREC_FP64_MFMA;
   if (p->pass_kind == SEVP_PASS_FUSED) {
     const size_t ww = (size_t)p->d.i2 * p->d.i2;
-    if (dbl) {
+    if (p->d.precision == SEVP_PREC_BF16) {
+      if (!p->rinv16c.p) {
+        CHK(p->rinv16c.alloc(ww * p->d.nb0));
+        CHK(p->rinv1_16c.alloc(ww * nb1));
+        CHK(p->w16c.alloc(ww * nb1));
+      }
+    } else if (dbl) {
       if (!p->w64c.p) CHK(p->w64c.alloc(ww * nb1));
     } else if (!p->rinv32c.p) {
       CHK(p->rinv32c.alloc(ww * p->d.nb0));
@@ -1480,6 +1539,7 @@ int sevp_set_pass(sevp_plan p, int kind) {
   if (p->d.form == SEVP_FORM_BIR) return fail(SEVP_E_ARG, "the BIR form has no correction products to fuse");
   if (kind == SEVP_PASS_FUSED && !fused_eligible(p))
     return fail(SEVP_E_ARG, "the fused pass needs block heights <= 4 and I2 divisible by 4 (>= 16)");
+  CHK(check_pass(p, kind, p->d.precision));
   if (p->storage == SEVP_STORE_PRODUCTS && p->have_inv)
     return fail(SEVP_E_STATE, "a products-only plan keeps the operand layout it was built with");
   HIPC(hipStreamSynchronize(p->stream));
@@ -1515,7 +1575,7 @@ int sevp_set_storage(sevp_plan p, int storage) {
   drop_graphs(p);
   p->rinv.release();
   p->rinv1.release();
-  p->has32 = p->has64 = p->has32c = p->has64c = false;
+  p->has32 = p->has64 = p->has32c = p->has64c = p->has16c = false;
   p->have_inv = false;
   p->storage = storage;
   return SEVP_OK;
@@ -1527,7 +1587,7 @@ int sevp_set_precision(sevp_plan p, int precision) {
     return fail(SEVP_E_ARG, "unknown precision");
   if (p->d.form == SEVP_FORM_BIR && precision != SEVP_PREC_FP64)
     return fail(SEVP_E_ARG, "BIR form runs the plain FP64 arithmetic only");
-  if (precision == SEVP_PREC_BF16) return fail(SEVP_E_ARG, "BF16 corrections are not built");
+  CHK(check_pass(p, p->pass_kind, precision));
   if (p->storage == SEVP_STORE_PRODUCTS && precision != SEVP_PREC_FP32)
     return fail(SEVP_E_ARG, "a plan that keeps only FP32 product operands runs SEVP_PREC_FP32 only");
   const int old = p->d.precision;

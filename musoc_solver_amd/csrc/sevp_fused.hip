@@ -43,17 +43,7 @@ namespace sevp {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
-
-template <class T>
-struct Vec4;
-template <>
-struct Vec4<float> {
-  typedef f32x4 type;
-};
-template <>
-struct Vec4<double> {
-  typedef f64x4 type;
-};
+typedef short s16x4 __attribute__((ext_vector_type(4)));
 
 // Lane i of each 16-lane row receives lane i-1's (row_shr:1) or lane i+1's
 // (row_shl:1) value; the row's edge lane keeps its own.  The edge lanes are
@@ -69,24 +59,67 @@ __device__ __forceinline__ double row_shl1(double v) {
                           __builtin_amdgcn_update_dpp(lo, lo, 0x101, 0xf, 0xf, false));
 }
 
-__device__ __forceinline__ f32x4 mfma16(f32x4 a, f32x4 b, f32x4 c, int t) {
-  return __builtin_amdgcn_mfma_f32_16x16x4f32(a[t], b[t], c, 0, 0, 0);
-}
-__device__ __forceinline__ f64x4 mfma16(f64x4 a, f64x4 b, f64x4 c, int t) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b[t], c, 0, 0, 0);
-}
-
-// Accumulator element i of lane group q holds row q4(q, i) of its column:
-// 4q + i for the FP32 instruction, q + 4i for the FP64 one.
-template <class T>
-__device__ __forceinline__ int acc_row(int q, int i) {
-  return sizeof(T) == 4 ? 4 * q + i : q + 4 * i;
+// BF16 bits of a float, rounded to nearest even (NaN stays a quiet NaN).
+__device__ __forceinline__ short bf16_rne(float f) {
+  const unsigned u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (short)((u >> 16) | 0x40);
+  return (short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
 
-// Four consecutive k of a run, zero at or past klen (klen a multiple of 4).
-template <class T>
-__device__ __forceinline__ typename Vec4<T>::type ld4(const T *base, int k, int klen) {
-  typedef typename Vec4<T>::type V;
+// The three product arithmetics.  Per chunk of 16 k a lane holds k = 4q ..
+// 4q+3 (q = lane / 16) of its A row and B column: FP32 and FP64 take four
+// 16x16x4 MFMA steps (one k each; A and B use the same permutation of k, so
+// the product is unchanged); BF16 rounds the FP32 error row to BF16 in
+// registers and takes one 16x16x16 step with FP32 accumulation.
+struct KF32 {
+  typedef float A;
+  typedef float B;
+  typedef float Acc;
+  typedef f32x4 AV;
+  typedef f32x4 BV;
+  typedef f32x4 CV;
+  static constexpr int G = 6;  // chunks per load group
+  __device__ static int row(int q, int i) { return 4 * q + i; }
+  __device__ static CV mma(AV a, BV b, CV c) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t], b[t], c, 0, 0, 0);
+    return c;
+  }
+};
+struct KF64 {
+  typedef double A;
+  typedef double B;
+  typedef double Acc;
+  typedef f64x4 AV;
+  typedef f64x4 BV;
+  typedef f64x4 CV;
+  static constexpr int G = 4;
+  __device__ static int row(int q, int i) { return q + 4 * i; }  // the FP64 instruction's D layout
+  __device__ static CV mma(AV a, BV b, CV c) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b[t], c, 0, 0, 0);
+    return c;
+  }
+};
+struct KB16 {
+  typedef float A;            // the error rows stay FP32 in memory
+  typedef unsigned short B;   // BF16 bits
+  typedef float Acc;
+  typedef f32x4 AV;
+  typedef s16x4 BV;
+  typedef f32x4 CV;
+  static constexpr int G = 8;
+  __device__ static int row(int q, int i) { return 4 * q + i; }
+  __device__ static CV mma(AV a, BV b, CV c) {
+    const s16x4 ab = {bf16_rne(a[0]), bf16_rne(a[1]), bf16_rne(a[2]), bf16_rne(a[3])};
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ab, b, c, 0, 0, 0);
+  }
+};
+
+// Four consecutive k of a run, zero at or past klen (klen a multiple of 4):
+// one unconditional vector load (from k = 0 when out of range) and a select.
+template <class V, class T>
+__device__ __forceinline__ V ld4(const T *base, int k, int klen) {
   const bool ok = k < klen;
   const V v = *reinterpret_cast<const V *>(base + (ok ? k : 0));
   return ok ? v : V{0, 0, 0, 0};
@@ -96,38 +129,40 @@ __device__ __forceinline__ typename Vec4<T>::type ld4(const T *base, int k, int 
 // column-major B column of this lane's column (lane % 16), over the chunks
 // [c_lo, c_hi) of 16 k.  Loads are issued G chunks at a time, one group
 // ahead of the MFMA steps (2G chunks in flight).
-template <class T, int G>
-__device__ __forceinline__ typename Vec4<T>::type part_tile(const T *a, int M, int K, int m0,
-                                                            const T *bcol, bool bok, int c_lo,
-                                                            int c_hi) {
-  typedef typename Vec4<T>::type V;
+template <class KT>
+__device__ __forceinline__ typename KT::CV part_tile(const typename KT::A *a, int M, int K, int m0,
+                                                     const typename KT::B *bcol, bool bok,
+                                                     int c_lo, int c_hi) {
+  typedef typename KT::AV AV;
+  typedef typename KT::BV BV;
+  constexpr int G = KT::G;
   const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
   const int m = m0 + r;
-  const T *arow = a + (long)(m < M ? m : 0) * K;
+  const typename KT::A *arow = a + (long)(m < M ? m : 0) * K;
   const int alen = m < M ? K : 0;
   const int blen = bok ? K : 0;
-  V acc = V{0, 0, 0, 0};
-  V av[G], bv[G];
+  typename KT::CV acc = typename KT::CV{0, 0, 0, 0};
+  AV av[G];
+  BV bv[G];
 #pragma unroll
   for (int g = 0; g < G; ++g) {
     const int k = 16 * (c_lo + g) + 4 * q;
     const bool in = c_lo + g < c_hi;
-    av[g] = ld4<T>(arow, k, in ? alen : 0);
-    bv[g] = ld4<T>(bcol, k, in ? blen : 0);
+    av[g] = ld4<AV>(arow, k, in ? alen : 0);
+    bv[g] = ld4<BV>(bcol, k, in ? blen : 0);
   }
   for (int c0 = c_lo; c0 < c_hi; c0 += G) {
-    V an[G], bn[G];
+    AV an[G];
+    BV bn[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) {  // the next group, issued before this group's MFMA steps
       const int k = 16 * (c0 + G + g) + 4 * q;
       const bool in = c0 + G + g < c_hi;
-      an[g] = ld4<T>(arow, k, in ? alen : 0);
-      bn[g] = ld4<T>(bcol, k, in ? blen : 0);
+      an[g] = ld4<AV>(arow, k, in ? alen : 0);
+      bn[g] = ld4<BV>(bcol, k, in ? blen : 0);
     }
 #pragma unroll
-    for (int g = 0; g < G; ++g)
-#pragma unroll
-      for (int t = 0; t < 4; ++t) acc = mfma16(av[g], bv[g], acc, t);
+    for (int g = 0; g < G; ++g) acc = KT::mma(av[g], bv[g], acc);
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       av[g] = an[g];
@@ -137,31 +172,21 @@ __device__ __forceinline__ typename Vec4<T>::type part_tile(const T *a, int M, i
   return acc;
 }
 
-template <class T>
-struct Chunks;
-template <>
-struct Chunks<float> {
-  static constexpr int G = 6;
-};
-template <>
-struct Chunks<double> {
-  static constexpr int G = 4;
-};
-
 // The product tile of a workgroup: W wavefronts, each its share of the K
 // chunks; their partial tiles go to LDS and thread t < 256 receives the FP64
 // sum over the wavefronts, in wavefront order, of row t / 16 and column
 // t % 16 of the tile.  Every thread of the workgroup must call it.
-template <class T, int W>
-__device__ __forceinline__ double product_tile(const T *a, int M, int K, int m0, const T *b,
-                                               int bcolumn, bool bok, T *red) {
+template <class KT, int W>
+__device__ __forceinline__ double product_tile(const typename KT::A *a, int M, int K, int m0,
+                                               const typename KT::B *b, int bcolumn, bool bok,
+                                               typename KT::Acc *red) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, q = lane >> 4;
   const int nchk = (K + 15) / 16;
   const int c_lo = (int)((long)nchk * wave / W), c_hi = (int)((long)nchk * (wave + 1) / W);
-  const T *bcol = b + (long)(bok ? bcolumn : 0) * K;
-  const auto acc = part_tile<T, Chunks<T>::G>(a, M, K, m0, bcol, bok, c_lo, c_hi);
+  const typename KT::B *bcol = b + (long)(bok ? bcolumn : 0) * K;
+  const auto acc = part_tile<KT>(a, M, K, m0, bcol, bok, c_lo, c_hi);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) red[wave * 256 + acc_row<T>(q, i) * 16 + (lane & 15)] = acc[i];
+  for (int i = 0; i < 4; ++i) red[wave * 256 + KT::row(q, i) * 16 + (lane & 15)] = acc[i];
   __syncthreads();
   double s = 0.0;
   if (threadIdx.x < 256)
@@ -185,9 +210,9 @@ __device__ __forceinline__ double rhs_at(const float *f, const double *f64, long
 }
 
 // ------------------------------------------------------------------ forward
-template <class T, int W>
+template <class KT, int W>
 __global__ __launch_bounds__(64 * W) void fused_fwd_kernel(FusedFwdArgs p) {
-  __shared__ T red[W * 256];
+  __shared__ typename KT::Acc red[W * 256];
   span_begin(p.tspan);
   int tile, mb;
   if (tile_of(blockIdx.x, p.ntiles, p.nmb, tile, mb)) {
@@ -233,10 +258,11 @@ __global__ __launch_bounds__(64 * W) void fused_fwd_kernel(FusedFwdArgs p) {
       fe = x[f2(i0, xce, js + 1)];
       xnorth = x[f2(i0, xc, ieb + 1)];
     }
-    const double d = product_tile<T, W>((const T *)p.a, p.M, i2, 16 * mb, (const T *)p.b, pc - 1,
-                                        !halo, red);
+    const double d = product_tile<KT, W>((const typename KT::A *)p.a, p.M, i2, 16 * mb,
+                                         (const typename KT::B *)p.b, pc - 1, !halo, red);
     if (act) {
       if (!halo) vs = vs - d;  // the south row as block b's correction leaves it
+      if (p.d_out && out) p.d_out[(long)m * i2 + pc - 1] = d;
       const long co = pc - 1;
       const long fo = (long)m * p.f_stride + co;
       auto cf = [&](const float *arr, int j) -> double { return (double)arr[co + (long)(j - 1) * i2]; };
@@ -270,9 +296,9 @@ __global__ __launch_bounds__(64 * W) void fused_fwd_kernel(FusedFwdArgs p) {
 // epilogue), then, when W_b is given, those of W_b (the coupling row, only
 // stored); then the homogeneous march of block b+1, one wavefront per
 // (tile, RHS).
-template <class T, int W, int K>
+template <class KT, int W, int K>
 __global__ __launch_bounds__(64 * W) void fused_bwd_kernel(FusedBwdArgs p) {
-  __shared__ T red[W * 256];
+  __shared__ typename KT::Acc red[W * 256];
   span_begin(p.tspan);
   const int wg = blockIdx.x;
   const int nprod = p.bw ? 2 * p.nprod_half : p.nprod_half;
@@ -307,8 +333,8 @@ __global__ __launch_bounds__(64 * W) void fused_bwd_kernel(FusedBwdArgs p) {
         xe = x[f2(i0, c + 2, js)];
       }
     }
-    const double v = product_tile<T, W>((const T *)p.a, p.M, i2, 16 * mb,
-                                        (const T *)(cpl ? p.bw : p.b), c - 1, ok, red);
+    const double v = product_tile<KT, W>((const typename KT::A *)p.a, p.M, i2, 16 * mb,
+                                         (const typename KT::B *)(cpl ? p.bw : p.b), c - 1, ok, red);
     if (act) {
       const long ro = (long)m * i2 + c - 1;
       if (cpl) {
@@ -318,6 +344,7 @@ __global__ __launch_bounds__(64 * W) void fused_bwd_kernel(FusedBwdArgs p) {
         const double north = xfree + v;
         x[f2(i0, c + 1, js + 1)] = north;
         if (resid) {
+          if (p.dsouth) xsouth = xsouth - p.dsouth[ro];
           const long co = (long)(c - 1) + (long)(js - 2) * i2;  // coefficient row js-1
           const double e = rhs_at(p.f, p.f64, (long)m * p.f_stride + co) - (double)p.ax[co] * xw -
                            (double)p.ay[co] * xsouth - (double)p.bb[co] * xcen -
@@ -370,42 +397,56 @@ int fused_bwd_prepare(FusedBwdArgs &a, int *K) {
   return 0;
 }
 
-hipError_t launch_fused_fwd(const FusedFwdArgs &a0, bool dbl, hipStream_t st, hipEvent_t ev0,
+template <class KT>
+static void fwd_go(const FusedFwdArgs &a, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+  const dim3 grid((unsigned)fused_fwd_workgroups(a)), block(64 * FUSED_WAVES);
+  if (ev0) hipExtLaunchKernelGGL((fused_fwd_kernel<KT, FUSED_WAVES>), grid, block, 0, st, ev0, ev1, 0, a);
+  else hipLaunchKernelGGL((fused_fwd_kernel<KT, FUSED_WAVES>), grid, block, 0, st, a);
+}
+
+hipError_t launch_fused_fwd(const FusedFwdArgs &a0, int arith, hipStream_t st, hipEvent_t ev0,
                             hipEvent_t ev1) {
   FusedFwdArgs a = a0;
   if (fused_fwd_prepare(a)) return hipErrorInvalidValue;
-  const dim3 grid((unsigned)fused_fwd_workgroups(a)), block(64 * FUSED_WAVES);
-  if (dbl) {
-    if (ev0) hipExtLaunchKernelGGL((fused_fwd_kernel<double, FUSED_WAVES>), grid, block, 0, st, ev0, ev1, 0, a);
-    else hipLaunchKernelGGL((fused_fwd_kernel<double, FUSED_WAVES>), grid, block, 0, st, a);
-  } else {
-    if (ev0) hipExtLaunchKernelGGL((fused_fwd_kernel<float, FUSED_WAVES>), grid, block, 0, st, ev0, ev1, 0, a);
-    else hipLaunchKernelGGL((fused_fwd_kernel<float, FUSED_WAVES>), grid, block, 0, st, a);
-  }
+  if (arith == FUSED_F64) fwd_go<KF64>(a, st, ev0, ev1);
+  else if (arith == FUSED_B16) fwd_go<KB16>(a, st, ev0, ev1);
+  else fwd_go<KF32>(a, st, ev0, ev1);
   return hipGetLastError();
 }
 
-template <class T, int K>
-static void bwd_launch_k(const FusedBwdArgs &a, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+template <class KT, int K>
+static void bwd_go(const FusedBwdArgs &a, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
   const dim3 grid((unsigned)fused_bwd_workgroups(a)), block(64 * FUSED_WAVES);
-  if (ev0) hipExtLaunchKernelGGL((fused_bwd_kernel<T, FUSED_WAVES, K>), grid, block, 0, st, ev0, ev1, 0, a);
-  else hipLaunchKernelGGL((fused_bwd_kernel<T, FUSED_WAVES, K>), grid, block, 0, st, a);
+  if (ev0) hipExtLaunchKernelGGL((fused_bwd_kernel<KT, FUSED_WAVES, K>), grid, block, 0, st, ev0, ev1, 0, a);
+  else hipLaunchKernelGGL((fused_bwd_kernel<KT, FUSED_WAVES, K>), grid, block, 0, st, a);
 }
 
-template <class T>
-static void bwd_launch(const FusedBwdArgs &a, int K, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
-  if (K == 1) bwd_launch_k<T, 1>(a, st, ev0, ev1);
-  else if (K == 2) bwd_launch_k<T, 2>(a, st, ev0, ev1);
-  else bwd_launch_k<T, 4>(a, st, ev0, ev1);
+template <class KT>
+static void bwd_lanes(const FusedBwdArgs &a, int K, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+  if (K == 1) bwd_go<KT, 1>(a, st, ev0, ev1);
+  else if (K == 2) bwd_go<KT, 2>(a, st, ev0, ev1);
+  else bwd_go<KT, 4>(a, st, ev0, ev1);
 }
 
-hipError_t launch_fused_bwd(const FusedBwdArgs &a0, bool dbl, hipStream_t st, hipEvent_t ev0,
+hipError_t launch_fused_bwd(const FusedBwdArgs &a0, int arith, hipStream_t st, hipEvent_t ev0,
                             hipEvent_t ev1) {
   FusedBwdArgs a = a0;
   int K = 1;
   if (fused_bwd_prepare(a, &K)) return hipErrorInvalidValue;
-  if (dbl) bwd_launch<double>(a, K, st, ev0, ev1);
-  else bwd_launch<float>(a, K, st, ev0, ev1);
+  if (arith == FUSED_F64) bwd_lanes<KF64>(a, K, st, ev0, ev1);
+  else if (arith == FUSED_B16) bwd_lanes<KB16>(a, K, st, ev0, ev1);
+  else bwd_lanes<KF32>(a, K, st, ev0, ev1);
+  return hipGetLastError();
+}
+
+// FP64 column-major matrices to BF16 bits (round to nearest even via FP32).
+__global__ void cvt_bf16_kernel(const double *src, unsigned short *dst, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = (unsigned short)bf16_rne((float)src[t]);
+}
+
+hipError_t launch_cvt_bf16(const double *src, unsigned short *dst, long n, hipStream_t st) {
+  hipLaunchKernelGGL(cvt_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, n);
   return hipGetLastError();
 }
 

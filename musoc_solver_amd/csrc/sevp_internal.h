@@ -225,6 +225,7 @@ struct FusedFwdArgs {
   long x_stride;
   double *e_out;       // e_(b+1) (M, i2), FP64 (may be null)
   float *e32_out;      // e_(b+1) rounded to FP32 (may be null)
+  double *d_out;       // d itself (M, i2), kept when block b+1 is one row high (may be null)
   int ntiles, nmb;     // set by the launcher
   unsigned long long *tspan;  // profiling: kernel span slot (null: off)
 };
@@ -247,6 +248,10 @@ struct FusedBwdArgs {
   long x_stride;
   double *e_out;       // e_(b-1) (M, i2)
   float *e32_out;
+  // block b-1 one row high: its closure row's south neighbour is row
+  // IE(b-2), which REP1 still holds corrected by d_(b-2) (restored from DUM0
+  // only at block b-2's step); d_(b-2) (M, i2), else null
+  const double *dsouth;
   double *hfree_out, *hcpl_out;  // c and cpl of block b (M, i2)
   MarchArgs hom;       // MODE_HOM march of block b+1 (hfree / hcpl of the previous launch)
   int has_hom;
@@ -260,11 +265,16 @@ int fused_fwd_prepare(FusedFwdArgs &a);
 int fused_bwd_prepare(FusedBwdArgs &a, int *K);
 long fused_fwd_workgroups(const FusedFwdArgs &a);  // after the prepare call
 long fused_bwd_workgroups(const FusedBwdArgs &a);
+// Product arithmetic of the fused kernels: FP32 operands (FP32 MFMA), FP64
+// (FP64 MFMA), or FP32 error rows against BF16 matrices, the rows rounded to
+// BF16 in registers (BF16 MFMA, FP32 accumulation).
+enum { FUSED_F32 = 0, FUSED_F64 = 1, FUSED_B16 = 2 };
 // ev0 / ev1 (optional): events with the kernel's own start and end times.
-hipError_t launch_fused_fwd(const FusedFwdArgs &a, bool dbl, hipStream_t st,
+hipError_t launch_fused_fwd(const FusedFwdArgs &a, int arith, hipStream_t st,
                             hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
-hipError_t launch_fused_bwd(const FusedBwdArgs &a, bool dbl, hipStream_t st,
+hipError_t launch_fused_bwd(const FusedBwdArgs &a, int arith, hipStream_t st,
                             hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+hipError_t launch_cvt_bf16(const double *src, unsigned short *dst, long n, hipStream_t st);
 
 // Step 14 of FS_GLOBAL (the barotropic correction around the solve); arrays
 // have the reference shapes, Fortran order (see include/sevp.h).

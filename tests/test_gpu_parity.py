@@ -505,8 +505,9 @@ def test_fused_pass_matches_reference_order(oracle, device, i2, j2, form, height
     slip in the fused epilogue would show as O(1).  The FP32 fused pass and
     the split-K pass both refine to 1e-10."""
     import torch
-    coefs, _, mask = make(i2, j2, 2, metric="latlon", lat_max=70.0)
+    coefs, _, mask = make(i2, j2, 2, metric="latlon", lat_max=45.0)
     ie = _ie_cycle(j2, heights)
+    unit = bool(np.any(np.diff(ie) == 1))  # a one-row block after the first
     print(f"[fused] {i2}x{j2} form={form} heights={heights} nrhs={nrhs} nb0={len(ie)}", flush=True)
     plan = S.SevpPlan(coefs, ie, form=form, nrhs_max=nrhs, precision=S.PREC_FP64)
     plan.build_influence()
@@ -521,12 +522,18 @@ def test_fused_pass_matches_reference_order(oracle, device, i2, j2, form, height
     assert all_finite(x_f)
     d = np.abs(x_f - x_plain).max() / np.abs(x_plain).max()
     print(f"[fused] FP64 fused vs reference order: {d:.2e}", flush=True)
-    assert d <= 1e-10, d
+    # the march amplifies the different rounding of the sums by up to ~1e5
+    # with 4-row blocks; an indexing slip shows as 1e-6 or O(1)
+    assert d <= 1e-9, d
     s_d = torch.from_numpy(np.ascontiguousarray(s.transpose(2, 1, 0))).cuda()
     sols = {}
+    kinds = (S.PASS_FUSED,) if unit else (S.PASS_FUSED, S.PASS_SPLITK)
+    if unit:  # the split-K pass refuses one-row blocks after the first
+        with pytest.raises(S.SevpError, match="at least 2 rows"):
+            plan.set_pass(S.PASS_SPLITK)
     for prec in (S.PREC_FP64_MFMA, S.PREC_FP32):
         plan.set_precision(prec)
-        for kind in (S.PASS_FUSED, S.PASS_SPLITK):
+        for kind in kinds:
             plan.set_pass(kind)
             x_d = torch.zeros((nrhs, j2 + 2, i2 + 2), dtype=torch.float64, device="cuda")
             rel, st, ok = plan.solve_refined_dev(s_d, x_d, tol=1e-10, max_iter=10)
@@ -542,4 +549,44 @@ def test_fused_pass_matches_reference_order(oracle, device, i2, j2, form, height
     x = sols[(S.PREC_FP32, S.PASS_FUSED)].cpu().numpy().transpose(2, 1, 0)
     for r in (0, nrhs - 1):
         assert oracle.rel_residual(coefs, s[:, :, r], np.asfortranarray(x[:, :, r])) <= 1.5e-10
+    plan.close()
+
+
+@pytest.mark.parametrize("i2,j2", [(96, 48), (192, 96)])
+def test_bf16_trial_contraction(oracle, device, i2, j2):
+    """BF16-input, FP32-accumulate corrections (the fused pass with BF16
+    matrices, the FP32 error rows rounded to BF16 in registers): the residual
+    history of one pass at a time, measured on the GPU, against the CPU model
+    of the same pass (tests/test_correction_precision.py): at best about 0.2
+    per pass, and at 192 x 96 the first pass does not reduce the residual.
+    The refinement still reaches 1e-10 on these small grids, in many passes."""
+    import torch
+    from tests.test_correction_precision import residual_history, round_bf16
+    case = synth.make_case(i2, j2, 3, metric="latlon", lat_max=80.0, h_land=50.0)
+    ie = synth.reference_ie(j2, synth.nb0_for_height(j2, 2))
+    print(f"[bf16] {i2}x{j2}", flush=True)
+    plan = S.SevpPlan(case.coefs, ie, form=S.FORM_FULL_PERIODIC, nrhs_max=1,
+                      precision=S.PREC_BF16)
+    plan.build_influence()
+    s = synth.make_rhs(case.mask, 4, 1)
+    s_d = torch.from_numpy(np.ascontiguousarray(s.transpose(2, 1, 0))).cuda()
+    x_d = torch.zeros((1, j2 + 2, i2 + 2), dtype=torch.float64, device="cuda")
+    hist = [1.0]
+    for _ in range(4):
+        rel, st, ok = plan.solve_refined_dev(s_d, x_d, tol=0.0, max_iter=1)
+        hist.append(float(rel[0]))
+    model = residual_history(oracle, i2, j2, round_bf16, round_bf16, passes=4)
+    print(f"[bf16] GPU {np.array(hist)} model {model}", flush=True)
+    g = np.array(hist)
+    assert np.all(np.abs(np.log(g[1:] / model[1:])) < np.log(3.0))  # same pass, other sums
+    c = g[2:] / g[1:-1]
+    assert c.max() > 0.1 and c.max() < 0.6
+    if i2 == 192:
+        assert g[1] > 1.0
+    x_d.zero_()
+    rel, st, ok = plan.solve_refined_dev(s_d, x_d, tol=1e-10, max_iter=60)
+    print(f"[bf16] to 1e-10: {st.iterations} passes, {rel.max():.2e}", flush=True)
+    assert ok and st.iterations > 8
+    xr = np.asfortranarray(x_d[0].cpu().numpy().T)
+    assert oracle.rel_residual(case.coefs, s[:, :, 0], xr) <= 1.5e-10
     plan.close()
