@@ -246,6 +246,30 @@ int sevp_set_rhs_groups(sevp_plan plan, int groups);
 int sevp_set_pass(sevp_plan plan, int kind);
 int sevp_get_pass(sevp_plan plan, int *kind);
 
+/* Performance parameters of the MFMA corrections, chosen by sevp_autotune
+ * or set directly:
+ *   pass_kind   SEVP_PASS_FUSED or SEVP_PASS_SPLITK;
+ *   waves       wavefronts per workgroup of the fused kernels (4 or 8): the
+ *               share of K each wavefront sums;
+ *   rhs_groups  RHS groups whose passes run concurrently (1..16).
+ * The results of each choice are deterministic; the pass kernels and the
+ * waves change the products' summation order (rounding-level differences).
+ * sevp_autotune times refined solves of the caller's problem (s_dev, x_dev,
+ * nrhs, tol, max_iter as for sevp_solve_refined_dev; x_dev is overwritten)
+ * for every applicable choice, keeps the fastest and returns it with its
+ * time per solve call.  Block height is a property of the plan (its IE):
+ * musoc_solver_amd.autotune tunes it across plans.  No counterpart in the
+ * reference. */
+typedef struct {
+  int pass_kind;
+  int waves;
+  int rhs_groups;
+} sevp_tuning;
+int sevp_get_tuning(sevp_plan plan, sevp_tuning *tuning);
+int sevp_set_tuning(sevp_plan plan, const sevp_tuning *tuning);
+int sevp_autotune(sevp_plan plan, const float *s_dev, double *x_dev, int nrhs, double tol,
+                  int max_iter, sevp_tuning *best, double *best_ms);
+
 /* Where the influence matrices live (set before sevp_build_influence or
  * sevp_load_influence; changing it drops the matrices built so far).
  *   SEVP_STORE_FP64      RINV / RINV1 in FP64 on the device (the default),

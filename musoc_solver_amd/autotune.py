@@ -82,3 +82,76 @@ def uniform_ie(j2: int, h: int) -> np.ndarray:
     reference's rule for NB0 = ANINT(J2/h))."""
     from .synth import nb0_for_height, reference_ie
     return reference_ie(j2, nb0_for_height(j2, h))
+
+
+# ------------------------------------------------------- timing on the device
+# The performance parameters of the MFMA corrections are chosen by timing
+# refined solves on the GPU: the pass kernels, the wavefronts per fused
+# workgroup and the RHS groups by sevp_autotune inside one plan, the block
+# height (the plan's IE) across plans built for each candidate.  Choices are
+# cached under the operator hash (sevp_operator_hash: grid, IE, form and the
+# coefficient bits) with the RHS count and precision, so a plan of the same
+# operator reuses them without timing.
+
+def _cache_key(plan, nrhs):
+    from . import sevp as S
+    h = S.operator_hash(plan.coefs, plan.ie, plan.form, plan.nbir)
+    return f"{h:016x}:nrhs={nrhs}:prec={plan.precision}"
+
+
+def tune(plan, s, x, tol=1e-10, max_iter=20, cache_dir=None):
+    """Apply the fastest (pass kernels, waves, RHS groups) for refined
+    solves of s (torch device tensors; x is overwritten while timing).  With
+    ``cache_dir`` the choice is read from / stored in
+    ``<cache_dir>/sevp_tuning.json``.  Returns the choice as a dict with the
+    time per solve call (ms) and whether it came from the cache."""
+    import json
+    import os
+    nrhs = s.shape[0]
+    path = os.path.join(str(cache_dir), "sevp_tuning.json") if cache_dir else None
+    key = _cache_key(plan, nrhs) if path else None
+    table = {}
+    if path and os.path.exists(path):
+        with open(path) as f:
+            table = json.load(f)
+        if key in table:
+            c = table[key]
+            from . import sevp as S
+            plan.set_tuning(S.PASS_FUSED if c["pass_kernels"] == "fused" else S.PASS_SPLITK,
+                            c["waves"], c["rhs_groups"])
+            return dict(c, cached=True)
+    t, ms = plan.autotune(s, x, tol=tol, max_iter=max_iter)
+    out = dict(t.as_dict(), ms=ms)
+    if path:
+        os.makedirs(str(cache_dir), exist_ok=True)
+        table[key] = out
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(table, f, indent=1)
+        os.replace(tmp, path)
+    return dict(out, cached=False)
+
+
+def tune_block_height(coefs, heights, s, make_plan, tol=1e-10, max_iter=20):
+    """Block height by timing: for each h, a plan of uniform h-row blocks
+    from ``make_plan(ie)`` (its influence matrices built), tuned with
+    ``tune`` on the right-hand sides s; returns (best h, {h: ms per solve
+    call}).  Plans whose refinement does not converge are skipped."""
+    import torch
+    j2 = coefs[0].shape[1]
+    times = {}
+    for h in heights:
+        plan = make_plan(uniform_ie(j2, h))
+        try:
+            x = torch.zeros((s.shape[0], j2 + 2, coefs[0].shape[0] + 2), dtype=torch.float64,
+                            device=s.device)
+            r = tune(plan, s, x, tol=tol, max_iter=max_iter)
+            x.zero_()
+            rel, st, ok = plan.solve_refined_dev(s, x, tol=tol, max_iter=max_iter)
+            if ok:
+                times[h] = r["ms"]
+        finally:
+            plan.close()
+    if not times:
+        raise RuntimeError("no block height converged")
+    return min(times, key=times.get), times

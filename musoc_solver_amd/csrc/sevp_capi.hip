@@ -202,7 +202,7 @@ struct KProf {
 // One captured refinement pass (zero the correction, one full-width SEVP pass
 // of the FP64 residual) for a batch size and precision.
 struct PassGraph {
-  int nrhs = 0, prec = 0;
+  int nrhs = 0, prec = 0, kind = 0, waves = 0;
   int r0 = 0, count = 0;  // the RHS group this pass covers
   bool prof = false;
   hipGraphExec_t exec = nullptr;
@@ -215,7 +215,7 @@ struct PassGraph {
 // after it is set are skipped, so the result is the loop's).  The caller's s
 // and x are baked in.
 struct RefineGraph {
-  int nrhs = 0, prec = 0, kind = 0, groups = 0, passes = 0;
+  int nrhs = 0, prec = 0, kind = 0, groups = 0, passes = 0, waves = 0;
   bool prof = false;
   const float *s = nullptr;
   double *x = nullptr;
@@ -278,6 +278,7 @@ struct sevp_plan_s {
   // FP32 copies of RINV, RINV1 and W, or (FP64 MFMA) RINV and RINV1
   // themselves and W in FP64
   int pass_kind = SEVP_PASS_SPLITK;
+  int fused_waves = FUSED_WAVES;  // wavefronts per fused workgroup (4 or 8)
   DBuf<float> rinv32c, rinv1_32c, w32c;
   DBuf<double> w64c;
   DBuf<unsigned short> rinv16c, rinv1_16c, w16c;  // BF16 (fused pass only)
@@ -389,6 +390,7 @@ struct FullCtx {
   // two-parity scratch rows (each parity half holds fhalf elements)
   bool fused;
   int farith;  // FUSED_F32 / FUSED_F64 / FUSED_B16
+  int fwaves;  // wavefronts per fused workgroup
   const float *rinv32c, *rinv1_32c, *w32c;
   const double *w64c;
   const unsigned short *rinv16c, *rinv1_16c, *w16c;
@@ -644,6 +646,7 @@ static int run_fused_pass(const FullCtx &c, const float *f32, const double *f64,
     a.e_out = dbl ? E[(b + 1) & 1] : nullptr;
     a.e32_out = dbl ? nullptr : E32[(b + 1) & 1];
     if (a.steps == 0) a.d_out = c.fd + (long)(b - 1) * c.fhalf;  // block b+1 is one row high
+    a.waves = c.fwaves;
     FusedFwdArgs sz = a;
     if (fused_fwd_prepare(sz)) return fail(SEVP_E_ARG, "block too tall for the fused pass");
     CHK(timed([&](hipEvent_t e0, hipEvent_t e1) { return launch_fused_fwd(a, c.farith, st, e0, e1); },
@@ -666,6 +669,7 @@ static int run_fused_pass(const FullCtx &c, const float *f32, const double *f64,
       a.dsouth = c.fd + (long)(b - 3) * c.fhalf;
     a.hfree_out = HF[b & 1];
     a.hcpl_out = HC[b & 1];
+    a.waves = c.fwaves;
     if (b != c.nb0) {  // the homogeneous march of block b+1, from the previous launch's rows
       MarchArgs m{};
       m.ax = c.ax; m.ay = c.ay; m.bb = c.bb; m.cx = c.cx; m.cy = c.cy;
@@ -809,21 +813,26 @@ __device__ __forceinline__ void resid_norm_body(const float *al, const float *ab
   __shared__ double red[2][256];
   const int t = threadIdx.x;
   const long n = (long)i2 * j2;
-  const long per = (n + nchunk - 1) / nchunk;
-  const long lo = ch * per, hi = min(n, lo + per);
+  // chunk ch takes whole rows [j_lo, j_hi) (1-based j = row + 1); each
+  // thread strides over the columns of each row, so no index division
+  const int j_lo = (int)((long)j2 * ch / nchunk), j_hi = (int)((long)j2 * (ch + 1) / nchunk);
   const double *xr = x + (long)r * x_stride;
   const int i0 = i2 + 2;
   double a0 = 0.0, a1 = 0.0;
-  for (long q = lo + t; q < hi; q += 256) {
-    const int i = (int)(q % i2) + 1, j = (int)(q / i2) + 1;
-    const double ax = (double)ab[q] * xr[f2(i0, i + 1, j)] + (double)al[q] * xr[f2(i0, i, j + 1)] +
-                      (double)ac[q] * xr[f2(i0, i + 1, j + 1)] + (double)ar[q] * xr[f2(i0, i + 2, j + 1)] +
-                      (double)at[q] * xr[f2(i0, i + 1, j + 2)];
-    const double sv = s64 ? s64[(long)r * s_stride + q] : (double)s[(long)r * s_stride + q];
-    const double rv = sv - ax;
-    if (rout) rout[(long)r * n + q] = rv;
-    a0 += rv * rv;
-    a1 += sv * sv;
+  for (int jj = j_lo; jj < j_hi; ++jj) {
+    const int j = jj + 1;
+    const double *xs = xr + f2(i0, 1, j), *xc = xr + f2(i0, 1, j + 1), *xn = xr + f2(i0, 1, j + 2);
+    for (int ii = t; ii < i2; ii += 256) {
+      const long q = (long)jj * i2 + ii;
+      const double ax = (double)ab[q] * xs[ii + 1] + (double)al[q] * xc[ii] +
+                        (double)ac[q] * xc[ii + 1] + (double)ar[q] * xc[ii + 2] +
+                        (double)at[q] * xn[ii + 1];
+      const double sv = s64 ? s64[(long)r * s_stride + q] : (double)s[(long)r * s_stride + q];
+      const double rv = sv - ax;
+      if (rout) rout[(long)r * n + q] = rv;
+      a0 += rv * rv;
+      a1 += sv * sv;
+    }
   }
   red[0][t] = a0;
   red[1][t] = a1;
@@ -1420,6 +1429,7 @@ static FullCtx full_ctx(sevp_plan p) {
   const bool bf16 = p->d.precision == SEVP_PREC_BF16;
   c.fused = (mixed || mfma64 || bf16) && p->pass_kind == SEVP_PASS_FUSED;
   c.farith = mfma64 ? FUSED_F64 : (bf16 ? FUSED_B16 : FUSED_F32);
+  c.fwaves = p->fused_waves;
   c.rinv32c = p->rinv32c.p; c.rinv1_32c = p->rinv1_32c.p; c.w32c = p->w32c.p;
   c.w64c = p->w64c.p;
   c.rinv16c = p->rinv16c.p; c.rinv1_16c = p->rinv1_16c.p; c.w16c = p->w16c.p;
@@ -1558,6 +1568,75 @@ int sevp_set_pass(sevp_plan p, int kind) {
 int sevp_get_pass(sevp_plan p, int *kind) {
   if (!p || !kind) return fail(SEVP_E_ARG, "null argument");
   *kind = p->pass_kind;
+  return SEVP_OK;
+}
+
+int sevp_get_tuning(sevp_plan p, sevp_tuning *t) {
+  if (!p || !t) return fail(SEVP_E_ARG, "null argument");
+  t->pass_kind = p->pass_kind;
+  t->waves = p->fused_waves;
+  t->rhs_groups = p->rhs_groups;
+  return SEVP_OK;
+}
+
+int sevp_set_tuning(sevp_plan p, const sevp_tuning *t) {
+  if (!p || !t) return fail(SEVP_E_ARG, "null argument");
+  if (t->waves != 4 && t->waves != 8) return fail(SEVP_E_ARG, "waves per workgroup must be 4 or 8");
+  if (t->rhs_groups < 1 || t->rhs_groups > 16) return fail(SEVP_E_ARG, "RHS groups must be 1..16");
+  CHK(sevp_set_pass(p, t->pass_kind));
+  HIPC(hipStreamSynchronize(p->stream));
+  if (p->fused_waves != t->waves) drop_graphs(p);
+  p->fused_waves = t->waves;
+  p->rhs_groups = t->rhs_groups;
+  return SEVP_OK;
+}
+
+// Time refined solves of the caller's problem over the pass kernels, the
+// waves per fused workgroup and the RHS groups, and keep the fastest.
+int sevp_autotune(sevp_plan p, const float *s, double *x, int nrhs, double tol, int max_iter,
+                  sevp_tuning *best, double *best_ms) {
+  if (!p || !s || !x) return fail(SEVP_E_ARG, "null argument");
+  if (p->d.form == SEVP_FORM_BIR || p->d.precision == SEVP_PREC_FP64)
+    return fail(SEVP_E_ARG, "the autotuner chooses among MFMA pass kernels: full forms, MFMA precision");
+  if (nrhs < 1 || nrhs > p->d.nrhs_max) return fail(SEVP_E_ARG, "nrhs out of range");
+  if (!p->have_inv) return fail(SEVP_E_STATE, "influence matrices not built or loaded");
+  const long nx = (long)(p->d.i2 + 2) * (p->d.j2 + 2);
+  std::vector<sevp_tuning> cand;
+  for (int kind : {SEVP_PASS_FUSED, SEVP_PASS_SPLITK}) {
+    if (kind != p->pass_kind && p->storage == SEVP_STORE_PRODUCTS) continue;
+    if (check_pass(p, kind, p->d.precision) != SEVP_OK) continue;
+    for (int waves : {8, 4}) {
+      if (kind == SEVP_PASS_SPLITK && waves != 8) continue;  // not a split-K parameter
+      for (int g : {1, 2, 4})
+        if (g <= nrhs) cand.push_back(sevp_tuning{kind, waves, g});
+    }
+  }
+  if (cand.empty()) return fail(SEVP_E_ARG, "no pass kernels apply to this plan");
+  sevp_tuning keep{};
+  CHK(sevp_get_tuning(p, &keep));
+  double bms = 1e300;
+  sevp_tuning b = keep;
+  for (const auto &t : cand) {
+    CHK(sevp_set_tuning(p, &t));
+    double ms = 1e300;
+    for (int rep = 0; rep < 3; ++rep) {  // the first run captures the graphs
+      CHK(zero_async(x, sizeof(double) * nx * nrhs, p->stream));
+      HIPC(hipStreamSynchronize(p->stream));
+      const auto t0 = std::chrono::steady_clock::now();
+      const int rc = sevp_solve_refined_dev(p, s, x, nrhs, tol, max_iter, nullptr, nullptr);
+      if (rc != SEVP_OK && rc != SEVP_E_NOTCONV) return rc;
+      const double dt =
+          1e3 * std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if (rep > 0) ms = std::min(ms, dt);
+    }
+    if (ms < bms) {
+      bms = ms;
+      b = t;
+    }
+  }
+  CHK(sevp_set_tuning(p, &b));
+  if (best) *best = b;
+  if (best_ms) *best_ms = bms;
   return SEVP_OK;
 }
 
@@ -1762,7 +1841,7 @@ static int get_pass_graph(sevp_plan p, const FullCtx &c0, int nrhs, int r0, int 
   const bool prof = p->prof.kclass >= 0;
   for (auto &g : p->graphs)
     if (g.nrhs == nrhs && g.r0 == r0 && g.count == count && g.prec == p->d.precision &&
-        g.prof == prof) {
+        g.kind == p->pass_kind && g.waves == p->fused_waves && g.prof == prof) {
       *out = &g;
       return SEVP_OK;
     }
@@ -1773,6 +1852,8 @@ static int get_pass_graph(sevp_plan p, const FullCtx &c0, int nrhs, int r0, int 
   g.r0 = r0;
   g.count = count;
   g.prec = p->d.precision;
+  g.kind = p->pass_kind;
+  g.waves = p->fused_waves;
   g.prof = prof;
   hipStream_t cs = p->cap_stream;
   HIPC(hipStreamSynchronize(p->stream));
@@ -1939,7 +2020,8 @@ static int get_refine_graph(sevp_plan p, const FullCtx &c, double *x, int nrhs, 
   const float *s = p->s_stage.p;
   for (auto &g : p->rgraphs)
     if (g.nrhs == nrhs && g.prec == p->d.precision && g.kind == p->pass_kind &&
-        g.groups == ngroups && g.passes == passes && g.prof == prof && g.x == x) {
+        g.groups == ngroups && g.passes == passes && g.prof == prof && g.x == x &&
+        g.waves == p->fused_waves) {
       *out = &g;
       return SEVP_OK;
     }
@@ -1957,7 +2039,7 @@ static int get_refine_graph(sevp_plan p, const FullCtx &c, double *x, int nrhs, 
   const long nx = (long)(i2 + 2) * (j2 + 2);
   RefineGraph g;
   g.nrhs = nrhs; g.prec = p->d.precision; g.kind = p->pass_kind; g.groups = ngroups;
-  g.passes = passes; g.prof = prof; g.s = s; g.x = x;
+  g.passes = passes; g.prof = prof; g.s = s; g.x = x; g.waves = p->fused_waves;
   hipStream_t cs = p->cap_stream;
   HIPC(hipStreamSynchronize(p->stream));
   HIPC(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));

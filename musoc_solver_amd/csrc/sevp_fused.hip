@@ -370,7 +370,8 @@ long fused_fwd_workgroups(const FusedFwdArgs &a) { return placed_workgroups(a.nt
 
 long fused_bwd_workgroups(const FusedBwdArgs &a) {
   const long hom_waves = (long)a.hom_tiles * (a.has_hom ? a.hom.nrhs : 0);
-  return (a.bw ? 2L : 1L) * a.nprod_half + (hom_waves + FUSED_WAVES - 1) / FUSED_WAVES;
+  const int w = a.waves > 0 ? a.waves : FUSED_WAVES;
+  return (a.bw ? 2L : 1L) * a.nprod_half + (hom_waves + w - 1) / w;
 }
 
 // Fill the launcher-set fields (tiles, row blocks) of the arguments.
@@ -397,35 +398,48 @@ int fused_bwd_prepare(FusedBwdArgs &a, int *K) {
   return 0;
 }
 
-template <class KT>
+template <class KT, int W>
 static void fwd_go(const FusedFwdArgs &a, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
-  const dim3 grid((unsigned)fused_fwd_workgroups(a)), block(64 * FUSED_WAVES);
-  if (ev0) hipExtLaunchKernelGGL((fused_fwd_kernel<KT, FUSED_WAVES>), grid, block, 0, st, ev0, ev1, 0, a);
-  else hipLaunchKernelGGL((fused_fwd_kernel<KT, FUSED_WAVES>), grid, block, 0, st, a);
+  const dim3 grid((unsigned)fused_fwd_workgroups(a)), block(64 * W);
+  if (ev0) hipExtLaunchKernelGGL((fused_fwd_kernel<KT, W>), grid, block, 0, st, ev0, ev1, 0, a);
+  else hipLaunchKernelGGL((fused_fwd_kernel<KT, W>), grid, block, 0, st, a);
+}
+
+template <class KT>
+static void fwd_waves(const FusedFwdArgs &a, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+  if (a.waves == 4) fwd_go<KT, 4>(a, st, ev0, ev1);
+  else fwd_go<KT, 8>(a, st, ev0, ev1);
 }
 
 hipError_t launch_fused_fwd(const FusedFwdArgs &a0, int arith, hipStream_t st, hipEvent_t ev0,
                             hipEvent_t ev1) {
   FusedFwdArgs a = a0;
   if (fused_fwd_prepare(a)) return hipErrorInvalidValue;
-  if (arith == FUSED_F64) fwd_go<KF64>(a, st, ev0, ev1);
-  else if (arith == FUSED_B16) fwd_go<KB16>(a, st, ev0, ev1);
-  else fwd_go<KF32>(a, st, ev0, ev1);
+  if (a.waves != 0 && a.waves != 4 && a.waves != 8) return hipErrorInvalidValue;
+  if (arith == FUSED_F64) fwd_waves<KF64>(a, st, ev0, ev1);
+  else if (arith == FUSED_B16) fwd_waves<KB16>(a, st, ev0, ev1);
+  else fwd_waves<KF32>(a, st, ev0, ev1);
   return hipGetLastError();
 }
 
-template <class KT, int K>
+template <class KT, int W, int K>
 static void bwd_go(const FusedBwdArgs &a, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
-  const dim3 grid((unsigned)fused_bwd_workgroups(a)), block(64 * FUSED_WAVES);
-  if (ev0) hipExtLaunchKernelGGL((fused_bwd_kernel<KT, FUSED_WAVES, K>), grid, block, 0, st, ev0, ev1, 0, a);
-  else hipLaunchKernelGGL((fused_bwd_kernel<KT, FUSED_WAVES, K>), grid, block, 0, st, a);
+  const dim3 grid((unsigned)fused_bwd_workgroups(a)), block(64 * W);
+  if (ev0) hipExtLaunchKernelGGL((fused_bwd_kernel<KT, W, K>), grid, block, 0, st, ev0, ev1, 0, a);
+  else hipLaunchKernelGGL((fused_bwd_kernel<KT, W, K>), grid, block, 0, st, a);
+}
+
+template <class KT, int W>
+static void bwd_lanes(const FusedBwdArgs &a, int K, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+  if (K == 1) bwd_go<KT, W, 1>(a, st, ev0, ev1);
+  else if (K == 2) bwd_go<KT, W, 2>(a, st, ev0, ev1);
+  else bwd_go<KT, W, 4>(a, st, ev0, ev1);
 }
 
 template <class KT>
-static void bwd_lanes(const FusedBwdArgs &a, int K, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
-  if (K == 1) bwd_go<KT, 1>(a, st, ev0, ev1);
-  else if (K == 2) bwd_go<KT, 2>(a, st, ev0, ev1);
-  else bwd_go<KT, 4>(a, st, ev0, ev1);
+static void bwd_waves(const FusedBwdArgs &a, int K, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+  if (a.waves == 4) bwd_lanes<KT, 4>(a, K, st, ev0, ev1);
+  else bwd_lanes<KT, 8>(a, K, st, ev0, ev1);
 }
 
 hipError_t launch_fused_bwd(const FusedBwdArgs &a0, int arith, hipStream_t st, hipEvent_t ev0,
@@ -433,9 +447,10 @@ hipError_t launch_fused_bwd(const FusedBwdArgs &a0, int arith, hipStream_t st, h
   FusedBwdArgs a = a0;
   int K = 1;
   if (fused_bwd_prepare(a, &K)) return hipErrorInvalidValue;
-  if (arith == FUSED_F64) bwd_lanes<KF64>(a, K, st, ev0, ev1);
-  else if (arith == FUSED_B16) bwd_lanes<KB16>(a, K, st, ev0, ev1);
-  else bwd_lanes<KF32>(a, K, st, ev0, ev1);
+  if (a.waves != 0 && a.waves != 4 && a.waves != 8) return hipErrorInvalidValue;
+  if (arith == FUSED_F64) bwd_waves<KF64>(a, K, st, ev0, ev1);
+  else if (arith == FUSED_B16) bwd_waves<KB16>(a, K, st, ev0, ev1);
+  else bwd_waves<KF32>(a, K, st, ev0, ev1);
   return hipGetLastError();
 }
 

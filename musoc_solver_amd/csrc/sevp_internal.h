@@ -207,7 +207,7 @@ long gemm_part_workgroups(const GemmPartArgs &a);  // grid size of launch_gemm_p
 // tiles in wavefront order) with the work that depends on it in the
 // epilogue.  T = float (FP32 MFMA, A the FP32 error rows, B FP32
 // column-major) or double (FP64 MFMA).
-constexpr int FUSED_WAVES = 8;
+constexpr int FUSED_WAVES = 8;  // the default; 4 is the other compiled choice
 
 // Forward step: d = e_b RINV1_b, then block b+1's march from the corrected
 // south row X(js) - d and its closure residual e_(b+1).
@@ -226,6 +226,7 @@ struct FusedFwdArgs {
   double *e_out;       // e_(b+1) (M, i2), FP64 (may be null)
   float *e32_out;      // e_(b+1) rounded to FP32 (may be null)
   double *d_out;       // d itself (M, i2), kept when block b+1 is one row high (may be null)
+  int waves;           // wavefronts per workgroup (4 or 8; 0: FUSED_WAVES)
   int ntiles, nmb;     // set by the launcher
   unsigned long long *tspan;  // profiling: kernel span slot (null: off)
 };
@@ -255,6 +256,7 @@ struct FusedBwdArgs {
   double *hfree_out, *hcpl_out;  // c and cpl of block b (M, i2)
   MarchArgs hom;       // MODE_HOM march of block b+1 (hfree / hcpl of the previous launch)
   int has_hom;
+  int waves;           // wavefronts per workgroup (4 or 8; 0: FUSED_WAVES)
   int ntiles, nmb, nprod_half, hom_tiles;  // set by the launcher
   unsigned long long *tspan;
 };

@@ -59,6 +59,14 @@ class _Desc(ctypes.Structure):
                 ("seam_fix", ctypes.c_int)]
 
 
+class SevpTuning(ctypes.Structure):
+    _fields_ = [("pass_kind", ctypes.c_int), ("waves", ctypes.c_int), ("rhs_groups", ctypes.c_int)]
+
+    def as_dict(self):
+        return {"pass_kernels": "fused" if self.pass_kind == PASS_FUSED else "splitk",
+                "waves": self.waves, "rhs_groups": self.rhs_groups}
+
+
 class SevpStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("max_rel_resid", ctypes.c_double),
                 ("seconds", ctypes.c_double)]
@@ -102,6 +110,11 @@ def lib():
         L.sevp_set_precision.argtypes = [VP, CI]
         L.sevp_set_storage.argtypes = [VP, CI]
         L.sevp_set_pass.argtypes = [VP, CI]
+        TP = ctypes.POINTER(SevpTuning)
+        L.sevp_get_tuning.argtypes = [VP, TP]
+        L.sevp_set_tuning.argtypes = [VP, TP]
+        L.sevp_autotune.argtypes = [VP, VP, VP, CI, ctypes.c_double, CI, TP,
+                                    ctypes.POINTER(ctypes.c_double)]
         L.sevp_get_pass.argtypes = [VP, ctypes.POINTER(CI)]
         L.sevp_set_stream.argtypes = [VP, VP]
         L.sevp_set_graphs.argtypes = [VP, CI]
@@ -297,6 +310,28 @@ class SevpPlan:
     def set_pass(self, kind):
         """PASS_SPLITK or PASS_FUSED kernels for the MFMA corrections."""
         _check(lib().sevp_set_pass(self._h, kind))
+
+    @property
+    def tuning(self) -> SevpTuning:
+        t = SevpTuning()
+        _check(lib().sevp_get_tuning(self._h, ctypes.byref(t)))
+        return t
+
+    def set_tuning(self, pass_kind, waves, rhs_groups):
+        t = SevpTuning(pass_kind, waves, rhs_groups)
+        _check(lib().sevp_set_tuning(self._h, ctypes.byref(t)))
+
+    def autotune(self, s, x, tol=1e-10, max_iter=20, nrhs=None):
+        """Time refined solves of (s, x) (torch device tensors; x is
+        overwritten) over the pass kernels, waves per fused workgroup and RHS
+        groups, keep the fastest; returns (SevpTuning, ms per solve call)."""
+        nrhs = nrhs or s.shape[0]
+        self._follow_torch_stream()
+        t = SevpTuning()
+        ms = ctypes.c_double(0.0)
+        _check(lib().sevp_autotune(self._h, _p(s), _p(x), nrhs, tol, max_iter, ctypes.byref(t),
+                                   ctypes.byref(ms)))
+        return t, ms.value
 
     @property
     def pass_kind(self) -> int:

@@ -590,3 +590,32 @@ def test_bf16_trial_contraction(oracle, device, i2, j2):
     xr = np.asfortranarray(x_d[0].cpu().numpy().T)
     assert oracle.rel_residual(case.coefs, s[:, :, 0], xr) <= 1.5e-10
     plan.close()
+
+
+def test_autotune_choices_and_cache(oracle, device, tmp_path):
+    """The timing autotuner tries the pass kernels, waves per fused
+    workgroup and RHS groups, keeps the fastest, and every choice refines to
+    the tolerance; the cached choice is applied without timing."""
+    import torch
+    from musoc_solver_amd import autotune
+    i2, j2, nrhs = 192, 96, 8
+    print("[autotune]", flush=True)
+    plan, coefs, ie, s, s_d, x_d = _device_case(i2, j2, 2, nrhs, S.PREC_FP32)
+    for kind in (S.PASS_FUSED, S.PASS_SPLITK):
+        for waves in (4, 8):
+            for groups in (1, 2):
+                plan.set_tuning(kind, waves, groups)
+                x_d.zero_()
+                rel, st, ok = plan.solve_refined_dev(s_d, x_d, tol=1e-10, max_iter=10)
+                torch.cuda.synchronize()
+                assert ok and rel.max() <= 1e-10, (kind, waves, groups)
+    r = autotune.tune(plan, s_d, x_d, cache_dir=tmp_path)
+    print(f"[autotune] {r}", flush=True)
+    assert not r["cached"] and r["ms"] > 0
+    t = plan.tuning.as_dict()
+    assert (t["pass_kernels"], t["waves"], t["rhs_groups"]) == (r["pass_kernels"], r["waves"],
+                                                               r["rhs_groups"])
+    plan.set_tuning(S.PASS_SPLITK if r["pass_kernels"] == "fused" else S.PASS_FUSED, 4, 2)
+    r2 = autotune.tune(plan, s_d, x_d, cache_dir=tmp_path)
+    assert r2["cached"] and plan.tuning.as_dict() == t
+    plan.close()
