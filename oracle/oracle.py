@@ -22,9 +22,11 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libsevpref.so")
+REF_FAST_PATH = os.path.join(HERE, "_ref", "libsevpref_fast.so")  # -O3, FMA: timing only
 
 _lib = None
 _ref = None
+_ref_fast = None
 
 
 def build(ref: bool = False) -> None:
@@ -55,6 +57,19 @@ def ref():
     if _ref is None:
         _ref = ctypes.CDLL(REF_PATH)
     return _ref
+
+
+def ref_fast_available() -> bool:
+    return os.path.exists(REF_FAST_PATH)
+
+
+def ref_fast():
+    """The reference routines built with -O3 and multiply-add contraction:
+    for timing the CPU path, not bitwise parity."""
+    global _ref_fast
+    if _ref_fast is None:
+        _ref_fast = ctypes.CDLL(REF_FAST_PATH)
+    return _ref_fast
 
 
 def P(a: np.ndarray, off: int = 0):
@@ -183,7 +198,9 @@ def _full_width(fn_c, fn_ref, coefs, rinv, rinv1, s, x, ie, use_ref):
     rinv, rinv1 = f64(rinv), f64(rinv1)
     ptrs = [P(al), P(ab), P(ac), P(ar), P(at), P(rinv), P(rinv1), P(dum0), P(dum1),
             P(dum2), P(s), P(h), P(x), P(ie)]
-    if use_ref:
+    if use_ref == "fast":
+        getattr(ref_fast(), fn_ref)(*ptrs, _ri(i2 + 2), _ri(i2), _ri(nb0))
+    elif use_ref:
         getattr(ref(), fn_ref)(*ptrs, _ri(i2 + 2), _ri(i2), _ri(nb0))
     else:
         getattr(lib(), fn_c)(*ptrs, _ci(i2 + 2), _ci(i2), _ci(nb0))
