@@ -249,7 +249,7 @@ int sevp_get_pass(sevp_plan plan, int *kind);
 /* Performance parameters of the MFMA corrections, chosen by sevp_autotune
  * or set directly:
  *   pass_kind   SEVP_PASS_FUSED or SEVP_PASS_SPLITK;
- *   waves       wavefronts per workgroup of the fused kernels (4 or 8): the
+ *   waves       wavefronts per workgroup of the fused kernels (8 or 16): the
  *               share of K each wavefront sums;
  *   rhs_groups  RHS groups whose passes run concurrently (1..16).
  * The results of each choice are deterministic; the pass kernels and the

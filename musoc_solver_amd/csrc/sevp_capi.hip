@@ -203,7 +203,7 @@ struct KProf {
 // of the FP64 residual) for a batch size and precision.
 struct PassGraph {
   int nrhs = 0, prec = 0, kind = 0, waves = 0;
-  int r0 = 0, count = 0;  // the RHS group this pass covers
+  int r0 = 0, count = 0, ng = 0;  // the RHS group this pass covers (groups of ng)
   bool prof = false;
   hipGraphExec_t exec = nullptr;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -273,19 +273,17 @@ struct sevp_plan_s {
   DBuf<float> rinv32, rinv1_32;  // mixed precision copies
   DBuf<float> w32;               // mixed: W_b = RINV_b diag(CY(:,IE(b-1)-1)) RINV1_(b-1), b >= 2
   DBuf<double> rinv64t, rinv1_64t, w64t;  // FP64 MFMA: the same three in FP64 lane tiles
-  bool has32 = false, has64 = false;      // the copies above match the influence matrices
-  // the fused pass (one launch per block step) reads column-major operands:
-  // FP32 copies of RINV, RINV1 and W, or (FP64 MFMA) RINV and RINV1
-  // themselves and W in FP64
+  DBuf<unsigned short> rinv16t, rinv1_16t, w16t;  // BF16 lane tiles (fused pass only)
+  // the copies above match the influence matrices; both pass kinds read them
+  bool has32 = false, has64 = false, has16 = false;
   int pass_kind = SEVP_PASS_SPLITK;
-  int fused_waves = FUSED_WAVES;  // wavefronts per fused workgroup (4 or 8)
-  DBuf<float> rinv32c, rinv1_32c, w32c;
-  DBuf<double> w64c;
-  DBuf<unsigned short> rinv16c, rinv1_16c, w16c;  // BF16 (fused pass only)
-  bool has32c = false, has64c = false, has16c = false;
-  DBuf<double> fe, fh, fhc;  // fused pass: error rows and c / cpl rows, two parities each
+  int fused_waves = FUSED_WAVES;  // wavefronts per fused workgroup (8 or 16)
+  // fused pass: error rows (lane tiles, a region per RHS group, two parities;
+  // finite padding, zeroed once) and c / cpl rows (nrhs, I2), two parities
+  DBuf<double> fe, fh, fhc;
   DBuf<double> fd;           // fused pass: d of the blocks below one-row blocks, (NB0-1) slices
   DBuf<float> fe32;
+  long fe_half = 0;          // elements per parity of fe / fe32
   // solve scratch
   DBuf<double> e, dum0, hfree, hcpl, g;  // full forms
   DBuf<float> e32;                       // mixed: FP32 copy of the error rows
@@ -386,17 +384,16 @@ struct FullCtx {
   float *e32;        // FP32 copy of e (mixed precision), else null
   float *kpart;      // mixed: split-K partial products
   double *kpart64;   // FP64 MFMA: split-K partial products (selects that arithmetic)
-  // fused pass (one launch per block step): column-major operands and the
-  // two-parity scratch rows (each parity half holds fhalf elements)
+  // fused pass (one launch per block step): lane-tile operands (the FP32 or
+  // FP64 ones above, or BF16) and the two-parity scratch rows: fe / fe32 in
+  // lane tiles (fe_half elements per parity), the others fhalf per parity
   bool fused;
   int farith;  // FUSED_F32 / FUSED_F64 / FUSED_B16
   int fwaves;  // wavefronts per fused workgroup
-  const float *rinv32c, *rinv1_32c, *w32c;
-  const double *w64c;
-  const unsigned short *rinv16c, *rinv1_16c, *w16c;
+  const unsigned short *rinv16t, *rinv1_16t, *w16t;
   double *fe, *fh, *fhc;
   float *fe32;
-  long fhalf;
+  long fhalf, fe_half;
   double *fd;  // d_b, slice b-1 of fhalf elements, for blocks b followed by a one-row block
   KProf *prof;
 };
@@ -584,16 +581,17 @@ static int run_mixed_pass(const FullCtx &c, const float *f32, const double *f64,
 static int run_fused_pass(const FullCtx &c, const float *f32, const double *f64, long f_stride,
                           double *x, long x_stride, int nrhs, hipStream_t st) {
   const bool dbl = c.farith == FUSED_F64;
-  const long ww = (long)c.i2 * c.i2;
-  // column-major operands of block b (RINV1, RINV, W) in the product's arithmetic
-  auto opnd = [&](const double *f64, const float *f32, const unsigned short *b16,
+  const long tw = lane_tile_elems(c.i2, c.i2);
+  const int nchk = (c.i2 + 15) / 16;
+  // lane-tile operands of block b (RINV1, RINV, W) in the product's arithmetic
+  auto opnd = [&](const double *f64t, const float *f32t, const unsigned short *b16t,
                   long off) -> const void * {
-    if (c.farith == FUSED_F64) return f64 + off;
-    if (c.farith == FUSED_B16) return b16 + off;
-    return f32 + off;
+    if (c.farith == FUSED_F64) return f64t + off;
+    if (c.farith == FUSED_B16) return b16t + off;
+    return f32t + off;
   };
-  double *E[2] = {c.fe, c.fe + c.fhalf};
-  float *E32[2] = {c.fe32, c.fe32 + c.fhalf};
+  double *E[2] = {c.fe, c.fe + c.fe_half};
+  float *E32[2] = {c.fe32, c.fe32 + c.fe_half};
   double *HF[2] = {c.fh, c.fh + c.fhalf}, *HC[2] = {c.fhc, c.fhc + c.fhalf};
   auto erow = [&](int b) -> const void * {
     return dbl ? (const void *)E[b & 1] : (const void *)E32[b & 1];
@@ -614,6 +612,7 @@ static int run_fused_pass(const FullCtx &c, const float *f32, const double *f64,
       m.resid = 1;
       m.e = dbl ? E[b & 1] : nullptr;
       m.e32 = dbl ? nullptr : E32[b & 1];
+      m.e_lt = nchk;
     } else {
       m.hfree = HF[b & 1];
       m.hcpl = (b != 1) ? HC[b & 1] : nullptr;
@@ -636,7 +635,7 @@ static int run_fused_pass(const FullCtx &c, const float *f32, const double *f64,
   for (int b = 1; b < c.nb0; ++b) {
     FusedFwdArgs a{};
     a.a = erow(b);
-    a.b = opnd(c.rinv1, c.rinv1_32c, c.rinv1_16c, (b - 1) * ww);
+    a.b = opnd(c.rinv1_64t, c.rinv1_32, c.rinv1_16t, (b - 1) * tw);
     a.M = nrhs; a.i2 = c.i2; a.periodic = c.periodic;
     a.js = c.ie[b - 1]; a.jf = c.ie[b] - 2; a.ieb = c.ie[b];
     a.steps = a.jf - a.js + 1;
@@ -648,15 +647,15 @@ static int run_fused_pass(const FullCtx &c, const float *f32, const double *f64,
     if (a.steps == 0) a.d_out = c.fd + (long)(b - 1) * c.fhalf;  // block b+1 is one row high
     a.waves = c.fwaves;
     FusedFwdArgs sz = a;
-    if (fused_fwd_prepare(sz)) return fail(SEVP_E_ARG, "block too tall for the fused pass");
+    if (fused_fwd_prepare(sz, c.farith)) return fail(SEVP_E_ARG, "block too tall for the fused pass");
     CHK(timed([&](hipEvent_t e0, hipEvent_t e1) { return launch_fused_fwd(a, c.farith, st, e0, e1); },
               fused_fwd_workgroups(sz), a.tspan));
   }
   for (int b = c.nb0; b >= 1; --b) {
     FusedBwdArgs a{};
     a.a = erow(b);
-    a.b = opnd(c.rinv, c.rinv32c, c.rinv16c, (b - 1) * ww);
-    a.bw = (b == 1) ? nullptr : opnd(c.w64c, c.w32c, c.w16c, (b - 2) * ww);
+    a.b = opnd(c.rinv64t, c.rinv32, c.rinv16t, (b - 1) * tw);
+    a.bw = (b == 1) ? nullptr : opnd(c.w64t, c.w32, c.w16t, (b - 2) * tw);
     a.M = nrhs; a.i2 = c.i2; a.periodic = c.periodic;
     a.js = (b != 1) ? c.ie[b - 2] : 1;
     a.do_resid = b != 1;
@@ -687,8 +686,7 @@ static int run_fused_pass(const FullCtx &c, const float *f32, const double *f64,
       a.has_hom = 1;
     }
     FusedBwdArgs sz = a;
-    int K = 1;
-    if (fused_bwd_prepare(sz, &K)) return fail(SEVP_E_ARG, "shape refused by the fused pass");
+    if (fused_bwd_prepare(sz, c.farith)) return fail(SEVP_E_ARG, "shape refused by the fused pass");
     CHK(timed([&](hipEvent_t e0, hipEvent_t e1) { return launch_fused_bwd(a, c.farith, st, e0, e1); },
               fused_bwd_workgroups(sz), a.tspan));
   }
@@ -1165,8 +1163,7 @@ size_t sevp_plan_device_bytes(sevp_plan p) {
          p->kpart.bytes() + p->xx.bytes() + p->rinv64t.bytes() + p->rinv1_64t.bytes() +
          p->w64t.bytes() + p->kpart64.bytes() +
          p->bdum0.bytes() + p->s14.bytes() + p->r64.bytes() + p->dx.bytes() + p->part.bytes() + p->b1s.bytes() +
-         p->zp.bytes() + p->rinv32c.bytes() + p->rinv1_32c.bytes() + p->w32c.bytes() + p->w64c.bytes() +
-         p->rinv16c.bytes() + p->rinv1_16c.bytes() + p->w16c.bytes() + p->fe.bytes() + p->fh.bytes() +
+         p->zp.bytes() + p->rinv16t.bytes() + p->rinv1_16t.bytes() + p->w16t.bytes() + p->fe.bytes() + p->fh.bytes() +
          p->fhc.bytes() + p->fe32.bytes() + p->fd.bytes() + p->ctl.bytes() + p->ctl_d.bytes() +
          p->s_stage.bytes();
 }
@@ -1192,10 +1189,9 @@ static int build_products(sevp_plan p, const double *h_rinv, const double *h_rin
   if (p->d.precision != SEVP_PREC_FP32) return fail(SEVP_E_ARG, "products-only plans are FP32");
   CHK(alloc_mixed(p));
   CHK(alloc_operands(p));
-  const bool fused = p->pass_kind == SEVP_PASS_FUSED;
   const int i2 = p->d.i2, nb0 = p->d.nb0;
   const long ww = (long)i2 * i2;
-  const long tw = lane_tiles_floats(i2, i2);
+  const long tw = lane_tile_elems(i2, i2);
   hipStream_t st = p->stream;
   DBuf<double> cur, tm, r1a, r1b, sc, wt;
   CHK(cur.alloc(ww));
@@ -1243,14 +1239,9 @@ static int build_products(sevp_plan p, const double *h_rinv, const double *h_rin
         HIPC(launch_gemm(gm, st));
       }
     }
-    if (fused) {
-      HIPC(launch_cvt_f32(cur.p, p->rinv32c.p + (long)(nb - 1) * ww, ww, st));
-      if (nb != nb0) HIPC(launch_cvt_f32(r1cur, p->rinv1_32c.p + (long)(nb - 1) * ww, ww, st));
-    } else {
-      HIPC(launch_cvt_lane_tiles(cur.p, p->rinv32.p + (long)(nb - 1) * tw, i2, i2, 1, st));
-      if (nb != nb0)
-        HIPC(launch_cvt_lane_tiles(r1cur, p->rinv1_32.p + (long)(nb - 1) * tw, i2, i2, 1, st));
-    }
+    HIPC(launch_cvt_lane_tiles(cur.p, p->rinv32.p + (long)(nb - 1) * tw, i2, i2, 1, st));
+    if (nb != nb0)
+      HIPC(launch_cvt_lane_tiles(r1cur, p->rinv1_32.p + (long)(nb - 1) * tw, i2, i2, 1, st));
     if (nb >= 2) {  // W_nb = RINV_nb diag(CY(:, IE(nb-1)-1)) RINV1_(nb-1)
       HIPC(launch_scale_cols(cur.p, p->at(), jl - 1, i2, sc.p, st));
       GemmArgs gm{};
@@ -1260,8 +1251,7 @@ static int build_products(sevp_plan p, const double *h_rinv, const double *h_rin
       gm.ep = EP_COLMAJ;
       gm.out = wt.p; gm.out_ld = i2;
       HIPC(launch_gemm(gm, st));
-      if (fused) HIPC(launch_cvt_f32(wt.p, p->w32c.p + (long)(nb - 2) * ww, ww, st));
-      else HIPC(launch_cvt_lane_tiles(wt.p, p->w32.p + (long)(nb - 2) * tw, i2, i2, 1, st));
+      HIPC(launch_cvt_lane_tiles(wt.p, p->w32.p + (long)(nb - 2) * tw, i2, i2, 1, st));
     }
     jl = jh;
   }
@@ -1269,46 +1259,41 @@ static int build_products(sevp_plan p, const double *h_rinv, const double *h_rin
   HIPC(hipMemcpyAsync(&z, p->zp.p, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPC(hipStreamSynchronize(st));
   if (zero_pivots) *zero_pivots = z;
-  p->has32 = !fused;
-  p->has32c = fused;
-  p->has64 = p->has64c = false;
+  p->has32 = true;
+  p->has64 = p->has16 = false;
   p->have_inv = true;
   return SEVP_OK;
 }
 
-// The operand copies of the current precision's products: FP32 lane tiles
-// (SEVP_PREC_FP32) or FP64 lane tiles (SEVP_PREC_FP64_MFMA) of RINV, RINV1
-// and the coupling operators W.
+// The operand copies of the current precision's products, in lane tiles:
+// FP32 (SEVP_PREC_FP32), FP64 (SEVP_PREC_FP64_MFMA) or BF16 (SEVP_PREC_BF16)
+// copies of RINV, RINV1 and the coupling operators W.
 static int make_low_precision_copies(sevp_plan p) {
   if (p->d.precision == SEVP_PREC_FP64 || p->d.form == SEVP_FORM_BIR) return SEVP_OK;
   if (p->storage == SEVP_STORE_PRODUCTS) return SEVP_OK;  // made while the blocks were built
   const bool dbl = p->d.precision == SEVP_PREC_FP64_MFMA;
   const bool b16 = p->d.precision == SEVP_PREC_BF16;
-  const bool fused = p->pass_kind == SEVP_PASS_FUSED;
-  if (b16 && !fused) return fail(SEVP_E_ARG, "BF16 corrections run in the fused pass only");
-  bool &have = b16 ? p->has16c
-                   : fused ? (dbl ? p->has64c : p->has32c) : (dbl ? p->has64 : p->has32);
+  if (b16 && p->pass_kind != SEVP_PASS_FUSED)
+    return fail(SEVP_E_ARG, "BF16 corrections run in the fused pass only");
+  bool &have = b16 ? p->has16 : (dbl ? p->has64 : p->has32);
   if (have) return SEVP_OK;
   CHK(alloc_operands(p));
   const int i2 = p->d.i2, nb1 = p->d.nb0 > 1 ? p->d.nb0 - 1 : 1;
   const long ww = (long)i2 * i2;
-  const long tw = lane_tiles_floats(i2, i2);
+  const long tw = lane_tile_elems(i2, i2);
   if (b16) {
-    HIPC(launch_cvt_bf16(p->rinv.p, p->rinv16c.p, ww * p->d.nb0, p->stream));
-    HIPC(launch_cvt_bf16(p->rinv1.p, p->rinv1_16c.p, ww * nb1, p->stream));
-  } else if (fused && !dbl) {
-    HIPC(launch_cvt_f32(p->rinv.p, p->rinv32c.p, ww * p->d.nb0, p->stream));
-    HIPC(launch_cvt_f32(p->rinv1.p, p->rinv1_32c.p, ww * nb1, p->stream));
-  } else if (!fused && dbl) {
+    HIPC(launch_cvt_lane_tiles_bf16(p->rinv.p, p->rinv16t.p, i2, i2, p->d.nb0, p->stream));
+    HIPC(launch_cvt_lane_tiles_bf16(p->rinv1.p, p->rinv1_16t.p, i2, i2, nb1, p->stream));
+  } else if (dbl) {
     HIPC(launch_cvt_lane_tiles64(p->rinv.p, p->rinv64t.p, i2, i2, p->d.nb0, p->stream));
     HIPC(launch_cvt_lane_tiles64(p->rinv1.p, p->rinv1_64t.p, i2, i2, nb1, p->stream));
-  } else if (!fused) {
+  } else {
     HIPC(launch_cvt_lane_tiles(p->rinv.p, p->rinv32.p, i2, i2, p->d.nb0, p->stream));
     HIPC(launch_cvt_lane_tiles(p->rinv1.p, p->rinv1_32.p, i2, i2, nb1, p->stream));
   }
   // W_b = RINV_b diag(CY(:, IE(b-1)-1)) RINV1_(b-1): the coupling row of the
-  // backward pass as one product with the error row (FP64, then the layout
-  // and precision of the products)
+  // backward pass as one product with the error row (FP64, then the
+  // precision of the products)
   if (p->d.nb0 > 1) {
     DBuf<double> t, w;
     CHK(t.alloc(ww));
@@ -1321,15 +1306,13 @@ static int make_low_precision_copies(sevp_plan p) {
       gm.b = p->rinv1.p + (long)(nb - 2) * ww; gm.b_sk = 1; gm.b_sn = i2;
       gm.M = i2; gm.N = i2; gm.K = i2;
       gm.ep = EP_COLMAJ;
-      gm.out = (fused && dbl) ? p->w64c.p + (long)(nb - 2) * ww : w.p;
+      gm.out = w.p;
       gm.out_ld = i2;
       HIPC(launch_gemm(gm, p->stream));
-      if (b16) HIPC(launch_cvt_bf16(w.p, p->w16c.p + (long)(nb - 2) * ww, ww, p->stream));
-      else if (fused && !dbl) HIPC(launch_cvt_f32(w.p, p->w32c.p + (long)(nb - 2) * ww, ww, p->stream));
-      else if (!fused && dbl)
-        HIPC(launch_cvt_lane_tiles64(w.p, p->w64t.p + (long)(nb - 2) * tw, i2, i2, 1, p->stream));
-      else if (!fused)
-        HIPC(launch_cvt_lane_tiles(w.p, p->w32.p + (long)(nb - 2) * tw, i2, i2, 1, p->stream));
+      const long off = (long)(nb - 2) * tw;
+      if (b16) HIPC(launch_cvt_lane_tiles_bf16(w.p, p->w16t.p + off, i2, i2, 1, p->stream));
+      else if (dbl) HIPC(launch_cvt_lane_tiles64(w.p, p->w64t.p + off, i2, i2, 1, p->stream));
+      else HIPC(launch_cvt_lane_tiles(w.p, p->w32.p + off, i2, i2, 1, p->stream));
     }
     HIPC(hipStreamSynchronize(p->stream));
   }
@@ -1362,7 +1345,7 @@ int sevp_build_influence(sevp_plan p, int *zero_pivots) {
     c.map = ColMap{1, 0, 0};
   }
   CHK(run_pre(c, p->stream));
-  p->has32 = p->has64 = p->has32c = p->has64c = p->has16c = false;
+  p->has32 = p->has64 = p->has16 = false;
   CHK(make_low_precision_copies(p));
   int z = 0;
   HIPC(hipMemcpyAsync(&z, p->zp.p, sizeof(int), hipMemcpyDeviceToHost, p->stream));
@@ -1378,7 +1361,7 @@ int sevp_load_influence(sevp_plan p, const double *rinv, const double *rinv1) {
   CHK(ensure_inverses(p));
   HIPC(hipMemcpy(p->rinv.p, rinv, p->rinv.bytes(), hipMemcpyHostToDevice));
   if (p->d.nb0 > 1) HIPC(hipMemcpy(p->rinv1.p, rinv1, p->rinv1.bytes(), hipMemcpyHostToDevice));
-  p->has32 = p->has64 = p->has32c = p->has64c = p->has16c = false;
+  p->has32 = p->has64 = p->has16 = false;
   CHK(make_low_precision_copies(p));
   HIPC(hipStreamSynchronize(p->stream));
   p->have_inv = true;
@@ -1430,19 +1413,24 @@ static FullCtx full_ctx(sevp_plan p) {
   c.fused = (mixed || mfma64 || bf16) && p->pass_kind == SEVP_PASS_FUSED;
   c.farith = mfma64 ? FUSED_F64 : (bf16 ? FUSED_B16 : FUSED_F32);
   c.fwaves = p->fused_waves;
-  c.rinv32c = p->rinv32c.p; c.rinv1_32c = p->rinv1_32c.p; c.w32c = p->w32c.p;
-  c.w64c = p->w64c.p;
-  c.rinv16c = p->rinv16c.p; c.rinv1_16c = p->rinv1_16c.p; c.w16c = p->w16c.p;
+  if (bf16) {
+    c.rinv16t = p->rinv16t.p;
+    c.rinv1_16t = p->rinv1_16t.p;
+    c.w16t = p->w16t.p;
+  }
   c.fe = p->fe.p; c.fh = p->fh.p; c.fhc = p->fhc.p; c.fe32 = p->fe32.p;
   c.fd = p->fd.p;
   c.fhalf = (long)p->d.nrhs_max * p->d.i2;
+  c.fe_half = p->fe_half;
   c.prof = &p->prof;
   return c;
 }
 
-// The context of RHS r0.. of a batch: every per-RHS scratch row offset by
-// r0 right-hand sides (the split-K partials by r0 times their per-RHS size).
-static FullCtx group_ctx(const FullCtx &c0, int r0) {
+// The context of RHS r0.. of a batch split in groups of ng: every per-RHS
+// scratch row offset by r0 right-hand sides (the split-K partials by r0
+// times their per-RHS size); the fused pass's lane-tile error rows take a
+// region of ceil(ng / 16) row tiles per group.
+static FullCtx group_ctx(const FullCtx &c0, int r0, int ng) {
   FullCtx c = c0;
   const long i2 = c.i2;
   const long dstride = i2 * (c.nb0 > 1 ? c.nb0 - 1 : 1);
@@ -1455,10 +1443,11 @@ static FullCtx group_ctx(const FullCtx &c0, int r0) {
   const long part_rhs = (long)gemm_part_slices((int)i2, (int)i2, nullptr) * 2 * ((i2 + 15) / 16 * 16);
   if (c.kpart) c.kpart += r0 * part_rhs;
   if (c.kpart64) c.kpart64 += r0 * part_rhs;
-  if (c.fe) c.fe += r0 * i2;
+  const long lt_off = (long)(r0 / ng) * lane_tile_elems(ng, (int)i2);
+  if (c.fe) c.fe += lt_off;
   if (c.fh) c.fh += r0 * i2;
   if (c.fhc) c.fhc += r0 * i2;
-  if (c.fe32) c.fe32 += r0 * i2;
+  if (c.fe32) c.fe32 += lt_off;
   if (c.fd) c.fd += r0 * i2;
   return c;
 }
@@ -1476,10 +1465,18 @@ static int alloc_mixed(sevp_plan p) {
   const size_t n2 = (size_t)gemm_part_slices(2 * np, i2, nullptr) * 2 * np;
   const size_t nf = 2 * (size_t)nr * p->d.i2;  // two parities of (nrhs, I2)
   if (!p->fe.p) {
-    CHK(p->fe.alloc(nf));
+    // lane-tile error rows: a region of ceil(ng / 16) row tiles per RHS
+    // group of ng, for any split into 1 to 16 groups
+    long half = 0;
+    for (int g = 1; g <= 16; ++g)
+      half = std::max(half, g * lane_tile_elems((nr + g - 1) / g, p->d.i2));
+    p->fe_half = half;
+    CHK(p->fe.alloc(2 * (size_t)half));
+    CHK(p->fe32.alloc(2 * (size_t)half));
+    CHK(zero_async(p->fe.p, p->fe.bytes(), p->stream));
+    CHK(zero_async(p->fe32.p, p->fe32.bytes(), p->stream));
     CHK(p->fh.alloc(nf));
     CHK(p->fhc.alloc(nf));
-    CHK(p->fe32.alloc(nf));
     if (has_unit_block(p))
       CHK(p->fd.alloc((size_t)(p->d.nb0 - 1) * nr * p->d.i2));
   }
@@ -1492,31 +1489,18 @@ static int alloc_mixed(sevp_plan p) {
   return SEVP_OK;
 }
 
-// The product operands of the plan's precision and pass kernels, allocated
-// on first use: lane tiles for the split-K pass, column-major arrays for the
-// fused pass (FP64 MFMA reads RINV / RINV1 themselves and needs only W).
+// The product operands of the plan's precision, in lane tiles (both pass
+// kinds read them), allocated on first use.
 static int alloc_operands(sevp_plan p) {
   const size_t nb1 = p->d.nb0 > 1 ? p->d.nb0 - 1 : 1;
-  const bool dbl = p->d.precision == SEVP_PREC_FP64_MFMA;
-  if (p->pass_kind == SEVP_PASS_FUSED) {
-    const size_t ww = (size_t)p->d.i2 * p->d.i2;
-    if (p->d.precision == SEVP_PREC_BF16) {
-      if (!p->rinv16c.p) {
-        CHK(p->rinv16c.alloc(ww * p->d.nb0));
-        CHK(p->rinv1_16c.alloc(ww * nb1));
-        CHK(p->w16c.alloc(ww * nb1));
-      }
-    } else if (dbl) {
-      if (!p->w64c.p) CHK(p->w64c.alloc(ww * nb1));
-    } else if (!p->rinv32c.p) {
-      CHK(p->rinv32c.alloc(ww * p->d.nb0));
-      CHK(p->rinv1_32c.alloc(ww * nb1));
-      CHK(p->w32c.alloc(ww * nb1));
+  const size_t tw = (size_t)lane_tile_elems(p->d.i2, p->d.i2);
+  if (p->d.precision == SEVP_PREC_BF16) {
+    if (!p->rinv16t.p) {
+      CHK(p->rinv16t.alloc(tw * p->d.nb0));
+      CHK(p->rinv1_16t.alloc(tw * nb1));
+      CHK(p->w16t.alloc(tw * nb1));
     }
-    return SEVP_OK;
-  }
-  const size_t tw = (size_t)lane_tiles_floats(p->d.i2, p->d.i2);
-  if (dbl) {
+  } else if (p->d.precision == SEVP_PREC_FP64_MFMA) {
     if (!p->rinv64t.p) {
       CHK(p->rinv64t.alloc(tw * p->d.nb0));
       CHK(p->rinv1_64t.alloc(tw * nb1));
@@ -1550,8 +1534,6 @@ int sevp_set_pass(sevp_plan p, int kind) {
   if (kind == SEVP_PASS_FUSED && !fused_eligible(p))
     return fail(SEVP_E_ARG, "the fused pass needs block heights <= 4 and I2 divisible by 4 (>= 16)");
   CHK(check_pass(p, kind, p->d.precision));
-  if (p->storage == SEVP_STORE_PRODUCTS && p->have_inv)
-    return fail(SEVP_E_STATE, "a products-only plan keeps the operand layout it was built with");
   HIPC(hipStreamSynchronize(p->stream));
   drop_graphs(p);
   const int old = p->pass_kind;
@@ -1581,7 +1563,7 @@ int sevp_get_tuning(sevp_plan p, sevp_tuning *t) {
 
 int sevp_set_tuning(sevp_plan p, const sevp_tuning *t) {
   if (!p || !t) return fail(SEVP_E_ARG, "null argument");
-  if (t->waves != 4 && t->waves != 8) return fail(SEVP_E_ARG, "waves per workgroup must be 4 or 8");
+  if (t->waves != 8 && t->waves != 16) return fail(SEVP_E_ARG, "waves per workgroup must be 8 or 16");
   if (t->rhs_groups < 1 || t->rhs_groups > 16) return fail(SEVP_E_ARG, "RHS groups must be 1..16");
   CHK(sevp_set_pass(p, t->pass_kind));
   HIPC(hipStreamSynchronize(p->stream));
@@ -1603,10 +1585,9 @@ int sevp_autotune(sevp_plan p, const float *s, double *x, int nrhs, double tol, 
   const long nx = (long)(p->d.i2 + 2) * (p->d.j2 + 2);
   std::vector<sevp_tuning> cand;
   for (int kind : {SEVP_PASS_FUSED, SEVP_PASS_SPLITK}) {
-    if (kind != p->pass_kind && p->storage == SEVP_STORE_PRODUCTS) continue;
     if (check_pass(p, kind, p->d.precision) != SEVP_OK) continue;
-    for (int waves : {8, 4}) {
-      if (kind == SEVP_PASS_SPLITK && waves != 8) continue;  // not a split-K parameter
+    for (int waves : {16, 8}) {
+      if (kind == SEVP_PASS_SPLITK && waves != FUSED_WAVES) continue;  // not a split-K parameter
       for (int g : {1, 2, 4})
         if (g <= nrhs) cand.push_back(sevp_tuning{kind, waves, g});
     }
@@ -1654,7 +1635,7 @@ int sevp_set_storage(sevp_plan p, int storage) {
   drop_graphs(p);
   p->rinv.release();
   p->rinv1.release();
-  p->has32 = p->has64 = p->has32c = p->has64c = p->has16c = false;
+  p->has32 = p->has64 = p->has16 = false;
   p->have_inv = false;
   p->storage = storage;
   return SEVP_OK;
@@ -1836,21 +1817,22 @@ static int add_timing_nodes(hipGraph_t graph, const std::vector<hipGraphNode_t> 
 }
 
 // The captured pass for (nrhs, precision, profiling), captured on first use.
-static int get_pass_graph(sevp_plan p, const FullCtx &c0, int nrhs, int r0, int count,
+static int get_pass_graph(sevp_plan p, const FullCtx &c0, int nrhs, int r0, int count, int ng,
                           PassGraph **out) {
   const bool prof = p->prof.kclass >= 0;
   for (auto &g : p->graphs)
-    if (g.nrhs == nrhs && g.r0 == r0 && g.count == count && g.prec == p->d.precision &&
+    if (g.nrhs == nrhs && g.r0 == r0 && g.count == count && g.ng == ng && g.prec == p->d.precision &&
         g.kind == p->pass_kind && g.waves == p->fused_waves && g.prof == prof) {
       *out = &g;
       return SEVP_OK;
     }
   const long ns = (long)p->d.i2 * p->d.j2, nx = (long)(p->d.i2 + 2) * (p->d.j2 + 2);
-  const FullCtx c = group_ctx(c0, r0);
+  const FullCtx c = group_ctx(c0, r0, ng);
   PassGraph g;
   g.nrhs = nrhs;
   g.r0 = r0;
   g.count = count;
+  g.ng = ng;
   g.prec = p->d.precision;
   g.kind = p->pass_kind;
   g.waves = p->fused_waves;
@@ -1980,12 +1962,12 @@ static int run_group_passes(sevp_plan p, const FullCtx &c, int nrhs, int ngroups
     hipStream_t gs = g == 0 ? st : p->gstreams[g - 1];
     if (g > 0) HIPC(hipStreamWaitEvent(gs, p->gevents[0], 0));
     PassGraph *pg = nullptr;
-    if (graph_groups) CHK(get_pass_graph(p, c, nrhs, r0, count, &pg));
+    if (graph_groups) CHK(get_pass_graph(p, c, nrhs, r0, count, ng, &pg));
     if (pg) {
       HIPC(hipGraphLaunch(pg->exec, gs));
       if (pg->prof) p->prof.pending.push_back(&pg->events);
     } else {
-      const FullCtx cg = group_ctx(c, r0);
+      const FullCtx cg = group_ctx(c, r0, ng);
       hipStream_t saved = p->prof.st;
       p->prof.st = gs;
       int rc = zero_async(p->dx.p + r0 * nx, sizeof(double) * nx * count, gs);

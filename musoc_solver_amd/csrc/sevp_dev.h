@@ -15,6 +15,14 @@ __device__ __forceinline__ long f2(int ld, int i, int j) {
   return (long)(i - 1) + (long)(j - 1) * (long)ld;
 }
 
+// Lane tiles of an (m, k) operand of the 16x16x4 MFMA steps: block
+// (m / 16, k / 16) of 256 elements holds, at lane l = m % 16 + 16 q and slot
+// t, the element (m, 16 (k / 16) + 4 q + t), so one wavefront's operand for
+// 16 rows and 16 k is one contiguous run.  nchk = ceil(columns / 16).
+__device__ __forceinline__ long lt_index(int m, int k, int nchk) {
+  return ((long)((m >> 4) * nchk + (k >> 4)) * 64 + (m & 15) + 16 * ((k & 15) >> 2)) * 4 + (k & 3);
+}
+
 // Lane i receives lane i-1's value (lane 0 receives `fill`): DPP wave_shr:1.
 __device__ __forceinline__ double wave_shr1(double v, double fill) {
   int lo = __double2loint(v), hi = __double2hiint(v);

@@ -1,14 +1,14 @@
 // One launch per block step: the correction product of a full-width SEVP
-// pass fused with what depends on it, for the MFMA corrections (FP32 with
-// FP64 refinement, or FP64).
+// pass fused with what depends on it, for the MFMA corrections (FP32 or BF16
+// with FP64 refinement, or FP64).
 //
 // The split-K pass (sevp_mixed.hip) takes four launches per block and writes
 // every product as slice partials that the next launch adds up.  Here a
 // workgroup computes whole dot products: its W wavefronts split K, each keeps
-// a 16 x 16 accumulator of the MFMA instruction, and the W partial tiles are
-// added in FP64 in wavefront order through LDS, so every output is one fixed
-// sum.  The epilogue then does the work that depends only on that output's
-// column:
+// one or two 16 x 16 accumulators of the MFMA instruction (two 16-row tiles
+// share the matrix operand), and the W partial tiles are added in FP64 in
+// wavefront order through LDS, so every output is one fixed sum.  The
+// epilogue then does the work that depends only on that output's column:
 //
 //   forward, block b:  d = e_b RINV1_b.  Block b+1's first march row uses the
 //     corrected south row X(IE(b)) - d (never written back: the backward pass
@@ -26,13 +26,17 @@
 //     closure residual of block b-1, which reads that row only at its own
 //     column, are done at once; c and cpl are stored for the homogeneous
 //     march of block b, which needs c at the neighbouring columns and runs in
-//     the next launch (extra workgroups there), when every column is complete.
+//     the next launch (its workgroups come first), when every column is
+//     complete.
 //
 // A pass is 2*NB0 + 1 launches instead of 4*NB0 + 2, each influence matrix is
-// read once for all right-hand sides, and no partial products go through
-// memory.  The workgroups of one column tile (one per 16 right-hand sides)
-// are placed on the same XCD so the tile's matrix columns are fetched from
-// HBM once.
+// read from HBM once for all right-hand sides, and no partial products go
+// through memory.  Both operands are in lane tiles, so a wavefront's operand
+// for 16 rows (or columns) and 16 k is one contiguous 1 KiB (FP32) run; every
+// load address is clamped in bounds instead of masked, so a wavefront issues
+// all its loads of a K range before its first wait.  The workgroups of one
+// column tile are placed on the same XCD so the tile's matrix columns are
+// fetched from HBM once.
 #include <hip/hip_ext.h>
 
 #include "sevp_dev.h"
@@ -116,82 +120,67 @@ struct KB16 {
   }
 };
 
-// Four consecutive k of a run, zero at or past klen (klen a multiple of 4):
-// one unconditional vector load (from k = 0 when out of range) and a select.
-template <class V, class T>
-__device__ __forceinline__ V ld4(const T *base, int k, int klen) {
-  const bool ok = k < klen;
-  const V v = *reinterpret_cast<const V *>(base + (ok ? k : 0));
-  return ok ? v : V{0, 0, 0, 0};
-}
-
-// Partial tile of one wavefront: rows m0 .. m0+15 of A (ld K) times the
-// column-major B column of this lane's column (lane % 16), over the chunks
-// [c_lo, c_hi) of 16 k.  Loads are issued G chunks at a time, one group
-// ahead of the MFMA steps (2G chunks in flight).
-template <class KT>
-__device__ __forceinline__ typename KT::CV part_tile(const typename KT::A *a, int M, int K, int m0,
-                                                     const typename KT::B *bcol, bool bok,
-                                                     int c_lo, int c_hi) {
+// Partial tile of one wavefront over the chunks [c_lo, c_hi) of 16 k: MBW
+// 16-row tiles of A (row tiles mb0 .. mb0+MBW-1, clamped to the last one,
+// whose rows are then never stored) times the 16 columns of B around this
+// lane's column n.  Chunks are loaded G at a time; a chunk past c_hi loads
+// the last one again and is skipped by the MFMA steps (the range is
+// wave-uniform), so no load is masked and none waits on another.
+template <class KT, int G, int MBW>
+__device__ __forceinline__ void part_tile(const typename KT::A *alt, int mb0, int mtiles,
+                                          const typename KT::B *blt, int n, int nchk, int c_lo,
+                                          int c_hi, typename KT::CV (&acc)[MBW]) {
   typedef typename KT::AV AV;
   typedef typename KT::BV BV;
-  constexpr int G = KT::G;
-  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
-  const int m = m0 + r;
-  const typename KT::A *arow = a + (long)(m < M ? m : 0) * K;
-  const int alen = m < M ? K : 0;
-  const int blen = bok ? K : 0;
-  typename KT::CV acc = typename KT::CV{0, 0, 0, 0};
-  AV av[G];
-  BV bv[G];
+  const int lane = threadIdx.x & 63, q = lane >> 4;
+  const typename KT::B *bb = blt + ((long)(n >> 4) * nchk * 64 + (n & 15) + 16 * q) * 4;
+  const typename KT::A *ab[MBW];
 #pragma unroll
-  for (int g = 0; g < G; ++g) {
-    const int k = 16 * (c_lo + g) + 4 * q;
-    const bool in = c_lo + g < c_hi;
-    av[g] = ld4<AV>(arow, k, in ? alen : 0);
-    bv[g] = ld4<BV>(bcol, k, in ? blen : 0);
-  }
+  for (int mb = 0; mb < MBW; ++mb) ab[mb] = alt + ((long)min(mb0 + mb, mtiles - 1) * nchk * 64 + lane) * 4;
   for (int c0 = c_lo; c0 < c_hi; c0 += G) {
-    AV an[G];
-    BV bn[G];
-#pragma unroll
-    for (int g = 0; g < G; ++g) {  // the next group, issued before this group's MFMA steps
-      const int k = 16 * (c0 + G + g) + 4 * q;
-      const bool in = c0 + G + g < c_hi;
-      an[g] = ld4<AV>(arow, k, in ? alen : 0);
-      bn[g] = ld4<BV>(bcol, k, in ? blen : 0);
-    }
-#pragma unroll
-    for (int g = 0; g < G; ++g) acc = KT::mma(av[g], bv[g], acc);
+    AV av[G][MBW];
+    BV bv[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-      av[g] = an[g];
-      bv[g] = bn[g];
+      const long ch = min(c0 + g, c_hi - 1);
+      bv[g] = *reinterpret_cast<const BV *>(bb + ch * 256);
+#pragma unroll
+      for (int mb = 0; mb < MBW; ++mb) av[g][mb] = *reinterpret_cast<const AV *>(ab[mb] + ch * 256);
     }
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      if (c0 + g < c_hi)
+#pragma unroll
+        for (int mb = 0; mb < MBW; ++mb) acc[mb] = KT::mma(av[g][mb], bv[g], acc[mb]);
   }
-  return acc;
 }
 
 // The product tile of a workgroup: W wavefronts, each its share of the K
-// chunks; their partial tiles go to LDS and thread t < 256 receives the FP64
-// sum over the wavefronts, in wavefront order, of row t / 16 and column
-// t % 16 of the tile.  Every thread of the workgroup must call it.
-template <class KT, int W>
-__device__ __forceinline__ double product_tile(const typename KT::A *a, int M, int K, int m0,
-                                               const typename KT::B *b, int bcolumn, bool bok,
+// chunks (96 / W chunks per load group: one group at K <= 1536); their
+// partial tiles go to LDS and thread t < 256 MBW receives the FP64 sum over
+// the wavefronts, in wavefront order, of row t / 16 (of the MBW row tiles)
+// and column t % 16 of the tile.  Every thread of the workgroup must call it.
+template <class KT, int W, int MBW>
+__device__ __forceinline__ double product_tile(const typename KT::A *alt, int mb0, int mtiles,
+                                               const typename KT::B *blt, int n, int K,
                                                typename KT::Acc *red) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, q = lane >> 4;
-  const int nchk = (K + 15) / 16;
-  const int c_lo = (int)((long)nchk * wave / W), c_hi = (int)((long)nchk * (wave + 1) / W);
-  const typename KT::B *bcol = b + (long)(bok ? bcolumn : 0) * K;
-  const auto acc = part_tile<KT>(a, M, K, m0, bcol, bok, c_lo, c_hi);
+  const int lane = threadIdx.x & 63, q = lane >> 4;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nchk = (K + 15) >> 4;
+  const int c_lo = nchk * wave / W, c_hi = nchk * (wave + 1) / W;
+  typename KT::CV acc[MBW];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) red[wave * 256 + KT::row(q, i) * 16 + (lane & 15)] = acc[i];
+  for (int mb = 0; mb < MBW; ++mb) acc[mb] = typename KT::CV{0, 0, 0, 0};
+  part_tile<KT, 96 / W, MBW>(alt, mb0, mtiles, blt, n, nchk, c_lo, c_hi, acc);
+#pragma unroll
+  for (int mb = 0; mb < MBW; ++mb)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) red[(wave * MBW + mb) * 256 + KT::row(q, i) * 16 + (lane & 15)] = acc[mb][i];
   __syncthreads();
   double s = 0.0;
-  if (threadIdx.x < 256)
+  if (threadIdx.x < 256 * MBW)
 #pragma unroll
-    for (int w = 0; w < W; ++w) s += (double)red[w * 256 + threadIdx.x];
+    for (int w = 0; w < W; ++w) s += (double)red[w * 256 * MBW + threadIdx.x];
   return s;
 }
 
@@ -210,17 +199,17 @@ __device__ __forceinline__ double rhs_at(const float *f, const double *f64, long
 }
 
 // ------------------------------------------------------------------ forward
-template <class KT, int W>
+template <class KT, int W, int MBW>
 __global__ __launch_bounds__(64 * W) void fused_fwd_kernel(FusedFwdArgs p) {
-  __shared__ typename KT::Acc red[W * 256];
+  __shared__ typename KT::Acc red[W * 256 * MBW];
   span_begin(p.tspan);
-  int tile, mb;
-  if (tile_of(blockIdx.x, p.ntiles, p.nmb, tile, mb)) {
+  int tile, mbg;
+  if (tile_of(blockIdx.x, p.ntiles, p.nmb, tile, mbg)) {
     const int t = threadIdx.x;
     const int col = t & 15;
-    const int m = 16 * mb + (t >> 4);
+    const int m = 16 * MBW * mbg + (t >> 4);
     const int steps = p.steps;
-    const int i2 = p.i2, i0 = i2 + 2;
+    const int i2 = p.i2, i0 = i2 + 2, nchk = (i2 + 15) >> 4;
     // interior column (1-based, before wrapping) of this lane's tile column
     const int c = tile * (16 - 2 * steps) - steps + 1 + col;
     int pc;
@@ -243,7 +232,7 @@ __global__ __launch_bounds__(64 * W) void fused_fwd_kernel(FusedFwdArgs p) {
       xce = max(min(c + 1, i2 + 1), 0) + 1;
     }
     const bool out = col >= steps && col < 16 - steps && c >= 1 && c <= i2;
-    const bool act = t < 256 && m < p.M;
+    const bool act = t < 256 * MBW && m < p.M;
     double *x = p.x + (long)(act ? m : 0) * p.x_stride;
     const int js = p.js, jf = p.jf, ieb = p.ieb;
     // the epilogue's rows that do not depend on the product, loaded first;
@@ -258,8 +247,8 @@ __global__ __launch_bounds__(64 * W) void fused_fwd_kernel(FusedFwdArgs p) {
       fe = x[f2(i0, xce, js + 1)];
       xnorth = x[f2(i0, xc, ieb + 1)];
     }
-    const double d = product_tile<KT, W>((const typename KT::A *)p.a, p.M, i2, 16 * mb,
-                                         (const typename KT::B *)p.b, pc - 1, !halo, red);
+    const double d = product_tile<KT, W, MBW>((const typename KT::A *)p.a, MBW * mbg, (p.M + 15) >> 4,
+                                              (const typename KT::B *)p.b, pc - 1, i2, red);
     if (act) {
       if (!halo) vs = vs - d;  // the south row as block b's correction leaves it
       if (p.d_out && out) p.d_out[(long)m * i2 + pc - 1] = d;
@@ -283,8 +272,9 @@ __global__ __launch_bounds__(64 * W) void fused_fwd_kernel(FusedFwdArgs p) {
                        cf(p.ay, jr) * vs - cf(p.bb, jr) * vc - cf(p.cx, jr) * xe -
                        cf(p.cy, jr) * xnorth;
       if (out) {
-        if (p.e_out) p.e_out[(long)m * i2 + co] = e;
-        if (p.e32_out) p.e32_out[(long)m * i2 + co] = (float)e;
+        const long eo = lt_index(m, (int)co, nchk);
+        if (p.e_out) p.e_out[eo] = e;
+        if (p.e32_out) p.e32_out[eo] = (float)e;
       }
     }
   }
@@ -292,65 +282,69 @@ __global__ __launch_bounds__(64 * W) void fused_fwd_kernel(FusedFwdArgs p) {
 }
 
 // ----------------------------------------------------------------- backward
-// Workgroups [0, nprod): product tiles, first those of RINV_b (with the
-// epilogue), then, when W_b is given, those of W_b (the coupling row, only
-// stored); then the homogeneous march of block b+1, one wavefront per
-// (tile, RHS).
-template <class KT, int W, int K>
+// Workgroups [0, nhom_wg): the homogeneous march of block b+1, one wavefront
+// per (tile, RHS), first so that the launch's longest dependent chain starts
+// at once; then the product tiles of RINV_b (with the epilogue), then, when
+// W_b is given, those of W_b (the coupling row, only stored).
+template <class KT, int W, int MBW>
 __global__ __launch_bounds__(64 * W) void fused_bwd_kernel(FusedBwdArgs p) {
-  __shared__ typename KT::Acc red[W * 256];
+  __shared__ typename KT::Acc red[W * 256 * MBW];
   span_begin(p.tspan);
   const int wg = blockIdx.x;
-  const int nprod = p.bw ? 2 * p.nprod_half : p.nprod_half;
-  int tile, mb;
-  if (wg >= nprod) {
-    const long wave = (long)(wg - nprod) * W + (threadIdx.x >> 6);
+  int tile, mbg;
+  if (wg < p.nhom_wg) {
+    const long wave = (long)wg * W + (threadIdx.x >> 6);
     if (wave < (long)p.hom_tiles * p.hom.nrhs)
-      march_body<K, MODE_HOM>(p.hom, (int)(wave % p.hom_tiles), (int)(wave / p.hom_tiles), 0, 1,
+      march_body<1, MODE_HOM>(p.hom, (int)(wave % p.hom_tiles), (int)(wave / p.hom_tiles), 0, 1,
                               threadIdx.x & 63);
-  } else if (tile_of(wg >= p.nprod_half ? wg - p.nprod_half : wg, p.ntiles, p.nmb, tile, mb)) {
-    const bool cpl = wg >= p.nprod_half;
-    const int t = threadIdx.x;
-    const int m = 16 * mb + (t >> 4);
-    const int i2 = p.i2, i0 = i2 + 2;
-    const int c = 16 * tile + 1 + (t & 15);
-    const bool ok = c <= i2;
-    const bool act = t < 256 && m < p.M && ok;
-    const int js = p.js;
-    double *x = p.x + (long)(act ? m : 0) * p.x_stride;
-    const bool resid = !cpl && p.do_resid && act;
-    // the epilogue's operands, loaded before the product
-    double xfree = 0.0, xsouth = 0.0, xw = 0.0, xcen = 0.0, xe = 0.0;
-    if (act && !cpl) xfree = x[f2(i0, c + 1, js + 1)];
-    if (resid) {  // closure row js-1 of block b-1; row js = IE(b-1)
-      xsouth = x[f2(i0, c + 1, js - 1)];
-      xcen = x[f2(i0, c + 1, js)];
-      if (p.periodic) {
-        xw = x[f2(i0, (c == 1 ? i2 : c - 1) + 1, js)];
-        xe = x[f2(i0, (c == i2 ? 1 : c + 1) + 1, js)];
-      } else {
-        xw = x[f2(i0, c, js)];
-        xe = x[f2(i0, c + 2, js)];
+  } else {
+    const int wp = wg - p.nhom_wg;
+    const bool cpl = wp >= p.nprod_half;
+    if (tile_of(cpl ? wp - p.nprod_half : wp, p.ntiles, p.nmb, tile, mbg)) {
+      const int t = threadIdx.x;
+      const int m = 16 * MBW * mbg + (t >> 4);
+      const int i2 = p.i2, i0 = i2 + 2, nchk = (i2 + 15) >> 4;
+      const int c = 16 * tile + 1 + (t & 15);
+      const bool ok = c <= i2;
+      const bool act = t < 256 * MBW && m < p.M && ok;
+      const int js = p.js;
+      double *x = p.x + (long)(act ? m : 0) * p.x_stride;
+      const bool resid = !cpl && p.do_resid && act;
+      // the epilogue's operands, loaded before the product
+      double xfree = 0.0, xsouth = 0.0, xw = 0.0, xcen = 0.0, xe = 0.0;
+      if (act && !cpl) xfree = x[f2(i0, c + 1, js + 1)];
+      if (resid) {  // closure row js-1 of block b-1; row js = IE(b-1)
+        xsouth = x[f2(i0, c + 1, js - 1)];
+        xcen = x[f2(i0, c + 1, js)];
+        if (p.periodic) {
+          xw = x[f2(i0, (c == 1 ? i2 : c - 1) + 1, js)];
+          xe = x[f2(i0, (c == i2 ? 1 : c + 1) + 1, js)];
+        } else {
+          xw = x[f2(i0, c, js)];
+          xe = x[f2(i0, c + 2, js)];
+        }
       }
-    }
-    const double v = product_tile<KT, W>((const typename KT::A *)p.a, p.M, i2, 16 * mb,
-                                         (const typename KT::B *)(cpl ? p.bw : p.b), c - 1, ok, red);
-    if (act) {
-      const long ro = (long)m * i2 + c - 1;
-      if (cpl) {
-        p.hcpl_out[ro] = v;
-      } else {
-        p.hfree_out[ro] = v;
-        const double north = xfree + v;
-        x[f2(i0, c + 1, js + 1)] = north;
-        if (resid) {
-          if (p.dsouth) xsouth = xsouth - p.dsouth[ro];
-          const long co = (long)(c - 1) + (long)(js - 2) * i2;  // coefficient row js-1
-          const double e = rhs_at(p.f, p.f64, (long)m * p.f_stride + co) - (double)p.ax[co] * xw -
-                           (double)p.ay[co] * xsouth - (double)p.bb[co] * xcen -
-                           (double)p.cx[co] * xe - (double)p.cy[co] * north;
-          if (p.e_out) p.e_out[ro] = e;
-          if (p.e32_out) p.e32_out[ro] = (float)e;
+      // lanes past I2 (last tile) read the zero-padded matrix columns
+      const double v = product_tile<KT, W, MBW>((const typename KT::A *)p.a, MBW * mbg, (p.M + 15) >> 4,
+                                                (const typename KT::B *)(cpl ? p.bw : p.b), c - 1, i2, red);
+      if (act) {
+        const long ro = (long)m * i2 + c - 1;
+        if (cpl) {
+          p.hcpl_out[ro] = v;
+        } else {
+          p.hfree_out[ro] = v;
+          const double north = xfree + v;
+          x[f2(i0, c + 1, js + 1)] = north;
+          if (resid) {
+            if (p.dsouth) xsouth = xsouth - p.dsouth[ro];
+            const long co = (long)(c - 1) + (long)(js - 2) * i2;  // coefficient row js-1
+            const double e = rhs_at(p.f, p.f64, (long)m * p.f_stride + co) - (double)p.ax[co] * xw -
+                             (double)p.ay[co] * xsouth - (double)p.bb[co] * xcen -
+                             (double)p.cx[co] * xe - (double)p.cy[co] * north;
+            const long eo = lt_index(m, c - 1, nchk);
+            if (p.e_out) p.e_out[eo] = e;
+            if (p.e32_out) p.e32_out[eo] = (float)e;
+          }
         }
       }
     }
@@ -366,102 +360,125 @@ int fused_tiles_fwd(int i2, int steps) {
 
 static long placed_workgroups(int ntiles, int nmb) { return 8L * nmb * ((ntiles + 7) / 8); }
 
+// Row tiles per workgroup: two share each matrix load (FP32 and BF16), one
+// for FP64, whose operands take twice the registers.
+static int rows_per_group(int arith, int M) { return (arith != FUSED_F64 && M > 16) ? 2 : 1; }
+
 long fused_fwd_workgroups(const FusedFwdArgs &a) { return placed_workgroups(a.ntiles, a.nmb); }
 
 long fused_bwd_workgroups(const FusedBwdArgs &a) {
-  const long hom_waves = (long)a.hom_tiles * (a.has_hom ? a.hom.nrhs : 0);
-  const int w = a.waves > 0 ? a.waves : FUSED_WAVES;
-  return (a.bw ? 2L : 1L) * a.nprod_half + (hom_waves + w - 1) / w;
+  return a.nhom_wg + (a.bw ? 2L : 1L) * a.nprod_half;
 }
 
-// Fill the launcher-set fields (tiles, row blocks) of the arguments.
-int fused_fwd_prepare(FusedFwdArgs &a) {
+static int waves_of(int w) { return w == 0 ? FUSED_WAVES : w; }
+
+// Fill the launcher-set fields (tiles, row-tile groups) of the arguments.
+int fused_fwd_prepare(FusedFwdArgs &a, int arith) {
   if (a.steps < 0 || 16 - 2 * a.steps < 2 || a.M < 1 || a.M > 64 || a.i2 % 4 != 0) return -1;
+  const int w = waves_of(a.waves);
+  if (w != 8 && w != 16) return -1;
   a.ntiles = fused_tiles_fwd(a.i2, a.steps);
-  a.nmb = (a.M + 15) / 16;
+  a.mbw = rows_per_group(arith, a.M);
+  a.nmb = ((a.M + 15) / 16 + a.mbw - 1) / a.mbw;
   return 0;
 }
 
-int fused_bwd_prepare(FusedBwdArgs &a, int *K) {
+int fused_bwd_prepare(FusedBwdArgs &a, int arith) {
   if (a.M < 1 || a.M > 64 || a.i2 % 4 != 0) return -1;
+  const int w = waves_of(a.waves);
+  if (w != 8 && w != 16) return -1;
   a.ntiles = (a.i2 + 15) / 16;
-  a.nmb = (a.M + 15) / 16;
+  a.mbw = rows_per_group(arith, a.M);
+  a.nmb = ((a.M + 15) / 16 + a.mbw - 1) / a.mbw;
   a.nprod_half = (int)placed_workgroups(a.ntiles, a.nmb);
-  *K = 1;
   a.hom_tiles = 0;
+  a.nhom_wg = 0;
   if (a.has_hom && a.hom.jf >= a.hom.js) {  // a one-row block has nothing to march
-    *K = march_lanes(a.hom.w, a.hom.jf - a.hom.js + 1);
-    if (*K < 0) return -1;
-    a.hom_tiles = march_tiles(a.hom, *K);
+    // one lane per column (any lane width whose tile is positive gives the
+    // same values; one keeps the march's registers within the product's)
+    a.hom_tiles = march_tiles(a.hom, 1);
     if (a.hom_tiles <= 0) return -1;
+    a.nhom_wg = (int)(((long)a.hom_tiles * a.hom.nrhs + w - 1) / w);
   }
   return 0;
 }
 
-template <class KT, int W>
+template <class KT, int W, int MBW>
 static void fwd_go(const FusedFwdArgs &a, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
   const dim3 grid((unsigned)fused_fwd_workgroups(a)), block(64 * W);
-  if (ev0) hipExtLaunchKernelGGL((fused_fwd_kernel<KT, W>), grid, block, 0, st, ev0, ev1, 0, a);
-  else hipLaunchKernelGGL((fused_fwd_kernel<KT, W>), grid, block, 0, st, a);
+  if (ev0) hipExtLaunchKernelGGL((fused_fwd_kernel<KT, W, MBW>), grid, block, 0, st, ev0, ev1, 0, a);
+  else hipLaunchKernelGGL((fused_fwd_kernel<KT, W, MBW>), grid, block, 0, st, a);
 }
 
-template <class KT>
-static void fwd_waves(const FusedFwdArgs &a, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
-  if (a.waves == 4) fwd_go<KT, 4>(a, st, ev0, ev1);
-  else fwd_go<KT, 8>(a, st, ev0, ev1);
+template <class KT, int W, int MBW>
+static void bwd_go(const FusedBwdArgs &a, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+  const dim3 grid((unsigned)fused_bwd_workgroups(a)), block(64 * W);
+  if (ev0) hipExtLaunchKernelGGL((fused_bwd_kernel<KT, W, MBW>), grid, block, 0, st, ev0, ev1, 0, a);
+  else hipLaunchKernelGGL((fused_bwd_kernel<KT, W, MBW>), grid, block, 0, st, a);
+}
+
+template <class KT, int W, int MBW>
+static void go(const FusedFwdArgs &a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+  fwd_go<KT, W, MBW>(a, st, e0, e1);
+}
+template <class KT, int W, int MBW>
+static void go(const FusedBwdArgs &a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+  bwd_go<KT, W, MBW>(a, st, e0, e1);
+}
+
+// (arithmetic, waves, row tiles per workgroup) -> the compiled kernel
+template <class Args>
+static void dispatch(const Args &a, int arith, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+  const bool w16 = waves_of(a.waves) == 16, two = a.mbw == 2;
+  if (arith == FUSED_F64) {
+    if (w16) go<KF64, 16, 1>(a, st, e0, e1);
+    else go<KF64, 8, 1>(a, st, e0, e1);
+  } else if (arith == FUSED_B16) {
+    if (w16) two ? go<KB16, 16, 2>(a, st, e0, e1) : go<KB16, 16, 1>(a, st, e0, e1);
+    else two ? go<KB16, 8, 2>(a, st, e0, e1) : go<KB16, 8, 1>(a, st, e0, e1);
+  } else {
+    if (w16) two ? go<KF32, 16, 2>(a, st, e0, e1) : go<KF32, 16, 1>(a, st, e0, e1);
+    else two ? go<KF32, 8, 2>(a, st, e0, e1) : go<KF32, 8, 1>(a, st, e0, e1);
+  }
 }
 
 hipError_t launch_fused_fwd(const FusedFwdArgs &a0, int arith, hipStream_t st, hipEvent_t ev0,
                             hipEvent_t ev1) {
   FusedFwdArgs a = a0;
-  if (fused_fwd_prepare(a)) return hipErrorInvalidValue;
-  if (a.waves != 0 && a.waves != 4 && a.waves != 8) return hipErrorInvalidValue;
-  if (arith == FUSED_F64) fwd_waves<KF64>(a, st, ev0, ev1);
-  else if (arith == FUSED_B16) fwd_waves<KB16>(a, st, ev0, ev1);
-  else fwd_waves<KF32>(a, st, ev0, ev1);
+  if (fused_fwd_prepare(a, arith)) return hipErrorInvalidValue;
+  dispatch(a, arith, st, ev0, ev1);
   return hipGetLastError();
-}
-
-template <class KT, int W, int K>
-static void bwd_go(const FusedBwdArgs &a, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
-  const dim3 grid((unsigned)fused_bwd_workgroups(a)), block(64 * W);
-  if (ev0) hipExtLaunchKernelGGL((fused_bwd_kernel<KT, W, K>), grid, block, 0, st, ev0, ev1, 0, a);
-  else hipLaunchKernelGGL((fused_bwd_kernel<KT, W, K>), grid, block, 0, st, a);
-}
-
-template <class KT, int W>
-static void bwd_lanes(const FusedBwdArgs &a, int K, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
-  if (K == 1) bwd_go<KT, W, 1>(a, st, ev0, ev1);
-  else if (K == 2) bwd_go<KT, W, 2>(a, st, ev0, ev1);
-  else bwd_go<KT, W, 4>(a, st, ev0, ev1);
-}
-
-template <class KT>
-static void bwd_waves(const FusedBwdArgs &a, int K, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
-  if (a.waves == 4) bwd_lanes<KT, 4>(a, K, st, ev0, ev1);
-  else bwd_lanes<KT, 8>(a, K, st, ev0, ev1);
 }
 
 hipError_t launch_fused_bwd(const FusedBwdArgs &a0, int arith, hipStream_t st, hipEvent_t ev0,
                             hipEvent_t ev1) {
   FusedBwdArgs a = a0;
-  int K = 1;
-  if (fused_bwd_prepare(a, &K)) return hipErrorInvalidValue;
-  if (a.waves != 0 && a.waves != 4 && a.waves != 8) return hipErrorInvalidValue;
-  if (arith == FUSED_F64) bwd_waves<KF64>(a, K, st, ev0, ev1);
-  else if (arith == FUSED_B16) bwd_waves<KB16>(a, K, st, ev0, ev1);
-  else bwd_waves<KF32>(a, K, st, ev0, ev1);
+  if (fused_bwd_prepare(a, arith)) return hipErrorInvalidValue;
+  dispatch(a, arith, st, ev0, ev1);
   return hipGetLastError();
 }
 
-// FP64 column-major matrices to BF16 bits (round to nearest even via FP32).
-__global__ void cvt_bf16_kernel(const double *src, unsigned short *dst, long n) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) dst[t] = (unsigned short)bf16_rne((float)src[t]);
+// FP64 column-major (K, N) matrices to BF16 lane tiles (round to nearest
+// even via FP32; zero past K or N).
+__global__ void cvt_lane_tiles_bf16_kernel(const double *src, unsigned short *dst, int K, int N,
+                                           long nmat) {
+  const int nchk = (K + 15) / 16;
+  const long per = lane_tile_elems(N, K);
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= per * nmat) return;
+  const long mat = i / per, r = i % per;
+  const int t = (int)(r & 3), l = (int)((r >> 2) & 63);
+  const long blk = r >> 8;
+  const int ch = (int)(blk % nchk), tile = (int)(blk / nchk);
+  const int k = 16 * ch + 4 * (l >> 4) + t, n = 16 * tile + (l & 15);
+  dst[i] = (k < K && n < N) ? (unsigned short)bf16_rne((float)src[mat * K * (long)N + k + (long)n * K]) : 0;
 }
 
-hipError_t launch_cvt_bf16(const double *src, unsigned short *dst, long n, hipStream_t st) {
-  hipLaunchKernelGGL(cvt_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, n);
+hipError_t launch_cvt_lane_tiles_bf16(const double *src, unsigned short *dst, int K, int N, long nmat,
+                                     hipStream_t st) {
+  const long n = lane_tile_elems(N, K) * nmat;
+  hipLaunchKernelGGL(cvt_lane_tiles_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                     src, dst, K, N, nmat);
   return hipGetLastError();
 }
 

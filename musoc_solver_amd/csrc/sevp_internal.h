@@ -109,6 +109,7 @@ struct MarchArgs {
   // receives the same values rounded to FP32 for the mixed-precision product
   double *e;
   float *e32;
+  int e_lt;         // nonzero: e / e32 are in lane tiles (lt_index, I2 columns), else (nrhs, i2)
   int i2, j2, periodic, js, jf, ieb, resid;
   // mixed precision, split-K corrections read directly from the partials:
   //   MODE_PART: subtracted from row js (the row the previous block corrected;
@@ -205,15 +206,17 @@ long gemm_part_workgroups(const GemmPartArgs &a);  // grid size of launch_gemm_p
 // One launch per block step (sevp_fused.hip): a product whose workgroups
 // sum whole dot products (W wavefronts split K, FP64 sums of their partial
 // tiles in wavefront order) with the work that depends on it in the
-// epilogue.  T = float (FP32 MFMA, A the FP32 error rows, B FP32
-// column-major) or double (FP64 MFMA).
-constexpr int FUSED_WAVES = 8;  // the default; 4 is the other compiled choice
+// epilogue.  Both operands are in lane tiles (lt_index): A the error rows of
+// one RHS group (FP32, or FP64 for the FP64 MFMA; rows past M and columns
+// past I2 of the last tiles are finite, zero until written), B an influence
+// matrix (the split-K copies: FP32, FP64 or BF16 lane tiles).
+constexpr int FUSED_WAVES = 16;  // the default; 8 is the other compiled choice
 
 // Forward step: d = e_b RINV1_b, then block b+1's march from the corrected
 // south row X(js) - d and its closure residual e_(b+1).
 struct FusedFwdArgs {
-  const void *a;      // e_b (M, i2) row-major, T
-  const void *b;      // RINV1_b (i2, i2) column-major, T
+  const void *a;      // e_b, lane tiles of (M, i2)
+  const void *b;      // RINV1_b, lane tiles of (i2, i2)
   int M, i2, periodic;
   // block b+1: march rows js .. jf (steps = jf - js + 1), closure row ieb - 1
   int js, jf, ieb, steps;
@@ -223,11 +226,11 @@ struct FusedFwdArgs {
   long f_stride;
   double *x;           // (i2+2, j2+2) per RHS
   long x_stride;
-  double *e_out;       // e_(b+1) (M, i2), FP64 (may be null)
-  float *e32_out;      // e_(b+1) rounded to FP32 (may be null)
-  double *d_out;       // d itself (M, i2), kept when block b+1 is one row high (may be null)
-  int waves;           // wavefronts per workgroup (4 or 8; 0: FUSED_WAVES)
-  int ntiles, nmb;     // set by the launcher
+  double *e_out;       // e_(b+1), FP64 lane tiles (may be null)
+  float *e32_out;      // e_(b+1) rounded to FP32, lane tiles (may be null)
+  double *d_out;       // d itself (M, i2) row-major, kept when block b+1 is one row high (may be null)
+  int waves;           // wavefronts per workgroup (8 or 16; 0: FUSED_WAVES)
+  int ntiles, nmb, mbw;  // set by the launcher: column tiles, row-tile groups, row tiles per group
   unsigned long long *tspan;  // profiling: kernel span slot (null: off)
 };
 
@@ -235,9 +238,9 @@ struct FusedFwdArgs {
 // closure residual of block b-1 (do_resid); c and cpl stored for the
 // homogeneous march of block b, which the next backward launch runs (hom).
 struct FusedBwdArgs {
-  const void *a;       // e_b (M, i2), T
-  const void *b;       // RINV_b column-major, T
-  const void *bw;      // W_b column-major, T (null for the first block)
+  const void *a;       // e_b, lane tiles
+  const void *b;       // RINV_b, lane tiles
+  const void *bw;      // W_b, lane tiles (null for the first block)
   int M, i2, periodic;
   int js;              // JS of block b (IE(b-1), or 1)
   int do_resid;        // b > 1: closure residual of block b-1 at row js-1
@@ -247,24 +250,29 @@ struct FusedBwdArgs {
   long f_stride;
   double *x;
   long x_stride;
-  double *e_out;       // e_(b-1) (M, i2)
-  float *e32_out;
+  double *e_out;       // e_(b-1), FP64 lane tiles (may be null)
+  float *e32_out;      // FP32 lane tiles (may be null)
   // block b-1 one row high: its closure row's south neighbour is row
   // IE(b-2), which REP1 still holds corrected by d_(b-2) (restored from DUM0
   // only at block b-2's step); d_(b-2) (M, i2), else null
   const double *dsouth;
-  double *hfree_out, *hcpl_out;  // c and cpl of block b (M, i2)
+  double *hfree_out, *hcpl_out;  // c and cpl of block b (M, i2) row-major
   MarchArgs hom;       // MODE_HOM march of block b+1 (hfree / hcpl of the previous launch)
   int has_hom;
-  int waves;           // wavefronts per workgroup (4 or 8; 0: FUSED_WAVES)
-  int ntiles, nmb, nprod_half, hom_tiles;  // set by the launcher
+  int waves;           // wavefronts per workgroup (8 or 16; 0: FUSED_WAVES)
+  int ntiles, nmb, mbw, nprod_half, hom_tiles, nhom_wg;  // set by the launcher
   unsigned long long *tspan;
 };
+// Elements of the lane tiles of an (m, k) operand: 16-row by 16-column
+// blocks of 256 elements.
+__host__ __device__ inline long lane_tile_elems(int m, int k) {
+  return (long)((m + 15) / 16) * ((k + 15) / 16) * 256;
+}
 int fused_tiles_fwd(int i2, int steps);
 // Set the launcher's fields (tiles, row blocks; K: the march lane width of
 // the backward launch's homogeneous march); nonzero if the shape is refused.
-int fused_fwd_prepare(FusedFwdArgs &a);
-int fused_bwd_prepare(FusedBwdArgs &a, int *K);
+int fused_fwd_prepare(FusedFwdArgs &a, int arith);
+int fused_bwd_prepare(FusedBwdArgs &a, int arith);
 long fused_fwd_workgroups(const FusedFwdArgs &a);  // after the prepare call
 long fused_bwd_workgroups(const FusedBwdArgs &a);
 // Product arithmetic of the fused kernels: FP32 operands (FP32 MFMA), FP64
@@ -276,7 +284,9 @@ hipError_t launch_fused_fwd(const FusedFwdArgs &a, int arith, hipStream_t st,
                             hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 hipError_t launch_fused_bwd(const FusedBwdArgs &a, int arith, hipStream_t st,
                             hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
-hipError_t launch_cvt_bf16(const double *src, unsigned short *dst, long n, hipStream_t st);
+// FP64 column-major (K, N) matrices to BF16 lane tiles (round to nearest even via FP32).
+hipError_t launch_cvt_lane_tiles_bf16(const double *src, unsigned short *dst, int K, int N, long nmat,
+                                     hipStream_t st);
 
 // Step 14 of FS_GLOBAL (the barotropic correction around the solve); arrays
 // have the reference shapes, Fortran order (see include/sevp.h).

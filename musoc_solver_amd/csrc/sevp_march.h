@@ -187,8 +187,9 @@ __device__ void march_body(const MarchArgs &p, int tile, int bidx, int strip, in
       nbr<K>(vc, k, xw, xe);
       const double r = cf[k] - ca[k] * xw - cb[k] * vs[k] - cc[k] * vc[k] - cd[k] * xe - ce[k] * xnorth[k];
       if (out[k]) {
-        if (p.e) p.e[(long)bidx * p.i2 + pc[k] - 1] = r;
-        if (p.e32) p.e32[(long)bidx * p.i2 + pc[k] - 1] = (float)r;
+        const long eo = p.e_lt ? lt_index(bidx, pc[k] - 1, p.e_lt) : (long)bidx * p.i2 + pc[k] - 1;
+        if (p.e) p.e[eo] = r;
+        if (p.e32) p.e32[eo] = (float)r;
       }
     }
   }

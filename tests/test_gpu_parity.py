@@ -471,6 +471,15 @@ def test_products_only_storage(oracle, device):
         with pytest.raises(S.SevpError):
             plan.set_precision(S.PREC_FP64)
         assert plan.device_bytes < ref.device_bytes
+        if mode == "build":  # both pass kinds read the same lane-tile operands
+            plan.set_pass(S.PASS_SPLITK)
+            ref.set_pass(S.PASS_SPLITK)
+            x_s, x_d = torch.zeros_like(x_ref), torch.zeros_like(x_ref)
+            assert ref.solve_refined_dev(s_d, x_s, tol=1e-10, max_iter=8)[2]
+            assert plan.solve_refined_dev(s_d, x_d, tol=1e-10, max_iter=8)[2]
+            torch.cuda.synchronize()
+            assert torch.equal(x_d, x_s)
+            ref.set_pass(S.PASS_FUSED)
         plan.close()
     ref.close()
 
@@ -483,6 +492,7 @@ FUSED_CASES = [
     (200, 61, S.FORM_FULL, (2, 3, 1, 4), 21),
     (96, 48, S.FORM_FULL_PERIODIC, (3, 4, 2), 3),
     (64, 20, S.FORM_FULL, (4,), 16),
+    (144, 40, S.FORM_FULL_PERIODIC, (2, 1), 40),  # three 16-row tiles: two per workgroup
 ]
 
 
@@ -613,8 +623,8 @@ def test_autotune_choices_and_cache(oracle, device, tmp_path):
     print("[autotune]", flush=True)
     plan, coefs, ie, s, s_d, x_d = _device_case(i2, j2, 2, nrhs, S.PREC_FP32)
     for kind in (S.PASS_FUSED, S.PASS_SPLITK):
-        for waves in (4, 8):
-            for groups in (1, 2):
+        for waves in (8, 16):
+            for groups in (1, 3):  # 3 groups of 3, 3 and 2 RHS
                 plan.set_tuning(kind, waves, groups)
                 x_d.zero_()
                 rel, st, ok = plan.solve_refined_dev(s_d, x_d, tol=1e-10, max_iter=10)
@@ -626,7 +636,7 @@ def test_autotune_choices_and_cache(oracle, device, tmp_path):
     t = plan.tuning.as_dict()
     assert (t["pass_kernels"], t["waves"], t["rhs_groups"]) == (r["pass_kernels"], r["waves"],
                                                                r["rhs_groups"])
-    plan.set_tuning(S.PASS_SPLITK if r["pass_kernels"] == "fused" else S.PASS_FUSED, 4, 2)
+    plan.set_tuning(S.PASS_SPLITK if r["pass_kernels"] == "fused" else S.PASS_FUSED, 8, 2)
     r2 = autotune.tune(plan, s_d, x_d, cache_dir=tmp_path)
     assert r2["cached"] and plan.tuning.as_dict() == t
     plan.close()
