@@ -39,6 +39,8 @@
 // fetched from HBM once.
 #include <hip/hip_ext.h>
 
+#include <type_traits>
+
 #include "sevp_dev.h"
 #include "sevp_internal.h"
 #include "sevp_march.h"
@@ -137,22 +139,36 @@ __device__ __forceinline__ void part_tile(const typename KT::A *alt, int mb0, in
   const typename KT::A *ab[MBW];
 #pragma unroll
   for (int mb = 0; mb < MBW; ++mb) ab[mb] = alt + ((long)min(mb0 + mb, mtiles - 1) * nchk * 64 + lane) * 4;
-  for (int c0 = c_lo; c0 < c_hi; c0 += G) {
+  const int clast = max(c_hi - 1, 0);
+  auto group = [&](int c0) {
     AV av[G][MBW];
     BV bv[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-      const long ch = min(c0 + g, c_hi - 1);
+      const long ch = min(c0 + g, clast);
       bv[g] = *reinterpret_cast<const BV *>(bb + ch * 256);
 #pragma unroll
       for (int mb = 0; mb < MBW; ++mb) av[g][mb] = *reinterpret_cast<const AV *>(ab[mb] + ch * 256);
     }
+    // every load of the group is issued before the first MFMA step, which
+    // then waits only for its own chunk (the FP64 operands' group takes too
+    // many registers for that: the compiler interleaves them)
+    if constexpr (!std::is_same<KT, KF64>::value) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int g = 0; g < G; ++g)
       if (c0 + g < c_hi)
 #pragma unroll
         for (int mb = 0; mb < MBW; ++mb) acc[mb] = KT::mma(av[g][mb], bv[g], acc[mb]);
+  };
+  // the first group outside the loop: straight-line code, so its loads
+  // follow the caller's earlier loads without waiting for them (the FP64
+  // operands keep the plain loop, which needs fewer registers)
+  int c0 = c_lo;
+  if constexpr (!std::is_same<KT, KF64>::value) {
+    group(c0);
+    c0 += G;
   }
+  for (; c0 < c_hi; c0 += G) group(c0);
 }
 
 // The product tile of a workgroup: W wavefronts, each its share of the K
@@ -194,14 +210,27 @@ __device__ __forceinline__ bool tile_of(int wg, int ntiles, int nmb, int &tile, 
   return tile < ntiles;
 }
 
-__device__ __forceinline__ double rhs_at(const float *f, const double *f64, long o) {
-  return f64 ? f64[o] : (double)f[o];
+// The right-hand sides of a launch: FP32 (a solve's s) or FP64 (a
+// refinement residual), a template parameter so that their loads need no
+// branch.
+template <class FT>
+__device__ __forceinline__ const FT *rhs_ptr(const float *f, const double *f64) {
+  if constexpr (std::is_same<FT, double>::value) return f64;
+  else return f;
 }
 
 // ------------------------------------------------------------------ forward
-template <class KT, int W, int MBW>
+// The epilogue operands that do not depend on the product are loaded before
+// it, from addresses valid for every thread (threads without an output read
+// right-hand side 0), so no load is predicated and the product's loads are
+// issued without waiting for them: the X rows always, and, with 8 wavefronts
+// of FP32 or BF16 operands (16 wavefronts leave no registers for them, and
+// the FP64 operands take them), the right-hand side and coefficients of the
+// first march row and of the closure row.
+template <class KT, int W, int MBW, class FT>
 __global__ __launch_bounds__(64 * W) void fused_fwd_kernel(FusedFwdArgs p) {
   __shared__ typename KT::Acc red[W * 256 * MBW];
+  constexpr bool PF = !std::is_same<KT, KF64>::value && W == 8;
   span_begin(p.tspan);
   int tile, mbg;
   if (tile_of(blockIdx.x, p.ntiles, p.nmb, tile, mbg)) {
@@ -233,44 +262,62 @@ __global__ __launch_bounds__(64 * W) void fused_fwd_kernel(FusedFwdArgs p) {
     }
     const bool out = col >= steps && col < 16 - steps && c >= 1 && c <= i2;
     const bool act = t < 256 * MBW && m < p.M;
-    double *x = p.x + (long)(act ? m : 0) * p.x_stride;
+    const int mm = act ? m : 0;
+    double *x = p.x + (long)mm * p.x_stride;
+    const FT *f = rhs_ptr<FT>(p.f, p.f64) + (long)mm * p.f_stride;
     const int js = p.js, jf = p.jf, ieb = p.ieb;
-    // the epilogue's rows that do not depend on the product, loaded first;
+    const int jr = ieb - 1;                  // closure row
+    const int j1 = steps > 0 ? js : jr;     // first march row (a valid row either way)
+    const long co = pc - 1;
+    auto cf = [&](const float *arr, int j) -> double { return (double)arr[co + (long)(j - 1) * i2]; };
+    auto fv = [&](int j) -> double { return (double)f[co + (long)(j - 1) * i2]; };
     // the free row's neighbours come from memory (that row is not written
     // in the forward pass), so the first row operation is exact in every
     // lane and each later one loses one lane per side
-    double vs = 0.0, vc = 0.0, xnorth = 0.0, fw = 0.0, fe = 0.0;
-    if (act) {
-      vs = x[f2(i0, xc, js)];
-      vc = x[f2(i0, xc, js + 1)];
-      fw = x[f2(i0, xcw, js + 1)];
-      fe = x[f2(i0, xce, js + 1)];
-      xnorth = x[f2(i0, xc, ieb + 1)];
+    double vs = x[f2(i0, xc, js)];
+    double vc = x[f2(i0, xc, js + 1)];
+    const double fw = x[f2(i0, xcw, js + 1)];
+    const double fe = x[f2(i0, xce, js + 1)];
+    const double xnorth = x[f2(i0, xc, ieb + 1)];
+    FT f1 = 0, fr = 0;
+    float a1 = 0.f, b1 = 0.f, c1 = 0.f, d1 = 0.f, e1 = 0.f;
+    float ar = 0.f, br = 0.f, cr = 0.f, dr = 0.f, er = 0.f;
+    if constexpr (PF) {  // kept in their own precision until used
+      const long o1 = co + (long)(j1 - 1) * i2, orr = co + (long)(jr - 1) * i2;
+      f1 = f[o1];
+      a1 = p.ax[o1]; b1 = p.ay[o1]; c1 = p.bb[o1]; d1 = p.cx[o1]; e1 = p.cy[o1];
+      fr = f[orr];
+      ar = p.ax[orr]; br = p.ay[orr]; cr = p.bb[orr]; dr = p.cx[orr]; er = p.cy[orr];
     }
     const double d = product_tile<KT, W, MBW>((const typename KT::A *)p.a, MBW * mbg, (p.M + 15) >> 4,
                                               (const typename KT::B *)p.b, pc - 1, i2, red);
     if (act) {
       if (!halo) vs = vs - d;  // the south row as block b's correction leaves it
       if (p.d_out && out) p.d_out[(long)m * i2 + pc - 1] = d;
-      const long co = pc - 1;
-      const long fo = (long)m * p.f_stride + co;
-      auto cf = [&](const float *arr, int j) -> double { return (double)arr[co + (long)(j - 1) * i2]; };
       for (int j = js; j <= jf; ++j) {
         const double sw = row_shr1(vc), se = row_shl1(vc);
         const double xw = j == js ? fw : sw, xe = j == js ? fe : se;
-        double r = (rhs_at(p.f, p.f64, fo + (long)(j - 1) * i2) - cf(p.ax, j) * xw -
-                    cf(p.ay, j) * vs - cf(p.bb, j) * vc - cf(p.cx, j) * xe) / cf(p.cy, j);
+        double r;
+        if (PF && j == js)
+          r = ((double)f1 - (double)a1 * xw - (double)b1 * vs - (double)c1 * vc - (double)d1 * xe) /
+              (double)e1;
+        else
+          r = (fv(j) - cf(p.ax, j) * xw - cf(p.ay, j) * vs - cf(p.bb, j) * vc - cf(p.cx, j) * xe) /
+              cf(p.cy, j);
         if (halo) r = x[f2(i0, xc, j + 2)];
         if (out) x[f2(i0, xc, j + 2)] = r;
         vs = vc;
         vc = r;
       }
-      const int jr = ieb - 1;  // closure row
       const double sw = row_shr1(vc), se = row_shl1(vc);
       const double xw = steps == 0 ? fw : sw, xe = steps == 0 ? fe : se;
-      const double e = rhs_at(p.f, p.f64, fo + (long)(jr - 1) * i2) - cf(p.ax, jr) * xw -
-                       cf(p.ay, jr) * vs - cf(p.bb, jr) * vc - cf(p.cx, jr) * xe -
-                       cf(p.cy, jr) * xnorth;
+      double e;
+      if constexpr (PF)
+        e = (double)fr - (double)ar * xw - (double)br * vs - (double)cr * vc - (double)dr * xe -
+            (double)er * xnorth;
+      else
+        e = fv(jr) - cf(p.ax, jr) * xw - cf(p.ay, jr) * vs - cf(p.bb, jr) * vc - cf(p.cx, jr) * xe -
+            cf(p.cy, jr) * xnorth;
       if (out) {
         const long eo = lt_index(m, (int)co, nchk);
         if (p.e_out) p.e_out[eo] = e;
@@ -285,10 +332,15 @@ __global__ __launch_bounds__(64 * W) void fused_fwd_kernel(FusedFwdArgs p) {
 // Workgroups [0, nhom_wg): the homogeneous march of block b+1, one wavefront
 // per (tile, RHS), first so that the launch's longest dependent chain starts
 // at once; then the product tiles of RINV_b (with the epilogue), then, when
-// W_b is given, those of W_b (the coupling row, only stored).
-template <class KT, int W, int MBW>
+// W_b is given, those of W_b (the coupling row, only stored).  The
+// epilogue's X operands (and, where the forward kernel loads them early, its
+// right-hand side and coefficients) are loaded before the product from
+// clamped addresses (column I2 for lanes past it, row 1 for the closure row
+// of the first block, which has none).
+template <class KT, int W, int MBW, class FT>
 __global__ __launch_bounds__(64 * W) void fused_bwd_kernel(FusedBwdArgs p) {
   __shared__ typename KT::Acc red[W * 256 * MBW];
+  constexpr bool PF = !std::is_same<KT, KF64>::value && W == 8;
   span_begin(p.tspan);
   const int wg = blockIdx.x;
   int tile, mbg;
@@ -307,22 +359,29 @@ __global__ __launch_bounds__(64 * W) void fused_bwd_kernel(FusedBwdArgs p) {
       const int c = 16 * tile + 1 + (t & 15);
       const bool ok = c <= i2;
       const bool act = t < 256 * MBW && m < p.M && ok;
-      const int js = p.js;
-      double *x = p.x + (long)(act ? m : 0) * p.x_stride;
+      const int mm = act ? m : 0, cc = min(c, i2);
+      const int js = p.js, jrow = max(js - 1, 1);
+      double *x = p.x + (long)mm * p.x_stride;
       const bool resid = !cpl && p.do_resid && act;
-      // the epilogue's operands, loaded before the product
-      double xfree = 0.0, xsouth = 0.0, xw = 0.0, xcen = 0.0, xe = 0.0;
-      if (act && !cpl) xfree = x[f2(i0, c + 1, js + 1)];
-      if (resid) {  // closure row js-1 of block b-1; row js = IE(b-1)
-        xsouth = x[f2(i0, c + 1, js - 1)];
-        xcen = x[f2(i0, c + 1, js)];
-        if (p.periodic) {
-          xw = x[f2(i0, (c == 1 ? i2 : c - 1) + 1, js)];
-          xe = x[f2(i0, (c == i2 ? 1 : c + 1) + 1, js)];
-        } else {
-          xw = x[f2(i0, c, js)];
-          xe = x[f2(i0, c + 2, js)];
-        }
+      const long co = (long)(cc - 1) + (long)(jrow - 1) * i2;  // coefficient row js-1
+      const int xcw = p.periodic ? (cc == 1 ? i2 : cc - 1) + 1 : cc;
+      const int xce = p.periodic ? (cc == i2 ? 1 : cc + 1) + 1 : cc + 2;
+      const double *dsp = p.dsouth ? p.dsouth : p.hfree_out;  // any valid (M, i2) array
+      const FT *fp = rhs_ptr<FT>(p.f, p.f64) + (long)mm * p.f_stride + co;
+      const double *dp = dsp + (long)mm * i2 + cc - 1;
+      // closure row js-1 of block b-1; row js = IE(b-1)
+      const double xfree = x[f2(i0, cc + 1, js + 1)];
+      double xsouth = x[f2(i0, cc + 1, jrow)];
+      const double xcen = x[f2(i0, cc + 1, js)];
+      const double xw = x[f2(i0, xcw, js)];
+      const double xe = x[f2(i0, xce, js)];
+      FT fr = 0;
+      float ar = 0.f, br = 0.f, cr = 0.f, dr = 0.f, er = 0.f;
+      double ds = 0.0;
+      if constexpr (PF) {
+        fr = *fp;
+        ar = p.ax[co]; br = p.ay[co]; cr = p.bb[co]; dr = p.cx[co]; er = p.cy[co];
+        ds = *dp;
       }
       // lanes past I2 (last tile) read the zero-padded matrix columns
       const double v = product_tile<KT, W, MBW>((const typename KT::A *)p.a, MBW * mbg, (p.M + 15) >> 4,
@@ -336,11 +395,14 @@ __global__ __launch_bounds__(64 * W) void fused_bwd_kernel(FusedBwdArgs p) {
           const double north = xfree + v;
           x[f2(i0, c + 1, js + 1)] = north;
           if (resid) {
-            if (p.dsouth) xsouth = xsouth - p.dsouth[ro];
-            const long co = (long)(c - 1) + (long)(js - 2) * i2;  // coefficient row js-1
-            const double e = rhs_at(p.f, p.f64, (long)m * p.f_stride + co) - (double)p.ax[co] * xw -
-                             (double)p.ay[co] * xsouth - (double)p.bb[co] * xcen -
-                             (double)p.cx[co] * xe - (double)p.cy[co] * north;
+            if constexpr (!PF) {
+              fr = *fp;
+              ar = p.ax[co]; br = p.ay[co]; cr = p.bb[co]; dr = p.cx[co]; er = p.cy[co];
+              ds = *dp;
+            }
+            if (p.dsouth) xsouth = xsouth - ds;
+            const double e = (double)fr - (double)ar * xw - (double)br * xsouth - (double)cr * xcen -
+                             (double)dr * xe - (double)er * north;
             const long eo = lt_index(m, c - 1, nchk);
             if (p.e_out) p.e_out[eo] = e;
             if (p.e32_out) p.e32_out[eo] = (float)e;
@@ -403,27 +465,30 @@ int fused_bwd_prepare(FusedBwdArgs &a, int arith) {
   return 0;
 }
 
-template <class KT, int W, int MBW>
+template <class KT, int W, int MBW, class FT>
 static void fwd_go(const FusedFwdArgs &a, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
   const dim3 grid((unsigned)fused_fwd_workgroups(a)), block(64 * W);
-  if (ev0) hipExtLaunchKernelGGL((fused_fwd_kernel<KT, W, MBW>), grid, block, 0, st, ev0, ev1, 0, a);
-  else hipLaunchKernelGGL((fused_fwd_kernel<KT, W, MBW>), grid, block, 0, st, a);
+  if (ev0) hipExtLaunchKernelGGL((fused_fwd_kernel<KT, W, MBW, FT>), grid, block, 0, st, ev0, ev1, 0, a);
+  else hipLaunchKernelGGL((fused_fwd_kernel<KT, W, MBW, FT>), grid, block, 0, st, a);
 }
 
-template <class KT, int W, int MBW>
+template <class KT, int W, int MBW, class FT>
 static void bwd_go(const FusedBwdArgs &a, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
   const dim3 grid((unsigned)fused_bwd_workgroups(a)), block(64 * W);
-  if (ev0) hipExtLaunchKernelGGL((fused_bwd_kernel<KT, W, MBW>), grid, block, 0, st, ev0, ev1, 0, a);
-  else hipLaunchKernelGGL((fused_bwd_kernel<KT, W, MBW>), grid, block, 0, st, a);
+  if (ev0) hipExtLaunchKernelGGL((fused_bwd_kernel<KT, W, MBW, FT>), grid, block, 0, st, ev0, ev1, 0, a);
+  else hipLaunchKernelGGL((fused_bwd_kernel<KT, W, MBW, FT>), grid, block, 0, st, a);
 }
 
+// the right-hand sides' precision -> the kernel that loads them
 template <class KT, int W, int MBW>
 static void go(const FusedFwdArgs &a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
-  fwd_go<KT, W, MBW>(a, st, e0, e1);
+  if (a.f64) fwd_go<KT, W, MBW, double>(a, st, e0, e1);
+  else fwd_go<KT, W, MBW, float>(a, st, e0, e1);
 }
 template <class KT, int W, int MBW>
 static void go(const FusedBwdArgs &a, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
-  bwd_go<KT, W, MBW>(a, st, e0, e1);
+  if (a.f64) bwd_go<KT, W, MBW, double>(a, st, e0, e1);
+  else bwd_go<KT, W, MBW, float>(a, st, e0, e1);
 }
 
 // (arithmetic, waves, row tiles per workgroup) -> the compiled kernel
