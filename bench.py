@@ -419,6 +419,11 @@ def roofline(leg, name, nrhs, i2, a, fused=True):
         kern = {"mixed": "gemm_part_kernel", "fast64": "gemm_part64_kernel"}.get(name, "gemm_seq_kernel")
     r.update({"traffic": measured_traffic(name, m, i2, fused),
               "kernel": kern,
+              "rhs_groups": groups,
+              "durations": ("kernel's own: the profiled solve runs the RHS groups one after "
+                            "another; the timed steps run them on %d concurrent streams, so "
+                            "corr_ms_per_step may exceed ms_per_step" % groups) if groups > 1
+                           else "kernel's own (one RHS group)",
               "compute_peak": "fp32 mfma" if name == "mixed" else (
                   "fp64 vector (separate multiply and add)" if name == "plain" else "fp64 mfma"),
               "mfma_frac": mfma_frac, "hbm_frac": hbm_frac,
@@ -519,7 +524,7 @@ def tenth_leg(a):
                "device_gib": plan.device_bytes / 2**30, "storage": "fp32 products only",
                "workload": f"SEVP {i2}x{j2} (0.1 deg zonal, +-{a.lat_max:g} deg), REP1 form, "
                            f"{nrhs} RHS per call, each to {a.tol:g} rel resid"}
-        leg["roofline"] = roofline(leg, "mixed", nrhs, i2, a, a.pass_kernels == "fused")
+        leg["roofline"] = roofline(leg, "mixed", nrhs, i2, a, plan.pass_kind == S.PASS_FUSED)
         log(f"tenth: {leg}")
         return leg
     finally:

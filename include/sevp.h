@@ -344,7 +344,10 @@ int sevp_step14_dev(sevp_plan plan, const sevp_step14_desc *desc, const sevp_ste
 /* -------------------------------------------------------- measurement
  * Kernel-class timing with HIP events recorded on the plan's stream around
  * every launch of that class while enabled (used by the benchmark to measure
- * the dominant kernel's average duration over its timed region). */
+ * the dominant kernel's average duration over its timed region).  While it
+ * is enabled, refinement passes launched outside a captured graph run their
+ * RHS groups one after another on the plan's stream, so each duration is the
+ * kernel's own. */
 #define SEVP_KCLASS_CORR 0   /* influence-matrix correction products */
 #define SEVP_KCLASS_MARCH 1  /* row marches (particular and homogeneous) */
 #define SEVP_KCLASS_RESID 2  /* closure residuals and FP64 refinement residuals */

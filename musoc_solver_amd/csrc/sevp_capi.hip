@@ -1954,13 +1954,20 @@ static int ensure_refine_ctl(sevp_plan p) {
 static int run_group_passes(sevp_plan p, const FullCtx &c, int nrhs, int ngroups, int ng,
                             hipStream_t st, bool graph_groups) {
   const long ns = (long)p->d.i2 * p->d.j2, nx = (long)(p->d.i2 + 2) * (p->d.j2 + 2);
-  if (ngroups > 1) HIPC(hipEventRecord(p->gevents[0], st));
+  // a profiled eager pass runs the groups one after another on st, so each
+  // kernel's timed duration is its own and not shared with another stream's
+  // (captured and graph-replayed passes keep them concurrent)
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  HIPC(hipStreamIsCapturing(st, &cap));
+  const bool concurrent = ngroups > 1 && (graph_groups || p->prof.kclass < 0 ||
+                                          cap == hipStreamCaptureStatusActive);
+  if (concurrent) HIPC(hipEventRecord(p->gevents[0], st));
   for (int g = 0; g < ngroups; ++g) {
     const int r0 = g * ng;
     if (r0 >= nrhs) break;
     const int count = std::min(ng, nrhs - r0);
-    hipStream_t gs = g == 0 ? st : p->gstreams[g - 1];
-    if (g > 0) HIPC(hipStreamWaitEvent(gs, p->gevents[0], 0));
+    hipStream_t gs = (g == 0 || !concurrent) ? st : p->gstreams[g - 1];
+    if (g > 0 && concurrent) HIPC(hipStreamWaitEvent(gs, p->gevents[0], 0));
     PassGraph *pg = nullptr;
     if (graph_groups) CHK(get_pass_graph(p, c, nrhs, r0, count, ng, &pg));
     if (pg) {
@@ -1976,7 +1983,7 @@ static int run_group_passes(sevp_plan p, const FullCtx &c, int nrhs, int ngroups
       p->prof.st = saved;
       CHK(rc);
     }
-    if (g > 0) {
+    if (g > 0 && concurrent) {
       HIPC(hipEventRecord(p->gevents[g], gs));
       HIPC(hipStreamWaitEvent(st, p->gevents[g], 0));
     }
