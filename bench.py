@@ -77,6 +77,9 @@ def parse():
                     help="also run the 0.1 degree leg: 3600x1800, 16 RHS, FP32 MFMA corrections "
                          "refined to the tolerance, products-only plan (0: skip)")
     ap.add_argument("--tenth-only", action="store_true", help="run the 0.1 degree leg alone")
+    ap.add_argument("--tenth-j2", type=int, default=1800,
+                    help="rows of the 0.1 degree leg (fewer rows keep every kernel's shape: counter "
+                         "runs, whose per-dispatch cost would make the 900-block build too slow)")
     ap.add_argument("--autotune", type=int, default=1,
                     help="choose pass kernels, waves per fused workgroup and RHS groups of the MFMA "
                          "legs by timing on the device (musoc_solver_amd.autotune); 0: the flags")
@@ -472,7 +475,7 @@ def tenth_leg(a):
     import torch
     from musoc_solver_amd import synth
     from musoc_solver_amd import sevp as S
-    i2, j2, nrhs = 3600, 1800, 16
+    i2, j2, nrhs = 3600, a.tenth_j2, 16
     case = synth.make_case(i2, j2, a.seed + 5000, metric="latlon", lat_max=a.lat_max,
                            h_land=a.h_land)
     ie = synth.reference_ie(j2, synth.nb0_for_height(j2, 2))
