@@ -251,9 +251,12 @@ int sevp_get_pass(sevp_plan plan, int *kind);
  *   pass_kind   SEVP_PASS_FUSED or SEVP_PASS_SPLITK;
  *   waves       wavefronts per workgroup of the fused kernels (8 or 16): the
  *               share of K each wavefront sums;
- *   rhs_groups  RHS groups whose passes run concurrently (1..16).
+ *   rhs_groups  RHS groups whose passes run concurrently (1..16);
+ *   rows        16-row tiles (right-hand sides) per fused workgroup, 1 or 2
+ *               (0: two for FP32 and BF16, one for FP64).
  * The results of each choice are deterministic; the pass kernels and the
- * waves change the products' summation order (rounding-level differences).
+ * waves change the products' summation order (rounding-level differences),
+ * the row tiles and RHS groups do not.
  * sevp_autotune times refined solves of the caller's problem (s_dev, x_dev,
  * nrhs, tol, max_iter as for sevp_solve_refined_dev; x_dev is overwritten)
  * for every applicable choice, keeps the fastest and returns it with its
@@ -264,6 +267,7 @@ typedef struct {
   int pass_kind;
   int waves;
   int rhs_groups;
+  int rows;
 } sevp_tuning;
 int sevp_get_tuning(sevp_plan plan, sevp_tuning *tuning);
 int sevp_set_tuning(sevp_plan plan, const sevp_tuning *tuning);

@@ -100,7 +100,7 @@ def _cache_key(plan, nrhs):
 
 
 def tune(plan, s, x, tol=1e-10, max_iter=20, cache_dir=None):
-    """Apply the fastest (pass kernels, waves, RHS groups) for refined
+    """Apply the fastest (pass kernels, waves, RHS groups, row tiles) for refined
     solves of s (torch device tensors; x is overwritten while timing).  With
     ``cache_dir`` the choice is read from / stored in
     ``<cache_dir>/sevp_tuning.json``.  Returns the choice as a dict with the
@@ -118,7 +118,7 @@ def tune(plan, s, x, tol=1e-10, max_iter=20, cache_dir=None):
             c = table[key]
             from . import sevp as S
             plan.set_tuning(S.PASS_FUSED if c["pass_kernels"] == "fused" else S.PASS_SPLITK,
-                            c["waves"], c["rhs_groups"])
+                            c["waves"], c["rhs_groups"], c.get("rows", 0))
             return dict(c, cached=True)
     t, ms = plan.autotune(s, x, tol=tol, max_iter=max_iter)
     out = dict(t.as_dict(), ms=ms)

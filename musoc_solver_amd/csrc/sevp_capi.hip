@@ -202,7 +202,7 @@ struct KProf {
 // One captured refinement pass (zero the correction, one full-width SEVP pass
 // of the FP64 residual) for a batch size and precision.
 struct PassGraph {
-  int nrhs = 0, prec = 0, kind = 0, waves = 0;
+  int nrhs = 0, prec = 0, kind = 0, waves = 0, rows = 0;
   int r0 = 0, count = 0, ng = 0;  // the RHS group this pass covers (groups of ng)
   bool prof = false;
   hipGraphExec_t exec = nullptr;
@@ -215,7 +215,7 @@ struct PassGraph {
 // after it is set are skipped, so the result is the loop's).  The caller's s
 // and x are baked in.
 struct RefineGraph {
-  int nrhs = 0, prec = 0, kind = 0, groups = 0, passes = 0, waves = 0;
+  int nrhs = 0, prec = 0, kind = 0, groups = 0, passes = 0, waves = 0, rows = 0;
   bool prof = false;
   const float *s = nullptr;
   double *x = nullptr;
@@ -278,6 +278,7 @@ struct sevp_plan_s {
   bool has32 = false, has64 = false, has16 = false;
   int pass_kind = SEVP_PASS_SPLITK;
   int fused_waves = FUSED_WAVES;  // wavefronts per fused workgroup (8 or 16)
+  int fused_rows = 0;             // 16-row tiles per fused workgroup (1, 2; 0: by arithmetic)
   // fused pass: error rows (lane tiles, a region per RHS group, two parities;
   // finite padding, zeroed once) and c / cpl rows (nrhs, I2), two parities
   DBuf<double> fe, fh, fhc;
@@ -390,6 +391,7 @@ struct FullCtx {
   bool fused;
   int farith;  // FUSED_F32 / FUSED_F64 / FUSED_B16
   int fwaves;  // wavefronts per fused workgroup
+  int frows;   // 16-row tiles per fused workgroup (0: by arithmetic)
   const unsigned short *rinv16t, *rinv1_16t, *w16t;
   double *fe, *fh, *fhc;
   float *fe32;
@@ -646,6 +648,7 @@ static int run_fused_pass(const FullCtx &c, const float *f32, const double *f64,
     a.e32_out = dbl ? nullptr : E32[(b + 1) & 1];
     if (a.steps == 0) a.d_out = c.fd + (long)(b - 1) * c.fhalf;  // block b+1 is one row high
     a.waves = c.fwaves;
+    a.rows = c.frows;
     FusedFwdArgs sz = a;
     if (fused_fwd_prepare(sz, c.farith)) return fail(SEVP_E_ARG, "block too tall for the fused pass");
     CHK(timed([&](hipEvent_t e0, hipEvent_t e1) { return launch_fused_fwd(a, c.farith, st, e0, e1); },
@@ -669,6 +672,7 @@ static int run_fused_pass(const FullCtx &c, const float *f32, const double *f64,
     a.hfree_out = HF[b & 1];
     a.hcpl_out = HC[b & 1];
     a.waves = c.fwaves;
+    a.rows = c.frows;
     if (b != c.nb0) {  // the homogeneous march of block b+1, from the previous launch's rows
       MarchArgs m{};
       m.ax = c.ax; m.ay = c.ay; m.bb = c.bb; m.cx = c.cx; m.cy = c.cy;
@@ -1413,6 +1417,7 @@ static FullCtx full_ctx(sevp_plan p) {
   c.fused = (mixed || mfma64 || bf16) && p->pass_kind == SEVP_PASS_FUSED;
   c.farith = mfma64 ? FUSED_F64 : (bf16 ? FUSED_B16 : FUSED_F32);
   c.fwaves = p->fused_waves;
+  c.frows = p->fused_rows;
   if (bf16) {
     c.rinv16t = p->rinv16t.p;
     c.rinv1_16t = p->rinv1_16t.p;
@@ -1558,6 +1563,7 @@ int sevp_get_tuning(sevp_plan p, sevp_tuning *t) {
   t->pass_kind = p->pass_kind;
   t->waves = p->fused_waves;
   t->rhs_groups = p->rhs_groups;
+  t->rows = p->fused_rows;
   return SEVP_OK;
 }
 
@@ -1565,16 +1571,19 @@ int sevp_set_tuning(sevp_plan p, const sevp_tuning *t) {
   if (!p || !t) return fail(SEVP_E_ARG, "null argument");
   if (t->waves != 8 && t->waves != 16) return fail(SEVP_E_ARG, "waves per workgroup must be 8 or 16");
   if (t->rhs_groups < 1 || t->rhs_groups > 16) return fail(SEVP_E_ARG, "RHS groups must be 1..16");
+  if (t->rows < 0 || t->rows > 2) return fail(SEVP_E_ARG, "row tiles per workgroup must be 0, 1 or 2");
   CHK(sevp_set_pass(p, t->pass_kind));
   HIPC(hipStreamSynchronize(p->stream));
-  if (p->fused_waves != t->waves) drop_graphs(p);
+  if (p->fused_waves != t->waves || p->fused_rows != t->rows) drop_graphs(p);
   p->fused_waves = t->waves;
+  p->fused_rows = t->rows;
   p->rhs_groups = t->rhs_groups;
   return SEVP_OK;
 }
 
 // Time refined solves of the caller's problem over the pass kernels, the
-// waves per fused workgroup and the RHS groups, and keep the fastest.
+// waves and row tiles per fused workgroup and the RHS groups, and keep the
+// fastest.
 int sevp_autotune(sevp_plan p, const float *s, double *x, int nrhs, double tol, int max_iter,
                   sevp_tuning *best, double *best_ms) {
   if (!p || !s || !x) return fail(SEVP_E_ARG, "null argument");
@@ -1586,11 +1595,13 @@ int sevp_autotune(sevp_plan p, const float *s, double *x, int nrhs, double tol, 
   std::vector<sevp_tuning> cand;
   for (int kind : {SEVP_PASS_FUSED, SEVP_PASS_SPLITK}) {
     if (check_pass(p, kind, p->d.precision) != SEVP_OK) continue;
-    for (int waves : {16, 8}) {
-      if (kind == SEVP_PASS_SPLITK && waves != FUSED_WAVES) continue;  // not a split-K parameter
-      for (int g : {1, 2, 4})
-        if (g <= nrhs) cand.push_back(sevp_tuning{kind, waves, g});
-    }
+    for (int waves : {16, 8})
+      for (int rows : {1, 2}) {
+        // waves and row tiles are not split-K parameters
+        if (kind == SEVP_PASS_SPLITK && (waves != FUSED_WAVES || rows != 1)) continue;
+        for (int g : {1, 2, 4})
+          if (g <= nrhs) cand.push_back(sevp_tuning{kind, waves, g, kind == SEVP_PASS_SPLITK ? 0 : rows});
+      }
   }
   if (cand.empty()) return fail(SEVP_E_ARG, "no pass kernels apply to this plan");
   sevp_tuning keep{};
@@ -1822,7 +1833,8 @@ static int get_pass_graph(sevp_plan p, const FullCtx &c0, int nrhs, int r0, int 
   const bool prof = p->prof.kclass >= 0;
   for (auto &g : p->graphs)
     if (g.nrhs == nrhs && g.r0 == r0 && g.count == count && g.ng == ng && g.prec == p->d.precision &&
-        g.kind == p->pass_kind && g.waves == p->fused_waves && g.prof == prof) {
+        g.kind == p->pass_kind && g.waves == p->fused_waves && g.rows == p->fused_rows &&
+        g.prof == prof) {
       *out = &g;
       return SEVP_OK;
     }
@@ -1836,6 +1848,7 @@ static int get_pass_graph(sevp_plan p, const FullCtx &c0, int nrhs, int r0, int 
   g.prec = p->d.precision;
   g.kind = p->pass_kind;
   g.waves = p->fused_waves;
+  g.rows = p->fused_rows;
   g.prof = prof;
   hipStream_t cs = p->cap_stream;
   HIPC(hipStreamSynchronize(p->stream));
@@ -2010,7 +2023,7 @@ static int get_refine_graph(sevp_plan p, const FullCtx &c, double *x, int nrhs, 
   for (auto &g : p->rgraphs)
     if (g.nrhs == nrhs && g.prec == p->d.precision && g.kind == p->pass_kind &&
         g.groups == ngroups && g.passes == passes && g.prof == prof && g.x == x &&
-        g.waves == p->fused_waves) {
+        g.waves == p->fused_waves && g.rows == p->fused_rows) {
       *out = &g;
       return SEVP_OK;
     }
@@ -2029,6 +2042,7 @@ static int get_refine_graph(sevp_plan p, const FullCtx &c, double *x, int nrhs, 
   RefineGraph g;
   g.nrhs = nrhs; g.prec = p->d.precision; g.kind = p->pass_kind; g.groups = ngroups;
   g.passes = passes; g.prof = prof; g.s = s; g.x = x; g.waves = p->fused_waves;
+  g.rows = p->fused_rows;
   hipStream_t cs = p->cap_stream;
   HIPC(hipStreamSynchronize(p->stream));
   HIPC(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));

@@ -172,7 +172,8 @@ __device__ __forceinline__ void part_tile(const typename KT::A *alt, int mb0, in
 }
 
 // The product tile of a workgroup: W wavefronts, each its share of the K
-// chunks (96 / W chunks per load group: one group at K <= 1536); their
+// chunks (96 / W chunks per load group: one group at K <= 1536; half that
+// for FP64 with two row tiles); their
 // partial tiles go to LDS and thread t < 256 MBW receives the FP64 sum over
 // the wavefronts, in wavefront order, of row t / 16 (of the MBW row tiles)
 // and column t % 16 of the tile.  Every thread of the workgroup must call it.
@@ -187,7 +188,8 @@ __device__ __forceinline__ double product_tile(const typename KT::A *alt, int mb
   typename KT::CV acc[MBW];
 #pragma unroll
   for (int mb = 0; mb < MBW; ++mb) acc[mb] = typename KT::CV{0, 0, 0, 0};
-  part_tile<KT, 96 / W, MBW>(alt, mb0, mtiles, blt, n, nchk, c_lo, c_hi, acc);
+  constexpr int G = (std::is_same<KT, KF64>::value && MBW == 2) ? 48 / W : 96 / W;
+  part_tile<KT, G, MBW>(alt, mb0, mtiles, blt, n, nchk, c_lo, c_hi, acc);
 #pragma unroll
   for (int mb = 0; mb < MBW; ++mb)
 #pragma unroll
@@ -422,9 +424,15 @@ int fused_tiles_fwd(int i2, int steps) {
 
 static long placed_workgroups(int ntiles, int nmb) { return 8L * nmb * ((ntiles + 7) / 8); }
 
-// Row tiles per workgroup: two share each matrix load (FP32 and BF16), one
-// for FP64, whose operands take twice the registers.
-static int rows_per_group(int arith, int M) { return (arith != FUSED_F64 && M > 16) ? 2 : 1; }
+// Row tiles per workgroup: the requested 1 or 2 (0: two for FP32 and BF16,
+// which share each matrix load, one for FP64), one when M <= 16.  Two row
+// tiles halve the matrix reads and the workgroups; FP64 then loads its
+// chunks in groups of half the size, within the registers of 16 wavefronts.
+static int rows_per_group(int arith, int M, int rows) {
+  if (M <= 16) return 1;
+  if (rows == 1 || rows == 2) return rows;
+  return arith != FUSED_F64 ? 2 : 1;
+}
 
 long fused_fwd_workgroups(const FusedFwdArgs &a) { return placed_workgroups(a.ntiles, a.nmb); }
 
@@ -439,8 +447,9 @@ int fused_fwd_prepare(FusedFwdArgs &a, int arith) {
   if (a.steps < 0 || 16 - 2 * a.steps < 2 || a.M < 1 || a.M > 64 || a.i2 % 4 != 0) return -1;
   const int w = waves_of(a.waves);
   if (w != 8 && w != 16) return -1;
+  if (a.rows < 0 || a.rows > 2) return -1;
   a.ntiles = fused_tiles_fwd(a.i2, a.steps);
-  a.mbw = rows_per_group(arith, a.M);
+  a.mbw = rows_per_group(arith, a.M, a.rows);
   a.nmb = ((a.M + 15) / 16 + a.mbw - 1) / a.mbw;
   return 0;
 }
@@ -449,8 +458,9 @@ int fused_bwd_prepare(FusedBwdArgs &a, int arith) {
   if (a.M < 1 || a.M > 64 || a.i2 % 4 != 0) return -1;
   const int w = waves_of(a.waves);
   if (w != 8 && w != 16) return -1;
+  if (a.rows < 0 || a.rows > 2) return -1;
   a.ntiles = (a.i2 + 15) / 16;
-  a.mbw = rows_per_group(arith, a.M);
+  a.mbw = rows_per_group(arith, a.M, a.rows);
   a.nmb = ((a.M + 15) / 16 + a.mbw - 1) / a.mbw;
   a.nprod_half = (int)placed_workgroups(a.ntiles, a.nmb);
   a.hom_tiles = 0;
@@ -496,8 +506,8 @@ template <class Args>
 static void dispatch(const Args &a, int arith, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
   const bool w16 = waves_of(a.waves) == 16, two = a.mbw == 2;
   if (arith == FUSED_F64) {
-    if (w16) go<KF64, 16, 1>(a, st, e0, e1);
-    else go<KF64, 8, 1>(a, st, e0, e1);
+    if (w16) two ? go<KF64, 16, 2>(a, st, e0, e1) : go<KF64, 16, 1>(a, st, e0, e1);
+    else two ? go<KF64, 8, 2>(a, st, e0, e1) : go<KF64, 8, 1>(a, st, e0, e1);
   } else if (arith == FUSED_B16) {
     if (w16) two ? go<KB16, 16, 2>(a, st, e0, e1) : go<KB16, 16, 1>(a, st, e0, e1);
     else two ? go<KB16, 8, 2>(a, st, e0, e1) : go<KB16, 8, 1>(a, st, e0, e1);

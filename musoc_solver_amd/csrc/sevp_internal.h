@@ -230,6 +230,7 @@ struct FusedFwdArgs {
   float *e32_out;      // e_(b+1) rounded to FP32, lane tiles (may be null)
   double *d_out;       // d itself (M, i2) row-major, kept when block b+1 is one row high (may be null)
   int waves;           // wavefronts per workgroup (8 or 16; 0: FUSED_WAVES)
+  int rows;            // 16-row tiles per workgroup (1 or 2; 0: by arithmetic)
   int ntiles, nmb, mbw;  // set by the launcher: column tiles, row-tile groups, row tiles per group
   unsigned long long *tspan;  // profiling: kernel span slot (null: off)
 };
@@ -260,6 +261,7 @@ struct FusedBwdArgs {
   MarchArgs hom;       // MODE_HOM march of block b+1 (hfree / hcpl of the previous launch)
   int has_hom;
   int waves;           // wavefronts per workgroup (8 or 16; 0: FUSED_WAVES)
+  int rows;            // 16-row tiles per workgroup (1 or 2; 0: by arithmetic)
   int ntiles, nmb, mbw, nprod_half, hom_tiles, nhom_wg;  // set by the launcher
   unsigned long long *tspan;
 };

@@ -60,11 +60,12 @@ class _Desc(ctypes.Structure):
 
 
 class SevpTuning(ctypes.Structure):
-    _fields_ = [("pass_kind", ctypes.c_int), ("waves", ctypes.c_int), ("rhs_groups", ctypes.c_int)]
+    _fields_ = [("pass_kind", ctypes.c_int), ("waves", ctypes.c_int), ("rhs_groups", ctypes.c_int),
+                ("rows", ctypes.c_int)]
 
     def as_dict(self):
         return {"pass_kernels": "fused" if self.pass_kind == PASS_FUSED else "splitk",
-                "waves": self.waves, "rhs_groups": self.rhs_groups}
+                "waves": self.waves, "rhs_groups": self.rhs_groups, "rows": self.rows}
 
 
 class SevpStats(ctypes.Structure):
@@ -317,14 +318,15 @@ class SevpPlan:
         _check(lib().sevp_get_tuning(self._h, ctypes.byref(t)))
         return t
 
-    def set_tuning(self, pass_kind, waves, rhs_groups):
-        t = SevpTuning(pass_kind, waves, rhs_groups)
+    def set_tuning(self, pass_kind, waves, rhs_groups, rows=0):
+        t = SevpTuning(pass_kind, waves, rhs_groups, rows)
         _check(lib().sevp_set_tuning(self._h, ctypes.byref(t)))
 
     def autotune(self, s, x, tol=1e-10, max_iter=20, nrhs=None):
         """Time refined solves of (s, x) (torch device tensors; x is
-        overwritten) over the pass kernels, waves per fused workgroup and RHS
-        groups, keep the fastest; returns (SevpTuning, ms per solve call)."""
+        overwritten) over the pass kernels, waves and row tiles per fused
+        workgroup and RHS groups, keep the fastest; returns (SevpTuning, ms per
+        solve call)."""
         nrhs = nrhs or s.shape[0]
         self._follow_torch_stream()
         t = SevpTuning()

@@ -614,28 +614,39 @@ def test_bf16_trial_contraction(oracle, device, i2, j2):
 
 
 def test_autotune_choices_and_cache(oracle, device, tmp_path):
-    """The timing autotuner tries the pass kernels, waves per fused
-    workgroup and RHS groups, keeps the fastest, and every choice refines to
-    the tolerance; the cached choice is applied without timing."""
+    """The timing autotuner tries the pass kernels, waves and row tiles per
+    fused workgroup and RHS groups, keeps the fastest, and every choice
+    refines to the tolerance (row tiles and RHS groups bitwise alike); the
+    cached choice is applied without timing."""
     import torch
     from musoc_solver_amd import autotune
-    i2, j2, nrhs = 192, 96, 8
+    i2, j2, nrhs = 192, 96, 40
     print("[autotune]", flush=True)
     plan, coefs, ie, s, s_d, x_d = _device_case(i2, j2, 2, nrhs, S.PREC_FP32)
-    for kind in (S.PASS_FUSED, S.PASS_SPLITK):
-        for waves in (8, 16):
-            for groups in (1, 3):  # 3 groups of 3, 3 and 2 RHS
-                plan.set_tuning(kind, waves, groups)
-                x_d.zero_()
-                rel, st, ok = plan.solve_refined_dev(s_d, x_d, tol=1e-10, max_iter=10)
-                torch.cuda.synchronize()
-                assert ok and rel.max() <= 1e-10, (kind, waves, groups)
+    for prec in (S.PREC_FP32, S.PREC_FP64_MFMA):
+        plan.set_precision(prec)
+        for kind in (S.PASS_FUSED, S.PASS_SPLITK):
+            for waves in (8, 16):
+                first = None
+                for groups, rows in ((1, 1), (1, 2), (3, 2), (3, 1)):  # 3 groups: 14, 14, 12 RHS
+                    if kind == S.PASS_SPLITK and rows == 2:
+                        continue
+                    plan.set_tuning(kind, waves, groups, rows)
+                    x_d.zero_()
+                    rel, st, ok = plan.solve_refined_dev(s_d, x_d, tol=1e-10, max_iter=10)
+                    torch.cuda.synchronize()
+                    assert ok and rel.max() <= 1e-10, (prec, kind, waves, groups, rows)
+                    if first is None:
+                        first = x_d.clone()
+                    else:
+                        assert torch.equal(x_d, first), (prec, kind, waves, groups, rows)
+    plan.set_precision(S.PREC_FP32)
     r = autotune.tune(plan, s_d, x_d, cache_dir=tmp_path)
     print(f"[autotune] {r}", flush=True)
     assert not r["cached"] and r["ms"] > 0
     t = plan.tuning.as_dict()
-    assert (t["pass_kernels"], t["waves"], t["rhs_groups"]) == (r["pass_kernels"], r["waves"],
-                                                               r["rhs_groups"])
+    assert (t["pass_kernels"], t["waves"], t["rhs_groups"], t["rows"]) == (
+        r["pass_kernels"], r["waves"], r["rhs_groups"], r["rows"])
     plan.set_tuning(S.PASS_SPLITK if r["pass_kernels"] == "fused" else S.PASS_FUSED, 8, 2)
     r2 = autotune.tune(plan, s_d, x_d, cache_dir=tmp_path)
     assert r2["cached"] and plan.tuning.as_dict() == t
