@@ -2355,10 +2355,10 @@ static int batch_build(sevp_batch b, const int *nrhs) {
       if (t == nb0) continue;
       if (dbl) {
         g64.push_back(ops[i].fprod64[t - 1]);
-        gw.push_back(gemm_part64_workgroups(g64.back()));
+        gw.push_back(batch_gemm_part64_workgroups(g64.back()));
       } else {
         g32.push_back(ops[i].fprod[t - 1]);
-        gw.push_back(gemm_part_workgroups(g32.back()));
+        gw.push_back(batch_gemm_part_workgroups(g32.back()));
       }
     }
     add_launch(img, b->launches, BL_MARCH, me, mw);
@@ -2386,10 +2386,10 @@ static int batch_build(sevp_batch b, const int *nrhs) {
       const int nb = nb0 - t + 1;
       if (dbl) {
         g64.push_back(ops[i].bprod64[nb - 1]);
-        gw.push_back(gemm_part64_workgroups(g64.back()));
+        gw.push_back(batch_gemm_part64_workgroups(g64.back()));
       } else {
         g32.push_back(ops[i].bprod[nb - 1]);
-        gw.push_back(gemm_part_workgroups(g32.back()));
+        gw.push_back(batch_gemm_part_workgroups(g32.back()));
       }
       HomResidEnt e;
       const long n = hom_resid_prepare(ops[i].bmarch[nb - 1], &ops[i].bresid[nb - 1], b->kmarch, e);

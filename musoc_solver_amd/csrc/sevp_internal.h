@@ -202,6 +202,9 @@ int gemm_part_slices(int N, int K, int *kslice);
 hipError_t launch_gemm_part(const GemmPartArgs &a, hipStream_t st, hipEvent_t ev0 = nullptr,
                             hipEvent_t ev1 = nullptr);
 long gemm_part_workgroups(const GemmPartArgs &a);  // grid size of launch_gemm_part
+// workgroups of one entry of a grouped product launch
+long batch_gemm_part_workgroups(const GemmPartArgs &a);
+long batch_gemm_part64_workgroups(const GemmPart64Args &a);
 
 // One launch per block step (sevp_fused.hip): a product whose workgroups
 // sum whole dot products (W wavefronts split K, FP64 sums of their partial

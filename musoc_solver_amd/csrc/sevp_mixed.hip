@@ -133,9 +133,29 @@ __device__ __forceinline__ void gemm_part_body(const GemmPartArgs &p, int bx, in
   span_end(p.tspan);
 }
 
+// XCD-aware placement of a product's (column group, slice, row group)
+// tiles over a 1-D grid: the dispatcher deals workgroups to the 8 XCDs in
+// turn, and the row groups of one (column group, slice) get ids that land on
+// the same XCD one after another, so its matrix tile comes from HBM once and
+// from that XCD's L2 for the other row groups.  Ids past the last tile only
+// record their span.
+__device__ __forceinline__ bool gp_tile_of(int wg, int ncs, int rg, int &cs, int &z) {
+  const int xcd = wg & 7, slot = wg >> 3;
+  cs = (slot / rg) * 8 + xcd;
+  z = slot % rg;
+  return cs < ncs;
+}
+
 template <int KS, int MB>
-__global__ __launch_bounds__(64 * GP_WAVES) void gemm_part_kernel(GemmPartArgs p) {
-  gemm_part_body<KS, MB>(p, blockIdx.x, blockIdx.y, blockIdx.z * 16 * MB);
+__global__ __launch_bounds__(64 * GP_WAVES) void gemm_part_kernel(GemmPartArgs p, int groups, int ncs,
+                                                                int rg) {
+  int cs, z;
+  if (gp_tile_of(blockIdx.x, ncs, rg, cs, z)) {
+    gemm_part_body<KS, MB>(p, cs % groups, cs / groups, z * 16 * MB);
+  } else {
+    span_begin(p.tspan);
+    span_end(p.tspan);
+  }
 }
 
 // Slice lengths with a compiled kernel, and the one used for a product: the
@@ -162,12 +182,17 @@ int gemm_part_slices(int N, int K, int *kslice) {
   return (K + ks - 1) / ks;
 }
 
+// Workgroups of the 1-D placement: 8 XCDs x row groups x ceil(tiles / 8).
+static long gp_placed(long ncs, int rg) { return 8L * rg * ((ncs + 7) / 8); }
+
 template <int KS, int MB>
 static void gp_launch(const GemmPartArgs &a, dim3 grid, hipStream_t st, hipEvent_t ev0,
                       hipEvent_t ev1) {
   const dim3 block(64 * GP_WAVES);
-  if (ev0) hipExtLaunchKernelGGL(gemm_part_kernel<KS, MB>, grid, block, 0, st, ev0, ev1, 0, a);
-  else hipLaunchKernelGGL((gemm_part_kernel<KS, MB>), grid, block, 0, st, a);
+  const int groups = (int)grid.x, ncs = (int)(grid.x * grid.y), rg = (int)grid.z;
+  const dim3 g1((unsigned)gp_placed(ncs, rg));
+  if (ev0) hipExtLaunchKernelGGL(gemm_part_kernel<KS, MB>, g1, block, 0, st, ev0, ev1, 0, a, groups, ncs, rg);
+  else hipLaunchKernelGGL((gemm_part_kernel<KS, MB>), g1, block, 0, st, a, groups, ncs, rg);
 }
 
 // Right-hand sides per workgroup: up to 32 (two 16-row MFMA blocks); more
@@ -184,9 +209,20 @@ static void gp_launch_mb(const GemmPartArgs &a, dim3 grid, hipStream_t st, hipEv
   else gp_launch<KS, 2>(a, grid, st, ev0, ev1);
 }
 
-long gemm_part_workgroups(const GemmPartArgs &a) {
+// Workgroups of one entry of a grouped launch (batch_gemm_part_kernel's own
+// dense numbering, no placement).
+long batch_gemm_part_workgroups(const GemmPartArgs &a) {
   return (long)((a.N + 16 * GP_WAVES - 1) / (16 * GP_WAVES)) * gemm_part_slices(a.N, a.K, nullptr) *
          gp_row_groups(a.M);
+}
+long batch_gemm_part64_workgroups(const GemmPart64Args &a) {
+  return (long)((a.N + 16 * GP_WAVES - 1) / (16 * GP_WAVES)) * gemm_part_slices(a.N, a.K, nullptr) *
+         gp_row_groups(a.M);
+}
+
+long gemm_part_workgroups(const GemmPartArgs &a) {
+  return gp_placed((long)((a.N + 16 * GP_WAVES - 1) / (16 * GP_WAVES)) * gemm_part_slices(a.N, a.K, nullptr),
+                   gp_row_groups(a.M));
 }
 
 hipError_t launch_gemm_part(const GemmPartArgs &a, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
@@ -299,16 +335,25 @@ __device__ __forceinline__ void gemm_part64_body(const GemmPart64Args &p, int bx
 }
 
 template <int KS, int MB>
-__global__ __launch_bounds__(64 * GP_WAVES) void gemm_part64_kernel(GemmPart64Args p) {
-  gemm_part64_body<KS, MB>(p, blockIdx.x, blockIdx.y, blockIdx.z * 16 * MB);
+__global__ __launch_bounds__(64 * GP_WAVES) void gemm_part64_kernel(GemmPart64Args p, int groups,
+                                                                  int ncs, int rg) {
+  int cs, z;
+  if (gp_tile_of(blockIdx.x, ncs, rg, cs, z)) {
+    gemm_part64_body<KS, MB>(p, cs % groups, cs / groups, z * 16 * MB);
+  } else {
+    span_begin(p.tspan);
+    span_end(p.tspan);
+  }
 }
 
 template <int KS, int MB>
 static void gp64_launch(const GemmPart64Args &a, dim3 grid, hipStream_t st, hipEvent_t ev0,
                         hipEvent_t ev1) {
   const dim3 block(64 * GP_WAVES);
-  if (ev0) hipExtLaunchKernelGGL(gemm_part64_kernel<KS, MB>, grid, block, 0, st, ev0, ev1, 0, a);
-  else hipLaunchKernelGGL((gemm_part64_kernel<KS, MB>), grid, block, 0, st, a);
+  const int groups = (int)grid.x, ncs = (int)(grid.x * grid.y), rg = (int)grid.z;
+  const dim3 g1((unsigned)gp_placed(ncs, rg));
+  if (ev0) hipExtLaunchKernelGGL(gemm_part64_kernel<KS, MB>, g1, block, 0, st, ev0, ev1, 0, a, groups, ncs, rg);
+  else hipLaunchKernelGGL((gemm_part64_kernel<KS, MB>), g1, block, 0, st, a, groups, ncs, rg);
 }
 
 template <int KS>
@@ -319,8 +364,8 @@ static void gp64_launch_mb(const GemmPart64Args &a, dim3 grid, hipStream_t st, h
 }
 
 long gemm_part64_workgroups(const GemmPart64Args &a) {
-  return (long)((a.N + 16 * GP_WAVES - 1) / (16 * GP_WAVES)) * gemm_part_slices(a.N, a.K, nullptr) *
-         gp_row_groups(a.M);
+  return gp_placed((long)((a.N + 16 * GP_WAVES - 1) / (16 * GP_WAVES)) * gemm_part_slices(a.N, a.K, nullptr),
+                   gp_row_groups(a.M));
 }
 
 hipError_t launch_gemm_part64(const GemmPart64Args &a, hipStream_t st, hipEvent_t ev0,
