@@ -73,6 +73,10 @@ def parse():
     ap.add_argument("--pass-kernels", choices=("fused", "splitk"), default="fused",
                     help="pass kernels of the MFMA legs: one launch per block step (fused) or "
                          "the split-K products with separate marches (splitk)")
+    ap.add_argument("--fused-waves", type=int, default=0,
+                    help="wavefronts per fused workgroup without the autotuner (8 or 16; 0: default)")
+    ap.add_argument("--fused-rows", type=int, default=0,
+                    help="16-row tiles per fused workgroup without the autotuner (1 or 2; 0: default)")
     ap.add_argument("--tenth", type=int, default=1,
                     help="also run the 0.1 degree leg: 3600x1800, 16 RHS, FP32 MFMA corrections "
                          "refined to the tolerance, products-only plan (0: skip)")
@@ -245,6 +249,9 @@ def main():
         torch.cuda.synchronize()
         setup[name] = setup.get(name, 0.0) + time.perf_counter() - t0
         plan.set_rhs_groups(plan_groups(name))
+        if name != "plain" and (a.fused_waves or a.fused_rows):
+            t = plan.tuning
+            plan.set_tuning(t.pass_kind, a.fused_waves or t.waves, t.rhs_groups, a.fused_rows)
         tuning = None
         if a.autotune and name != "plain":
             from musoc_solver_amd import autotune
