@@ -558,7 +558,8 @@ def sequence_leg(a):
     each solve starts from the previous step's solution and is refined to the
     tolerance with the REP1 form (block height 2), as a model's time loop
     calls the solver.  Reports milliseconds per solve for each correction
-    arithmetic on one plan."""
+    arithmetic on one plan (the MFMA ones autotuned once on the first
+    forcing)."""
     import torch
     from musoc_solver_amd import synth
     from musoc_solver_amd import sevp as S
@@ -576,6 +577,11 @@ def sequence_leg(a):
                            ("mixed", S.PREC_FP32)):
             plan.set_precision(prec)
             x = torch.zeros((1, j2 + 2, i2 + 2), dtype=torch.float64, device="cuda")
+            tuning = None
+            if a.autotune and name != "plain":  # once per plan, as a model's setup would
+                from musoc_solver_amd import autotune
+                tuning = autotune.tune(plan, sd[0], x, tol=a.tol, max_iter=a.max_iter)
+                x.zero_()
             plan.solve_refined_dev(sd[0], x, tol=a.tol, max_iter=a.max_iter)  # warm-up
             x.zero_()
             torch.cuda.synchronize()
@@ -589,6 +595,8 @@ def sequence_leg(a):
             dt = (time.perf_counter() - t0) / n
             out[name] = {"ms_per_solve": 1e3 * dt, "passes_mean": float(np.mean(passes)),
                          "max_rel_resid": worst}
+            if tuning:
+                out[name]["tuning"] = tuning
     log(f"sequence: {out}")
     return out
 
