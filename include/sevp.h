@@ -58,7 +58,7 @@ typedef struct sevp_plan_s *sevp_plan;
 
 typedef struct {
   int i2, j2;      /* interior grid size (I2 x J2) */
-  int nb0;         /* number of block rows NB0 */
+  int nb0;         /* number of block rows NB0 (at most 4096 for SEVP_FORM_BIR) */
   const int *ie;   /* IE(1..NB0), host pointer, copied */
   int form;        /* SEVP_FORM_* */
   int nbir;        /* number of BIR strips (SEVP_FORM_BIR); IBIR = 2*I2/NBIR */
@@ -162,7 +162,7 @@ int sevp_residual_dev(sevp_plan plan, const float *s_dev, const double *x_dev, i
  * H) are accepted for signature compatibility; the device keeps its own
  * scratch and does not write them. */
 
-/* REPBIR, MAIN/template/GLOBAL/src/sevp.f90:218-328. */
+/* REPBIR, MAIN/template/GLOBAL/src/sevp.f90:218-328.  2 <= M2 <= 64, NBLK <= 4096. */
 int sevp_repbir(const float *ax, const float *ay, const float *bb, const float *cx,
                 const float *cy, const double *rinv, const double *rinv1, double *dum0,
                 double *dum1, double *dum2, const float *f, double *h, double *x, const int *ie,

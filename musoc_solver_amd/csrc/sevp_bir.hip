@@ -1,20 +1,44 @@
 // Plain FP64 strip SEVP (REPBIR) and the BIR multiplicative strip sweeps.
 //
-// One wavefront owns one right-hand side and runs the whole chain the
+// One workgroup owns one right-hand side and runs the whole chain the
 // reference runs on one CPU core: nsweep sweeps x NBIR overlapping strips x
 // (forward + backward block march).  The chain is a true data dependency
 // (each strip reads the halo columns and interior guess written by the one
 // before it, and each block's march starts from the corrected last row of the
 // block below), so the only parallelism inside one solve is the strip's
-// columns: lane l marches strip column l+1.  Independent right-hand sides run
-// as independent wavefronts.
+// columns: lane l of the marching wavefront marches strip column l+1.
+// Independent right-hand sides run as independent workgroups.
 //
-// The march keeps the two live rows in registers and exchanges the stencil's
-// east/west neighbours with DPP wave shifts.  The influence-matrix columns a
-// lane needs (column n of RINV/RINV1 for lane n) are loaded into registers at
-// the start of the block, so their latency hides behind the march.  The
-// reference's scratch arrays H, DUM1 and DUM2 live in registers; DUM0 (the
-// saved last rows) is per-RHS global scratch.
+// A row step is a dependent FP64 chain (two wave shifts, four products and
+// the ordered differences, a correctly rounded division), so the workgroup's
+// two wavefronts split the work to keep memory off that chain:
+//  * wave 0, the marcher, reads every row operand from LDS, keeps the live
+//    rows in registers, exchanges the stencil's east/west neighbours with DPP
+//    wave shifts and writes its results to LDS.  Its only global accesses are
+//    the influence-matrix columns (column n of RINV/RINV1 for lane n), loaded
+//    into registers a whole block before the closure that uses them.
+//  * wave 1, the stager, streams each pass's operands into LDS with
+//    global_load_lds DMAs (no registers, no waits while they fly) and stores
+//    the march's results back to X and DUM0.
+// A pass is a sequence of records, one per march row or closure, in the order
+// the marcher consumes them.  Records travel in chunks of R through three
+// input buffers: while the marcher works on chunk c the stager fills chunk
+// c+1 and drains chunk c-1, and one workgroup barrier ends each chunk, on
+// both wavefronts, so the two agree on the barrier count by construction
+// (1 + ceil(records / R) per pass).
+//
+// A record is 8 slots of WMAX dwords; lane l's value of slot s is dword
+// s*WMAX + l.  A double per lane takes two slots: low words, then high words.
+//   forward header       slots 0-1 X(., 1), 2-3 X(., 2)
+//   march row, closure   slots 0-5 AX, AY, BB, CX, CY, F of the row (REAL(4));
+//                        slots 6-7 the halo of a forward march row (lane 0
+//                        X(1, J+1), lane W-1 X(W+2, J+1)), the forward
+//                        closure's north row X(., IE+1), the backward
+//                        closure's X(., JR) or the backward march row's old
+//                        X(., J+2)
+//   backward closure B   slots 0-1 the restored row X(., IE), 2-3 X(., JS+1),
+//                        4 CY(., JS-1), 6-7 X(., IE+1) of the top block
+// plus 4 dwords of halo for the closure records (X(1, row) and X(W+2, row)).
 //
 // Arithmetic follows MAIN/template/GLOBAL/src/sevp.f90:218-328 operation for
 // operation (see sevp_dev.h), so results are bitwise equal to the reference.
@@ -23,262 +47,593 @@
 
 namespace sevp {
 
-struct StripCtx {
-  // Coefficients: value for lane's column at row j is base[col + (j-1)*ld].
-  const float *ax, *ay, *bb, *cx, *cy;
-  long ccol;
-  int cld;
-  // Right-hand side F, same convention.
-  const float *f;
-  long fcol;
-  int fld;
-  // Strip view of X: X(i,j) = xv[(i-1) + (j-1)*ldx], i = 1..w+2.
+// Stager wavefronts per workgroup: a single wavefront's stream of DMAs and
+// stores is bound by how many vector memory operations it may have in flight,
+// so the records of each chunk are dealt round-robin to NSTG stagers.
+constexpr int NSTG = 3;
+constexpr int BIR_THREADS = 64 * (1 + NSTG);
+
+template <int WMAX>
+struct alignas(16) Stage {
+  static constexpr int R = WMAX == 32 ? 32 : 16;  // records per chunk
+  static constexpr int NIN = 3;                     // input buffers
+  static constexpr int REC = 8 * WMAX;              // dwords per record
+  static constexpr int NF = 6 * WMAX / 64;          // DMAs of slots 0-5
+  static constexpr int ND = 2 * WMAX / 64;          // DMAs of one double per lane
+  uint32_t in[NIN][R][REC];
+  uint32_t halo[NIN][R][4];
+  double out[2][R][2][WMAX];  // up to two results per record
+  double e[WMAX];             // the marcher's residual row, read back as broadcasts
+  int desc[4][R][4];          // record-table entries of four chunks
+};
+
+// One strip: strip column c (0-based) of the coefficient arrays is physical
+// column col(c); X(i, j) = xv[(i-1) + (j-1)*ldx] with the halo columns at
+// i = 1 and i = w+2.
+struct Strip {
+  const float *arr[6];  // AX, AY, BB, CX, CY, F
+  int ld[6];
+  int il, inc, seam;
   double *xv;
   int ldx;
-  // Influence matrices of this strip: RINV(w,w,nblk), RINV1(w,w,nblk-1).
   const double *rinv, *rinv1;
   double *dum0;  // (w, nblk-1)
   const int *ie;
-  int nblk;
-  int w;
+  int nblk, w;
+  __device__ int col(int c) const {
+    c = c < w ? c : 0;
+    return (seam && c >= inc) ? c - inc : il - 1 + c;
+  }
+  __device__ double *xrow(int j) const { return xv + 1 + (long)(j - 1) * ldx; }  // &X(2, j)
 };
+
+// IE(k+1) from the LDS copy, as a wave-uniform (scalar) value.
+__device__ __forceinline__ int ie_at(const int *ie, int k) { return __builtin_amdgcn_readfirstlane(ie[k]); }
+
+__device__ __forceinline__ int fwd_jlast(const int *ie, int nblk) {
+  const int ieb = ie_at(ie, nblk - 1), js = nblk > 1 ? ie_at(ie, nblk - 2) : 1;
+  return ieb - 2 >= js ? ieb - 2 : js - 1;
+}
+
+// Chunk hand-over.  The marcher's outstanding global loads are influence
+// columns for a later closure, so its barrier waits for LDS only; the
+// stager's also waits for its DMAs and stores.
+__device__ __forceinline__ void bar_lds() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+__device__ __forceinline__ void bar_all() {
+  asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+__device__ __forceinline__ void wait_vm() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// Lane t's dword of a 64-dword run at dst (wave-uniform) comes from src.
+__device__ __forceinline__ void dma(const void *src, uint32_t *dst) {
+  __builtin_amdgcn_global_load_lds((__attribute__((address_space(1))) void *)(src),
+                                   (__attribute__((address_space(3))) void *)(dst), 4, 0, 0);
+}
 
 template <int WMAX>
 __device__ __forceinline__ void load_col(double (&r)[WMAX], const double *mat, int w, int n) {
+  const double *c = mat + (long)n * w;
 #pragma unroll
-  for (int m = 0; m < WMAX; ++m) r[m] = (m < w) ? mat[m + (long)n * w] : 0.0;
+  for (int m = 0; m < WMAX; ++m) r[m] = c[m < w ? m : w - 1];
 }
 
 // sum_{m=1..w} e(m) * R(m, n), sequential from zero (lane n holds column n).
+// e goes through LDS (eb, the marcher's own buffer) and comes back two
+// entries per broadcast read.
 template <int WMAX>
-__device__ __forceinline__ double dot_col(double e, const double (&r)[WMAX], int w) {
+__device__ __forceinline__ double dot_col(double e, const double (&r)[WMAX], int w, double *eb, int lane) {
+  eb[lane] = e;
   double s = 0.0;
 #pragma unroll
-  for (int m = 0; m < WMAX; ++m)
-    if (m < w) s = s + read_lane(e, m) * r[m];
+  for (int m = 0; m < WMAX; m += 2) {
+    if (m < w) {
+      const double2 em = *reinterpret_cast<const double2 *>(eb + m);
+      s = s + em.x * r[m];
+      if (m + 1 < w) s = s + em.y * r[m + 1];
+    }
+  }
   return s;
 }
 
-#define LD_C(p, j) ((double)(p)[c.ccol + (long)((j) - 1) * c.cld])
-#define LD_F(j) ((double)c.f[c.fcol + (long)((j) - 1) * c.fld])
-#define XV(i, j) c.xv[(long)((i) - 1) + (long)((j) - 1) * c.ldx]
-#define LDC32(p, j) ((p)[c.ccol + (long)((j) - 1) * c.cld])
+// ------------------------------------------------------------ record order
+// One record of a pass (an entry of bir_record_table): its kind, block nb,
+// row j and the block's IE and JS, all wave-uniform.
+struct Cursor {
+  int kind, j, nb, js, ieb;
+};
 
-// REPBIR on one strip for the calling wavefront.
-//
-// Each march reads its row operands from a ring of D rows loaded D rows
-// ahead, so no row waits for a load issued in the row before it.  Measured
-// at 720x360 with 24-row blocks, rings of 2, 4 and 8 rows all take about
-// 0.35 us per row step (a one-row prefetch took 0.43 us): the row's own
-// dependent chain (wave shifts, four products and differences, the
-// correctly rounded FP64 division) sets the pace; with 2-row blocks the
-// shallower rings are faster, and D = 4 is used throughout.  The forward pass
-// consumes rows 1 .. jlast strictly in order (each a march row of its block
-// or the block's closure row), so one ring runs across the block boundaries;
-// the next block's RINV1 columns and first guess row are requested at each
-// closure, a whole block before they are needed.  The backward H march fills
-// a ring per block, after the block's restore of row IE (which its last row
-// reads: same lane, so program order makes the store visible).
-template <int WMAX, int D>
-__device__ __forceinline__ void repbir_wave(const StripCtx &c) {
-  const int lane = threadIdx.x;
-  const int w = c.w;
-  const bool act = lane < w;
-  const int li = act ? lane + 2 : 2;  // this lane's X column (1-based, strip view)
-  const int wi = w + 2;               // east halo column
-  const int n = act ? lane : 0;
-  const int nblk = c.nblk;
+// ------------------------------------------------------------------ stager
+template <int WMAX>
+struct StagerLane {
+  const float *fp[Stage<WMAX>::NF];  // this lane's source at row 1, per slot DMA
+  int fld[Stage<WMAX>::NF];
+  int doff[Stage<WMAX>::ND];  // byte offset of this lane's dword in a row of doubles
+  int hoff[Stage<WMAX>::ND];  // the same for the halo pair: X(1, .) or, at lane w-1, X(w+2, .)
+  __device__ void init(const Strip &s, int t) {
+#pragma unroll
+    for (int d = 0; d < Stage<WMAX>::NF; ++d) {
+      const int u = 64 * d + t, slot = u / WMAX, c = u % WMAX;
+      const float *a = s.arr[0];
+      int l = s.ld[0];
+#pragma unroll
+      for (int k = 1; k < 6; ++k)
+        if (slot == k) { a = s.arr[k]; l = s.ld[k]; }
+      fp[d] = a + s.col(c);
+      fld[d] = l;
+    }
+#pragma unroll
+    for (int e = 0; e < Stage<WMAX>::ND; ++e) {
+      const int u = 64 * e + t, half = u / WMAX, k = min(u % WMAX, s.w - 1);
+      doff[e] = (2 * k + half) * 4;
+      hoff[e] = (k == s.w - 1 ? (s.w + 1) * 8 : 0) + half * 4;
+    }
+  }
+};
+
+template <int WMAX>
+__device__ __forceinline__ void fill_record(Stage<WMAX> &S, const Strip &s, const StagerLane<WMAX> &L,
+                                            int b, int p, const Cursor &k, int t) {
+  using G = Stage<WMAX>;
+  uint32_t *rec = S.in[b][p];
+  auto frows = [&](int j) {
+#pragma unroll
+    for (int d = 0; d < G::NF; ++d) dma(L.fp[d] + (long)(j - 1) * L.fld[d], rec + 64 * d);
+  };
+  auto drow = [&](uint32_t *dst, const double *row) {
+#pragma unroll
+    for (int e = 0; e < G::ND; ++e) dma((const char *)row + L.doff[e], dst + 64 * e);
+  };
+  auto halo = [&](int j) {
+    if (t < 4) dma((const char *)(s.xv + (long)(j - 1) * s.ldx + (t < 2 ? 0 : s.w + 1)) + 4 * (t & 1),
+                   S.halo[b][p]);
+  };
+  const double *xr = (k.nb != s.nblk) ? s.dum0 + (long)(k.nb - 1) * s.w : s.xrow(k.ieb);
+  switch (k.kind) {
+    case BIR_HEAD:
+      drow(rec, s.xrow(1));
+      drow(rec + 2 * WMAX, s.xrow(2));
+      break;
+    case BIR_ROW: {
+      const char *h = (const char *)(s.xv + (long)k.j * s.ldx);  // &X(1, j+1)
+      frows(k.j);
+#pragma unroll
+      for (int e = 0; e < G::ND; ++e) dma(h + L.hoff[e], rec + 6 * WMAX + 64 * e);
+      break;
+    }
+    case BIR_CLOS:
+      frows(k.j);
+      halo(k.j + 1);
+      drow(rec + 6 * WMAX, s.xrow(k.ieb + 1));
+      break;
+    case BIR_A:
+      frows(k.ieb - 1);
+      halo(k.ieb);
+      drow(rec + 6 * WMAX, s.xrow(k.ieb - 1));
+      break;
+    case BIR_B: {
+      constexpr int d4 = 4 * WMAX / 64;  // the DMA holding slot 4
+      drow(rec, xr);
+      // with a one-row block X(., JS+1) is row IE, read after its restore
+      drow(rec + 2 * WMAX, k.js + 1 == k.ieb ? xr : s.xrow(k.js + 1));
+      if (k.nb != 1) dma(L.fp[d4] + (long)(k.js - 2) * L.fld[d4], rec + 64 * d4);
+      if (k.nb == s.nblk) drow(rec + 6 * WMAX, s.xrow(k.ieb + 1));
+      break;
+    }
+    default:  // BIR_BROW: the last row reads row IE after its restore
+      frows(k.j);
+      drow(rec + 6 * WMAX, k.j + 2 == k.ieb ? xr : s.xrow(k.j + 2));
+      break;
+  }
+}
+
+// The record's results to memory.  The destination rows are wave-uniform and
+// chosen first; only the stores themselves are limited to the strip's lanes.
+template <int WMAX>
+__device__ __forceinline__ void drain_record(Stage<WMAX> &S, const Strip &s, int b, int ob, int p,
+                                             const Cursor &k, int t, double o0) {
+  double *d0 = nullptr, *d1 = nullptr, *dr = nullptr;
+  if (k.kind == BIR_ROW) {  // row IE(nb) of a closing block waits for its correction
+    if (!(k.nb < s.nblk && k.j == k.ieb - 2)) d0 = s.xrow(k.j + 2);
+  } else if (k.kind == BIR_CLOS) {
+    d0 = s.xrow(k.ieb);
+    d1 = s.dum0 + (long)(k.nb - 1) * s.w;
+  } else if (k.kind == BIR_B) {
+    // restore row IE from DUM0 (still in the record); a block with march
+    // rows overwrites it with its last row, so only a one-row block keeps it
+    if (k.nb != s.nblk && k.ieb - 2 < k.js) dr = s.xrow(k.ieb);
+    d0 = s.xrow(k.js + 1);
+  } else if (k.kind == BIR_BROW) {
+    d0 = s.xrow(k.j + 2);
+  }
+  const bool act = t < s.w;
+  if (dr && act) dr[t] = __hiloint2double((int)S.in[b][p][WMAX + t], (int)S.in[b][p][t]);
+  if (d0 && act) d0[t] = o0;
+  if (d1 && act) d1[t] = S.out[ob][p][1][t];
+}
+
+// One pass for stager `sid`: it fills and drains the records p of each chunk
+// with p % NSTG == sid.  The record-table entries of each chunk come into an
+// LDS ring by DMA two chunks ahead (stager 0 issues them; the first two
+// chunks' every stager loads for itself).  Records of one pass read only rows
+// that no record of the same pass writes, and write distinct rows, so the
+// stagers need no ordering among themselves inside a pass; the barrier after
+// the last drain makes every result visible to the next pass's DMAs.
+template <int WMAX>
+__device__ void stage_pass(Stage<WMAX> &S, const Strip &s, const StagerLane<WMAX> &L, int t, int sid,
+                           bool fwd, const int *recs) {
+  constexpr int R = Stage<WMAX>::R;
+  const int T = fwd ? 1 + fwd_jlast(s.ie, s.nblk) : s.nblk + ie_at(s.ie, s.nblk - 1) - 1;
+  const int C = (T + R - 1) / R;
+  auto load_desc = [&](int c) {
+#pragma unroll
+    for (int k = 0; k < R * 4 / 64; ++k) dma(recs + (long)c * R * 4 + 64 * k + t, reinterpret_cast<uint32_t *>(&S.desc[c & 3][0][0]) + 64 * k);
+  };
+  // This stager's entries of chunk c, one per lane (lane i: record sid +
+  // i*NSTG), read at once; the i-th is then taken with lane reads.
+  auto descs = [&](int c, int n) {
+    const int p = sid + NSTG * t;
+    int4 d = make_int4(0, 0, 0, 0);
+    if (p < n) d = *reinterpret_cast<const int4 *>(S.desc[c & 3][p]);
+    return d;
+  };
+  auto cur_at = [&](const int4 &d, int i) {
+    Cursor k;
+    const int kn = __builtin_amdgcn_readlane(d.x, i);
+    k.kind = kn & 255;
+    k.nb = kn >> 8;
+    k.j = __builtin_amdgcn_readlane(d.y, i);
+    k.ieb = __builtin_amdgcn_readlane(d.z, i);
+    k.js = __builtin_amdgcn_readlane(d.w, i);
+    return k;
+  };
+  auto fill = [&](int c) {
+    const int n = min(R, T - c * R);
+    const int4 d = descs(c, n);
+    for (int i = 0, p = sid; p < n; ++i, p += NSTG) fill_record<WMAX>(S, s, L, c % 3, p, cur_at(d, i), t);
+  };
+  // Each result is read from LDS a record ahead of its store.
+  auto drain = [&](int c) {
+    const int n = min(R, T - c * R);
+    const int4 d = descs(c, n);
+    const int tl = t & (WMAX - 1);
+    double on = sid < n ? S.out[c & 1][sid][0][tl] : 0.0;
+    for (int i = 0, p = sid; p < n; ++i, p += NSTG) {
+      const double o0 = on;
+      if (p + NSTG < n) on = S.out[c & 1][p + NSTG][0][tl];
+      drain_record<WMAX>(S, s, c % 3, c & 1, p, cur_at(d, i), t, o0);
+    }
+  };
+  load_desc(0);
+  if (C > 1) load_desc(1);
+  wait_vm();
+  fill(0);
+  bar_all();
+  for (int c = 0; c < C; ++c) {
+    if (sid == 0 && c + 2 < C) load_desc(c + 2);
+    if (c + 1 < C) fill(c + 1);
+    if (c >= 1) drain(c - 1);
+    bar_all();
+  }
+  drain(C - 1);
+  bar_all();  // pass end
+}
+
+// ----------------------------------------------------------------- marcher
+template <int WMAX>
+struct Rec {
+  uint32_t v[8];
+  __device__ double fl(int s) const { return (double)__uint_as_float(v[s]); }
+  __device__ double dbl(int s) const { return __hiloint2double((int)v[s + 1], (int)v[s]); }
+};
+
+// The marcher's walk through the records.  Byte offsets of the current
+// record (input, halo, output) advance by one record per step and wrap with
+// the buffers at chunk ends.
+template <int WMAX>
+struct MarchIO {
+  using G = Stage<WMAX>;
+  static constexpr int R = G::R;
+  static constexpr int RB = G::REC * 4;    // bytes per input record
+  static constexpr int OB = 2 * WMAX * 8;  // bytes per output record
+  G &S;
+  int l, qo = 0, ho = 0, oo = 0, p = 0;
+  Rec<WMAX> c0, c1;  // c0 is the current record between calls
+  __device__ MarchIO(G &s, int lane) : S(s), l(lane) {}
+  template <int K>
+  __device__ Rec<WMAX> &set() { return K ? c1 : c0; }
+  // Record at input offset q (past the third buffer's end it reads the halo
+  // array that follows: a discarded prefetch).
+  template <int K>
+  __device__ void read(int q) {
+    Rec<WMAX> &c = set<K>();
+    const uint32_t *r = reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(&S) + q) + l;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c.v[k] = r[k * WMAX];
+  }
+  __device__ void start() {
+    bar_lds();
+    qo = 0; ho = 0; oo = 0; p = 0;
+    read<0>(0);
+  }
+  // Record set K is current: return it and request the next record into the
+  // other set at once, so its LDS latency hides behind this row's chain.
+  // Rows alternate the two sets, so a row's record is never copied; other
+  // records go through take(), which moves the prefetched record back into
+  // set 0.
+  template <int K>
+  __device__ const Rec<WMAX> &take_row() {
+    read<1 - K>(qo + RB);
+    return set<K>();
+  }
+  __device__ Rec<WMAX> take() {
+    const Rec<WMAX> r = c0;
+    read<0>(qo + RB);
+    return r;
+  }
+  // The current closure record's halo: X(1, row) and X(W+2, row).
+  __device__ void halo(double &hw, double &he) const {
+    const uint4 h = *reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(&S.halo[0][0][0]) + ho);
+    hw = __hiloint2double((int)h.y, (int)h.x);
+    he = __hiloint2double((int)h.w, (int)h.z);
+  }
+  __device__ void put(int k, double v) {
+    *reinterpret_cast<double *>(reinterpret_cast<char *>(&S.out[0][0][0][0]) + oo + k * WMAX * 8 + l * 8) = v;
+  }
+  // After the record: at the chunk's end its barrier, then the new chunk's
+  // first record into set K (the set that becomes current).
+  template <int K = 0>
+  __device__ void next() {
+    qo += RB;
+    ho += 16;
+    oo += OB;
+    if (++p == R) {
+      bar_lds();
+      p = 0;
+      if (qo == G::NIN * R * RB) { qo = 0; ho = 0; }
+      if (oo == 2 * R * OB) oo = 0;
+      read<K>(qo);
+    }
+  }
+  // The last chunk's barrier, then the pass-end barrier that waits for the
+  // stagers' last results to reach memory.
+  __device__ void finish() {
+    if (p != 0) bar_lds();
+    bar_lds();
+  }
+};
+
+// REPBIR on one strip (MAIN/template/GLOBAL/src/sevp.f90:218-328), consuming
+// the forward and backward record sequences.
+template <int WMAX>
+__device__ void march_strip(Stage<WMAX> &S, const Strip &s, int lane) {
+  const int w = s.w, nblk = s.nblk;
+  const int n = lane < w ? lane : 0;
+  const long ww = (long)w * w;
   double rc[WMAX], rc2[WMAX];
+  MarchIO<WMAX> io(S, lane);
 
   // ---------------- forward ----------------
-  const int ieb_last = c.ie[nblk - 1];
-  const int js_last = nblk > 1 ? c.ie[nblk - 2] : 1;
-  const int jlast = (ieb_last - 2 >= js_last) ? ieb_last - 2 : js_last - 1;
-  float fa[D], fb[D], fc[D], fd[D], fe[D], ff_[D];
-  double fhw[D], fhe[D];
-#define FFETCH(k, jj)                                                                  \
-  do {                                                                                 \
-    const int j_ = (jj);                                                               \
-    fa[k] = LDC32(c.ax, j_); fb[k] = LDC32(c.ay, j_); fc[k] = LDC32(c.bb, j_);         \
-    fd[k] = LDC32(c.cx, j_); fe[k] = LDC32(c.cy, j_);                                  \
-    ff_[k] = c.f[c.fcol + (long)(j_ - 1) * c.fld];                                     \
-    fhw[k] = XV(1, j_ + 1); fhe[k] = XV(wi, j_ + 1);                                   \
-  } while (0)
-#pragma unroll
-  for (int k = 0; k < D; ++k) FFETCH(k, min(1 + k, max(jlast, 1)));
-  int nb = 1, ieb = c.ie[0];
-  if (nblk > 1) load_col<WMAX>(rc, c.rinv1, w, n);
-  double xs = XV(li, 1), xc = XV(li, 2);
-  double xnn = (nblk > 1) ? XV(li, ieb + 1) : 0.0;
-  for (int j0 = 1; j0 <= jlast; j0 += D) {
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-      const int j = j0 + k;
-      if (j <= jlast) {
-        const double a = fa[k], b = fb[k], cc = fc[k], d = fd[k], e = fe[k], ff = ff_[k];
-        const double hw = fhw[k], he = fhe[k];
-        FFETCH(k, min(j + D, jlast));
-        double xw = wave_shr1(xc, hw);
-        double xe = wave_shl1(xc, he);
-        if (lane == w - 1) xe = he;
-        if (j == ieb - 1) {
-          // closure residual of block nb at row ieb-1, correction of row ieb
-          const double r = ff - a * xw - b * xs - cc * xc - d * xe - e * xnn;
-          const double dd = dot_col<WMAX>(r, rc, w);
-          if (act) c.dum0[n + (long)(nb - 1) * w] = xc;
-          const double xcor = xc - dd;
-          if (act) XV(li, ieb) = xcor;
-          xs = xcor;
-          xc = xnn;
-          ++nb;
-          ieb = c.ie[nb - 1];
-          if (nb < nblk) {
-            load_col<WMAX>(rc, c.rinv1 + (long)(nb - 1) * w * w, w, n);
-            xnn = XV(li, ieb + 1);
-          }
-        } else {
-          const double xn = (ff - a * xw - b * xs - cc * xc - d * xe) / e;
-          if (act && j + 2 < ieb) XV(li, j + 2) = xn;
-          xs = xc;
-          xc = xn;
-        }
+  if (nblk > 1) load_col<WMAX>(rc, s.rinv1, w, n);
+  else load_col<WMAX>(rc, s.rinv, w, n);
+  io.start();
+  const int jlast = fwd_jlast(s.ie, nblk);
+  double xs, xc;
+  {
+    const Rec<WMAX> h = io.take();
+    xs = h.dbl(0);
+    xc = h.dbl(2);
+    io.next();
+  }
+  int js = 1, ieb = ie_at(s.ie, 0);
+  for (int nb = 1;; ++nb) {
+    const int jend = nb < nblk ? ieb - 2 : jlast;
+    auto row = [&](const Rec<WMAX> &r) {
+      const double a = r.fl(0), b = r.fl(1), cc = r.fl(2), d = r.fl(3), e = r.fl(4), ff = r.fl(5);
+      const double h = r.dbl(6);  // lane 0: X(1, J+1); lane w-1: X(w+2, J+1)
+      double xw = wave_shr1_zero(xc);
+      if (lane == 0) xw = h;
+      double xe = wave_shl1_zero(xc);
+      if (lane == w - 1) xe = h;
+      const double xn = (ff - a * xw - b * xs - cc * xc - d * xe) / e;
+      io.put(0, xn);
+      xs = xc;
+      xc = xn;
+    };
+    for (int j = js; j <= jend; j += 2) {  // rows in pairs: record sets 0, 1
+      row(io.template take_row<0>());
+      io.template next<1>();
+      if (j == jend) {
+        io.c0 = io.c1;
+        break;
+      }
+      row(io.template take_row<1>());
+      io.template next<0>();
+    }
+    if (nb == nblk) break;
+    {  // closure residual of block nb at row IE-1, correction of row IE
+      const Rec<WMAX> r = io.take();
+      const double a = r.fl(0), b = r.fl(1), cc = r.fl(2), d = r.fl(3), e = r.fl(4), ff = r.fl(5);
+      const double xnn = r.dbl(6);
+      double hw, he;
+      io.halo(hw, he);
+      double xw = wave_shr1_zero(xc);
+      if (lane == 0) xw = hw;
+      double xe = wave_shl1_zero(xc);
+      if (lane == w - 1) xe = he;
+      const double res = ff - a * xw - b * xs - cc * xc - d * xe - e * xnn;
+      const double dd = dot_col<WMAX>(res, rc, w, S.e, lane);
+      const double xcor = xc - dd;
+      io.put(0, xcor);
+      io.put(1, xc);  // DUM0
+      io.next();
+      xs = xcor;
+      xc = xnn;
+      if (nb + 1 < nblk) {
+        load_col<WMAX>(rc, s.rinv1 + nb * ww, w, n);
+      } else {  // no forward closure left: the backward pass's first block
+        load_col<WMAX>(rc, s.rinv + (nblk - 1) * ww, w, n);
+        load_col<WMAX>(rc2, s.rinv1 + (nblk - 2) * ww, w, n);
       }
     }
+    js = ieb;
+    ieb = ie_at(s.ie, nb);
   }
-  if (act) XV(li, ieb) = xc;  // the last block's row IE(NBLK)
-#undef FFETCH
+  io.finish();
 
   // ---------------- backward ----------------
-  float ba[D], bb_[D], bc[D], bd[D], be[D];
-  double bxo[D];
-#define BFETCH(k, jj)                                                                  \
-  do {                                                                                 \
-    const int j_ = (jj);                                                               \
-    ba[k] = LDC32(c.ax, j_); bb_[k] = LDC32(c.ay, j_); bc[k] = LDC32(c.bb, j_);        \
-    bd[k] = LDC32(c.cx, j_); be[k] = LDC32(c.cy, j_); bxo[k] = XV(li, j_ + 2);         \
-  } while (0)
-  for (int nbs = 1; nbs <= nblk; ++nbs) {
-    const int nbk = nblk - nbs + 1;
-    const int iebk = c.ie[nbk - 1];
-    const int js = (nbk != 1) ? c.ie[nbk - 2] : 1;
+  io.start();
+  double xnn2 = 0.0;  // X(., IE+1): the block above's corrected row JS+1
+  for (int nbk = nblk; nbk >= 1; --nbk) {
+    const int iebk = ie_at(s.ie, nbk - 1);
+    const int jsb = nbk != 1 ? ie_at(s.ie, nbk - 2) : 1;
     const int jf = iebk - 2;
-    load_col<WMAX>(rc, c.rinv + (long)(nbk - 1) * w * w, w, n);
-    if (nbk != 1) load_col<WMAX>(rc2, c.rinv1 + (long)(nbk - 2) * w * w, w, n);
-    double xrow;  // X(li, ieb) after the restore
-    if (nbk != nblk) {
-      xrow = c.dum0[n + (long)(nbk - 1) * w];
-      if (act) XV(li, iebk) = xrow;
-    } else {
-      xrow = XV(li, iebk);
-    }
-    // the H march's operands, requested with the closure's
-    if (jf >= js) {
-#pragma unroll
-      for (int k = 0; k < D; ++k) BFETCH(k, min(js + k, jf));
-    }
-    const int jr = iebk - 1;
-    const double xsr = XV(li, jr), xnn2 = XV(li, iebk + 1);
-    const double hw = XV(1, iebk), he = XV(wi, iebk);
-    double xw = wave_shr1(xrow, hw);
-    double xe = wave_shl1(xrow, he);
+    const Rec<WMAX> A = io.take();
+    double hw, he;
+    io.halo(hw, he);  // record A's
+    io.next();
+    const Rec<WMAX> B = io.take();
+    const double xrow = B.dbl(0), xf = B.dbl(2);
+    if (nbk == nblk) xnn2 = B.dbl(6);
+    double xw = wave_shr1_zero(xrow);
+    if (lane == 0) xw = hw;
+    double xe = wave_shl1_zero(xrow);
     if (lane == w - 1) xe = he;
-    const double r = LD_F(jr) - LD_C(c.ax, jr) * xw - LD_C(c.ay, jr) * xsr -
-                     LD_C(c.bb, jr) * xrow - LD_C(c.cx, jr) * xe - LD_C(c.cy, jr) * xnn2;
-    const double cv = dot_col<WMAX>(r, rc, w);
-    const double xf = XV(li, js + 1);
-    if (act) XV(li, js + 1) = xf + cv;
+    const double res = A.fl(5) - A.fl(0) * xw - A.fl(1) * A.dbl(6) - A.fl(2) * xrow -
+                       A.fl(3) * xe - A.fl(4) * xnn2;
+    const double cv = dot_col<WMAX>(res, rc, w, S.e, lane);
+    const double xnew = xf + cv;
+    io.put(0, xnew);
     double hs = 0.0;
     if (nbk != 1) {
-      const double g = cv * LD_C(c.cy, js - 1);
-      hs = dot_col<WMAX>(g, rc2, w);
+      const double g = cv * B.fl(4);
+      hs = dot_col<WMAX>(g, rc2, w, S.e, lane);
     }
+    io.next();
+    if (nbk > 1) {  // the next block's matrices, a block ahead
+      load_col<WMAX>(rc, s.rinv + (nbk - 2) * ww, w, n);
+      if (nbk > 2) load_col<WMAX>(rc2, s.rinv1 + (nbk - 3) * ww, w, n);
+    }
+    xnn2 = xnew;
     double hc = cv;
-    for (int j0 = js; j0 <= jf; j0 += D) {
-#pragma unroll
-      for (int k = 0; k < D; ++k) {
-        const int j = j0 + k;
-        if (j <= jf) {
-          const double a = ba[k], b = bb_[k], cc = bc[k], d = bd[k], e = be[k], xo = bxo[k];
-          BFETCH(k, min(j + D, jf));
-          double hwv = wave_shr1(hc, 0.0);
-          double hev = wave_shl1(hc, 0.0);
-          if (lane == w - 1) hev = 0.0;
-          const double hn = (-(a * hwv) - b * hs - cc * hc - d * hev) / e;
-          if (act) XV(li, j + 2) = xo + hn;
-          hs = hc;
-          hc = hn;
-        }
+    auto row = [&](const Rec<WMAX> &m) {
+      const double a = m.fl(0), b = m.fl(1), cc = m.fl(2), d = m.fl(3), e = m.fl(4), xo = m.dbl(6);
+      const double hwv = wave_shr1_zero(hc);
+      double hev = wave_shl1_zero(hc);
+      if (lane == w - 1) hev = 0.0;
+      const double hn = (-(a * hwv) - b * hs - cc * hc - d * hev) / e;
+      io.put(0, xo + hn);
+      hs = hc;
+      hc = hn;
+    };
+    for (int j = jsb; j <= jf; j += 2) {
+      row(io.template take_row<0>());
+      io.template next<1>();
+      if (j == jf) {
+        io.c0 = io.c1;
+        break;
       }
+      row(io.template take_row<1>());
+      io.template next<0>();
     }
   }
-#undef BFETCH
+  io.finish();
 }
-#undef LDC32
-#undef XV
+
+// ----------------------------------------------------------------- kernels
+// The block rows IE live in LDS for the whole launch: the stager copies them
+// before its first chunk, so the marcher (which reads them only after the
+// first chunk barrier) and the stager's cursors never wait on global memory
+// for them.
+__device__ inline void load_ie(int *dst, const int *src, int nblk, int t) {
+  for (int b = t; b < nblk; b += SEVP_WAVE) dst[b] = src[b];
+  wait_vm();
+}
+
+__device__ inline Strip bir_strip(const BirArgs &p, const int *ie, int r, int n) {
+  Strip s;
+  const int w = p.ibir, inc = p.ibir / 2;
+  const int i0 = p.i2 + 2;
+  const int il = (n - 1) * inc + 1;
+  s.arr[0] = p.al; s.arr[1] = p.ab; s.arr[2] = p.ac; s.arr[3] = p.ar; s.arr[4] = p.at;
+  s.arr[5] = p.s + (long)r * p.s_stride;
+  for (int k = 0; k < 6; ++k) s.ld[k] = p.i2;
+  s.il = il;
+  s.inc = inc;
+  s.seam = n == p.nbir;
+  if (!s.seam) {
+    s.xv = p.x + (long)r * p.x_stride + f2(i0, il, 1);
+    s.ldx = i0;
+  } else {
+    s.xv = p.xx + (long)r * p.xx_stride;
+    s.ldx = w + 2;
+  }
+  const long ibr2 = (long)w * w;
+  s.rinv = p.rinv + (long)(n - 1) * ibr2 * p.nb0;
+  s.rinv1 = p.rinv1 + (long)(n - 1) * ibr2 * (p.nb0 - 1);
+  s.dum0 = p.dum0 + (long)r * p.dum0_stride;
+  s.ie = ie;
+  s.nblk = p.nb0;
+  s.w = w;
+  return s;
+}
 
 // Whole FS_SOLVER BIR loop (MAIN/template/GLOBAL/src/fs_GLOBAL.f90:1902-1966)
-// for one RHS per wavefront.
+// for one RHS per workgroup.  The seam strip's halo gather and write-back
+// (and the periodic halo refresh) are done by every stager in full: the
+// stagers write identical values, and each reads only what it has itself
+// written before its next pass's DMAs.
 template <int WMAX>
-__global__ __launch_bounds__(64) void bir_sweeps_kernel(BirArgs p) {
+__global__ __launch_bounds__(BIR_THREADS) void bir_sweeps_kernel(BirArgs p) {
+  __shared__ Stage<WMAX> S;
+  __shared__ int ie[BIR_MAX_BLOCKS];
   const int r = blockIdx.x;
-  const int lane = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int t = threadIdx.x & 63;
+  if (wave == 0) {
+    if (t < WMAX)
+      for (int it = 0; it < p.nsweep; ++it)
+        for (int n = 1; n <= p.nbir; ++n) march_strip<WMAX>(S, bir_strip(p, ie, r, n), t);
+    return;
+  }
+  const int sid = wave - 1;
+  load_ie(ie, p.ie, p.nb0, t);
   const int w = p.ibir, inc = p.ibir / 2;
   const int i0 = p.i2 + 2, m0 = w + 2;
   const int j1 = p.j2 + 1;
   double *x = p.x + (long)r * p.x_stride;
   double *xx = p.xx + (long)r * p.xx_stride;
-  const float *s = p.s + (long)r * p.s_stride;
-  StripCtx c;
-  c.ax = p.al; c.ay = p.ab; c.bb = p.ac; c.cx = p.ar; c.cy = p.at;
-  c.cld = p.i2;
-  c.f = s;
-  c.fld = p.i2;
-  c.ie = p.ie;
-  c.nblk = p.nb0;
-  c.w = w;
-  c.dum0 = p.dum0 + (long)r * p.dum0_stride;
-  const long ibr2 = (long)w * w;
   for (int it = 0; it < p.nsweep; ++it) {
     for (int n = 1; n <= p.nbir; ++n) {
-      const int il = (n - 1) * inc + 1;
-      c.rinv = p.rinv + (long)(n - 1) * ibr2 * p.nb0;
-      c.rinv1 = p.rinv1 + (long)(n - 1) * ibr2 * (p.nb0 - 1);
-      const int lc = lane < w ? lane : 0;
-      if (n != p.nbir) {
-        c.ccol = il - 1 + lc;
-        c.fcol = c.ccol;
-        c.xv = x + f2(i0, il, 1);
-        c.ldx = i0;
-        repbir_wave<WMAX, 4>(c);
-        __threadfence_block();
-      } else {
+      const Strip s = bir_strip(p, ie, r, n);
+      const int il = s.il;
+      if (s.seam) {
         // seam strip: halo columns of XX for rows 1..J2 as the reference does
         // (row J1 keeps its old value), or 1..J1 with seam_fix
         const int jg = p.seam_fix ? j1 : p.j2;
-        for (int j = 1 + lane; j <= jg; j += SEVP_WAVE) {
+        for (int j = 1 + t; j <= jg; j += SEVP_WAVE) {
           xx[f2(m0, 1, j)] = x[f2(i0, il, j)];
           xx[f2(m0, w + 2, j)] = x[f2(i0, inc + 2, j)];
         }
-        __threadfence_block();
-        c.ccol = (lc < inc) ? (il - 1 + lc) : (lc - inc);
-        c.fcol = c.ccol;
-        c.xv = xx;
-        c.ldx = m0;
-        repbir_wave<WMAX, 4>(c);
-        __threadfence_block();
-        for (long t = lane; t < (long)w * (j1 - 1); t += SEVP_WAVE) {
-          const int i = (int)(t % w) + 1, j = (int)(t / w) + 2;
+        wait_vm();
+      }
+      StagerLane<WMAX> L;
+      L.init(s, t);
+      stage_pass<WMAX>(S, s, L, t, sid, true, p.recs);
+      stage_pass<WMAX>(S, s, L, t, sid, false, p.recs + 4L * p.fwd_records);
+      if (s.seam) {
+        for (long q = t; q < (long)w * (j1 - 1); q += SEVP_WAVE) {
+          const int i = (int)(q % w) + 1, j = (int)(q / w) + 2;
           const double v = xx[f2(m0, i + 1, j)];
           if (i <= inc) x[f2(i0, il + i, j)] = v;
           else x[f2(i0, i - inc + 1, j)] = v;
         }
-        __threadfence_block();
+        wait_vm();
         if (p.periodic) {
-          for (int j = 2 + lane; j <= j1; j += SEVP_WAVE) {
+          for (int j = 2 + t; j <= j1; j += SEVP_WAVE) {
             x[f2(i0, 1, j)] = x[f2(i0, i0 - 1, j)];
             x[f2(i0, i0, j)] = x[f2(i0, 2, j)];
           }
-          __threadfence_block();
+          wait_vm();
         }
       }
     }
@@ -288,40 +643,52 @@ __global__ __launch_bounds__(64) void bir_sweeps_kernel(BirArgs p) {
 // One REPBIR call with the reference's argument conventions, batched over
 // independent right-hand sides (F and X strided by f_stride / x_stride).
 template <int WMAX>
-__global__ __launch_bounds__(64) void repbir_kernel(RepbirArgs p) {
+__global__ __launch_bounds__(BIR_THREADS) void repbir_kernel(RepbirArgs p) {
+  __shared__ Stage<WMAX> S;
+  __shared__ int ie[BIR_MAX_BLOCKS];
   const int r = blockIdx.x;
-  const int lane = threadIdx.x;
-  StripCtx c;
-  c.ax = p.ax; c.ay = p.ay; c.bb = p.bb; c.cx = p.cx; c.cy = p.cy;
-  c.ccol = lane < p.m2 ? lane : 0;
-  c.cld = p.i2;
-  c.f = p.f + (long)r * p.f_stride;
-  c.fcol = c.ccol;
-  c.fld = p.m2;
-  c.xv = p.x + (long)r * p.x_stride;
-  c.ldx = p.i0;
-  c.rinv = p.rinv;
-  c.rinv1 = p.rinv1;
-  c.dum0 = p.dum0 + (long)r * p.dum0_stride;
-  c.ie = p.ie;
-  c.nblk = p.nblk;
-  c.w = p.m2;
-  repbir_wave<WMAX, 4>(c);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int t = threadIdx.x & 63;
+  Strip s;
+  s.arr[0] = p.ax; s.arr[1] = p.ay; s.arr[2] = p.bb; s.arr[3] = p.cx; s.arr[4] = p.cy;
+  s.arr[5] = p.f + (long)r * p.f_stride;
+  for (int k = 0; k < 5; ++k) s.ld[k] = p.i2;
+  s.ld[5] = p.m2;
+  s.il = 1;
+  s.inc = 0;
+  s.seam = 0;
+  s.xv = p.x + (long)r * p.x_stride;
+  s.ldx = p.i0;
+  s.rinv = p.rinv;
+  s.rinv1 = p.rinv1;
+  s.dum0 = p.dum0 + (long)r * p.dum0_stride;
+  s.ie = ie;
+  s.nblk = p.nblk;
+  s.w = p.m2;
+  if (wave == 0) {
+    if (t < WMAX) march_strip<WMAX>(S, s, t);
+    return;
+  }
+  load_ie(ie, p.ie, p.nblk, t);
+  StagerLane<WMAX> L;
+  L.init(s, t);
+  stage_pass<WMAX>(S, s, L, t, wave - 1, true, p.recs);
+  stage_pass<WMAX>(S, s, L, t, wave - 1, false, p.recs + 4L * p.fwd_records);
 }
 
 hipError_t launch_bir_sweeps(const BirArgs &a, int nrhs, hipStream_t st) {
   if (a.ibir <= 32)
-    hipLaunchKernelGGL(bir_sweeps_kernel<32>, dim3(nrhs), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(bir_sweeps_kernel<32>, dim3(nrhs), dim3(BIR_THREADS), 0, st, a);
   else
-    hipLaunchKernelGGL(bir_sweeps_kernel<64>, dim3(nrhs), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(bir_sweeps_kernel<64>, dim3(nrhs), dim3(BIR_THREADS), 0, st, a);
   return hipGetLastError();
 }
 
 hipError_t launch_repbir(const RepbirArgs &a, int nrhs, hipStream_t st) {
   if (a.m2 <= 32)
-    hipLaunchKernelGGL(repbir_kernel<32>, dim3(nrhs), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(repbir_kernel<32>, dim3(nrhs), dim3(BIR_THREADS), 0, st, a);
   else
-    hipLaunchKernelGGL(repbir_kernel<64>, dim3(nrhs), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(repbir_kernel<64>, dim3(nrhs), dim3(BIR_THREADS), 0, st, a);
   return hipGetLastError();
 }
 

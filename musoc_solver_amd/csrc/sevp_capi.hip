@@ -269,6 +269,8 @@ struct sevp_plan_s {
   int storage = SEVP_STORE_FP64;
   DBuf<float> coef;     // 5 * i2*j2 (AL, AB, AC, AR, AT)
   DBuf<int> ie_d;
+  DBuf<int> brec;       // BIR: the strip record table (bir_record_table)
+  int brec_fwd = 0;
   DBuf<double> rinv, rinv1;      // reference layout
   DBuf<float> rinv32, rinv1_32;  // mixed precision copies
   DBuf<float> w32;               // mixed: W_b = RINV_b diag(CY(:,IE(b-1)-1)) RINV1_(b-1), b >= 2
@@ -1052,6 +1054,10 @@ int sevp_plan_create(const sevp_desc *desc, const float *al, const float *ab, co
       delete p;
       return fail(SEVP_E_ARG, "BIR strip width IBIR must be even and <= 64");
     }
+    if (d.nb0 > BIR_MAX_BLOCKS) {
+      delete p;
+      return fail(SEVP_E_ARG, "BIR form supports at most 4096 blocks (NB0)");
+    }
     if (d.precision != SEVP_PREC_FP64) {
       delete p;
       return fail(SEVP_E_ARG, "BIR form runs the plain FP64 arithmetic only");
@@ -1089,6 +1095,13 @@ int sevp_plan_create(const sevp_desc *desc, const float *al, const float *ab, co
   if ((rc = p->ie_d.alloc(d.nb0))) return bail(rc);
   if (hipMemcpy(p->ie_d.p, p->ie.data(), d.nb0 * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
     return bail(fail(SEVP_E_HIP, "IE upload failed"));
+  if (d.form == SEVP_FORM_BIR) {
+    std::vector<int> rt;
+    bir_record_table(p->ie.data(), d.nb0, rt, p->brec_fwd);
+    if ((rc = p->brec.alloc(rt.size()))) return bail(rc);
+    if (hipMemcpy(p->brec.p, rt.data(), rt.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(SEVP_E_HIP, "record table upload failed"));
+  }
   if ((rc = p->zp.alloc(1))) return bail(rc);
   if (!matinv_uses_lds(p->w))
     if ((rc = p->b1s.alloc(matinv_blocked_scratch(p->w, p->nmat)))) return bail(rc);
@@ -1161,7 +1174,7 @@ int sevp_set_stream(sevp_plan p, void *stream) {
 
 size_t sevp_plan_device_bytes(sevp_plan p) {
   if (!p) return 0;
-  return p->coef.bytes() + p->ie_d.bytes() + p->rinv.bytes() + p->rinv1.bytes() +
+  return p->coef.bytes() + p->ie_d.bytes() + p->brec.bytes() + p->rinv.bytes() + p->rinv1.bytes() +
          p->rinv32.bytes() + p->rinv1_32.bytes() + p->w32.bytes() + p->e.bytes() + p->dum0.bytes() +
          p->hfree.bytes() + p->hcpl.bytes() + p->g.bytes() + p->e32.bytes() +
          p->kpart.bytes() + p->xx.bytes() + p->rinv64t.bytes() + p->rinv1_64t.bytes() +
@@ -1737,6 +1750,7 @@ int sevp_solve_dev(sevp_plan p, const float *s, double *x, int nrhs, void *strea
     a.al = p->al(); a.ab = p->ab(); a.ac = p->ac(); a.ar = p->ar(); a.at = p->at();
     a.s = s; a.rinv = p->rinv.p; a.rinv1 = p->rinv1.p;
     a.x = x; a.xx = p->xx.p; a.dum0 = p->bdum0.p; a.ie = p->ie_d.p;
+    a.recs = p->brec.p; a.fwd_records = p->brec_fwd;
     a.i2 = p->d.i2; a.j2 = p->d.j2; a.nbir = p->d.nbir; a.ibir = p->ibir; a.nb0 = p->d.nb0;
     a.nsweep = p->d.nsweep; a.periodic = p->d.periodic; a.seam_fix = p->d.seam_fix;
     a.s_stride = s_stride; a.x_stride = x_stride;
@@ -2647,7 +2661,8 @@ extern "C" int sevp_repbir(const float *ax, const float *ay, const float *bb, co
                 int i0, int i2, int m0, int m2, int nblk) {
   (void)dum0; (void)dum1; (void)dum2; (void)h; (void)m0;
   if (!ax || !ay || !bb || !cx || !cy || !rinv || !f || !x || !ie) return fail(SEVP_E_ARG, "null argument");
-  if (m2 < 1 || m2 > 64 || nblk < 1 || i2 < m2 || i0 < m2 + 2) return fail(SEVP_E_ARG, "bad REPBIR shape");
+  if (m2 < 2 || m2 > 64 || nblk < 1 || i2 < m2 || i0 < m2 + 2) return fail(SEVP_E_ARG, "bad REPBIR shape");
+  if (nblk > BIR_MAX_BLOCKS) return fail(SEVP_E_ARG, "REPBIR supports at most 4096 blocks (NBLK)");
   if (nblk > 1 && !rinv1) return fail(SEVP_E_ARG, "null RINV1");
   for (int b = 0; b < nblk; ++b)
     if (ie[b] < 2 || (b && ie[b] <= ie[b - 1])) return fail(SEVP_E_ARG, "IE must be increasing from 2");
@@ -2664,12 +2679,18 @@ extern "C" int sevp_repbir(const float *ax, const float *ay, const float *bb, co
   CHK(rv1.alloc((size_t)m2 * m2 * (nblk > 1 ? nblk - 1 : 1)));
   if (nblk > 1) HIPC(hipMemcpy(rv1.p, rinv1, rv1.bytes(), hipMemcpyHostToDevice));
   CHK(d0.alloc((size_t)m2 * (nblk > 1 ? nblk - 1 : 1)));
-  DBuf<int> ied;
+  DBuf<int> ied, recd;
   CHK(ied.alloc(nblk));
   HIPC(hipMemcpy(ied.p, ie, nblk * sizeof(int), hipMemcpyHostToDevice));
+  std::vector<int> rt;
+  int rt_fwd = 0;
+  bir_record_table(ie, nblk, rt, rt_fwd);
+  CHK(recd.alloc(rt.size()));
+  HIPC(hipMemcpy(recd.p, rt.data(), rt.size() * sizeof(int), hipMemcpyHostToDevice));
   RepbirArgs a{};
   a.ax = c[0].p; a.ay = c[1].p; a.bb = c[2].p; a.cx = c[3].p; a.cy = c[4].p;
   a.f = fd.p; a.x = xd.p; a.rinv = rv.p; a.rinv1 = rv1.p; a.dum0 = d0.p; a.ie = ied.p;
+  a.recs = recd.p; a.fwd_records = rt_fwd;
   a.i0 = m2 + 2; a.i2 = m2; a.m2 = m2; a.nblk = nblk;
   HIPC(launch_repbir(a, 1, aux_stream()));
   HIPC(hipMemcpy2D(x, (size_t)i0 * sizeof(double), xd.p, (m2 + 2) * sizeof(double),

@@ -41,6 +41,21 @@ __device__ __forceinline__ double wave_shl1(double v, double fill) {
   return __hiloint2double(hi, lo);
 }
 
+// Lane i receives lane i-1's value, lane 0 receives +0 (DPP wave_shr:1 with
+// bound_ctrl): no fill operand, so no copy of one.
+__device__ __forceinline__ double wave_shr1_zero(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x138, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x138, 0xf, 0xf, true);
+  return __hiloint2double(hi, lo);
+}
+
+// Lane i receives lane i+1's value, lane 63 receives +0 (DPP wave_shl:1).
+__device__ __forceinline__ double wave_shl1_zero(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x130, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x130, 0xf, 0xf, true);
+  return __hiloint2double(hi, lo);
+}
+
 // Broadcast lane `src` (wave-uniform) of a double.
 __device__ __forceinline__ double read_lane(double v, int src) {
   int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
