@@ -110,11 +110,14 @@ __device__ __forceinline__ void dma(const void *src, uint32_t *dst) {
                                    (__attribute__((address_space(3))) void *)(dst), 4, 0, 0);
 }
 
+// Column n of a w x w matrix into registers, WMAX entries from one base (the
+// entries past w belong to the next column or the allocation's padding and
+// are never used).
 template <int WMAX>
 __device__ __forceinline__ void load_col(double (&r)[WMAX], const double *mat, int w, int n) {
   const double *c = mat + (long)n * w;
 #pragma unroll
-  for (int m = 0; m < WMAX; ++m) r[m] = c[m < w ? m : w - 1];
+  for (int m = 0; m < WMAX; ++m) r[m] = c[m];
 }
 
 // sum_{m=1..w} e(m) * R(m, n), sequential from zero (lane n holds column n).

@@ -1185,10 +1185,13 @@ size_t sevp_plan_device_bytes(sevp_plan p) {
          p->s_stage.bytes();
 }
 
-// FP64 influence matrices of an SEVP_STORE_FP64 plan, allocated on first use.
+// FP64 influence matrices of an SEVP_STORE_FP64 plan, allocated on first use,
+// with RINV_PAD spare doubles: the strip kernels load a whole register column
+// of 32 or 64 entries per lane, past the last column's end.
+constexpr size_t RINV_PAD = 64;
 static int ensure_inverses(sevp_plan p) {
-  if (!p->rinv.p) CHK(p->rinv.alloc(p->rinv_count()));
-  if (!p->rinv1.p) CHK(p->rinv1.alloc(p->rinv1_count()));
+  if (!p->rinv.p) CHK(p->rinv.alloc(p->rinv_count() + RINV_PAD));
+  if (!p->rinv1.p) CHK(p->rinv1.alloc(p->rinv1_count() + RINV_PAD));
   return SEVP_OK;
 }
 
@@ -1376,8 +1379,9 @@ int sevp_load_influence(sevp_plan p, const double *rinv, const double *rinv1) {
   if (!p || !rinv || !rinv1) return fail(SEVP_E_ARG, "null argument");
   if (p->storage == SEVP_STORE_PRODUCTS) return build_products(p, rinv, rinv1, nullptr);
   CHK(ensure_inverses(p));
-  HIPC(hipMemcpy(p->rinv.p, rinv, p->rinv.bytes(), hipMemcpyHostToDevice));
-  if (p->d.nb0 > 1) HIPC(hipMemcpy(p->rinv1.p, rinv1, p->rinv1.bytes(), hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(p->rinv.p, rinv, p->rinv_count() * sizeof(double), hipMemcpyHostToDevice));
+  if (p->d.nb0 > 1)
+    HIPC(hipMemcpy(p->rinv1.p, rinv1, p->rinv1_count() * sizeof(double), hipMemcpyHostToDevice));
   p->has32 = p->has64 = p->has16 = false;
   CHK(make_low_precision_copies(p));
   HIPC(hipStreamSynchronize(p->stream));
@@ -1391,8 +1395,9 @@ int sevp_get_influence(sevp_plan p, double *rinv, double *rinv1) {
   if (p->storage == SEVP_STORE_PRODUCTS)
     return fail(SEVP_E_STATE, "the plan keeps only the FP32 product operands (SEVP_STORE_PRODUCTS)");
   HIPC(hipStreamSynchronize(p->stream));
-  HIPC(hipMemcpy(rinv, p->rinv.p, p->rinv.bytes(), hipMemcpyDeviceToHost));
-  if (p->d.nb0 > 1) HIPC(hipMemcpy(rinv1, p->rinv1.p, p->rinv1.bytes(), hipMemcpyDeviceToHost));
+  HIPC(hipMemcpy(rinv, p->rinv.p, p->rinv_count() * sizeof(double), hipMemcpyDeviceToHost));
+  if (p->d.nb0 > 1)
+    HIPC(hipMemcpy(rinv1, p->rinv1.p, p->rinv1_count() * sizeof(double), hipMemcpyDeviceToHost));
   return SEVP_OK;
 }
 
@@ -2674,10 +2679,11 @@ extern "C" int sevp_repbir(const float *ax, const float *ay, const float *bb, co
   CHK(up2d(fd, f, m2, m2, jr));
   DBuf<double> xd, rv, rv1, d0;
   CHK(up2d(xd, x, i0, m2 + 2, jx));
-  CHK(rv.alloc((size_t)m2 * m2 * nblk));
-  HIPC(hipMemcpy(rv.p, rinv, rv.bytes(), hipMemcpyHostToDevice));
-  CHK(rv1.alloc((size_t)m2 * m2 * (nblk > 1 ? nblk - 1 : 1)));
-  if (nblk > 1) HIPC(hipMemcpy(rv1.p, rinv1, rv1.bytes(), hipMemcpyHostToDevice));
+  const size_t nrv = (size_t)m2 * m2 * nblk, nrv1 = (size_t)m2 * m2 * (nblk > 1 ? nblk - 1 : 1);
+  CHK(rv.alloc(nrv + RINV_PAD));
+  HIPC(hipMemcpy(rv.p, rinv, nrv * sizeof(double), hipMemcpyHostToDevice));
+  CHK(rv1.alloc(nrv1 + RINV_PAD));
+  if (nblk > 1) HIPC(hipMemcpy(rv1.p, rinv1, nrv1 * sizeof(double), hipMemcpyHostToDevice));
   CHK(d0.alloc((size_t)m2 * (nblk > 1 ? nblk - 1 : 1)));
   DBuf<int> ied, recd;
   CHK(ied.alloc(nblk));
