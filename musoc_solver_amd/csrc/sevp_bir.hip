@@ -110,29 +110,49 @@ __device__ __forceinline__ void dma(const void *src, uint32_t *dst) {
                                    (__attribute__((address_space(3))) void *)(dst), 4, 0, 0);
 }
 
-// Column n of a w x w matrix into registers, WMAX entries from one base (the
-// entries past w belong to the next column or the allocation's padding and
-// are never used).
-template <int WMAX>
+// Column n of a w x w matrix into registers from one base: WX entries when
+// the kernel is specialised for strips of width WX, else WMAX (the entries
+// past w belong to the next column or the allocation's padding and are never
+// used).
+template <int WMAX, int WX>
 __device__ __forceinline__ void load_col(double (&r)[WMAX], const double *mat, int w, int n) {
   const double *c = mat + (long)n * w;
 #pragma unroll
-  for (int m = 0; m < WMAX; ++m) r[m] = c[m];
+  for (int m = 0; m < (WX > 0 ? WX : WMAX); ++m) r[m] = c[m];
 }
 
 // sum_{m=1..w} e(m) * R(m, n), sequential from zero (lane n holds column n).
-// e goes through LDS (eb, the marcher's own buffer) and comes back two
-// entries per broadcast read.
-template <int WMAX>
+// e goes through LDS (eb, the marcher's own buffer) and comes back as
+// broadcast reads of two entries.  Specialised for strips of width WX, the
+// reads are all issued first and the sum is one straight chain; otherwise
+// they go a group of eight entries at a time (unconditional: eb holds WMAX
+// entries) and each term is guarded by m < w.
+template <int WMAX, int WX>
 __device__ __forceinline__ double dot_col(double e, const double (&r)[WMAX], int w, double *eb, int lane) {
   eb[lane] = e;
+  const double2 *e2 = reinterpret_cast<const double2 *>(eb);
   double s = 0.0;
+  if constexpr (WX > 0) {
+    double2 em[(WX + 1) / 2];
 #pragma unroll
-  for (int m = 0; m < WMAX; m += 2) {
-    if (m < w) {
-      const double2 em = *reinterpret_cast<const double2 *>(eb + m);
-      s = s + em.x * r[m];
-      if (m + 1 < w) s = s + em.y * r[m + 1];
+    for (int i = 0; i < (WX + 1) / 2; ++i) em[i] = e2[i];
+#pragma unroll
+    for (int m = 0; m < WX; ++m) s = s + ((m & 1) ? em[m >> 1].y : em[m >> 1].x) * r[m];
+  } else {
+#pragma unroll
+    for (int g = 0; g < WMAX / 8; ++g) {
+      if (8 * g >= w) break;
+      double2 cur[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) cur[i] = e2[4 * g + i];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int m = 8 * g + 2 * i;
+        if (m < w) {
+          s = s + cur[i].x * r[m];
+          if (m + 1 < w) s = s + cur[i].y * r[m + 1];
+        }
+      }
     }
   }
   return s;
@@ -406,8 +426,8 @@ struct MarchIO {
 };
 
 // REPBIR on one strip (MAIN/template/GLOBAL/src/sevp.f90:218-328), consuming
-// the forward and backward record sequences.
-template <int WMAX>
+// the forward and backward record sequences.  WX > 0: the strip width is WX.
+template <int WMAX, int WX>
 __device__ void march_strip(Stage<WMAX> &S, const Strip &s, int lane) {
   const int w = s.w, nblk = s.nblk;
   const int n = lane < w ? lane : 0;
@@ -416,8 +436,8 @@ __device__ void march_strip(Stage<WMAX> &S, const Strip &s, int lane) {
   MarchIO<WMAX> io(S, lane);
 
   // ---------------- forward ----------------
-  if (nblk > 1) load_col<WMAX>(rc, s.rinv1, w, n);
-  else load_col<WMAX>(rc, s.rinv, w, n);
+  if (nblk > 1) load_col<WMAX, WX>(rc, s.rinv1, w, n);
+  else load_col<WMAX, WX>(rc, s.rinv, w, n);
   io.start();
   const int jlast = fwd_jlast(s.ie, nblk);
   double xs, xc;
@@ -464,7 +484,7 @@ __device__ void march_strip(Stage<WMAX> &S, const Strip &s, int lane) {
       double xe = wave_shl1_zero(xc);
       if (lane == w - 1) xe = he;
       const double res = ff - a * xw - b * xs - cc * xc - d * xe - e * xnn;
-      const double dd = dot_col<WMAX>(res, rc, w, S.e, lane);
+      const double dd = dot_col<WMAX, WX>(res, rc, w, S.e, lane);
       const double xcor = xc - dd;
       io.put(0, xcor);
       io.put(1, xc);  // DUM0
@@ -472,10 +492,10 @@ __device__ void march_strip(Stage<WMAX> &S, const Strip &s, int lane) {
       xs = xcor;
       xc = xnn;
       if (nb + 1 < nblk) {
-        load_col<WMAX>(rc, s.rinv1 + nb * ww, w, n);
+        load_col<WMAX, WX>(rc, s.rinv1 + nb * ww, w, n);
       } else {  // no forward closure left: the backward pass's first block
-        load_col<WMAX>(rc, s.rinv + (nblk - 1) * ww, w, n);
-        load_col<WMAX>(rc2, s.rinv1 + (nblk - 2) * ww, w, n);
+        load_col<WMAX, WX>(rc, s.rinv + (nblk - 1) * ww, w, n);
+        load_col<WMAX, WX>(rc2, s.rinv1 + (nblk - 2) * ww, w, n);
       }
     }
     js = ieb;
@@ -503,18 +523,18 @@ __device__ void march_strip(Stage<WMAX> &S, const Strip &s, int lane) {
     if (lane == w - 1) xe = he;
     const double res = A.fl(5) - A.fl(0) * xw - A.fl(1) * A.dbl(6) - A.fl(2) * xrow -
                        A.fl(3) * xe - A.fl(4) * xnn2;
-    const double cv = dot_col<WMAX>(res, rc, w, S.e, lane);
+    const double cv = dot_col<WMAX, WX>(res, rc, w, S.e, lane);
     const double xnew = xf + cv;
     io.put(0, xnew);
     double hs = 0.0;
     if (nbk != 1) {
       const double g = cv * B.fl(4);
-      hs = dot_col<WMAX>(g, rc2, w, S.e, lane);
+      hs = dot_col<WMAX, WX>(g, rc2, w, S.e, lane);
     }
     io.next();
     if (nbk > 1) {  // the next block's matrices, a block ahead
-      load_col<WMAX>(rc, s.rinv + (nbk - 2) * ww, w, n);
-      if (nbk > 2) load_col<WMAX>(rc2, s.rinv1 + (nbk - 3) * ww, w, n);
+      load_col<WMAX, WX>(rc, s.rinv + (nbk - 2) * ww, w, n);
+      if (nbk > 2) load_col<WMAX, WX>(rc2, s.rinv1 + (nbk - 3) * ww, w, n);
     }
     xnn2 = xnew;
     double hc = cv;
@@ -585,7 +605,7 @@ __device__ inline Strip bir_strip(const BirArgs &p, const int *ie, int r, int n)
 // (and the periodic halo refresh) are done by every stager in full: the
 // stagers write identical values, and each reads only what it has itself
 // written before its next pass's DMAs.
-template <int WMAX>
+template <int WMAX, int WX>
 __global__ __launch_bounds__(BIR_THREADS) void bir_sweeps_kernel(BirArgs p) {
   __shared__ Stage<WMAX> S;
   __shared__ int ie[BIR_MAX_BLOCKS];
@@ -595,7 +615,7 @@ __global__ __launch_bounds__(BIR_THREADS) void bir_sweeps_kernel(BirArgs p) {
   if (wave == 0) {
     if (t < WMAX)
       for (int it = 0; it < p.nsweep; ++it)
-        for (int n = 1; n <= p.nbir; ++n) march_strip<WMAX>(S, bir_strip(p, ie, r, n), t);
+        for (int n = 1; n <= p.nbir; ++n) march_strip<WMAX, WX>(S, bir_strip(p, ie, r, n), t);
     return;
   }
   const int sid = wave - 1;
@@ -645,7 +665,7 @@ __global__ __launch_bounds__(BIR_THREADS) void bir_sweeps_kernel(BirArgs p) {
 
 // One REPBIR call with the reference's argument conventions, batched over
 // independent right-hand sides (F and X strided by f_stride / x_stride).
-template <int WMAX>
+template <int WMAX, int WX>
 __global__ __launch_bounds__(BIR_THREADS) void repbir_kernel(RepbirArgs p) {
   __shared__ Stage<WMAX> S;
   __shared__ int ie[BIR_MAX_BLOCKS];
@@ -669,7 +689,7 @@ __global__ __launch_bounds__(BIR_THREADS) void repbir_kernel(RepbirArgs p) {
   s.nblk = p.nblk;
   s.w = p.m2;
   if (wave == 0) {
-    if (t < WMAX) march_strip<WMAX>(S, s, t);
+    if (t < WMAX) march_strip<WMAX, WX>(S, s, t);
     return;
   }
   load_ie(ie, p.ie, p.nblk, t);
@@ -679,19 +699,23 @@ __global__ __launch_bounds__(BIR_THREADS) void repbir_kernel(RepbirArgs p) {
   stage_pass<WMAX>(S, s, L, t, wave - 1, false, p.recs + 4L * p.fwd_records);
 }
 
+// Strips of width 24 and 30 (the 0.5, 0.25 and 0.1 degree configurations)
+// run kernels specialised for their width.
 hipError_t launch_bir_sweeps(const BirArgs &a, int nrhs, hipStream_t st) {
-  if (a.ibir <= 32)
-    hipLaunchKernelGGL(bir_sweeps_kernel<32>, dim3(nrhs), dim3(BIR_THREADS), 0, st, a);
-  else
-    hipLaunchKernelGGL(bir_sweeps_kernel<64>, dim3(nrhs), dim3(BIR_THREADS), 0, st, a);
+  const dim3 g(nrhs), b(BIR_THREADS);
+  if (a.ibir == 24) hipLaunchKernelGGL((bir_sweeps_kernel<32, 24>), g, b, 0, st, a);
+  else if (a.ibir == 30) hipLaunchKernelGGL((bir_sweeps_kernel<32, 30>), g, b, 0, st, a);
+  else if (a.ibir <= 32) hipLaunchKernelGGL((bir_sweeps_kernel<32, 0>), g, b, 0, st, a);
+  else hipLaunchKernelGGL((bir_sweeps_kernel<64, 0>), g, b, 0, st, a);
   return hipGetLastError();
 }
 
 hipError_t launch_repbir(const RepbirArgs &a, int nrhs, hipStream_t st) {
-  if (a.m2 <= 32)
-    hipLaunchKernelGGL(repbir_kernel<32>, dim3(nrhs), dim3(BIR_THREADS), 0, st, a);
-  else
-    hipLaunchKernelGGL(repbir_kernel<64>, dim3(nrhs), dim3(BIR_THREADS), 0, st, a);
+  const dim3 g(nrhs), b(BIR_THREADS);
+  if (a.m2 == 24) hipLaunchKernelGGL((repbir_kernel<32, 24>), g, b, 0, st, a);
+  else if (a.m2 == 30) hipLaunchKernelGGL((repbir_kernel<32, 30>), g, b, 0, st, a);
+  else if (a.m2 <= 32) hipLaunchKernelGGL((repbir_kernel<32, 0>), g, b, 0, st, a);
+  else hipLaunchKernelGGL((repbir_kernel<64, 0>), g, b, 0, st, a);
   return hipGetLastError();
 }
 
