@@ -450,6 +450,7 @@ __device__ void march_strip(Stage<WMAX> &S, const Strip &s, int lane) {
   int js = 1, ieb = ie_at(s.ie, 0);
   for (int nb = 1;; ++nb) {
     const int jend = nb < nblk ? ieb - 2 : jlast;
+    const int ieb_next = nb < nblk ? ie_at(s.ie, nb) : 0;  // requested a block ahead
     auto row = [&](const Rec<WMAX> &r) {
       const double a = r.fl(0), b = r.fl(1), cc = r.fl(2), d = r.fl(3), e = r.fl(4), ff = r.fl(5);
       const double h = r.dbl(6);  // lane 0: X(1, J+1); lane w-1: X(w+2, J+1)
@@ -499,16 +500,16 @@ __device__ void march_strip(Stage<WMAX> &S, const Strip &s, int lane) {
       }
     }
     js = ieb;
-    ieb = ie_at(s.ie, nb);
+    ieb = ieb_next;
   }
   io.finish();
 
   // ---------------- backward ----------------
   io.start();
   double xnn2 = 0.0;  // X(., IE+1): the block above's corrected row JS+1
+  int iebk = ie_at(s.ie, nblk - 1), jsb = nblk != 1 ? ie_at(s.ie, nblk - 2) : 1;
   for (int nbk = nblk; nbk >= 1; --nbk) {
-    const int iebk = ie_at(s.ie, nbk - 1);
-    const int jsb = nbk != 1 ? ie_at(s.ie, nbk - 2) : 1;
+    const int jsb_next = nbk > 2 ? ie_at(s.ie, nbk - 3) : 1;  // block nbk-1's JS, a block ahead
     const int jf = iebk - 2;
     const Rec<WMAX> A = io.take();
     double hw, he;
@@ -558,6 +559,8 @@ __device__ void march_strip(Stage<WMAX> &S, const Strip &s, int lane) {
       row(io.template take_row<1>());
       io.template next<0>();
     }
+    iebk = jsb;
+    jsb = jsb_next;
   }
   io.finish();
 }
