@@ -205,6 +205,61 @@ def test_entry_points_bitwise(oracle, device):
     assert fields_equal(xg, xo)
 
 
+# Block rows the reference's IE rule never produces: one-row blocks among
+# taller ones, a one-row first and last block, every block one row high, and
+# a single block.  They exercise every record kind of the strip kernels'
+# record order (closures right after closures, backward blocks with no march
+# rows, restores that no march row overwrites).
+IRREGULAR_IE = [
+    (40, [2, 3, 5, 6, 9, 14, 15, 22, 30, 31, 41]),
+    (40, list(range(4, 41, 3)) + [41]),
+    (40, list(range(2, 42))),
+    (12, [13]),
+]
+
+
+@pytest.mark.parametrize("m2", [10, 24, 30, 40])
+@pytest.mark.parametrize("case", range(len(IRREGULAR_IE)))
+def test_repbir_irregular_blocks_bitwise(oracle, device, m2, case):
+    """REPBIR through the C ABI on one strip of width M2 (the generic and the
+    width-specialised strip kernels, 32 and 64 lanes) with irregular block
+    rows, bitwise against the oracle, influence matrices from the strip PRE."""
+    from oracle.oracle import P, _ci
+    j2, ie = IRREGULAR_IE[case]
+    i2, off = 48, 4
+    print(f"[repbir-irregular] m2={m2} ie={ie}", flush=True)
+    coefs, _, mask = make(i2, j2, 2, seed=31 + case, metric="latlon", lat_max=70.0)
+    al, ab, ac, ar, at = coefs
+    ie = np.asarray(ie, np.int32)
+    nblk = len(ie)
+    h = np.zeros((i2 + 2, j2 + 2), order="F")
+    rv = np.zeros((m2, m2, nblk), order="F")
+    rv1 = np.zeros((m2, m2, max(nblk - 1, 1)), order="F")
+    S.pre(al, ab, ac, ar, at, rv, rv1, h, ie, i2, m2 + 2, m2, nblk, coef_off=off)
+    rvo = np.zeros((m2, m2, nblk), order="F")
+    rv1o = np.zeros((m2, m2, nblk), order="F")
+    ho = np.zeros((m2 + 2, j2 + 2), order="F")
+    oracle.lib().orc_pre(*[P(c, off) for c in coefs], P(rvo), P(rv1o), P(ho),
+                         P(np.ascontiguousarray(ie, np.int32)), _ci(i2), _ci(m2 + 2), _ci(m2),
+                         _ci(nblk))
+    assert fields_equal(rv, rvo)
+    if nblk > 1:
+        assert fields_equal(rv1, rv1o[:, :, :nblk - 1])
+    s = synth.make_rhs(mask, 41 + case, 1)[:, :, 0]
+    src = np.asfortranarray(s[off:off + m2, :])
+    x0 = rand_x(i2, j2, 1, 51 + case)[:, :, 0].copy(order="F")
+    xg = x0.copy(order="F")
+    dd0 = np.zeros((m2, nblk), order="F")
+    dd1 = np.zeros(m2)
+    S.repbir(al, ab, ac, ar, at, rv, rv1, dd0, dd1, dd1, src, h, xg, ie, i2 + 2, i2, m2 + 2, m2,
+             nblk, coef_off=off, x_off=off)
+    xo = x0.copy(order="F")
+    oracle.repbir_call([(c, off) for c in coefs], rvo, rv1o, dd0.copy(order="F"), dd1.copy(),
+                       dd1.copy(), src, ho, (xo, off), ie, i2 + 2, i2, m2 + 2, m2, nblk)
+    assert all_finite(xo)
+    assert fields_equal(xg, xo)
+
+
 @pytest.mark.parametrize("n", [89, 96, 150, 300, 1024, 1440])
 def test_matinv_blocked_bitwise(oracle, device, n):
     rng = np.random.default_rng(n)
