@@ -107,6 +107,10 @@ def parse():
     ap.add_argument("--sequence", type=int, default=200,
                     help="sequential single-RHS solves at 720x360, warm-started (0: skip)")
     ap.add_argument("--sequence-only", action="store_true", help="run the sequence leg alone")
+    ap.add_argument("--bir", type=int, default=1,
+                    help="also time the reference's default form (5 BIR sweeps) at 720x360, "
+                         "NBIR 60, 2-row blocks, 1 and 64 RHS (0: skip)")
+    ap.add_argument("--bir-only", action="store_true", help="run the BIR leg alone")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device (reference kernel for HBM counters)")
     return ap.parse_args()
@@ -219,6 +223,9 @@ def main():
         return
     if a.sequence_only:
         print(json.dumps({"sequence": sequence_leg(a)}), flush=True)
+        return
+    if a.bir_only:
+        print(json.dumps({"bir": bir_leg(a)}), flush=True)
         return
     if a.tenth_only:
         print(json.dumps({"tenth": tenth_leg(a)}), flush=True)
@@ -350,6 +357,7 @@ def main():
         "krylov": krylov,
         "basins": basins_or_error(a),
         "sequence": sequence_or_error(a),
+        "bir": bir_or_error(a),
     }
     for plan, _ in plans.values():
         plan.close()
@@ -537,6 +545,67 @@ def tenth_leg(a):
         leg["roofline"] = roofline(leg, "mixed", nrhs, i2, a, plan.pass_kind == S.PASS_FUSED)
         log(f"tenth: {leg}")
         return leg
+    finally:
+        plan.close()
+
+
+def bir_or_error(a):
+    """The BIR leg after the main legs, isolated like the basins leg."""
+    if a.bir <= 0:
+        return None
+    try:
+        return bir_leg(a)
+    except Exception as e:  # noqa: BLE001 - reported, not hidden
+        log(f"bir leg failed: {e!r}")
+        return {"error": repr(e)}
+
+
+def bir_leg(a):
+    """The reference's default solver form (FL_EVP_STP = 1): 5 multiplicative
+    BIR sweeps over 60 overlapping 24-column strips of the 0.5 degree grid
+    (720x360), REPBIR per strip, 2-row blocks, plain FP64 and bitwise the
+    reference's arithmetic.  One solve is a dependent chain of 5 x 60 x 720 row
+    steps (each a 77 ns FP64 chain on its own), so it is timed for one RHS and
+    for 64 independent RHS (one workgroup each); the first RHS of the batch
+    must equal the single solve bit for bit."""
+    import torch
+    from musoc_solver_amd import synth
+    from musoc_solver_amd import sevp as S
+    i2, j2, nbir, h, nb = 720, 360, 60, 2, 64
+    case = synth.make_case(i2, j2, a.seed + 720, metric="latlon", lat_max=a.lat_max, h_land=a.h_land)
+    ie = synth.reference_ie(j2, synth.nb0_for_height(j2, h))
+    plan = S.SevpPlan(case.coefs, ie, form=S.FORM_BIR, nbir=nbir, nsweep=5, nrhs_max=nb)
+    try:
+        t0 = time.perf_counter()
+        plan.build_influence()
+        torch.cuda.synchronize()
+        setup = time.perf_counter() - t0
+        s_np = synth.make_rhs(case.mask, a.seed + 721, nb)
+        sd = torch.from_numpy(np.ascontiguousarray(s_np.transpose(2, 1, 0))).cuda()
+        out = {"workload": f"{i2}x{j2} NBIR {nbir} (24-column strips), block height {h}, 5 sweeps",
+               "setup_s": setup}
+        xs = {}
+        for n in (1, nb):
+            x = torch.zeros((n, j2 + 2, i2 + 2), dtype=torch.float64, device="cuda")
+            plan.solve_dev(sd[:n], x)  # warm-up
+            ts = []
+            for _ in range(3):
+                x.zero_()
+                plan.reset_seam()  # the seam scratch XX carries over between solves
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                plan.solve_dev(sd[:n], x)
+                torch.cuda.synchronize()
+                ts.append(time.perf_counter() - t1)
+            xs[n] = x
+            out[f"ms_per_call_{n}rhs"] = 1e3 * min(ts)
+            out[f"solves_per_s_{n}rhs"] = n / min(ts)
+        row_steps = 5 * nbir * 2 * j2
+        out["row_step_ns_1rhs"] = 1e6 * out["ms_per_call_1rhs"] / row_steps
+        out["chain_ns"] = 77.0
+        out["batch_first_equals_single"] = bool(torch.equal(xs[1][0], xs[nb][0]))
+        log(f"bir: {out}")
+        return out
     finally:
         plan.close()
 
