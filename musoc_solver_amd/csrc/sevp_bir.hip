@@ -11,21 +11,22 @@
 //
 // A row step is a dependent FP64 chain (two wave shifts, four products and
 // the ordered differences, a correctly rounded division), so the workgroup's
-// two wavefronts split the work to keep memory off that chain:
+// four wavefronts split the work to keep memory off that chain:
 //  * wave 0, the marcher, reads every row operand from LDS, keeps the live
 //    rows in registers, exchanges the stencil's east/west neighbours with DPP
 //    wave shifts and writes its results to LDS.  Its only global accesses are
 //    the influence-matrix columns (column n of RINV/RINV1 for lane n), loaded
 //    into registers a whole block before the closure that uses them.
-//  * wave 1, the stager, streams each pass's operands into LDS with
-//    global_load_lds DMAs (no registers, no waits while they fly) and stores
-//    the march's results back to X and DUM0.
+//  * waves 1-3, the stagers, stream each pass's operands into LDS with
+//    global_load_lds DMAs (no registers, no waits while they fly) and store
+//    the march's results back to X and DUM0, each taking every third record.
 // A pass is a sequence of records, one per march row or closure, in the order
-// the marcher consumes them.  Records travel in chunks of R through three
-// input buffers: while the marcher works on chunk c the stager fills chunk
-// c+1 and drains chunk c-1, and one workgroup barrier ends each chunk, on
-// both wavefronts, so the two agree on the barrier count by construction
-// (1 + ceil(records / R) per pass).
+// the marcher consumes them (the host-built record table, sevp_records.h).
+// Records travel in chunks of R through three input buffers: while the
+// marcher works on chunk c the stagers fill chunk c+1 and drain chunk c-1,
+// and one workgroup barrier ends each chunk on all four wavefronts, so they
+// agree on the barrier count by construction (1 + ceil(records / R) per
+// pass, plus a pass-end barrier).
 //
 // A record is 8 slots of WMAX dwords; lane l's value of slot s is dword
 // s*WMAX + l.  A double per lane takes two slots: low words, then high words.
@@ -161,7 +162,7 @@ __device__ __forceinline__ double dot_col(double e, const double (&r)[WMAX], int
 // ------------------------------------------------------------ record order
 // One record of a pass (an entry of bir_record_table): its kind, block nb,
 // row j and the block's IE and JS, all wave-uniform.
-struct Cursor {
+struct RecInfo {
   int kind, j, nb, js, ieb;
 };
 
@@ -195,7 +196,7 @@ struct StagerLane {
 
 template <int WMAX>
 __device__ __forceinline__ void fill_record(Stage<WMAX> &S, const Strip &s, const StagerLane<WMAX> &L,
-                                            int b, int p, const Cursor &k, int t) {
+                                            int b, int p, const RecInfo &k, int t) {
   using G = Stage<WMAX>;
   uint32_t *rec = S.in[b][p];
   auto frows = [&](int j) {
@@ -253,7 +254,7 @@ __device__ __forceinline__ void fill_record(Stage<WMAX> &S, const Strip &s, cons
 // chosen first; only the stores themselves are limited to the strip's lanes.
 template <int WMAX>
 __device__ __forceinline__ void drain_record(Stage<WMAX> &S, const Strip &s, int b, int ob, int p,
-                                             const Cursor &k, int t, double o0) {
+                                             const RecInfo &k, int t, double o0) {
   double *d0 = nullptr, *d1 = nullptr, *dr = nullptr;
   if (k.kind == BIR_ROW) {  // row IE(nb) of a closing block waits for its correction
     if (!(k.nb < s.nblk && k.j == k.ieb - 2)) d0 = s.xrow(k.j + 2);
@@ -276,8 +277,8 @@ __device__ __forceinline__ void drain_record(Stage<WMAX> &S, const Strip &s, int
 
 // One pass for stager `sid`: it fills and drains the records p of each chunk
 // with p % NSTG == sid.  The record-table entries of each chunk come into an
-// LDS ring by DMA two chunks ahead (stager 0 issues them; the first two
-// chunks' every stager loads for itself).  Records of one pass read only rows
+// LDS ring by DMA two chunks ahead: stager 0 issues them, except the first two
+// chunks' entries, which every stager loads for itself.  Records of one pass read only rows
 // that no record of the same pass writes, and write distinct rows, so the
 // stagers need no ordering among themselves inside a pass; the barrier after
 // the last drain makes every result visible to the next pass's DMAs.
@@ -289,7 +290,8 @@ __device__ void stage_pass(Stage<WMAX> &S, const Strip &s, const StagerLane<WMAX
   const int C = (T + R - 1) / R;
   auto load_desc = [&](int c) {
 #pragma unroll
-    for (int k = 0; k < R * 4 / 64; ++k) dma(recs + (long)c * R * 4 + 64 * k + t, reinterpret_cast<uint32_t *>(&S.desc[c & 3][0][0]) + 64 * k);
+    for (int k = 0; k < R * 4 / 64; ++k)
+      dma(recs + (long)c * R * 4 + 64 * k + t, reinterpret_cast<uint32_t *>(&S.desc[c & 3][0][0]) + 64 * k);
   };
   // This stager's entries of chunk c, one per lane (lane i: record sid +
   // i*NSTG), read at once; the i-th is then taken with lane reads.
@@ -300,7 +302,7 @@ __device__ void stage_pass(Stage<WMAX> &S, const Strip &s, const StagerLane<WMAX
     return d;
   };
   auto cur_at = [&](const int4 &d, int i) {
-    Cursor k;
+    RecInfo k;
     const int kn = __builtin_amdgcn_readlane(d.x, i);
     k.kind = kn & 255;
     k.nb = kn >> 8;
@@ -566,10 +568,10 @@ __device__ void march_strip(Stage<WMAX> &S, const Strip &s, int lane) {
 }
 
 // ----------------------------------------------------------------- kernels
-// The block rows IE live in LDS for the whole launch: the stager copies them
-// before its first chunk, so the marcher (which reads them only after the
-// first chunk barrier) and the stager's cursors never wait on global memory
-// for them.
+// The block rows IE live in LDS for the whole launch: every stager copies
+// them (the same values) before its first chunk, so neither the marcher, which
+// reads them only after the first chunk barrier, nor the stagers wait on
+// global memory for them.
 __device__ inline void load_ie(int *dst, const int *src, int nblk, int t) {
   for (int b = t; b < nblk; b += SEVP_WAVE) dst[b] = src[b];
   wait_vm();
