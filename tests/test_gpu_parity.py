@@ -42,6 +42,8 @@ BIR_CASES = [
     # 23-row march grows slowly and the solution stays finite
     (192, 96, 16, 24, dict(metric="mercator", dydx=0.1)),
     (120, 60, 8, 5, dict(metric="latlon", lat_max=80.0)),  # IBIR = 30
+    (96, 48, 4, 4, dict(metric="latlon", lat_max=70.0)),  # IBIR = 48: the 64-lane kernel
+    (16, 12, 16, 3, dict(metric="latlon", lat_max=60.0)),  # IBIR = 2, the narrowest strip
 ]
 
 
