@@ -604,10 +604,37 @@ def bir_leg(a):
         out["row_step_ns_1rhs"] = 1e6 * out["ms_per_call_1rhs"] / row_steps
         out["chain_ns"] = 77.0
         out["batch_first_equals_single"] = bool(torch.equal(xs[1][0], xs[nb][0]))
+        if a.cpu_rhs > 0:
+            out["cpu_baseline"] = bir_cpu_baseline(plan, case, ie, nbir, s_np, xs[1][0])
         log(f"bir: {out}")
         return out
     finally:
         plan.close()
+
+
+def bir_cpu_baseline(plan, case, ie, nbir, s_np, x_gpu):
+    """The BIR leg's CPU baseline: one solve of the same workload (the first
+    right-hand side, 5 sweeps from x = 0) by the oracle's C FS_SOLVER loop
+    around its REPBIR restatement, on one host core, with the influence
+    matrices of the GPU build (bitwise the reference's).  Its result must
+    equal the GPU's single solve bit for bit."""
+    from oracle import oracle as O
+    O.build()
+    rinv, rinv1 = plan.get_influence()
+    i2, j2 = case.shape
+    ts = []
+    for _ in range(2):
+        x = np.zeros((i2 + 2, j2 + 2), order="F")
+        xx = np.zeros((2 * i2 // nbir + 2, j2 + 2), order="F")
+        t0 = time.perf_counter()
+        O.bir_solve(case.coefs, s_np[:, :, 0], rinv, rinv1, x, xx, ie, nbir)
+        ts.append(time.perf_counter() - t0)
+    t = min(ts)
+    xg = x_gpu.cpu().numpy().T
+    return {"value": 1.0 / t, "unit": "solves/s", "cores": 1, "kind": "port",
+            "ms_per_solve": 1e3 * t,
+            "equals_gpu": bool(np.array_equal(x.view(np.int64), xg.view(np.int64))),
+            "sample": "one 1-RHS solve (5 sweeps from x = 0), best of two, oracle C loop"}
 
 
 def sequence_or_error(a):
