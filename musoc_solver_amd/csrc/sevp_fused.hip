@@ -84,7 +84,6 @@ struct KF32 {
   typedef f32x4 AV;
   typedef f32x4 BV;
   typedef f32x4 CV;
-  static constexpr int G = 6;  // chunks per load group
   __device__ static int row(int q, int i) { return 4 * q + i; }
   __device__ static CV mma(AV a, BV b, CV c) {
 #pragma unroll
@@ -99,7 +98,6 @@ struct KF64 {
   typedef f64x4 AV;
   typedef f64x4 BV;
   typedef f64x4 CV;
-  static constexpr int G = 4;
   __device__ static int row(int q, int i) { return q + 4 * i; }  // the FP64 instruction's D layout
   __device__ static CV mma(AV a, BV b, CV c) {
 #pragma unroll
@@ -114,13 +112,27 @@ struct KB16 {
   typedef f32x4 AV;
   typedef s16x4 BV;
   typedef f32x4 CV;
-  static constexpr int G = 8;
   __device__ static int row(int q, int i) { return 4 * q + i; }
   __device__ static CV mma(AV a, BV b, CV c) {
     const s16x4 ab = {bf16_rne(a[0]), bf16_rne(a[1]), bf16_rne(a[2]), bf16_rne(a[3])};
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ab, b, c, 0, 0, 0);
   }
 };
+
+// Two workgroups per CU for the FP32 product with 16 wavefronts and one row
+// tile (the mixed precision default): at most 64 VGPRs (8 waves per SIMD),
+// which the kernels reach without spills when their load groups are halved
+// (3 chunks instead of 6: the same loads in flight per CU).  A backward
+// launch of two 32-RHS row tiles has about 400 workgroups, more than one per
+// CU; with two per CU it runs in one round.  The sums are unchanged.
+template <class KT, int W, int MBW>
+constexpr bool two_per_cu() {
+  return std::is_same<KT, KF32>::value && W == 16 && MBW == 1;
+}
+template <class KT, int W, int MBW>
+constexpr int min_waves_per_simd() {
+  return two_per_cu<KT, W, MBW>() ? 8 : 1;
+}
 
 // Partial tile of one wavefront over the chunks [c_lo, c_hi) of 16 k: MBW
 // 16-row tiles of A (row tiles mb0 .. mb0+MBW-1, clamped to the last one,
@@ -173,7 +185,7 @@ __device__ __forceinline__ void part_tile(const typename KT::A *alt, int mb0, in
 
 // The product tile of a workgroup: W wavefronts, each its share of the K
 // chunks (96 / W chunks per load group: one group at K <= 1536; half that
-// for FP64 with two row tiles); their
+// for FP64 with two row tiles and for the two-per-CU FP32 kernels); their
 // partial tiles go to LDS and thread t < 256 MBW receives the FP64 sum over
 // the wavefronts, in wavefront order, of row t / 16 (of the MBW row tiles)
 // and column t % 16 of the tile.  Every thread of the workgroup must call it.
@@ -188,7 +200,8 @@ __device__ __forceinline__ double product_tile(const typename KT::A *alt, int mb
   typename KT::CV acc[MBW];
 #pragma unroll
   for (int mb = 0; mb < MBW; ++mb) acc[mb] = typename KT::CV{0, 0, 0, 0};
-  constexpr int G = (std::is_same<KT, KF64>::value && MBW == 2) ? 48 / W : 96 / W;
+  constexpr int G =
+      ((std::is_same<KT, KF64>::value && MBW == 2) || two_per_cu<KT, W, MBW>()) ? 48 / W : 96 / W;
   part_tile<KT, G, MBW>(alt, mb0, mtiles, blt, n, nchk, c_lo, c_hi, acc);
 #pragma unroll
   for (int mb = 0; mb < MBW; ++mb)
@@ -230,7 +243,7 @@ __device__ __forceinline__ const FT *rhs_ptr(const float *f, const double *f64) 
 // the FP64 operands take them), the right-hand side and coefficients of the
 // first march row and of the closure row.
 template <class KT, int W, int MBW, class FT>
-__global__ __launch_bounds__(64 * W) void fused_fwd_kernel(FusedFwdArgs p) {
+__global__ __launch_bounds__(64 * W, (min_waves_per_simd<KT, W, MBW>())) void fused_fwd_kernel(FusedFwdArgs p) {
   __shared__ typename KT::Acc red[W * 256 * MBW];
   constexpr bool PF = !std::is_same<KT, KF64>::value && W == 8;
   span_begin(p.tspan);
@@ -340,7 +353,7 @@ __global__ __launch_bounds__(64 * W) void fused_fwd_kernel(FusedFwdArgs p) {
 // clamped addresses (column I2 for lanes past it, row 1 for the closure row
 // of the first block, which has none).
 template <class KT, int W, int MBW, class FT>
-__global__ __launch_bounds__(64 * W) void fused_bwd_kernel(FusedBwdArgs p) {
+__global__ __launch_bounds__(64 * W, (min_waves_per_simd<KT, W, MBW>())) void fused_bwd_kernel(FusedBwdArgs p) {
   __shared__ typename KT::Acc red[W * 256 * MBW];
   constexpr bool PF = !std::is_same<KT, KF64>::value && W == 8;
   span_begin(p.tspan);
