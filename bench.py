@@ -109,7 +109,7 @@ def parse():
     ap.add_argument("--sequence-only", action="store_true", help="run the sequence leg alone")
     ap.add_argument("--bir", type=int, default=1,
                     help="also time the reference's default form (5 BIR sweeps) at 720x360, "
-                         "NBIR 60, 2-row blocks, 1 and 64 RHS (0: skip)")
+                         "NBIR 60, 2-row blocks, 1, 64 and 256 RHS (0: skip)")
     ap.add_argument("--bir-only", action="store_true", help="run the BIR leg alone")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device (reference kernel for HBM counters)")
@@ -566,26 +566,26 @@ def bir_leg(a):
     (720x360), REPBIR per strip, 2-row blocks, plain FP64 and bitwise the
     reference's arithmetic.  One solve is a dependent chain of 5 x 60 x 720 row
     steps (each a 77 ns FP64 chain on its own), so it is timed for one RHS and
-    for 64 independent RHS (one workgroup each); the first RHS of the batch
-    must equal the single solve bit for bit."""
+    for 64 and 256 independent RHS (one workgroup, i.e. one CU, each); the
+    first RHS of the 64-RHS batch must equal the single solve bit for bit."""
     import torch
     from musoc_solver_amd import synth
     from musoc_solver_amd import sevp as S
-    i2, j2, nbir, h, nb = 720, 360, 60, 2, 64
+    i2, j2, nbir, h, nb, nbig = 720, 360, 60, 2, 64, 256
     case = synth.make_case(i2, j2, a.seed + 720, metric="latlon", lat_max=a.lat_max, h_land=a.h_land)
     ie = synth.reference_ie(j2, synth.nb0_for_height(j2, h))
-    plan = S.SevpPlan(case.coefs, ie, form=S.FORM_BIR, nbir=nbir, nsweep=5, nrhs_max=nb)
+    plan = S.SevpPlan(case.coefs, ie, form=S.FORM_BIR, nbir=nbir, nsweep=5, nrhs_max=nbig)
     try:
         t0 = time.perf_counter()
         plan.build_influence()
         torch.cuda.synchronize()
         setup = time.perf_counter() - t0
-        s_np = synth.make_rhs(case.mask, a.seed + 721, nb)
+        s_np = synth.make_rhs(case.mask, a.seed + 721, nbig)
         sd = torch.from_numpy(np.ascontiguousarray(s_np.transpose(2, 1, 0))).cuda()
         out = {"workload": f"{i2}x{j2} NBIR {nbir} (24-column strips), block height {h}, 5 sweeps",
                "setup_s": setup}
         xs = {}
-        for n in (1, nb):
+        for n in (1, nb, nbig):
             x = torch.zeros((n, j2 + 2, i2 + 2), dtype=torch.float64, device="cuda")
             plan.solve_dev(sd[:n], x)  # warm-up
             ts = []

@@ -316,7 +316,48 @@ __global__ __launch_bounds__(256) void gemm_seq_kernel(GemmArgs p) {
       const double2 *ar = reinterpret_cast<const double2 *>(&As[tm][0]);
       const double2 *br = reinterpret_cast<const double2 *>(&Bs[tn][0]);
       const int kc2 = kc / 2;
-      if (neg) {
+      if (kc == GT_K) {
+        // a whole chunk in groups of 16 k: the group's LDS reads are issued
+        // one group ahead and its 16 products (independent of the sum) are
+        // formed before the 16 dependent additions, which then follow one
+        // another at the add latency instead of each waiting for its
+        // multiply.  Same products, same order of additions.
+        constexpr int GP = 8;  // k pairs per group
+        double2 a_c[GP], b_c[GP], a_n[GP], b_n[GP];
+#pragma unroll
+        for (int j = 0; j < GP; ++j) {
+          a_c[j] = ar[j];
+          b_c[j] = br[j];
+        }
+#pragma unroll
+        for (int g = 0; g < GT_K / 2 / GP; ++g) {
+          if (g + 1 < GT_K / 2 / GP) {
+#pragma unroll
+            for (int j = 0; j < GP; ++j) {
+              a_n[j] = ar[(g + 1) * GP + j];
+              b_n[j] = br[(g + 1) * GP + j];
+            }
+          }
+          double pr[2 * GP];
+#pragma unroll
+          for (int j = 0; j < GP; ++j) {
+            pr[2 * j] = a_c[j].x * b_c[j].x;
+            pr[2 * j + 1] = a_c[j].y * b_c[j].y;
+          }
+          if (neg) {
+#pragma unroll
+            for (int j = 0; j < 2 * GP; ++j) acc = acc - pr[j];
+          } else {
+#pragma unroll
+            for (int j = 0; j < 2 * GP; ++j) acc = acc + pr[j];
+          }
+#pragma unroll
+          for (int j = 0; j < GP; ++j) {
+            a_c[j] = a_n[j];
+            b_c[j] = b_n[j];
+          }
+        }
+      } else if (neg) {
         for (int k2 = 0; k2 < kc2; ++k2) {
           const double2 av = ar[k2], bv = br[k2];
           acc = acc - av.x * bv.x;
