@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <deque>
 #include <vector>
 
@@ -822,22 +823,50 @@ __device__ __forceinline__ void resid_norm_body(const float *al, const float *ab
   const int j_lo = (int)((long)j2 * ch / nchunk), j_hi = (int)((long)j2 * (ch + 1) / nchunk);
   const double *xr = x + (long)r * x_stride;
   const int i0 = i2 + 2;
+  const float *sr = s ? s + (long)r * s_stride : nullptr;
+  const double *s64r = s64 ? s64 + (long)r * s_stride : nullptr;
+  double *ro = rout ? rout + (long)r * n : nullptr;
   double a0 = 0.0, a1 = 0.0;
-  for (int jj = j_lo; jj < j_hi; ++jj) {
-    const int j = jj + 1;
-    const double *xs = xr + f2(i0, 1, j), *xc = xr + f2(i0, 1, j + 1), *xn = xr + f2(i0, 1, j + 2);
-    for (int ii = t; ii < i2; ii += 256) {
-      const long q = (long)jj * i2 + ii;
-      const double ax = (double)ab[q] * xs[ii + 1] + (double)al[q] * xc[ii] +
-                        (double)ac[q] * xc[ii + 1] + (double)ar[q] * xc[ii + 2] +
-                        (double)at[q] * xn[ii + 1];
-      const double sv = s64 ? s64[(long)r * s_stride + q] : (double)s[(long)r * s_stride + q];
-      const double rv = sv - ax;
-      if (rout) rout[(long)r * n + q] = rv;
-      a0 += rv * rv;
-      a1 += sv * sv;
+  // the operands of U columns (t, t + 256, ...) are loaded before any of
+  // them is used, so their loads are in flight together; the columns are
+  // then summed in the same order as one at a time
+  constexpr int U = 3;
+  auto rows = [&](auto wide) {
+    constexpr bool W64 = decltype(wide)::value;
+    for (int jj = j_lo; jj < j_hi; ++jj) {
+      const int j = jj + 1;
+      const double *xs = xr + f2(i0, 1, j), *xc = xr + f2(i0, 1, j + 1), *xn = xr + f2(i0, 1, j + 2);
+      for (int i00 = t; i00 < i2; i00 += 256 * U) {
+        float cl[U], cb[U], cc[U], cr[U], ct[U];
+        double x0[U], x1[U], x2[U], x3[U], x4[U], sv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int ii = min(i00 + 256 * u, i2 - 1);  // a column past I2 repeats the last
+          const long q = (long)jj * i2 + ii;
+          cb[u] = ab[q]; cl[u] = al[q]; cc[u] = ac[q]; cr[u] = ar[q]; ct[u] = at[q];
+          x0[u] = xs[ii + 1]; x1[u] = xc[ii]; x2[u] = xc[ii + 1]; x3[u] = xc[ii + 2];
+          x4[u] = xn[ii + 1];
+          if constexpr (W64) sv[u] = s64r[q];
+          else sv[u] = (double)sr[q];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int ii = i00 + 256 * u;
+          if (ii < i2) {
+            const double ax = (double)cb[u] * x0[u] + (double)cl[u] * x1[u] +
+                              (double)cc[u] * x2[u] + (double)cr[u] * x3[u] +
+                              (double)ct[u] * x4[u];
+            const double rv = sv[u] - ax;
+            if (ro) ro[(long)jj * i2 + ii] = rv;
+            a0 += rv * rv;
+            a1 += sv[u] * sv[u];
+          }
+        }
+      }
     }
-  }
+  };
+  if (s64r) rows(std::true_type{});
+  else rows(std::false_type{});
   red[0][t] = a0;
   red[1][t] = a1;
   __syncthreads();
