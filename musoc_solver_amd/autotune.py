@@ -19,16 +19,27 @@ from __future__ import annotations
 import numpy as np
 
 
+def _test_rows(i2: int, nvec: int, seed: int) -> np.ndarray:
+    """The march's free rows: nvec random ones and the most oscillatory mode."""
+    rng = np.random.default_rng(seed)
+    v = rng.standard_normal((i2, nvec))
+    return np.concatenate([v, ((-1.0) ** np.arange(i2))[:, None]], axis=1)
+
+
 def march_growth(coefs, js: int, hmax: int, periodic: bool = True, nvec: int = 4,
                  seed: int = 0) -> np.ndarray:
     """Growth g[h-1] = max over test vectors of |H(js+h)| / |H(js+1)|, for
     block heights h = 1..hmax starting at row js (1-based, the row below the
     free row), with H(js) = 0 as in the march of an influence column."""
-    al, ab, ac, ar, at = [np.asarray(c, np.float64) for c in coefs]
+    c64 = [np.asarray(c, np.float64) for c in coefs]
+    return _growth(c64, _test_rows(c64[0].shape[0], nvec, seed), js, hmax, periodic)
+
+
+def _growth(c64, v, js: int, hmax: int, periodic: bool, stop: float = np.inf) -> np.ndarray:
+    """march_growth on FP64 coefficients and given free rows v; the heights
+    past the first whose growth exceeds ``stop`` are not marched (inf)."""
+    al, ab, ac, ar, at = c64
     i2, j2 = al.shape
-    rng = np.random.default_rng(seed)
-    v = rng.standard_normal((i2, nvec))
-    v = np.concatenate([v, ((-1.0) ** np.arange(i2))[:, None]], axis=1)
     h0 = np.zeros_like(v)       # row js
     h1 = v / np.abs(v).max(axis=0)  # free row js+1
     base = np.abs(h1).max(axis=0)
@@ -49,19 +60,24 @@ def march_growth(coefs, js: int, hmax: int, periodic: bool = True, nvec: int = 4
                + ar[:, jj, None] * he) / at[:, jj, None]
         hs, hc = hc, hn
         g[h - 1] = float((np.abs(hn).max(axis=0) / base).max())
+        if g[h - 1] > stop:
+            g[h:] = np.inf
+            break
     return g
 
 
 def adaptive_ie(coefs, max_growth: float = 1e3, hmax: int = 8, hmin: int = 2,
                 periodic: bool = True) -> np.ndarray:
     """IE(1..NB0) with each block as tall as the growth bound allows."""
-    al = coefs[0]
-    j2 = al.shape[1]
+    # the coefficients in FP64 and the free rows once, not per block
+    c64 = [np.asarray(c, np.float64) for c in coefs]
+    v = _test_rows(c64[0].shape[0], 4, 0)
+    j2 = c64[0].shape[1]
     j1 = j2 + 1
     ie = []
     prev = 1  # IE(0) := 1 (the first block starts at row 1)
     while prev < j1:
-        g = march_growth(coefs, prev, hmax, periodic)
+        g = _growth(c64, v, prev, hmax, periodic, stop=max_growth)
         h = hmin
         for hh in range(hmin, hmax + 1):
             if g[hh - 1] <= max_growth:
