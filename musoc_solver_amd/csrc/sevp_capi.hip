@@ -810,6 +810,7 @@ static int run_full_pass(const FullCtx &c, const float *f32, const double *f64, 
 // ------------------------------------------------------------- residual
 // part[(r*nchunk + chunk)*2 + {0,1}] = partial sums of (s - A x)^2 and s^2.
 namespace sevp {
+template <int U>
 __device__ __forceinline__ void resid_norm_body(const float *al, const float *ab, const float *ac,
                                                 const float *ar, const float *at, const float *s,
                                                 const double *s64, long s_stride, const double *x,
@@ -830,7 +831,6 @@ __device__ __forceinline__ void resid_norm_body(const float *al, const float *ab
   // the operands of U columns (t, t + 256, ...) are loaded before any of
   // them is used, so their loads are in flight together; the columns are
   // then summed in the same order as one at a time
-  constexpr int U = 3;
   auto rows = [&](auto wide) {
     constexpr bool W64 = decltype(wide)::value;
     for (int jj = j_lo; jj < j_hi; ++jj) {
@@ -883,6 +883,8 @@ __device__ __forceinline__ void resid_norm_body(const float *al, const float *ab
   }
 }
 
+// U: columns per thread loaded together (resid_norm_cols).
+template <int U>
 __global__ __launch_bounds__(256) void resid_norm_kernel(const float *al, const float *ab,
                                                          const float *ac, const float *ar,
                                                          const float *at, const float *s,
@@ -891,7 +893,7 @@ __global__ __launch_bounds__(256) void resid_norm_kernel(const float *al, const 
                                                          double *rout, int i2, int j2, int nchunk,
                                                          double *part, const int *gate) {
   if (gate && gate[1]) return;  // captured refinement: converged, the residual stays
-  resid_norm_body(al, ab, ac, ar, at, s, s64, s_stride, x, x_stride, rout, i2, j2, nchunk, part,
+  resid_norm_body<U>(al, ab, ac, ar, at, s, s64, s_stride, x, x_stride, rout, i2, j2, nchunk, part,
                   blockIdx.y, blockIdx.x);
 }
 
@@ -911,7 +913,7 @@ __global__ __launch_bounds__(256) void batch_resid_norm_kernel(const NormEnt *__
   const int e = group_entry(start, n, b);
   const NormEnt &q = tab[e];
   const int l = b - start[e];
-  resid_norm_body(q.al, q.ab, q.ac, q.ar, q.at, q.s, nullptr, q.s_stride, q.x, q.x_stride, q.rout,
+  resid_norm_body<1>(q.al, q.ab, q.ac, q.ar, q.at, q.s, nullptr, q.s_stride, q.x, q.x_stride, q.rout,
                   q.i2, q.j2, q.nchunk, q.part, l / q.nchunk, l % q.nchunk);
 }
 
@@ -996,14 +998,34 @@ static int zero_async(void *ptr, size_t bytes, hipStream_t st) {
   return SEVP_OK;
 }
 
+// The residual-norm launch with U = the columns per thread of a row (at most
+// 3) loaded together: a kernel's registers are those of its widest path, so
+// narrow grids keep the one-column kernel.
+static void launch_resid_norm_cols(sevp_plan p, const float *s, const double *s64, long s_stride,
+                                   const double *x, long x_stride, double *rout, int nrhs,
+                                   const int *gate, hipStream_t st) {
+  const int i2 = p->d.i2, j2 = p->d.j2;
+  const dim3 grid(NCHUNK, nrhs);
+  if (i2 > 512)
+    hipLaunchKernelGGL(resid_norm_kernel<3>, grid, dim3(256), 0, st, p->al(), p->ab(), p->ac(),
+                       p->ar(), p->at(), s, s64, s_stride, x, x_stride, rout, i2, j2, NCHUNK,
+                       p->part.p, gate);
+  else if (i2 > 256)
+    hipLaunchKernelGGL(resid_norm_kernel<2>, grid, dim3(256), 0, st, p->al(), p->ab(), p->ac(),
+                       p->ar(), p->at(), s, s64, s_stride, x, x_stride, rout, i2, j2, NCHUNK,
+                       p->part.p, gate);
+  else
+    hipLaunchKernelGGL(resid_norm_kernel<1>, grid, dim3(256), 0, st, p->al(), p->ab(), p->ac(),
+                       p->ar(), p->at(), s, s64, s_stride, x, x_stride, rout, i2, j2, NCHUNK,
+                       p->part.p, gate);
+}
+
 static int residual_norms(sevp_plan p, const float *s, const double *s64, long s_stride,
                           const double *x, long x_stride, double *rout, int nrhs,
                           std::vector<double> &rel) {
   const int i2 = p->d.i2, j2 = p->d.j2;
   p->prof.start(SEVP_KCLASS_RESID);
-  hipLaunchKernelGGL(resid_norm_kernel, dim3(NCHUNK, nrhs), dim3(256), 0, p->stream, p->al(),
-                     p->ab(), p->ac(), p->ar(), p->at(), s, s64, s_stride, x, x_stride, rout, i2,
-                     j2, NCHUNK, p->part.p, (const int *)nullptr);
+  launch_resid_norm_cols(p, s, s64, s_stride, x, x_stride, rout, nrhs, nullptr, p->stream);
   HIPC(hipGetLastError());
   p->prof.stop(SEVP_KCLASS_RESID);
   std::vector<double> h((size_t)nrhs * NCHUNK * 2);
@@ -2055,9 +2077,8 @@ static int run_group_passes(sevp_plan p, const FullCtx &c, int nrhs, int ngroups
 static void launch_resid_norm(sevp_plan p, const float *s, const double *x, int nrhs, const int *gate,
                               hipStream_t st) {
   const int i2 = p->d.i2, j2 = p->d.j2;
-  hipLaunchKernelGGL(resid_norm_kernel, dim3(NCHUNK, nrhs), dim3(256), 0, st, p->al(), p->ab(),
-                     p->ac(), p->ar(), p->at(), s, (const double *)nullptr, (long)i2 * j2, x,
-                     (long)(i2 + 2) * (j2 + 2), p->r64.p, i2, j2, NCHUNK, p->part.p, gate);
+  launch_resid_norm_cols(p, s, nullptr, (long)i2 * j2, x, (long)(i2 + 2) * (j2 + 2), p->r64.p,
+                         nrhs, gate, st);
 }
 
 // Capture the whole refinement for `passes` passes (see RefineGraph).  It
