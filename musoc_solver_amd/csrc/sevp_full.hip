@@ -245,7 +245,17 @@ __global__ __launch_bounds__(256) void gemm_seq_kernel(GemmArgs p) {
   __shared__ __align__(16) double As[GT_M][GT_K + 2];
   __shared__ __align__(16) double Bs[GT_N][GT_K + 2];
   const int t = threadIdx.x;
-  const int n0 = blockIdx.x * GT_N, m0 = blockIdx.y * GT_M;
+  // XCD-aware placement: the dispatcher deals workgroups to the 8 XCDs in
+  // turn, so the row tiles of one column tile get ids on the same XCD, one
+  // after another, and read that tile of B from HBM once (the others from
+  // the XCD's L2); ids past the last column tile have no tile
+  const int ntn = (p.N + GT_N - 1) / GT_N, nmb = (p.M + GT_M - 1) / GT_M;
+  const int slot = blockIdx.x >> 3, tcol = (slot / nmb) * 8 + (blockIdx.x & 7);
+  if (tcol >= ntn) {
+    span_end(p.tspan);
+    return;
+  }
+  const int n0 = tcol * GT_N, m0 = (slot % nmb) * GT_M;
   const int tn = t % GT_N, tm = t / GT_N;
   // staging: each thread moves 4 elements of A and 4 of B per chunk
   double ra[4], rb[4];
@@ -422,11 +432,11 @@ hipError_t launch_scale_cols(const double *r, const float *cy, int cyrow, int i2
 }
 
 long gemm_workgroups(const GemmArgs &a) {
-  return (long)((a.N + GT_N - 1) / GT_N) * ((a.M + GT_M - 1) / GT_M);
+  return 8L * ((a.M + GT_M - 1) / GT_M) * (((a.N + GT_N - 1) / GT_N + 7) / 8);
 }
 
 hipError_t launch_gemm(const GemmArgs &a, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
-  dim3 grid((a.N + GT_N - 1) / GT_N, (a.M + GT_M - 1) / GT_M);
+  const dim3 grid((unsigned)gemm_workgroups(a));
   if (ev0) hipExtLaunchKernelGGL(gemm_seq_kernel, grid, dim3(256), 0, st, ev0, ev1, 0, a);
   else hipLaunchKernelGGL(gemm_seq_kernel, grid, dim3(256), 0, st, a);
   return hipGetLastError();
