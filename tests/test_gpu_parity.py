@@ -136,6 +136,29 @@ def test_full_width_bitwise(oracle, device, i2, j2, n0, form):
     plan.close()
 
 
+@pytest.mark.parametrize("i2,j2,n0,nrhs", [(96, 48, 3, 40), (192, 40, 3, 20)])
+def test_full_width_many_rhs_bitwise(oracle, device, i2, j2, n0, nrhs):
+    """The plain pass with several 16-row tiles of right-hand sides per product
+    (3 and 2; the tiles of one column tile share an XCD) and K past one
+    64-term chunk (the grouped sums and the chunk tail): the first, middle
+    and last right-hand sides bit for bit against the oracle's REP1."""
+    coefs, ie, mask = make(i2, j2, n0, metric="latlon", lat_max=80.0)
+    print(f"[full-many] i2={i2} j2={j2} n0={n0} nrhs={nrhs}", flush=True)
+    plan = S.SevpPlan(coefs, ie, form=S.FORM_FULL_PERIODIC, nrhs_max=nrhs)
+    plan.build_influence()
+    r_o, r1_o, _ = oracle.pre1(coefs, ie)
+    s = synth.make_rhs(mask, 11, nrhs)
+    x0 = rand_x(i2, j2, nrhs, 12)
+    xg = x0.copy(order="F")
+    plan.solve(s, xg)
+    for r in (0, nrhs // 2, nrhs - 1):
+        xr = np.asfortranarray(x0[:, :, r])
+        oracle.rep1(coefs, r_o, r1_o, s[:, :, r], xr, ie)
+        assert all_finite(xr)
+        assert fields_equal(xg[:, :, r], xr), f"rhs {r}"
+    plan.close()
+
+
 def test_entry_points_bitwise(oracle, device):
     i2, j2, n0 = 48, 24, 4
     coefs, ie, mask = make(i2, j2, n0, metric="mercator")
