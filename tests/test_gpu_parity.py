@@ -334,9 +334,14 @@ def test_refined_solve_reaches_tolerance(oracle, device, precision):
     plan.close()
 
 
-def test_residual_dev_matches_oracle(oracle, device):
+@pytest.mark.parametrize("i2", [64, 300, 600])
+def test_residual_dev_matches_oracle(oracle, device, i2):
+    """The FP64 residual norms against the oracle's p_sqprod, at widths whose
+    rows give a thread one, two and three columns (the kernels that load
+    that many columns' operands together)."""
     import torch
-    i2, j2 = 64, 30
+    j2 = 30
+    print(f"[resid] i2={i2}", flush=True)
     plan, coefs, ie, s, s_d, x_d = _device_case(i2, j2, 3, 2, S.PREC_FP64)
     x = rand_x(i2, j2, 2, 1)
     x_d.copy_(torch.from_numpy(np.ascontiguousarray(x.transpose(2, 1, 0))))
