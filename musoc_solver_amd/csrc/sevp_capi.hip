@@ -883,7 +883,7 @@ __device__ __forceinline__ void resid_norm_body(const float *al, const float *ab
   }
 }
 
-// U: columns per thread loaded together (resid_norm_cols).
+// U: columns per thread loaded together (chosen by launch_resid_norm_cols).
 template <int U>
 __global__ __launch_bounds__(256) void resid_norm_kernel(const float *al, const float *ab,
                                                          const float *ac, const float *ar,
@@ -1023,7 +1023,6 @@ static void launch_resid_norm_cols(sevp_plan p, const float *s, const double *s6
 static int residual_norms(sevp_plan p, const float *s, const double *s64, long s_stride,
                           const double *x, long x_stride, double *rout, int nrhs,
                           std::vector<double> &rel) {
-  const int i2 = p->d.i2, j2 = p->d.j2;
   p->prof.start(SEVP_KCLASS_RESID);
   launch_resid_norm_cols(p, s, s64, s_stride, x, x_stride, rout, nrhs, nullptr, p->stream);
   HIPC(hipGetLastError());
